@@ -1,0 +1,200 @@
+/*
+ * openmoc_amd.h -- C ABI of the MI355X (gfx950) method-of-characteristics
+ * source-iteration library.
+ *
+ * The library replaces the reference OpenMOC Solver back-end (CPUSolver /
+ * GPUSolver, src/Solver.h:121-585) on the hot path: transport sweep, FSR
+ * sources, scalar-flux update, keff, normalisation and residual all run in
+ * hand-written HIP kernels on one GPU.  The CPU producer (problem parsing,
+ * quadrature, track laydown and ray tracing) is exposed too, as the
+ * counterpart of the reference TrackGenerator.
+ *
+ * Conventions
+ *   - Every function returns MOC_OK (0) or a negative MOC_ERR_* code; the
+ *     message of the last failure on the calling thread is moc_last_error().
+ *     Nothing throws across this boundary (reference: log_printf(ERROR)
+ *     throws std::logic_error, src/log.cpp:588-599).
+ *   - Groups are 1-based where the reference is 1-based (getFlux), 0-based in
+ *     flat arrays ([fsr*G + g]).
+ *   - Inputs are copied at call time; no caller buffer is retained
+ *     (deviation: reference CPUSolver::setFluxes aliases the caller's buffer,
+ *     src/CPUSolver.cpp:180-192).
+ *   - A solver handle may be used from one host thread at a time.
+ */
+#ifndef OPENMOC_AMD_H_
+#define OPENMOC_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOC_OK 0
+#define MOC_ERR_INVALID (-1)   /* bad argument / id / size */
+#define MOC_ERR_STATE (-2)     /* call out of order (e.g. no tracks set) */
+#define MOC_ERR_HIP (-3)       /* HIP runtime failure */
+#define MOC_ERR_MEMORY (-4)    /* allocation / admission refused */
+#define MOC_ERR_NO_DEVICE (-5) /* no GPU visible */
+
+/* residualType (src/Solver.h / constants) */
+#define MOC_SCALAR_FLUX 0
+#define MOC_FISSION_SOURCE 1
+#define MOC_TOTAL_SOURCE 2
+
+/* exponential evaluation in the sweep */
+#define MOC_EXP_RATIONAL 0   /* the reference's 5/6 rational fit (default) */
+#define MOC_EXP_HW 1         /* hardware exp2: 1 - 2^(-x log2 e) */
+
+/* Thread-local text of the last error ("" if none). */
+const char* moc_last_error(void);
+
+/* Library version string. */
+const char* moc_version(void);
+
+/* ========================================================================
+ * CPU producer: counterpart of TrackGenerator::generateTracks
+ * (src/TrackGenerator.cpp:796-880) for a Cartesian pin lattice described by
+ * a "moc-problem 1" text.
+ * ====================================================================== */
+typedef struct moc_tracks moc_tracks_t;
+
+typedef struct {
+  int32_t num_groups;
+  int32_t num_materials;
+  int32_t num_azim;        /* full circle */
+  int32_t num_polar;       /* full sphere */
+  int64_t num_tracks;
+  int64_t num_segments;
+  int64_t num_fsrs;
+  double max_optical_length;
+  double tolerance;        /* from the problem's solver line */
+  int32_t max_iters;
+  int32_t res_type;
+} moc_tracks_info_t;
+
+/* Parses the problem, lays down and traces the tracks (threads<=0: all
+ * cores).  Replaces Geometry::initializeFlatSourceRegions +
+ * TrackGenerator::generateTracks. */
+int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** out);
+void moc_tracks_destroy(moc_tracks_t* tracks);
+int moc_tracks_info(const moc_tracks_t* tracks, moc_tracks_info_t* info);
+
+/* Copies track arrays out (any pointer may be NULL to skip). Sizes follow
+ * moc_tracks_info. */
+int moc_tracks_export_tracks(const moc_tracks_t* tracks, int64_t* seg_offset /*T+1*/,
+                             int32_t* azim /*T*/, int32_t* xy /*T*/,
+                             double* start_xy /*2T*/, double* end_xy /*2T*/,
+                             int64_t* next_fwd, int64_t* next_bwd,
+                             uint8_t* next_fwd_is_fwd, uint8_t* next_bwd_is_fwd,
+                             uint8_t* bc_fwd, uint8_t* bc_bwd);
+int moc_tracks_export_segments(const moc_tracks_t* tracks, double* length /*S*/,
+                               int32_t* fsr /*S*/);
+int moc_tracks_export_fsrs(const moc_tracks_t* tracks, int32_t* material /*N*/,
+                           double* volume /*N*/, double* centroid /*2N*/, int64_t* key /*N*/);
+/* Quadrature after laydown: phi/spacing/azim_weight [num_azim/2],
+ * sin_theta/polar_weight [num_polar], total_weight [num_azim/2 * num_polar]. */
+int moc_tracks_export_quadrature(const moc_tracks_t* tracks, double* phi, double* spacing,
+                                 double* azim_weight, double* sin_theta,
+                                 double* polar_weight, double* total_weight);
+/* Materials: sigma_t/nu_sigma_f/chi [M*G], sigma_s [M*G*G] dest-major. */
+int moc_tracks_export_materials(const moc_tracks_t* tracks, double* sigma_t,
+                                double* sigma_s, double* nu_sigma_f, double* chi);
+
+/* ========================================================================
+ * Solver: counterpart of the reference Solver API implemented on one GPU.
+ * ====================================================================== */
+typedef struct moc_solver moc_solver_t;
+
+/* Track data handed to the solver (Solver::setTrackGenerator,
+ * src/Solver.cpp:412-453; device upload as GPUSolver::initializeTracks,
+ * src/accel/cuda/GPUSolver.cu:1312-1353). 2D tracks, uid order.  Segment
+ * lengths are stored on the device as float; FSR ids as int32. */
+typedef struct {
+  int32_t num_azim;                 /* full circle */
+  int32_t num_polar;                /* full sphere; the 2D sweep uses num_polar/2 */
+  int64_t num_tracks;               /* T */
+  int64_t num_segments;             /* S */
+  const int64_t* seg_offset;        /* [T+1] */
+  const double* seg_length;         /* [S] cm */
+  const int32_t* seg_fsr;           /* [S] */
+  const int32_t* azim;              /* [T] azimuthal index in [0, num_azim/2) */
+  const int64_t* next_fwd;          /* [T] Track::getTrackNextFwd */
+  const int64_t* next_bwd;          /* [T] Track::getTrackNextBwd */
+  const uint8_t* next_fwd_is_fwd;   /* [T] Track::getNextFwdFwd */
+  const uint8_t* next_bwd_is_fwd;   /* [T] Track::getNextBwdFwd */
+  const uint8_t* bc_fwd;            /* [T] 0 vacuum, 1 reflective, 2 periodic */
+  const uint8_t* bc_bwd;            /* [T] */
+  const double* total_weight;       /* [num_azim/2 * num_polar] Quadrature w(a,p) */
+  const double* sin_theta;          /* [num_polar] */
+} moc_track_data_t;
+
+/* device < 0: current device. */
+int moc_solver_create(int device, moc_solver_t** out);
+void moc_solver_destroy(moc_solver_t* solver);
+
+/* Solver::setTrackGenerator */
+int moc_solver_set_tracks(moc_solver_t* solver, const moc_track_data_t* tracks);
+/* Geometry materials (Material::setSigma*, src/Material.cpp:628-979):
+ * sigma_t/nu_sigma_f/chi [M*G], sigma_s [M*G*G] dest-major s[dest*G+orig]. */
+int moc_solver_set_materials(moc_solver_t* solver, int32_t num_materials, int32_t num_groups,
+                             const double* sigma_t, const double* sigma_s,
+                             const double* nu_sigma_f, const double* chi);
+/* Solver::initializeFSRs (volumes and materials, src/Solver.cpp:820-873). */
+int moc_solver_set_fsrs(moc_solver_t* solver, int64_t num_fsrs, const int32_t* material,
+                        const double* volume);
+
+int moc_solver_set_convergence_threshold(moc_solver_t* solver, double threshold);
+int moc_solver_set_exponential(moc_solver_t* solver, int mode /* MOC_EXP_* */);
+int moc_solver_allow_negative_fluxes(moc_solver_t* solver, int allow);
+/* Solver::setFixedSourceByFSR (1-based group). */
+int moc_solver_set_fixed_source_by_fsr(moc_solver_t* solver, int64_t fsr, int32_t group,
+                                       double source);
+
+/* Solver::computeEigenvalue(max_iters, res_type) (src/Solver.cpp:1542-1689). */
+int moc_solver_compute_eigenvalue(moc_solver_t* solver, int32_t max_iters, int32_t res_type);
+/* Solver::computeFlux(max_iters, only_fixed_source) (src/Solver.cpp:1352-1420). */
+int moc_solver_compute_flux(moc_solver_t* solver, int32_t max_iters, int32_t only_fixed_source);
+/* Solver::computeSource(max_iters, k_eff, res_type) (src/Solver.cpp:1459-1516). */
+int moc_solver_compute_source(moc_solver_t* solver, int32_t max_iters, double k_eff,
+                              int32_t res_type);
+
+int moc_solver_get_keff(const moc_solver_t* solver, double* keff);
+int moc_solver_get_num_iterations(const moc_solver_t* solver, int32_t* iters);
+/* Solver::getFlux(fsr_id, group) with 1-based group (src/Solver.cpp:298-321). */
+int moc_solver_get_flux(moc_solver_t* solver, int64_t fsr, int32_t group, double* flux);
+/* CPUSolver::getFluxes(out, N*G) (src/CPUSolver.cpp:66-130). */
+int moc_solver_get_fluxes(moc_solver_t* solver, double* out, int64_t num_fluxes);
+/* CPUSolver::setFluxes: copied in, used as the initial guess of the next
+ * computeEigenvalue / computeSource (src/CPUSolver.cpp:166-192). */
+int moc_solver_set_fluxes(moc_solver_t* solver, const double* in, int64_t num_fluxes);
+/* CPUSolver::computeFSRFissionRates(rates, N) (src/CPUSolver.cpp:2850-2890). */
+int moc_solver_compute_fsr_fission_rates(moc_solver_t* solver, double* rates, int64_t num_fsrs);
+
+/* Timing of the last solve: device time of all sweeps (s), total wall time
+ * (s), number of integrations (2 * fluxes_per_track * segments * iters,
+ * src/Solver.cpp:1901-1905). */
+int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
+                          double* total_seconds, int64_t* num_integrations);
+
+/* One transport sweep on caller-given reduced sources Q [N*G] (per
+ * steradian, CPUSolver::_reduced_sources) and start angular fluxes
+ * psi [T*2*(P*G)] in the reference layout ((t*2+d)*fpt + p*G + g), P =
+ * num_polar/2.  Writes the raw FSR tally [N*G] (CPUSolver::_scalar_flux right
+ * after transportSweep, before addSourceToScalarFlux) and replaces psi with
+ * the transferred outgoing fluxes (CPUSolver::_start_flux after the sweep).
+ * Host pointers; for parity checks of the sweep kernel in isolation. */
+int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
+                               double* tally);
+
+/* Device-side sweep timing: runs `repeats` sweeps back to back on the
+ * solver's current state and returns the mean kernel time (s) measured with
+ * HIP events on the sweep's stream. */
+int moc_solver_time_sweeps(moc_solver_t* solver, int32_t repeats, double* seconds_per_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENMOC_AMD_H_ */

@@ -1,0 +1,12 @@
+"""openmoc_amd: the OpenMOC method-of-characteristics source iteration
+(transport sweep, FSR sources, scalar-flux update, keff, normalisation,
+residual) on one AMD MI355X (gfx950) in hand-written HIP kernels, behind the
+reference Solver interface.
+
+The HIP library is built in-tree (``python -m openmoc_amd.build``); there is
+no CPU fallback for the solver.
+"""
+from .workloads import Problem, get as get_workload  # noqa: F401
+from .solver import (EXP_HW, EXP_RATIONAL, FISSION_SOURCE, SCALAR_FLUX,  # noqa: F401
+                     TOTAL_SOURCE, GPUSolver, HipSolver, MOCError, TrackGenerator)
+from . import process  # noqa: F401

@@ -1,0 +1,299 @@
+// extern "C" boundary of libopenmoc_amd: status codes + thread-local error
+// text; no exception crosses this file.
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/openmoc_amd.h"
+#include "hip_solver.h"
+#include "tracks.h"
+
+struct moc_tracks {
+  moc::Problem problem;
+  moc::TrackSet ts;
+};
+
+struct moc_solver {
+  moc::HipSolver* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return MOC_OK;
+  } catch (const moc::DeviceError& e) {
+    return fail(e.code(), e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(MOC_ERR_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(MOC_ERR_INVALID, e.what());
+  } catch (...) {
+    return fail(MOC_ERR_INVALID, "unknown error");
+  }
+}
+
+int residual_code(const std::string& s) {
+  if (s == "SCALAR_FLUX") return MOC_SCALAR_FLUX;
+  if (s == "TOTAL_SOURCE") return MOC_TOTAL_SOURCE;
+  return MOC_FISSION_SOURCE;
+}
+
+template <typename T>
+void copy_out(T* dst, const std::vector<T>& src) {
+  if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(T));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* moc_last_error(void) { return g_last_error.c_str(); }
+
+const char* moc_version(void) { return "openmoc_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------- tracks
+int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** out) {
+  if (!problem_text || !out) return fail(MOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto* t = new moc_tracks;
+    try {
+      t->problem = moc::parse_problem(problem_text);
+      t->ts = moc::generate_tracks(t->problem, threads);
+      // the exponential evaluators support optical lengths up to 100
+      // (MAX_OPTICAL_LENGTH, src/constants.h; Solver::initializeExpEvaluators)
+      moc::split_segments(t->ts, t->problem, 100.0);
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+void moc_tracks_destroy(moc_tracks_t* tracks) { delete tracks; }
+
+int moc_tracks_info(const moc_tracks_t* t, moc_tracks_info_t* info) {
+  if (!t || !info) return fail(MOC_ERR_INVALID, "null argument");
+  info->num_groups = t->problem.num_groups;
+  info->num_materials = (int32_t)t->problem.materials.size();
+  info->num_azim = t->problem.num_azim;
+  info->num_polar = t->problem.num_polar;
+  info->num_tracks = t->ts.num_tracks;
+  info->num_segments = (int64_t)t->ts.seg_length.size();
+  info->num_fsrs = t->ts.num_fsrs;
+  info->max_optical_length = t->ts.max_optical_length;
+  info->tolerance = t->problem.tolerance;
+  info->max_iters = t->problem.max_iters;
+  info->res_type = residual_code(t->problem.res_type);
+  return MOC_OK;
+}
+
+int moc_tracks_export_tracks(const moc_tracks_t* t, int64_t* seg_offset, int32_t* azim,
+                             int32_t* xy, double* start_xy, double* end_xy, int64_t* next_fwd,
+                             int64_t* next_bwd, uint8_t* next_fwd_is_fwd,
+                             uint8_t* next_bwd_is_fwd, uint8_t* bc_fwd, uint8_t* bc_bwd) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  const moc::TrackSet& s = t->ts;
+  copy_out(seg_offset, s.seg_offset);
+  copy_out(azim, s.azim);
+  copy_out(xy, s.xy);
+  if (start_xy)
+    for (long i = 0; i < s.num_tracks; i++) {
+      start_xy[2 * i] = s.start_x[i];
+      start_xy[2 * i + 1] = s.start_y[i];
+    }
+  if (end_xy)
+    for (long i = 0; i < s.num_tracks; i++) {
+      end_xy[2 * i] = s.end_x[i];
+      end_xy[2 * i + 1] = s.end_y[i];
+    }
+  copy_out(next_fwd, s.next_fwd);
+  copy_out(next_bwd, s.next_bwd);
+  copy_out(next_fwd_is_fwd, s.next_fwd_is_fwd);
+  copy_out(next_bwd_is_fwd, s.next_bwd_is_fwd);
+  copy_out(bc_fwd, s.bc_fwd);
+  copy_out(bc_bwd, s.bc_bwd);
+  return MOC_OK;
+}
+
+int moc_tracks_export_segments(const moc_tracks_t* t, double* length, int32_t* fsr) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  copy_out(length, t->ts.seg_length);
+  copy_out(fsr, t->ts.seg_fsr);
+  return MOC_OK;
+}
+
+int moc_tracks_export_fsrs(const moc_tracks_t* t, int32_t* material, double* volume,
+                           double* centroid, int64_t* key) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  copy_out(material, t->ts.fsr_material);
+  copy_out(volume, t->ts.fsr_volume);
+  copy_out(centroid, t->ts.fsr_centroid);
+  copy_out(key, t->ts.fsr_key);
+  return MOC_OK;
+}
+
+int moc_tracks_export_quadrature(const moc_tracks_t* t, double* phi, double* spacing,
+                                 double* azim_weight, double* sin_theta, double* polar_weight,
+                                 double* total_weight) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  const moc::Quadrature& q = t->ts.quad;
+  copy_out(phi, q.phi);
+  copy_out(spacing, q.azim_spacing);
+  copy_out(azim_weight, q.azim_weight);
+  copy_out(sin_theta, q.sin_theta);
+  copy_out(polar_weight, q.polar_weight);
+  copy_out(total_weight, q.total_weight);
+  return MOC_OK;
+}
+
+int moc_tracks_export_materials(const moc_tracks_t* t, double* sigma_t, double* sigma_s,
+                                double* nu_sigma_f, double* chi) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  const int G = t->problem.num_groups;
+  size_t m = 0;
+  for (const moc::MaterialData& md : t->problem.materials) {
+    if (sigma_t) std::memcpy(sigma_t + m * G, md.sigma_t.data(), G * sizeof(double));
+    if (sigma_s) std::memcpy(sigma_s + m * G * G, md.sigma_s.data(), (size_t)G * G * sizeof(double));
+    if (nu_sigma_f) std::memcpy(nu_sigma_f + m * G, md.nu_sigma_f.data(), G * sizeof(double));
+    if (chi) std::memcpy(chi + m * G, md.chi.data(), G * sizeof(double));
+    m++;
+  }
+  return MOC_OK;
+}
+
+// ---------------------------------------------------------------- solver
+int moc_solver_create(int device, moc_solver_t** out) {
+  if (!out) return fail(MOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto* s = new moc_solver;
+    s->impl = nullptr;
+    try {
+      s->impl = new moc::HipSolver(device);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+void moc_solver_destroy(moc_solver_t* s) {
+  if (!s) return;
+  delete s->impl;
+  delete s;
+}
+
+#define MOC_SOLVER_CALL(solver, body)                                   \
+  do {                                                                  \
+    if (!(solver) || !(solver)->impl) return fail(MOC_ERR_INVALID, "null solver"); \
+    return guarded([&] { body; });                                      \
+  } while (0)
+
+int moc_solver_set_tracks(moc_solver_t* s, const moc_track_data_t* tracks) {
+  if (!tracks) return fail(MOC_ERR_INVALID, "null track data");
+  MOC_SOLVER_CALL(s, s->impl->setTracks(*tracks));
+}
+
+int moc_solver_set_materials(moc_solver_t* s, int32_t num_materials, int32_t num_groups,
+                             const double* sigma_t, const double* sigma_s,
+                             const double* nu_sigma_f, const double* chi) {
+  MOC_SOLVER_CALL(s, s->impl->setMaterials(num_materials, num_groups, sigma_t, sigma_s,
+                                           nu_sigma_f, chi));
+}
+
+int moc_solver_set_fsrs(moc_solver_t* s, int64_t num_fsrs, const int32_t* material,
+                        const double* volume) {
+  MOC_SOLVER_CALL(s, s->impl->setFSRs(num_fsrs, material, volume));
+}
+
+int moc_solver_set_convergence_threshold(moc_solver_t* s, double threshold) {
+  MOC_SOLVER_CALL(s, s->impl->setConvergenceThreshold(threshold));
+}
+
+int moc_solver_set_exponential(moc_solver_t* s, int mode) {
+  MOC_SOLVER_CALL(s, s->impl->setExponential(mode));
+}
+
+int moc_solver_allow_negative_fluxes(moc_solver_t* s, int allow) {
+  MOC_SOLVER_CALL(s, s->impl->allowNegativeFluxes(allow != 0));
+}
+
+int moc_solver_set_fixed_source_by_fsr(moc_solver_t* s, int64_t fsr, int32_t group,
+                                       double source) {
+  MOC_SOLVER_CALL(s, s->impl->setFixedSourceByFSR(fsr, group, source));
+}
+
+int moc_solver_compute_eigenvalue(moc_solver_t* s, int32_t max_iters, int32_t res_type) {
+  MOC_SOLVER_CALL(s, s->impl->computeEigenvalue(max_iters, res_type));
+}
+
+int moc_solver_compute_flux(moc_solver_t* s, int32_t max_iters, int32_t only_fixed_source) {
+  MOC_SOLVER_CALL(s, s->impl->computeFlux(max_iters, only_fixed_source != 0));
+}
+
+int moc_solver_compute_source(moc_solver_t* s, int32_t max_iters, double k_eff,
+                              int32_t res_type) {
+  MOC_SOLVER_CALL(s, s->impl->computeSource(max_iters, k_eff, res_type));
+}
+
+int moc_solver_get_keff(const moc_solver_t* s, double* keff) {
+  if (!s || !s->impl || !keff) return fail(MOC_ERR_INVALID, "null argument");
+  *keff = s->impl->getKeff();
+  return MOC_OK;
+}
+
+int moc_solver_get_num_iterations(const moc_solver_t* s, int32_t* iters) {
+  if (!s || !s->impl || !iters) return fail(MOC_ERR_INVALID, "null argument");
+  *iters = s->impl->getNumIterations();
+  return MOC_OK;
+}
+
+int moc_solver_get_flux(moc_solver_t* s, int64_t fsr, int32_t group, double* flux) {
+  if (!flux) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_SOLVER_CALL(s, *flux = s->impl->getFlux(fsr, group));
+}
+
+int moc_solver_get_fluxes(moc_solver_t* s, double* out, int64_t num_fluxes) {
+  MOC_SOLVER_CALL(s, s->impl->getFluxes(out, num_fluxes));
+}
+
+int moc_solver_set_fluxes(moc_solver_t* s, const double* in, int64_t num_fluxes) {
+  MOC_SOLVER_CALL(s, s->impl->setFluxes(in, num_fluxes));
+}
+
+int moc_solver_compute_fsr_fission_rates(moc_solver_t* s, double* rates, int64_t num_fsrs) {
+  MOC_SOLVER_CALL(s, s->impl->computeFSRFissionRates(rates, num_fsrs));
+}
+
+int moc_solver_get_timing(const moc_solver_t* s, double* sweep_seconds, double* total_seconds,
+                          int64_t* num_integrations) {
+  if (!s || !s->impl) return fail(MOC_ERR_INVALID, "null solver");
+  s->impl->timing(sweep_seconds, total_seconds, num_integrations);
+  return MOC_OK;
+}
+
+int moc_solver_transport_sweep(moc_solver_t* s, const double* Q, float* psi, double* tally) {
+  MOC_SOLVER_CALL(s, s->impl->transportSweepOnce(Q, psi, tally));
+}
+
+int moc_solver_time_sweeps(moc_solver_t* s, int32_t repeats, double* seconds_per_sweep) {
+  if (!seconds_per_sweep) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_SOLVER_CALL(s, *seconds_per_sweep = s->impl->timeSweeps(repeats));
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// HipSolver: the MOC source iteration on one MI355X (gfx950).
+//
+// Host-side counterpart of the reference Solver API (src/Solver.h:121-585),
+// with every per-iteration step in hand-written HIP kernels:
+//   transportSweep            -> sweep2d_kernel  (FP32, LDS-free, atomics)
+//   addSourceToScalarFlux +
+//   fission-rate partials     -> post_sweep_kernel (FP64)
+//   computeKeff + normalise   -> keff_kernel (FP64, 1 block, fixed order)
+//   normalise + residual +
+//   storeFSRFluxes +
+//   computeFSRSources (next)  -> update_kernel (FP64)
+//   residual / convergence    -> residual_kernel (FP64, 1 block)
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/openmoc_amd.h"
+
+namespace moc {
+
+// Iteration state kept on the device; the host reads it once per batch.
+struct IterState {
+  double k;             // current eigenvalue
+  double k_prev;        // eigenvalue of the previous iteration
+  double norm_factor;   // last normalisation factor (N / total fission)
+  double psi_scale;     // factor applied to start fluxes on load
+  double residual;
+  double fission_total;
+  int iteration;        // completed iterations of the current solve
+  int converged;        // 1 once the stopping rule is met
+  int dk;               // int(1e5 * (k - k_prev)) of the last iteration
+  int reported_iters;   // Solver::_num_iterations semantics
+  int64_t negative_fluxes;
+  int64_t negative_sources;
+};
+
+struct LaunchShape {
+  int groups_per_lane = 1;   // template parameter of the sweep
+  int lanes_per_track = 1;   // ceil(G / groups_per_lane)
+  int tracks_per_wave = 1;   // 64 / lanes_per_track
+  int block = 256;           // threads per block (multiple of 64)
+};
+
+class DeviceError : public std::exception {
+ public:
+  DeviceError(int code, std::string msg) : code_(code), msg_(std::move(msg)) {}
+  const char* what() const noexcept override { return msg_.c_str(); }
+  int code() const { return code_; }
+
+ private:
+  int code_;
+  std::string msg_;
+};
+
+class HipSolver {
+ public:
+  explicit HipSolver(int device);
+  ~HipSolver();
+  HipSolver(const HipSolver&) = delete;
+  HipSolver& operator=(const HipSolver&) = delete;
+
+  void setTracks(const moc_track_data_t& tracks);
+  void setMaterials(int num_materials, int num_groups, const double* sigma_t,
+                    const double* sigma_s, const double* nu_sigma_f, const double* chi);
+  void setFSRs(int64_t num_fsrs, const int32_t* material, const double* volume);
+  void setConvergenceThreshold(double t);
+  void setExponential(int mode);
+  void allowNegativeFluxes(bool allow) { allow_negative_ = allow; }
+  void setFixedSourceByFSR(int64_t fsr, int group, double source);
+
+  void computeEigenvalue(int max_iters, int res_type);
+  void computeFlux(int max_iters, bool only_fixed_source);
+  void computeSource(int max_iters, double k_eff, int res_type);
+
+  double getKeff() const { return keff_; }
+  int getNumIterations() const { return num_iterations_; }
+  double getFlux(int64_t fsr, int group);
+  void getFluxes(double* out, int64_t n);
+  void setFluxes(const double* in, int64_t n);
+  void computeFSRFissionRates(double* out, int64_t n);
+  void timing(double* sweep_s, double* total_s, int64_t* integrations) const;
+
+  void transportSweepOnce(const double* Q, float* psi, double* tally);
+  double timeSweeps(int repeats);
+
+  const LaunchShape& shape() const { return shape_; }
+  void setShape(const LaunchShape& s);
+
+ private:
+  enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
+  void requireReady() const;
+  void allocateIterationArrays();
+  void runSolve(Mode mode, int max_iters, int res_type, double k_fixed);
+  void launchSweep(const float* psi_in, float* psi_out, hipStream_t s);
+  void launchIteration(Mode mode, int res_type, bool recompute_sources);
+  void pickShape();
+  void countFissionable();
+
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+
+  // sizes
+  int G_ = 0, M_ = 0, P_ = 0, num_azim_ = 0, num_polar_ = 0;
+  int64_t T_ = 0, S_ = 0, N_ = 0;
+  int64_t num_fissionable_ = 0;
+  bool tracks_set_ = false, materials_set_ = false, fsrs_set_ = false;
+  double threshold_ = 1e-5;
+  int exp_mode_ = MOC_EXP_RATIONAL;
+  bool allow_negative_ = false;
+  LaunchShape shape_;
+  bool shape_forced_ = false;
+
+  // device arrays (raw pointers owned by this object)
+  float* d_seg_len_ = nullptr;
+  int32_t* d_seg_fsr_ = nullptr;
+  int32_t* d_work_ = nullptr;          // directed tracks, longest first
+  int32_t* d_trk_ = nullptr;           // [T][4] seg_begin, seg_count, out_fwd, out_bwd
+  int32_t* d_trk_azim_ = nullptr;
+  float* d_weights_ = nullptr;         // [num_azim/2][P]
+  float inv_sin_[4] = {0, 0, 0, 0};
+  float* d_psi_[2] = {nullptr, nullptr};
+  float* d_tally_ = nullptr;           // [N*G]
+  float2* d_qs_ = nullptr;             // [N*G] (Q/sigma_t, sigma_t)
+  double* d_phi_ = nullptr;            // [N*G]
+  double* d_phi_old_ = nullptr;        // [N*G]
+  double* d_Q_ = nullptr;              // [N*G]
+  double* d_fixed_ = nullptr;          // [N*G]
+  double* d_sigma_t_ = nullptr;        // [M*G]
+  double* d_sigma_s_ = nullptr;        // [M*G*G]
+  double* d_nsf_ = nullptr;            // [M*G]
+  double* d_chi_ = nullptr;            // [M*G]
+  int32_t* d_mat_fissionable_ = nullptr;  // [M]
+  int32_t* d_fsr_mat_ = nullptr;       // [N]
+  double* d_fsr_vol_ = nullptr;        // [N]
+  double* d_partials_ = nullptr;       // [max blocks]
+  IterState* d_state_ = nullptr;
+  IterState* h_state_ = nullptr;       // pinned
+  int num_partials_ = 0;
+  int psi_cur_ = 0;                    // which psi buffer holds the start fluxes
+
+  int32_t max_seg_fsr_ = -1;
+  std::vector<int32_t> mat_fissionable_;  // host copy, per material
+  std::vector<int32_t> h_fsr_mat_;        // host copy, per FSR
+  std::vector<double> h_fixed_;        // host copy of fixed sources
+  bool fixed_on_ = false;
+
+  // results
+  double keff_ = 1.0;
+  int num_iterations_ = 0;
+  double sweep_seconds_ = 0, total_seconds_ = 0;
+  int64_t integrations_ = 0;
+};
+
+}  // namespace moc

@@ -1,0 +1,304 @@
+"""Host-side mirror of the reference OpenMOC Python interface for the hot path.
+
+``TrackGenerator`` is the counterpart of ``openmoc.TrackGenerator`` (CPU
+producer: laydown + ray tracing) and ``HipSolver`` (alias ``GPUSolver``) the
+counterpart of ``openmoc.CPUSolver`` / ``openmoc.cuda.GPUSolver``: same method
+names, argument meaning and error behaviour (errors raise ``RuntimeError``
+subclasses, as SWIG maps ``log_printf(ERROR)``, openmoc/swig/openmoc.i:106-112).
+
+Usage, as in the reference's sample inputs
+(sample-input/benchmarks/c5g7/c5g7-2d.py:88-95)::
+
+    tg = TrackGenerator(problem)        # problem: workloads.Problem or text
+    tg.generateTracks()
+    solver = HipSolver(tg)
+    solver.setConvergenceThreshold(1e-5)
+    solver.computeEigenvalue(1000)
+    keff = solver.getKeff()
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import (EXP_HW, EXP_RATIONAL, FISSION_SOURCE, SCALAR_FLUX, TOTAL_SOURCE,
+                   MOCError, check, lib)
+from .workloads import Problem
+
+__all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "SCALAR_FLUX", "FISSION_SOURCE",
+           "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "MOCError"]
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class TrackGenerator:
+    """Lays down and ray-traces 2D tracks over a Cartesian pin lattice."""
+
+    def __init__(self, problem: Union[Problem, str], num_threads: int = 0):
+        self.problem = problem if isinstance(problem, Problem) else Problem.from_text(problem)
+        self._text = self.problem.to_text() if isinstance(problem, Problem) else problem
+        self._threads = int(num_threads)
+        self._h = None
+        self.info = None
+
+    def setNumThreads(self, num_threads: int) -> None:
+        self._threads = int(num_threads)
+
+    def generateTracks(self) -> None:
+        h = C.c_void_p()
+        check(lib().moc_tracks_generate(self._text.encode(), self._threads, C.byref(h)))
+        self._free()
+        self._h = h
+        info = _lib.TracksInfo()
+        check(lib().moc_tracks_info(h, C.byref(info)))
+        self.info = info
+        self._export()
+
+    def _export(self) -> None:
+        i = self.info
+        T, S, N, G, M = i.num_tracks, i.num_segments, i.num_fsrs, i.num_groups, i.num_materials
+        na2, npol = i.num_azim // 2, i.num_polar
+        self.seg_offset = np.zeros(T + 1, np.int64)
+        self.azim = np.zeros(T, np.int32)
+        self.xy = np.zeros(T, np.int32)
+        self.start = np.zeros((T, 2))
+        self.end = np.zeros((T, 2))
+        self.next_fwd = np.zeros(T, np.int64)
+        self.next_bwd = np.zeros(T, np.int64)
+        self.next_fwd_is_fwd = np.zeros(T, np.uint8)
+        self.next_bwd_is_fwd = np.zeros(T, np.uint8)
+        self.bc_fwd = np.zeros(T, np.uint8)
+        self.bc_bwd = np.zeros(T, np.uint8)
+        check(lib().moc_tracks_export_tracks(
+            self._h, _ptr(self.seg_offset), _ptr(self.azim), _ptr(self.xy), _ptr(self.start),
+            _ptr(self.end), _ptr(self.next_fwd), _ptr(self.next_bwd), _ptr(self.next_fwd_is_fwd),
+            _ptr(self.next_bwd_is_fwd), _ptr(self.bc_fwd), _ptr(self.bc_bwd)))
+        self.seg_length = np.zeros(S)
+        self.seg_fsr = np.zeros(S, np.int32)
+        check(lib().moc_tracks_export_segments(self._h, _ptr(self.seg_length), _ptr(self.seg_fsr)))
+        self.fsr_material = np.zeros(N, np.int32)
+        self.fsr_volume = np.zeros(N)
+        self.fsr_centroid = np.zeros((N, 2))
+        self.fsr_key = np.zeros(N, np.int64)
+        check(lib().moc_tracks_export_fsrs(self._h, _ptr(self.fsr_material), _ptr(self.fsr_volume),
+                                           _ptr(self.fsr_centroid), _ptr(self.fsr_key)))
+        self.phi = np.zeros(na2)
+        self.azim_spacing = np.zeros(na2)
+        self.azim_weight = np.zeros(na2)
+        self.sin_theta = np.zeros(npol)
+        self.polar_weight = np.zeros(npol)
+        self.total_weight = np.zeros((na2, npol))
+        check(lib().moc_tracks_export_quadrature(
+            self._h, _ptr(self.phi), _ptr(self.azim_spacing), _ptr(self.azim_weight),
+            _ptr(self.sin_theta), _ptr(self.polar_weight), _ptr(self.total_weight)))
+        self.sigma_t = np.zeros((M, G))
+        self.sigma_s = np.zeros((M, G * G))
+        self.nu_sigma_f = np.zeros((M, G))
+        self.chi = np.zeros((M, G))
+        check(lib().moc_tracks_export_materials(self._h, _ptr(self.sigma_t), _ptr(self.sigma_s),
+                                                _ptr(self.nu_sigma_f), _ptr(self.chi)))
+
+    # reference TrackGenerator getters
+    def getNumTracks(self) -> int:
+        return int(self.info.num_tracks)
+
+    getNum2DTracks = getNumTracks
+
+    def getNumSegments(self) -> int:
+        return int(self.info.num_segments)
+
+    def getNumFSRs(self) -> int:
+        return int(self.info.num_fsrs)
+
+    def getNumEnergyGroups(self) -> int:
+        return int(self.info.num_groups)
+
+    def getFSRVolumes(self) -> np.ndarray:
+        return self.fsr_volume
+
+    def getMaxOpticalLength(self) -> float:
+        return float(self.info.max_optical_length)
+
+    def containsSegments(self) -> bool:
+        return self._h is not None
+
+    def _free(self) -> None:
+        if self._h is not None:
+            lib().moc_tracks_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+
+class HipSolver:
+    """MOC source iteration on one MI355X; Solver API of the reference."""
+
+    def __init__(self, track_generator: Optional[TrackGenerator] = None, device: int = -1):
+        h = C.c_void_p()
+        check(lib().moc_solver_create(int(device), C.byref(h)))
+        self._h = h
+        self._tg = None
+        self._num_fsrs = 0
+        self._num_groups = 0
+        if track_generator is not None:
+            self.setTrackGenerator(track_generator)
+
+    # --- setup -----------------------------------------------------------
+    def setTrackGenerator(self, tg: TrackGenerator) -> None:
+        if not tg.containsSegments():
+            raise MOCError(-2, "Unable to set the TrackGenerator for the Solver since the "
+                               "TrackGenerator has not yet generated tracks")
+        i = tg.info
+        td = _lib.TrackData()
+        td.num_azim, td.num_polar = i.num_azim, i.num_polar
+        td.num_tracks, td.num_segments = i.num_tracks, i.num_segments
+        keep = [tg.seg_offset, tg.seg_length, tg.seg_fsr, tg.azim, tg.next_fwd, tg.next_bwd,
+                tg.next_fwd_is_fwd, tg.next_bwd_is_fwd, tg.bc_fwd, tg.bc_bwd,
+                np.ascontiguousarray(tg.total_weight), tg.sin_theta]
+        (td.seg_offset, td.seg_length, td.seg_fsr, td.azim, td.next_fwd, td.next_bwd,
+         td.next_fwd_is_fwd, td.next_bwd_is_fwd, td.bc_fwd, td.bc_bwd, td.total_weight,
+         td.sin_theta) = [a.ctypes.data for a in keep]
+        check(lib().moc_solver_set_materials(
+            self._h, i.num_materials, i.num_groups, _ptr(tg.sigma_t), _ptr(tg.sigma_s),
+            _ptr(tg.nu_sigma_f), _ptr(tg.chi)))
+        check(lib().moc_solver_set_fsrs(self._h, i.num_fsrs, _ptr(tg.fsr_material),
+                                        _ptr(tg.fsr_volume)))
+        check(lib().moc_solver_set_tracks(self._h, C.byref(td)))
+        del keep
+        self._tg = tg
+        self._num_fsrs = int(i.num_fsrs)
+        self._num_groups = int(i.num_groups)
+        self.setConvergenceThreshold(i.tolerance)
+
+    def setArrays(self, *, num_azim, num_polar, seg_offset, seg_length, seg_fsr, azim,
+                  next_fwd, next_bwd, next_fwd_is_fwd, next_bwd_is_fwd, bc_fwd, bc_bwd,
+                  total_weight, sin_theta, sigma_t, sigma_s, nu_sigma_f, chi, fsr_material,
+                  fsr_volume) -> None:
+        """Set tracks/materials/FSRs from raw arrays (e.g. tracks produced by
+        another tracer).  sigma_s is dest-major [M, G*G]."""
+        c = np.ascontiguousarray
+        arrs = dict(seg_offset=c(seg_offset, np.int64), seg_length=c(seg_length, np.float64),
+                    seg_fsr=c(seg_fsr, np.int32), azim=c(azim, np.int32),
+                    next_fwd=c(next_fwd, np.int64), next_bwd=c(next_bwd, np.int64),
+                    next_fwd_is_fwd=c(next_fwd_is_fwd, np.uint8),
+                    next_bwd_is_fwd=c(next_bwd_is_fwd, np.uint8), bc_fwd=c(bc_fwd, np.uint8),
+                    bc_bwd=c(bc_bwd, np.uint8), total_weight=c(total_weight, np.float64),
+                    sin_theta=c(sin_theta, np.float64))
+        td = _lib.TrackData()
+        td.num_azim, td.num_polar = int(num_azim), int(num_polar)
+        td.num_tracks, td.num_segments = len(arrs["azim"]), len(arrs["seg_length"])
+        for k, v in arrs.items():
+            setattr(td, k, v.ctypes.data)
+        st, ss, nsf, ch = (c(x, np.float64) for x in (sigma_t, sigma_s, nu_sigma_f, chi))
+        M, G = st.shape
+        check(lib().moc_solver_set_materials(self._h, M, G, _ptr(st), _ptr(ss), _ptr(nsf),
+                                             _ptr(ch)))
+        fm, fv = c(fsr_material, np.int32), c(fsr_volume, np.float64)
+        check(lib().moc_solver_set_fsrs(self._h, len(fm), _ptr(fm), _ptr(fv)))
+        check(lib().moc_solver_set_tracks(self._h, C.byref(td)))
+        self._num_fsrs, self._num_groups = len(fm), G
+
+    def setConvergenceThreshold(self, threshold: float) -> None:
+        check(lib().moc_solver_set_convergence_threshold(self._h, float(threshold)))
+
+    def setExponential(self, mode: int) -> None:
+        check(lib().moc_solver_set_exponential(self._h, int(mode)))
+
+    def allowNegativeFluxes(self, allow: bool) -> None:
+        check(lib().moc_solver_allow_negative_fluxes(self._h, int(bool(allow))))
+
+    def setFixedSourceByFSR(self, fsr_id: int, group: int, source: float) -> None:
+        check(lib().moc_solver_set_fixed_source_by_fsr(self._h, int(fsr_id), int(group),
+                                                       float(source)))
+
+    # --- solves ----------------------------------------------------------
+    def computeEigenvalue(self, max_iters: int = 1000, res_type: int = FISSION_SOURCE) -> None:
+        check(lib().moc_solver_compute_eigenvalue(self._h, int(max_iters), int(res_type)))
+
+    def computeFlux(self, max_iters: int = 1000, only_fixed_source: bool = True) -> None:
+        check(lib().moc_solver_compute_flux(self._h, int(max_iters), int(bool(only_fixed_source))))
+
+    def computeSource(self, max_iters: int = 1000, k_eff: float = 1.0,
+                      res_type: int = TOTAL_SOURCE) -> None:
+        check(lib().moc_solver_compute_source(self._h, int(max_iters), float(k_eff),
+                                              int(res_type)))
+
+    # --- results ---------------------------------------------------------
+    def getKeff(self) -> float:
+        v = C.c_double()
+        check(lib().moc_solver_get_keff(self._h, C.byref(v)))
+        return v.value
+
+    def getNumIterations(self) -> int:
+        v = C.c_int32()
+        check(lib().moc_solver_get_num_iterations(self._h, C.byref(v)))
+        return v.value
+
+    def getFlux(self, fsr_id: int, group: int) -> float:
+        v = C.c_double()
+        check(lib().moc_solver_get_flux(self._h, int(fsr_id), int(group), C.byref(v)))
+        return v.value
+
+    def getFluxes(self, num_fluxes: Optional[int] = None) -> np.ndarray:
+        n = self._num_fsrs * self._num_groups if num_fluxes is None else int(num_fluxes)
+        out = np.zeros(n)
+        check(lib().moc_solver_get_fluxes(self._h, _ptr(out), n))
+        return out
+
+    def setFluxes(self, fluxes: np.ndarray) -> None:
+        a = np.ascontiguousarray(fluxes, np.float64).ravel()
+        check(lib().moc_solver_set_fluxes(self._h, _ptr(a), a.size))
+
+    def computeFSRFissionRates(self, num_fsrs: Optional[int] = None) -> np.ndarray:
+        n = self._num_fsrs if num_fsrs is None else int(num_fsrs)
+        out = np.zeros(n)
+        check(lib().moc_solver_compute_fsr_fission_rates(self._h, _ptr(out), n))
+        return out
+
+    def getTiming(self):
+        sw, tot, ni = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().moc_solver_get_timing(self._h, C.byref(sw), C.byref(tot), C.byref(ni)))
+        return {"iteration_seconds": sw.value, "total_seconds": tot.value,
+                "num_integrations": ni.value}
+
+    def printTimerReport(self) -> None:
+        t = self.getTiming()
+        n = max(t["num_integrations"], 1)
+        print(f"Total time to solution ...................... {t['total_seconds']:.4e} sec")
+        print(f"Device time of source iterations ............. {t['iteration_seconds']:.4e} sec")
+        print(f"Integrations ................................. {t['num_integrations']}")
+        print(f"Integration time (device) .................... {t['iteration_seconds'] / n:.4e} sec")
+
+    # --- kernel-level entry points --------------------------------------
+    def transportSweep(self, Q: np.ndarray, psi: np.ndarray):
+        """One sweep on given reduced sources; returns (tally, psi_out)."""
+        q = np.ascontiguousarray(Q, np.float64).ravel()
+        p = np.array(psi, np.float32, copy=True).ravel()
+        tally = np.zeros(q.size)
+        check(lib().moc_solver_transport_sweep(self._h, _ptr(q), _ptr(p), _ptr(tally)))
+        return tally, p
+
+    def timeSweeps(self, repeats: int = 10) -> float:
+        v = C.c_double()
+        check(lib().moc_solver_time_sweeps(self._h, int(repeats), C.byref(v)))
+        return v.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                lib().moc_solver_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+GPUSolver = HipSolver

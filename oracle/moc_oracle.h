@@ -1,0 +1,114 @@
+/*
+ * moc_oracle: CPU restatement (FP64, OpenMP) of the reference OpenMOC
+ * source iteration, used ONLY as a checker by tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg.  It is never linked into the product.
+ *
+ * Every entry point takes plain arrays (the same SoA track layout the product
+ * consumes), so the oracle can run either on this package's tracks or on the
+ * tracks dumped from the compiled reference (oracle/ref_driver.cpp).
+ *
+ * Parity status of each function (how it is pinned) is listed in the header
+ * comment of moc_oracle.cpp.
+ */
+#ifndef MOC_ORACLE_H_
+#define MOC_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int num_groups;            /* G */
+  int num_polar_half;        /* P = num_polar/2 (2D) */
+  int num_azim_half;         /* num_azim/2 */
+  int64_t num_tracks;        /* T */
+  int64_t num_fsrs;          /* N */
+  int num_materials;         /* M */
+  /* tracks, uid order */
+  const int64_t* seg_offset;       /* [T+1] */
+  const double* seg_length;        /* [S] */
+  const int32_t* seg_fsr;          /* [S] */
+  const int32_t* trk_azim;         /* [T] */
+  const int64_t* next_fwd;         /* [T] */
+  const int64_t* next_bwd;         /* [T] */
+  const uint8_t* next_fwd_is_fwd;  /* [T] */
+  const uint8_t* next_bwd_is_fwd;  /* [T] */
+  const uint8_t* bc_fwd;           /* [T] 0 vacuum, 1 reflective, 2 periodic */
+  const uint8_t* bc_bwd;           /* [T] */
+  /* quadrature */
+  const double* total_weight;      /* [num_azim_half][P] w(a,p) */
+  const double* inv_sin_theta;     /* [P] */
+  /* materials, per material m: */
+  const double* sigma_t;           /* [M][G] */
+  const double* sigma_s;           /* [M][G*G] dest-major s[dest*G+orig] */
+  const double* nu_sigma_f;        /* [M][G] */
+  const double* chi;               /* [M][G] */
+  /* FSRs */
+  const int32_t* fsr_material;     /* [N] */
+  const double* fsr_volume;        /* [N] */
+  const double* fixed_source;      /* [N*G] or NULL */
+} OracleProblem;
+
+enum { ORACLE_SCALAR_FLUX = 0, ORACLE_FISSION_SOURCE = 1, ORACLE_TOTAL_SOURCE = 2 };
+
+typedef struct {
+  int max_iters;
+  double tolerance;
+  int res_type;               /* ORACLE_* */
+  int psi_fp64;               /* 0: float angular fluxes (reference), 1: double */
+  int threads;                /* <=0: OpenMP default */
+  int keff_from_fission_rates;/* 1: reference default */
+  int allow_negative_fluxes;  /* 0: reference default */
+} OracleOptions;
+
+typedef struct {
+  double keff;
+  int num_iterations;
+  double residual;
+  double sweep_seconds;       /* wall time inside transport sweeps */
+  int64_t num_integrations;   /* 2 * G * P * S * iterations */
+} OracleResult;
+
+/* Solver::computeEigenvalue (src/Solver.cpp:1542-1689) with CPUSolver steps.
+ * flux_out: [N*G] converged, normalised scalar flux. Returns 0 on success. */
+int oracle_compute_eigenvalue(const OracleProblem* p, const OracleOptions* o,
+                              OracleResult* r, double* flux_out);
+
+/* Solver::computeFlux(max_iters, only_fixed_source=true):
+ * src/Solver.cpp:1352-1420 (sources computed once from phi = 0). */
+int oracle_compute_flux(const OracleProblem* p, const OracleOptions* o,
+                        OracleResult* r, double* flux_out);
+
+/* Solver::computeSource(max_iters, k_eff, res_type): src/Solver.cpp:1459-1516. */
+int oracle_compute_source(const OracleProblem* p, const OracleOptions* o, double k_eff,
+                          OracleResult* r, double* flux_out);
+
+/* One CPUSolver::transportSweep (src/CPUSolver.cpp:2338-2389) on given
+ * reduced sources Q [N*G] and start fluxes psi_start [T*2*P*G] (float, the
+ * reference layout (t*2+d)*fpt + p*G + g).  Writes the raw scalar-flux tally
+ * [N*G] (before addSourceToScalarFlux) and updates psi_start in place with the
+ * transferred outgoing fluxes. */
+int oracle_transport_sweep(const OracleProblem* p, const double* Q, float* psi_start,
+                           double* tally_out, int threads);
+
+/* CPUSolver::computeFSRSources (src/CPUSolver.cpp:1939-2023). */
+int oracle_compute_fsr_sources(const OracleProblem* p, const double* phi, double keff,
+                               int iteration, int allow_negative, double* Q_out);
+
+/* CPUSolver::addSourceToScalarFlux (src/CPUSolver.cpp:2608-2659), in place. */
+int oracle_add_source_to_scalar_flux(const OracleProblem* p, const double* Q,
+                                     int allow_negative, double* phi_inout);
+
+/* Rational (1 - exp(-x))/x fit used by the reference (src/exponentials.h:156-192). */
+double oracle_exp_f1(double x);
+
+/* pairwise_sum (src/pairwise_sum.h:19-39) with VEC_LENGTH = 4 (FP64 build). */
+double oracle_pairwise_sum(const double* v, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

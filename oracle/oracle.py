@@ -1,0 +1,233 @@
+"""ctypes binding of oracle/liboracle.so (the FP64 CPU restatement).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_F64 = HERE / "_ref" / "ref_driver_f64"
+REF_F32 = HERE / "_ref" / "ref_driver_f32"
+
+SCALAR_FLUX, FISSION_SOURCE, TOTAL_SOURCE = 0, 1, 2
+
+
+class OracleProblemC(C.Structure):
+    _fields_ = [("num_groups", C.c_int), ("num_polar_half", C.c_int),
+                ("num_azim_half", C.c_int), ("num_tracks", C.c_int64),
+                ("num_fsrs", C.c_int64), ("num_materials", C.c_int)] + [
+        (n, C.c_void_p) for n in (
+            "seg_offset", "seg_length", "seg_fsr", "trk_azim", "next_fwd", "next_bwd",
+            "next_fwd_is_fwd", "next_bwd_is_fwd", "bc_fwd", "bc_bwd", "total_weight",
+            "inv_sin_theta", "sigma_t", "sigma_s", "nu_sigma_f", "chi", "fsr_material",
+            "fsr_volume", "fixed_source")]
+
+
+class OracleOptionsC(C.Structure):
+    _fields_ = [("max_iters", C.c_int), ("tolerance", C.c_double), ("res_type", C.c_int),
+                ("psi_fp64", C.c_int), ("threads", C.c_int),
+                ("keff_from_fission_rates", C.c_int), ("allow_negative_fluxes", C.c_int)]
+
+
+class OracleResultC(C.Structure):
+    _fields_ = [("keff", C.c_double), ("num_iterations", C.c_int), ("residual", C.c_double),
+                ("sweep_seconds", C.c_double), ("num_integrations", C.c_int64)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        for name in ("oracle_compute_eigenvalue", "oracle_compute_flux"):
+            getattr(h, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(h, name).restype = C.c_int
+        h.oracle_compute_source.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        h.oracle_compute_source.restype = C.c_int
+        h.oracle_transport_sweep.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        h.oracle_transport_sweep.restype = C.c_int
+        h.oracle_compute_fsr_sources.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                                 C.c_int, C.c_void_p]
+        h.oracle_compute_fsr_sources.restype = C.c_int
+        h.oracle_add_source_to_scalar_flux.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        h.oracle_add_source_to_scalar_flux.restype = C.c_int
+        h.oracle_exp_f1.argtypes = [C.c_double]
+        h.oracle_exp_f1.restype = C.c_double
+        h.oracle_pairwise_sum.argtypes = [C.c_void_p, C.c_int64]
+        h.oracle_pairwise_sum.restype = C.c_double
+        _lib = h
+    return _lib
+
+
+@dataclass
+class OracleSolution:
+    keff: float
+    num_iterations: int
+    residual: float
+    sweep_seconds: float
+    num_integrations: int
+    flux: np.ndarray          # [N, G]
+
+
+class OracleProblem:
+    """Owns contiguous copies of the arrays and the C struct pointing at them."""
+
+    def __init__(self, *, num_azim, num_polar, seg_offset, seg_length, seg_fsr, azim,
+                 next_fwd, next_bwd, next_fwd_is_fwd, next_bwd_is_fwd, bc_fwd, bc_bwd,
+                 total_weight, sin_theta, sigma_t, sigma_s, nu_sigma_f, chi, fsr_material,
+                 fsr_volume, fixed_source=None):
+        c = np.ascontiguousarray
+        P = num_polar // 2
+        tw = np.asarray(total_weight, np.float64).reshape(num_azim // 2, -1)
+        if tw.shape[1] != P:
+            tw = tw[:, :P]
+        self.a = dict(
+            seg_offset=c(seg_offset, np.int64), seg_length=c(seg_length, np.float64),
+            seg_fsr=c(seg_fsr, np.int32), trk_azim=c(azim, np.int32),
+            next_fwd=c(next_fwd, np.int64), next_bwd=c(next_bwd, np.int64),
+            next_fwd_is_fwd=c(next_fwd_is_fwd, np.uint8),
+            next_bwd_is_fwd=c(next_bwd_is_fwd, np.uint8),
+            bc_fwd=c(bc_fwd, np.uint8), bc_bwd=c(bc_bwd, np.uint8),
+            total_weight=c(tw, np.float64),
+            inv_sin_theta=c(1.0 / np.asarray(sin_theta, np.float64)[:P]),
+            sigma_t=c(sigma_t, np.float64), sigma_s=c(sigma_s, np.float64),
+            nu_sigma_f=c(nu_sigma_f, np.float64), chi=c(chi, np.float64),
+            fsr_material=c(fsr_material, np.int32), fsr_volume=c(fsr_volume, np.float64))
+        if fixed_source is not None:
+            self.a["fixed_source"] = c(fixed_source, np.float64).ravel()
+        self.G = self.a["sigma_t"].shape[-1] if self.a["sigma_t"].ndim > 1 else len(self.a["sigma_t"])
+        self.N = len(self.a["fsr_material"])
+        self.T = len(self.a["trk_azim"])
+        self.P = P
+        s = OracleProblemC()
+        s.num_groups, s.num_polar_half, s.num_azim_half = self.G, P, num_azim // 2
+        s.num_tracks, s.num_fsrs = self.T, self.N
+        s.num_materials = self.a["sigma_t"].size // self.G
+        for k, v in self.a.items():
+            setattr(s, k, v.ctypes.data)
+        self.c = s
+
+    @classmethod
+    def from_track_generator(cls, tg, fixed_source=None) -> "OracleProblem":
+        i = tg.info
+        return cls(num_azim=i.num_azim, num_polar=i.num_polar, seg_offset=tg.seg_offset,
+                   seg_length=tg.seg_length, seg_fsr=tg.seg_fsr, azim=tg.azim,
+                   next_fwd=tg.next_fwd, next_bwd=tg.next_bwd,
+                   next_fwd_is_fwd=tg.next_fwd_is_fwd, next_bwd_is_fwd=tg.next_bwd_is_fwd,
+                   bc_fwd=tg.bc_fwd, bc_bwd=tg.bc_bwd, total_weight=tg.total_weight,
+                   sin_theta=tg.sin_theta, sigma_t=tg.sigma_t, sigma_s=tg.sigma_s,
+                   nu_sigma_f=tg.nu_sigma_f, chi=tg.chi, fsr_material=tg.fsr_material,
+                   fsr_volume=tg.fsr_volume, fixed_source=fixed_source)
+
+    def _opts(self, max_iters, tol, res_type, psi_fp64, threads):
+        o = OracleOptionsC()
+        o.max_iters, o.tolerance, o.res_type = int(max_iters), float(tol), int(res_type)
+        o.psi_fp64, o.threads = int(psi_fp64), int(threads)
+        o.keff_from_fission_rates, o.allow_negative_fluxes = 1, 0
+        return o
+
+    def _run(self, fn, *extra, max_iters=1000, tol=1e-5, res_type=FISSION_SOURCE,
+             psi_fp64=False, threads=0) -> OracleSolution:
+        o = self._opts(max_iters, tol, res_type, psi_fp64, threads)
+        r = OracleResultC()
+        flux = np.zeros(self.N * self.G)
+        rc = fn(C.byref(self.c), C.byref(o), *extra, C.byref(r), flux.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oracle failed with code {rc}")
+        return OracleSolution(r.keff, r.num_iterations, r.residual, r.sweep_seconds,
+                              r.num_integrations, flux.reshape(self.N, self.G))
+
+    def eigenvalue(self, **kw) -> OracleSolution:
+        return self._run(lib().oracle_compute_eigenvalue, **kw)
+
+    def flux(self, **kw) -> OracleSolution:
+        kw.setdefault("res_type", SCALAR_FLUX)
+        return self._run(lib().oracle_compute_flux, **kw)
+
+    def source(self, k_eff: float, **kw) -> OracleSolution:
+        kw.setdefault("res_type", TOTAL_SOURCE)
+        return self._run(lib().oracle_compute_source, C.c_double(k_eff), **kw)
+
+    def transport_sweep(self, Q: np.ndarray, psi: np.ndarray, threads: int = 0):
+        q = np.ascontiguousarray(Q, np.float64).ravel()
+        p = np.array(psi, np.float32, copy=True).ravel()
+        tally = np.zeros(self.N * self.G)
+        rc = lib().oracle_transport_sweep(C.byref(self.c), q.ctypes.data, p.ctypes.data,
+                                          tally.ctypes.data, int(threads))
+        if rc != 0:
+            raise RuntimeError("oracle sweep failed")
+        return tally, p
+
+    def fsr_sources(self, phi: np.ndarray, keff: float, iteration: int = 100) -> np.ndarray:
+        f = np.ascontiguousarray(phi, np.float64).ravel()
+        Q = np.zeros_like(f)
+        lib().oracle_compute_fsr_sources(C.byref(self.c), f.ctypes.data, float(keff),
+                                         int(iteration), 0, Q.ctypes.data)
+        return Q
+
+    def add_source(self, tally: np.ndarray, Q: np.ndarray) -> np.ndarray:
+        phi = np.array(tally, np.float64, copy=True).ravel()
+        q = np.ascontiguousarray(Q, np.float64).ravel()
+        lib().oracle_add_source_to_scalar_flux(C.byref(self.c), q.ctypes.data, 0, phi.ctypes.data)
+        return phi
+
+
+def exp_f1(x: float) -> float:
+    return lib().oracle_exp_f1(float(x))
+
+
+def pairwise_sum(v: np.ndarray) -> float:
+    a = np.ascontiguousarray(v, np.float64)
+    return lib().oracle_pairwise_sum(a.ctypes.data, a.size)
+
+
+def from_ref_dump(d: dict, problem) -> OracleProblem:
+    """OracleProblem on the tracks dumped by ref_driver (reference tracks)."""
+    from openmoc_amd.workloads import Problem  # noqa: F401  (materials come from the problem)
+    G = problem.num_groups
+    mats = problem.materials
+    st = np.array([np.where(np.abs(m.sigma_t) < 1e-6, 1e-6, m.sigma_t) for m in mats])
+    ss = np.array([m.sigma_s.reshape(G, G).T.ravel() for m in mats])  # origin-major -> dest-major
+    nsf = np.array([m.nu_sigma_f for m in mats])
+    chi = np.array([m.chi / m.chi.sum() if abs(m.chi.sum()) >= 1e-12 else m.chi for m in mats])
+    T = len(d["trk_azim"])
+    off = np.zeros(T + 1, np.int64)
+    off[1:] = np.cumsum(d["trk_num_segments"])
+    na = int(d["num_azim"])
+    npol = int(d["num_polar"])
+    sin_theta = d["quad_sin_theta"].reshape(na // 2, npol)[0]
+    return OracleProblem(
+        num_azim=na, num_polar=npol, seg_offset=off, seg_length=d["seg_length"],
+        seg_fsr=d["seg_fsr"].astype(np.int32), azim=d["trk_azim"],
+        next_fwd=d["trk_next_fwd"], next_bwd=d["trk_next_bwd"],
+        next_fwd_is_fwd=d["trk_next_fwd_is_fwd"], next_bwd_is_fwd=d["trk_next_bwd_is_fwd"],
+        bc_fwd=d["trk_bc_fwd"], bc_bwd=d["trk_bc_bwd"],
+        total_weight=d["quad_total_weight"].reshape(na // 2, npol), sin_theta=sin_theta,
+        sigma_t=st, sigma_s=ss, nu_sigma_f=nsf, chi=chi,
+        fsr_material=d["fsr_material"], fsr_volume=d["fsr_volume"])
+
+
+def run_reference(problem_text_path: str, out_path: str, threads: int = 1, fp: str = "f64",
+                  extra=()) -> None:
+    exe = REF_F64 if fp == "f64" else REF_F32
+    if not exe.exists():
+        raise FileNotFoundError(f"{exe} not built (make -C oracle ref)")
+    subprocess.run([str(exe), problem_text_path, out_path, "--threads", str(threads), *extra],
+                   check=True, stdout=subprocess.DEVNULL)

@@ -1,0 +1,388 @@
+/*
+ * ref_driver: runs the *reference* OpenMOC CPUSolver on a "moc-problem 1"
+ * text file and writes everything a checker needs into a binary record file.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This driver is compiled together with the
+ * reference's own C++ sources (src/*.cpp under the reference tree) by
+ * oracle/Makefile into oracle/_ref/.  It is used to:
+ *   - pin the oracle restatement (oracle/moc_oracle.cpp) against the real
+ *     CPUSolver on the reference's own tracks,
+ *   - pin this package's tracer against the reference TrackGenerator
+ *     (laydown, links, segment lengths, FSR numbering and volumes),
+ *   - time the reference CPUSolver sweep as the CPU baseline.
+ * It is never linked into, or called by, the product library.
+ *
+ * Model construction mirrors how the reference's own drivers build a pin
+ * lattice (profile/models/pin-cell/pin-cell-3d.cpp:88-150): materials via
+ * Material::setSigma*, ZCylinder-bounded fuel cells with setNumRings /
+ * setNumSectors, a Lattice of pin universes inside a root cell bounded by
+ * X/Y planes carrying the boundary conditions, TrackGenerator(geometry,
+ * num_azim, spacing) with a TY polar quadrature, then
+ * CPUSolver::computeEigenvalue (src/Solver.cpp:1542-1689).
+ *
+ * Output ("MOCDUMP1"): a sequence of records
+ *   u32 name_len, name bytes, u8 dtype ('d' f64, 'f' f32, 'i' i32, 'l' i64),
+ *   u64 count, count * sizeof(dtype) bytes.
+ */
+#include "CPUSolver.h"
+#include "Geometry.h"
+#include "Quadrature.h"
+#include "TrackGenerator.h"
+#include "log.h"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Cyl { double r; int mat; int rings; int sectors; };
+struct Pin { std::vector<Cyl> cyl; int outer_mat; int outer_sectors; };
+
+struct Spec {
+  std::string name;
+  int groups = 0;
+  std::vector<std::vector<double>> sigma_t, sigma_s, nu_sigma_f, chi;
+  std::vector<std::string> mat_names;
+  std::vector<Pin> pins;
+  int nx = 0, ny = 0;
+  double px = 0, py = 0;
+  std::vector<int> cells;
+  std::string bc[4];
+  int num_azim = 4;
+  double spacing = 0.1;
+  int num_polar = 6;
+  std::string quad = "TY";
+  double tol = 1e-5;
+  int max_iters = 1000;
+  std::string res_type = "FISSION_SOURCE";
+};
+
+std::vector<double> read_doubles(std::istringstream& ss) {
+  std::vector<double> v;
+  double x;
+  while (ss >> x) v.push_back(x);
+  return v;
+}
+
+Spec parse(const char* path) {
+  std::ifstream in(path);
+  if (!in) { fprintf(stderr, "cannot open %s\n", path); exit(2); }
+  Spec s;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ss(line);
+    std::string key;
+    ss >> key;
+    if (key == "moc-problem") continue;
+    if (key == "name") ss >> s.name;
+    else if (key == "groups") ss >> s.groups;
+    else if (key == "material") { int id; std::string nm; ss >> id >> nm; s.mat_names.push_back(nm); }
+    else if (key == "sigma_t") s.sigma_t.push_back(read_doubles(ss));
+    else if (key == "sigma_s") s.sigma_s.push_back(read_doubles(ss));
+    else if (key == "nu_sigma_f") s.nu_sigma_f.push_back(read_doubles(ss));
+    else if (key == "chi") s.chi.push_back(read_doubles(ss));
+    else if (key == "endmaterial") continue;
+    else if (key == "pin") {
+      int id, n; ss >> id >> n;
+      Pin p;
+      for (int i = 0; i < n; i++) { Cyl c; ss >> c.r >> c.mat >> c.rings >> c.sectors; p.cyl.push_back(c); }
+      ss >> p.outer_mat >> p.outer_sectors;
+      s.pins.push_back(p);
+    }
+    else if (key == "grid") ss >> s.nx >> s.ny >> s.px >> s.py;
+    else if (key == "cells") { int v; while (ss >> v) s.cells.push_back(v); }
+    else if (key == "bc") ss >> s.bc[0] >> s.bc[1] >> s.bc[2] >> s.bc[3];
+    else if (key == "tracks") ss >> s.num_azim >> s.spacing >> s.num_polar >> s.quad;
+    else if (key == "solver") ss >> s.tol >> s.max_iters >> s.res_type;
+    else if (key == "end") break;
+    else { fprintf(stderr, "unknown key %s\n", key.c_str()); exit(2); }
+  }
+  return s;
+}
+
+boundaryType bc_of(const std::string& c) {
+  if (c == "V") return VACUUM;
+  if (c == "P") return PERIODIC;
+  return REFLECTIVE;
+}
+
+struct Writer {
+  FILE* f;
+  explicit Writer(const char* path) {
+    f = fopen(path, "wb");
+    if (!f) { fprintf(stderr, "cannot write %s\n", path); exit(2); }
+    fwrite("MOCDUMP1", 1, 8, f);
+  }
+  ~Writer() { fclose(f); }
+  template <typename T> void put(const char* name, char dt, const T* data, uint64_t n) {
+    uint32_t len = (uint32_t)strlen(name);
+    fwrite(&len, 4, 1, f);
+    fwrite(name, 1, len, f);
+    fwrite(&dt, 1, 1, f);
+    fwrite(&n, 8, 1, f);
+    if (n) fwrite(data, sizeof(T), n, f);
+  }
+  void d(const char* n, const std::vector<double>& v) { put(n, 'd', v.data(), v.size()); }
+  void i(const char* n, const std::vector<int32_t>& v) { put(n, 'i', v.data(), v.size()); }
+  void l(const char* n, const std::vector<int64_t>& v) { put(n, 'l', v.data(), v.size()); }
+  void d1(const char* n, double x) { put(n, 'd', &x, 1); }
+  void l1(const char* n, int64_t x) { put(n, 'l', &x, 1); }
+};
+
+/* Subclass only to read the protected timer. */
+class TimedCPUSolver : public CPUSolver {
+ public:
+  explicit TimedCPUSolver(TrackGenerator* tg) : CPUSolver(tg) {}
+  double split(const char* name) { return _timer->getSplit(name); }
+  void sweep_once() { transportSweep(); }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: %s problem.txt out.bin [--threads N] [--no-solve] "
+                    "[--max-iters N] [--tol X] [--sweeps N]\n", argv[0]);
+    return 2;
+  }
+  int threads = 1;
+  bool solve = true;
+  int max_iters_override = -1;
+  double tol_override = -1;
+  int timed_sweeps = 0;
+  for (int a = 3; a < argc; a++) {
+    if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--no-solve")) solve = false;
+    else if (!strcmp(argv[a], "--max-iters")) max_iters_override = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--tol")) tol_override = atof(argv[++a]);
+    else if (!strcmp(argv[a], "--sweeps")) timed_sweeps = atoi(argv[++a]);
+  }
+  Spec s = parse(argv[1]);
+  if (max_iters_override > 0) s.max_iters = max_iters_override;
+  if (tol_override > 0) s.tol = tol_override;
+  set_log_level("WARNING");
+
+  const int G = s.groups;
+  std::vector<Material*> mats;
+  for (size_t m = 0; m < s.sigma_t.size(); m++) {
+    Material* mat = new Material(0, s.mat_names[m].c_str());
+    mat->setNumEnergyGroups(G);
+    mat->setSigmaT(s.sigma_t[m].data(), G);
+    mat->setSigmaS(s.sigma_s[m].data(), G * G);
+    mat->setNuSigmaF(s.nu_sigma_f[m].data(), G);
+    mat->setChi(s.chi[m].data(), G);
+    mats.push_back(mat);
+  }
+
+  std::vector<Universe*> pin_univ;
+  for (const Pin& p : s.pins) {
+    Universe* u = new Universe();
+    std::vector<ZCylinder*> zc;
+    for (const Cyl& c : p.cyl) zc.push_back(new ZCylinder(0.0, 0.0, c.r));
+    for (size_t i = 0; i < p.cyl.size(); i++) {
+      Cell* c = new Cell();
+      c->setFill(mats[p.cyl[i].mat]);
+      c->addSurface(-1, zc[i]);
+      if (i > 0) c->addSurface(+1, zc[i - 1]);
+      if (p.cyl[i].rings > 1) c->setNumRings(p.cyl[i].rings);
+      if (p.cyl[i].sectors > 1) c->setNumSectors(p.cyl[i].sectors);
+      u->addCell(c);
+    }
+    Cell* outer = new Cell();
+    outer->setFill(mats[p.outer_mat]);
+    if (!zc.empty()) outer->addSurface(+1, zc.back());
+    if (p.outer_sectors > 1) outer->setNumSectors(p.outer_sectors);
+    u->addCell(outer);
+    pin_univ.push_back(u);
+  }
+
+  Lattice* lat = new Lattice();
+  lat->setWidth(s.px, s.py);
+  std::vector<Universe*> cell_univ(s.nx * s.ny);
+  for (int k = 0; k < s.nx * s.ny; k++) cell_univ[k] = pin_univ[s.cells[k]];
+  lat->setUniverses(1, s.ny, s.nx, cell_univ.data());
+
+  double hx = 0.5 * s.nx * s.px, hy = 0.5 * s.ny * s.py;
+  XPlane* xmin = new XPlane(-hx);
+  XPlane* xmax = new XPlane(+hx);
+  YPlane* ymin = new YPlane(-hy);
+  YPlane* ymax = new YPlane(+hy);
+  xmin->setBoundaryType(bc_of(s.bc[0]));
+  xmax->setBoundaryType(bc_of(s.bc[1]));
+  ymin->setBoundaryType(bc_of(s.bc[2]));
+  ymax->setBoundaryType(bc_of(s.bc[3]));
+  Cell* root_cell = new Cell();
+  root_cell->addSurface(+1, xmin);
+  root_cell->addSurface(-1, xmax);
+  root_cell->addSurface(+1, ymin);
+  root_cell->addSurface(-1, ymax);
+  root_cell->setFill(lat);
+  Universe* root = new Universe();
+  root->addCell(root_cell);
+
+  Geometry* geometry = new Geometry();
+  geometry->setRootUniverse(root);
+  geometry->initializeFlatSourceRegions();
+
+  TrackGenerator* tg = new TrackGenerator(geometry, s.num_azim, s.spacing);
+  Quadrature* quad = new TYPolarQuad();
+  quad->setNumAzimAngles(s.num_azim);
+  quad->setNumPolarAngles(s.num_polar);
+  tg->setQuadrature(quad);
+  tg->setNumThreads(1);
+  tg->generateTracks();
+
+  Writer w(argv[2]);
+  w.l1("num_groups", G);
+  w.l1("num_azim", s.num_azim);
+  w.l1("num_polar", s.num_polar);
+  w.l1("threads", threads);
+
+  /* Quadrature after laydown (effective angles and spacings). */
+  {
+    int na2 = s.num_azim / 2;
+    std::vector<double> phi(na2), spc(na2), aw(na2), tw(na2 * s.num_polar), st(na2 * s.num_polar);
+    for (int a = 0; a < na2; a++) {
+      phi[a] = quad->getPhi(a);
+      spc[a] = quad->getAzimSpacing(a);
+      aw[a] = quad->getAzimWeight(a);
+      for (int p = 0; p < s.num_polar; p++) {
+        tw[a * s.num_polar + p] = quad->getWeight(a, p);
+        st[a * s.num_polar + p] = quad->getSinTheta(a, p);
+      }
+    }
+    w.d("quad_phi", phi);
+    w.d("quad_azim_spacing", spc);
+    w.d("quad_azim_weight", aw);
+    w.d("quad_total_weight", tw);
+    w.d("quad_sin_theta", st);
+  }
+
+  /* Tracks and segments in uid order. */
+  long T = tg->getNum2DTracks();
+  Track** tracks = tg->get2DTracksArray();
+  std::vector<int32_t> t_azim(T), t_xy(T), t_nseg(T), t_nff(T), t_nbf(T), t_bcf(T), t_bcb(T);
+  std::vector<int64_t> t_nf(T), t_nb(T), t_uid(T);
+  std::vector<double> t_start(2 * T), t_end(2 * T), t_phi(T);
+  std::vector<double> s_len;
+  std::vector<int64_t> s_fsr;
+  std::vector<int32_t> s_mat;
+  for (long t = 0; t < T; t++) {
+    Track* tr = tracks[t];
+    t_uid[t] = tr->getUid();
+    t_azim[t] = tr->getAzimIndex();
+    t_xy[t] = tr->getXYIndex();
+    t_phi[t] = tr->getPhi();
+    t_start[2 * t] = tr->getStart()->getX();
+    t_start[2 * t + 1] = tr->getStart()->getY();
+    t_end[2 * t] = tr->getEnd()->getX();
+    t_end[2 * t + 1] = tr->getEnd()->getY();
+    t_nf[t] = tr->getTrackNextFwd();
+    t_nb[t] = tr->getTrackNextBwd();
+    t_nff[t] = tr->getNextFwdFwd();
+    t_nbf[t] = tr->getNextBwdFwd();
+    t_bcf[t] = tr->getBCFwd();
+    t_bcb[t] = tr->getBCBwd();
+    t_nseg[t] = tr->getNumSegments();
+    segment* seg = tr->getSegments();
+    for (int k = 0; k < tr->getNumSegments(); k++) {
+      s_len.push_back(seg[k]._length);
+      s_fsr.push_back(seg[k]._region_id);
+      int mid = -1;
+      for (size_t m = 0; m < mats.size(); m++)
+        if (mats[m] == seg[k]._material) mid = (int)m;
+      s_mat.push_back(mid);
+    }
+  }
+  w.l("trk_uid", t_uid);
+  w.i("trk_azim", t_azim);
+  w.i("trk_xy", t_xy);
+  w.d("trk_phi", t_phi);
+  w.d("trk_start", t_start);
+  w.d("trk_end", t_end);
+  w.l("trk_next_fwd", t_nf);
+  w.l("trk_next_bwd", t_nb);
+  w.i("trk_next_fwd_is_fwd", t_nff);
+  w.i("trk_next_bwd_is_fwd", t_nbf);
+  w.i("trk_bc_fwd", t_bcf);
+  w.i("trk_bc_bwd", t_bcb);
+  w.i("trk_num_segments", t_nseg);
+  w.d("seg_length", s_len);
+  w.l("seg_fsr", s_fsr);
+  w.i("seg_material", s_mat);
+
+  TimedCPUSolver solver(tg);
+  solver.setNumThreads(threads);
+  solver.setConvergenceThreshold(s.tol);
+
+  long N = geometry->getNumFSRs();
+  w.l1("num_fsrs", N);
+
+  if (solve) {
+    residualType rt = FISSION_SOURCE;
+    if (s.res_type == "SCALAR_FLUX") rt = SCALAR_FLUX;
+    else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
+    auto t0 = std::chrono::steady_clock::now();
+    solver.computeEigenvalue(s.max_iters, rt);
+    auto t1 = std::chrono::steady_clock::now();
+    std::vector<double> flux(N * G);
+    for (long r = 0; r < N; r++)
+      for (int g = 0; g < G; g++) flux[r * G + g] = solver.getFlux(r, g + 1);
+    w.d1("keff", solver.getKeff());
+    w.l1("num_iterations", solver.getNumIterations());
+    w.d("flux", flux);
+    w.d1("time_sweep", solver.split("Transport Sweep"));
+    w.d1("time_total", std::chrono::duration<double>(t1 - t0).count());
+    long S = tg->getNumSegments();
+    w.l1("num_segments", S);
+    w.l1("num_integrations", 2L * G * (s.num_polar / 2) * S * solver.getNumIterations());
+    fprintf(stdout, "keff %.10f iterations %d sweep_s %.6f total_s %.6f\n",
+            solver.getKeff(), solver.getNumIterations(),
+            solver.split("Transport Sweep"),
+            std::chrono::duration<double>(t1 - t0).count());
+  }
+
+  /* FSR data exists once the solver initialised the FSRs (volumes and
+   * centroids are computed in Solver::initializeFSRs, src/Solver.cpp:820-873). */
+  if (solve) {
+    FP_PRECISION* vol = tg->getFSRVolumes();
+    std::vector<double> v(N), cen(2 * N), pt(2 * N);
+    std::vector<int32_t> fm(N);
+    for (long r = 0; r < N; r++) {
+      v[r] = vol[r];
+      Point* c = geometry->getFSRCentroid(r);
+      cen[2 * r] = c->getX();
+      cen[2 * r + 1] = c->getY();
+      Point* p = geometry->getFSRPoint(r);
+      pt[2 * r] = p->getX();
+      pt[2 * r + 1] = p->getY();
+      Material* m = geometry->findFSRMaterial(r);
+      fm[r] = -1;
+      for (size_t k = 0; k < mats.size(); k++) if (mats[k] == m) fm[r] = (int)k;
+    }
+    w.d("fsr_volume", v);
+    w.d("fsr_centroid", cen);
+    w.d("fsr_point", pt);
+    w.i("fsr_material", fm);
+  }
+
+  /* Optional: time a fixed number of extra sweeps on the converged state
+   * (CPU-baseline sample); integrations per sweep = 2*G*P/2*S. */
+  if (solve && timed_sweeps > 0) {
+    double before = solver.split("Transport Sweep");
+    for (int k = 0; k < timed_sweeps; k++) solver.sweep_once();
+    double dt = solver.split("Transport Sweep") - before;
+    w.d1("timed_sweeps_s", dt);
+    w.l1("timed_sweeps", timed_sweeps);
+    fprintf(stdout, "timed_sweeps %d seconds %.6f\n", timed_sweeps, dt);
+  }
+  return 0;
+}
