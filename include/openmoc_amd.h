@@ -188,6 +188,13 @@ int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
 int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
                                double* tally);
 
+/* Benchmark form of the eigenvalue iteration: runs num_iterations source
+ * iterations (sweep + all per-FSR steps) from the current state with the
+ * stopping rule disabled, synchronises, and returns the mean device time of
+ * the sweep kernel measured with HIP events on the solver's stream. */
+int moc_solver_run_iterations(moc_solver_t* solver, int32_t num_iterations,
+                              double* sweep_seconds_avg);
+
 /* Device-side sweep timing: runs `repeats` sweeps back to back on the
  * solver's current state and returns the mean kernel time (s) measured with
  * HIP events on the sweep's stream. */

@@ -291,6 +291,12 @@ int moc_solver_transport_sweep(moc_solver_t* s, const double* Q, float* psi, dou
   MOC_SOLVER_CALL(s, s->impl->transportSweepOnce(Q, psi, tally));
 }
 
+int moc_solver_run_iterations(moc_solver_t* s, int32_t num_iterations,
+                              double* sweep_seconds_avg) {
+  if (!sweep_seconds_avg) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_SOLVER_CALL(s, *sweep_seconds_avg = s->impl->runIterations(num_iterations));
+}
+
 int moc_solver_time_sweeps(moc_solver_t* s, int32_t repeats, double* seconds_per_sweep) {
   if (!seconds_per_sweep) return fail(MOC_ERR_INVALID, "null argument");
   MOC_SOLVER_CALL(s, *seconds_per_sweep = s->impl->timeSweeps(repeats));

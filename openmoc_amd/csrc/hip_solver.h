@@ -85,6 +85,9 @@ class HipSolver {
   void timing(double* sweep_s, double* total_s, int64_t* integrations) const;
 
   void transportSweepOnce(const double* Q, float* psi, double* tally);
+  // Runs n eigenvalue source iterations with the stopping rule disabled
+  // (benchmark form).  Returns the mean device time of the sweep kernel.
+  double runIterations(int n);
   double timeSweeps(int repeats);
 
   const LaunchShape& shape() const { return shape_; }
@@ -96,7 +99,9 @@ class HipSolver {
   void allocateIterationArrays();
   void runSolve(Mode mode, int max_iters, int res_type, double k_fixed);
   void launchSweep(const float* psi_in, float* psi_out, hipStream_t s);
-  void launchIteration(Mode mode, int res_type, bool recompute_sources);
+  void launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+  void launchInit(Mode mode, double k0);
   void pickShape();
   void countFissionable();
 
@@ -141,6 +146,7 @@ class HipSolver {
   IterState* h_state_ = nullptr;       // pinned
   int num_partials_ = 0;
   int psi_cur_ = 0;                    // which psi buffer holds the start fluxes
+  bool state_valid_ = false;           // an iteration state exists on the device
 
   int32_t max_seg_fsr_ = -1;
   std::vector<int32_t> mat_fissionable_;  // host copy, per material

@@ -842,13 +842,16 @@ FsrArgs make_fsr_args(int64_t N, int G, int32_t* fsr_mat, double* vol, double* s
 }  // namespace
 
 // One source iteration; all launches on stream_, no host synchronisation.
-void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources) {
+void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
+                                hipEvent_t ev0, hipEvent_t ev1) {
   FsrArgs fa = make_fsr_args(N_, G_, d_fsr_mat_, d_fsr_vol_, d_sigma_t_, d_sigma_s_, d_nsf_,
                              d_chi_, d_mat_fissionable_, fixed_on_ ? d_fixed_ : nullptr, d_phi_,
                              d_phi_old_, d_Q_, d_qs_, d_tally_, d_partials_, d_state_,
                              allow_negative_);
   const unsigned nb = blocks_for(N_, kReduceBlock);
+  if (ev0) HIP_CHECK(hipEventRecord(ev0, stream_));
   launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
+  if (ev1) HIP_CHECK(hipEventRecord(ev1, stream_));
   psi_cur_ ^= 1;
   hipLaunchKernelGGL(post_sweep_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa, 1);
   if (mode == MODE_EIGEN)
@@ -862,28 +865,18 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources)
                      mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1);
   hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
                      num_partials_, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
-                     threshold_, mode == MODE_EIGEN ? 0 : 1);
+                     tol, mode == MODE_EIGEN ? 0 : 1);
   HIP_CHECK(hipGetLastError());
 }
 
-void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed) {
-  requireReady();
-  HIP_CHECK(hipSetDevice(device_));
-  if (max_iters <= 0) throw DeviceError(MOC_ERR_INVALID, "max_iters must be > 0");
-  if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
-    throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
-  if (mode != MODE_FLUX && res_type == MOC_FISSION_SOURCE && num_fissionable_ == 0)
-    throw DeviceError(MOC_ERR_INVALID,
-                      "cannot compute a FISSION_SOURCE residual without fissionable FSRs");
-  if (mode == MODE_EIGEN && num_fissionable_ == 0)
-    throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
-  const auto t0 = std::chrono::steady_clock::now();
+// Fresh state for a solve: zero angular fluxes and tally, initial guess,
+// its normalisation (eigen) and the sources of iteration 0.
+void HipSolver::launchInit(Mode mode, double k0) {
   const size_t NG = (size_t)N_ * G_;
   const size_t psi_n = (size_t)2 * T_ * P_ * G_;
-
   IterState st;
   std::memset(&st, 0, sizeof(st));
-  st.k = mode == MODE_SOURCE ? k_fixed : 1.0;
+  st.k = k0;
   st.k_prev = st.k;
   st.norm_factor = 1.0;
   st.psi_scale = 1.0;
@@ -914,6 +907,23 @@ void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed)
   // start fluxes are zero: the scale of the first sweep is irrelevant
   HIP_CHECK(hipGetLastError());
 
+  state_valid_ = true;
+}
+
+void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed) {
+  requireReady();
+  HIP_CHECK(hipSetDevice(device_));
+  if (max_iters <= 0) throw DeviceError(MOC_ERR_INVALID, "max_iters must be > 0");
+  if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
+    throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
+  if (mode != MODE_FLUX && res_type == MOC_FISSION_SOURCE && num_fissionable_ == 0)
+    throw DeviceError(MOC_ERR_INVALID,
+                      "cannot compute a FISSION_SOURCE residual without fissionable FSRs");
+  if (mode == MODE_EIGEN && num_fissionable_ == 0)
+    throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
+  const auto t0 = std::chrono::steady_clock::now();
+  launchInit(mode, mode == MODE_SOURCE ? k_fixed : 1.0);
+
   const bool recompute = mode != MODE_FLUX;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
@@ -924,7 +934,7 @@ void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed)
   while (done < max_iters) {
     const int nb_it = std::min(batch, max_iters - done);
     HIP_CHECK(hipEventRecord(e0, stream_));
-    for (int i = 0; i < nb_it; i++) launchIteration(mode, res_type, recompute);
+    for (int i = 0; i < nb_it; i++) launchIteration(mode, res_type, recompute, threshold_);
     HIP_CHECK(hipEventRecord(e1, stream_));
     HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1036,6 +1046,35 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)NG);
   HIP_CHECK(hipStreamSynchronize(stream_));
   psi_cur_ = 0;
+}
+
+double HipSolver::runIterations(int n) {
+  requireReady();
+  if (n <= 0) throw DeviceError(MOC_ERR_INVALID, "number of iterations must be > 0");
+  if (num_fissionable_ == 0) throw DeviceError(MOC_ERR_INVALID, "no fissionable material");
+  HIP_CHECK(hipSetDevice(device_));
+  if (!state_valid_) launchInit(MODE_EIGEN, 1.0);
+  // keep iterating from the current state; the stopping rule is disabled
+  // (tolerance 0) and a converged flag from an earlier solve is cleared
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  h_state_->converged = 0;
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  std::vector<hipEvent_t> ev(2 * n);
+  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+  for (int i = 0; i < n; i++)
+    launchIteration(MODE_EIGEN, MOC_FISSION_SOURCE, true, 0.0, ev[2 * i], ev[2 * i + 1]);
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  double total_ms = 0;
+  for (int i = 0; i < n; i++) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+    total_ms += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  keff_ = h_state_->k;
+  return total_ms * 1e-3 / n;
 }
 
 double HipSolver::timeSweeps(int repeats) {

@@ -40,8 +40,15 @@ class TrackGenerator:
     """Lays down and ray-traces 2D tracks over a Cartesian pin lattice."""
 
     def __init__(self, problem: Union[Problem, str], num_threads: int = 0):
-        self.problem = problem if isinstance(problem, Problem) else Problem.from_text(problem)
-        self._text = self.problem.to_text() if isinstance(problem, Problem) else problem
+        if isinstance(problem, Problem):
+            self.problem, self._text = problem, problem.to_text()
+        else:
+            # the C++ parser is authoritative and reports malformed input
+            self._text = problem
+            try:
+                self.problem = Problem.from_text(problem)
+            except (ValueError, KeyError, IndexError):
+                self.problem = None
         self._threads = int(num_threads)
         self._h = None
         self.info = None
@@ -206,6 +213,8 @@ class HipSolver:
         check(lib().moc_solver_set_fsrs(self._h, len(fm), _ptr(fm), _ptr(fv)))
         check(lib().moc_solver_set_tracks(self._h, C.byref(td)))
         self._num_fsrs, self._num_groups = len(fm), G
+        self._tg = None
+        self._psi_size = 2 * len(arrs["azim"]) * (int(num_polar) // 2) * G
 
     def setConvergenceThreshold(self, threshold: float) -> None:
         check(lib().moc_solver_set_convergence_threshold(self._h, float(threshold)))
@@ -283,9 +292,23 @@ class HipSolver:
         """One sweep on given reduced sources; returns (tally, psi_out)."""
         q = np.ascontiguousarray(Q, np.float64).ravel()
         p = np.array(psi, np.float32, copy=True).ravel()
+        if self._tg is None and not hasattr(self, "_psi_size"):
+            raise MOCError(-2, "tracks have not been set")
+        n_psi = (2 * self._tg.getNumTracks() * (self._tg.info.num_polar // 2) * self._num_groups
+                 if self._tg is not None else self._psi_size)
+        if q.size != self._num_fsrs * self._num_groups or p.size != n_psi:
+            raise MOCError(-1, f"transportSweep expects Q of {self._num_fsrs * self._num_groups} "
+                               f"and psi of {n_psi} values")
         tally = np.zeros(q.size)
         check(lib().moc_solver_transport_sweep(self._h, _ptr(q), _ptr(p), _ptr(tally)))
         return tally, p
+
+    def runIterations(self, num_iterations: int) -> float:
+        """Benchmark form: num_iterations full source iterations from the
+        current state, no stopping rule; returns mean sweep-kernel seconds."""
+        v = C.c_double()
+        check(lib().moc_solver_run_iterations(self._h, int(num_iterations), C.byref(v)))
+        return v.value
 
     def timeSweeps(self, repeats: int = 10) -> float:
         v = C.c_double()

@@ -1,0 +1,76 @@
+"""Generates the golden fixtures under tests/golden/ from the compiled
+reference OpenMOC (oracle/_ref/ref_driver_f64, built by `make -C oracle ref`
+from the reference's own C++ sources).  Run in the build container only:
+
+    python tests/golden/make_golden.py
+
+Outputs (data only: inputs and the reference's outputs):
+  pin_cell_7g_ref.npz       full tracks/segments/links/volumes + keff/flux
+  lattice_17x17_7g_ref.npz  per-track checksums + FSR volumes + keff/flux (f32)
+  c5g7_pin_cell_ref.npz     keff/flux/iterations (matches the reference's
+                            tests/test_forward_pin_cell/results_true.dat)
+  hom_inf_medium_ref.npz    keff/flux/iterations
+  *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
+"""
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from openmoc_amd import workloads  # noqa: E402
+from oracle import mocdump, oracle  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def run(name: str, threads: int = 8):
+    p = workloads.get(name)
+    txt = OUT / f"{name}.txt"
+    txt.write_text(p.to_text())
+    with tempfile.TemporaryDirectory() as td:
+        dump = Path(td) / "ref.bin"
+        oracle.run_reference(str(txt), str(dump), threads=threads, fp="f64")
+        return p, mocdump.read(str(dump))
+
+
+def base(d):
+    return dict(keff=np.float64(d["keff"]), num_iterations=np.int64(d["num_iterations"]),
+                num_fsrs=np.int64(d["num_fsrs"]), num_segments=np.int64(len(d["seg_length"])),
+                num_tracks=np.int64(len(d["trk_azim"])))
+
+
+def main():
+    for name in ("c5g7_pin_cell", "hom_inf_medium"):
+        _, d = run(name, threads=1)
+        np.savez_compressed(OUT / f"{name}_ref.npz", flux=d["flux"], **base(d))
+
+    _, d = run("pin_cell_7g", threads=1)
+    np.savez_compressed(
+        OUT / "pin_cell_7g_ref.npz", flux=d["flux"], seg_length=d["seg_length"],
+        seg_fsr=d["seg_fsr"].astype(np.int32), trk_num_segments=d["trk_num_segments"],
+        trk_next_fwd=d["trk_next_fwd"], trk_next_bwd=d["trk_next_bwd"],
+        trk_next_fwd_is_fwd=d["trk_next_fwd_is_fwd"], trk_next_bwd_is_fwd=d["trk_next_bwd_is_fwd"],
+        trk_bc_fwd=d["trk_bc_fwd"], trk_bc_bwd=d["trk_bc_bwd"], trk_azim=d["trk_azim"],
+        trk_start=d["trk_start"], quad_total_weight=d["quad_total_weight"],
+        quad_phi=d["quad_phi"], fsr_volume=d["fsr_volume"], fsr_material=d["fsr_material"],
+        **base(d))
+
+    _, d = run("lattice_17x17_7g", threads=8)
+    ns = d["trk_num_segments"]
+    off = np.concatenate([[0], np.cumsum(ns)])
+    len_sum = np.add.reduceat(d["seg_length"], off[:-1])
+    fsr_sum = np.add.reduceat(d["seg_fsr"].astype(np.int64), off[:-1])
+    np.savez_compressed(
+        OUT / "lattice_17x17_7g_ref.npz", flux=d["flux"].astype(np.float32),
+        trk_num_segments=ns.astype(np.int32), trk_length_sum=len_sum, trk_fsr_sum=fsr_sum,
+        trk_next_fwd=d["trk_next_fwd"], trk_next_bwd=d["trk_next_bwd"],
+        fsr_volume=d["fsr_volume"], fsr_material=d["fsr_material"], **base(d))
+    print("golden fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

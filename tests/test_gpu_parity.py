@@ -1,0 +1,197 @@
+"""HIP path vs the oracle and the compiled reference's fixtures, through the
+C ABI.  Tolerances (FP32 sweep vs FP64 oracle): keff within 1 pcm, FSR scalar
+flux relative L2 <= 1e-4; single-sweep tallies relative L2 <= 2e-5."""
+import numpy as np
+import pytest
+
+import openmoc_amd as om
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+K_TOL = 1e-5        # 1 pcm
+FLUX_RL2 = 1e-4
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def solver_for(tg, exp_mode=om.EXP_RATIONAL):
+    s = om.HipSolver(tg)
+    s.setExponential(exp_mode)
+    return s
+
+
+@pytest.mark.parametrize("name", ["c5g7_pin_cell", "pin_cell_7g", "lattice_17x17_7g"])
+@pytest.mark.parametrize("exp_mode", [om.EXP_RATIONAL, om.EXP_HW])
+def test_single_sweep_matches_oracle(tracks, name, exp_mode):
+    tg = tracks(name)
+    N, G, T = tg.getNumFSRs(), tg.getNumEnergyGroups(), tg.getNumTracks()
+    P = tg.info.num_polar // 2
+    rng = np.random.default_rng(7)
+    Q = rng.uniform(0.01, 1.0, N * G)
+    psi = rng.uniform(0.0, 2.0, T * 2 * P * G).astype(np.float32)
+    s = solver_for(tg, exp_mode)
+    tally, psi_out = s.transportSweep(Q, psi)
+    op = orc.OracleProblem.from_track_generator(tg)
+    t_ref, p_ref = op.transport_sweep(Q, psi)
+    tol = 2e-5 if exp_mode == om.EXP_RATIONAL else 5e-5
+    assert rel_l2(tally, t_ref) < tol
+    assert rel_l2(psi_out, p_ref) < tol
+    np.testing.assert_allclose(psi_out, p_ref, rtol=5e-4, atol=1e-6)
+
+
+def test_sweep_vacuum_slots_keep_inflow(tracks):
+    tg = tracks("minicore_70g", num_azim=4, spacing=0.5)
+    N, G, T = tg.getNumFSRs(), tg.getNumEnergyGroups(), tg.getNumTracks()
+    P = tg.info.num_polar // 2
+    rng = np.random.default_rng(3)
+    Q = rng.uniform(0.01, 1.0, N * G)
+    psi = rng.uniform(0.0, 1.0, T * 2 * P * G).astype(np.float32)
+    tally, psi_out = solver_for(tg).transportSweep(Q, psi)
+    t_ref, p_ref = orc.OracleProblem.from_track_generator(tg).transport_sweep(Q, psi)
+    assert rel_l2(tally, t_ref) < 2e-5
+    np.testing.assert_allclose(psi_out, p_ref, rtol=5e-4, atol=1e-6)
+
+
+def test_eigen_infinite_medium_closed_form(tracks, golden):
+    tg = tracks("hom_inf_medium")
+    s = solver_for(tg)
+    s.setConvergenceThreshold(1e-8)
+    s.computeEigenvalue(1000)
+    s11, s12, s22 = 0.1, 0.117, 1.42
+    k_inf = (0.0015 + 0.325 * s12 / (1.604 - s22)) / (0.2208 - s11 - s12 + s12)
+    assert abs(s.getKeff() - k_inf) < 2e-5
+    f = om.process.get_scalar_fluxes(s)
+    np.testing.assert_allclose(f[:, 0] / f[:, 1], (1.604 - s22) / s12, rtol=1e-4)
+
+
+@pytest.mark.parametrize("name", ["c5g7_pin_cell", "hom_inf_medium", "pin_cell_7g"])
+def test_eigen_matches_reference_fixture(tracks, golden, name):
+    g = golden(name)
+    tg = tracks(name)
+    s = solver_for(tg)
+    s.computeEigenvalue(tg.problem.max_iters)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+    assert abs(s.getNumIterations() - int(g["num_iterations"])) <= 3
+
+
+def test_c5g7_pin_cell_regression_values(tracks):
+    # tests/test_forward_pin_cell/results_true.dat, printed as the harness does
+    tg = tracks("c5g7_pin_cell")
+    s = solver_for(tg)
+    s.computeEigenvalue(500)
+    assert f"{s.getKeff():12.5E}" == " 1.04666E+00"
+    truth = np.array([6.428760E-01, 1.099273E+00, 5.710249E-01, 2.507108E-01, 1.962766E-01,
+                      4.993489E-01, 1.287533E+00, 1.104038E+00, 1.411933E+00, 5.559559E-01,
+                      2.268887E-01, 1.900581E-01, 4.444841E-01, 9.440460E-01])
+    np.testing.assert_allclose(s.getFluxes(), truth, rtol=2e-5)
+
+
+def test_eigen_tight_tolerance_matches_oracle(tracks):
+    tg = tracks("pin_cell_7g")
+    s = solver_for(tg)
+    s.setConvergenceThreshold(1e-9)
+    s.computeEigenvalue(2000)
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=2000, tol=1e-9)
+    assert abs(s.getKeff() - sol.keff) < 1e-6
+    assert rel_l2(s.getFluxes(), sol.flux) < 2e-5
+
+
+def test_lattice_17x17_matches_reference(tracks, golden):
+    g = golden("lattice_17x17_7g")
+    tg = tracks("lattice_17x17_7g")
+    s = solver_for(tg)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert rel_l2(s.getFluxes(), g["flux"].astype(np.float64)) < FLUX_RL2
+
+
+def test_hw_exponential_within_tolerance(tracks, golden):
+    g = golden("pin_cell_7g")
+    tg = tracks("pin_cell_7g")
+    s = solver_for(tg, om.EXP_HW)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+
+
+def _fixed_source_problem(tg):
+    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
+    fixed = np.zeros((N, G))
+    fixed[:, 0] = 1.0
+    fixed[: N // 2, 2] = 0.5
+    return fixed
+
+
+def test_compute_source_matches_oracle(tracks):
+    tg = tracks("c5g7_pin_cell")
+    fixed = _fixed_source_problem(tg)
+    s = solver_for(tg)
+    for r in range(fixed.shape[0]):
+        for gg in range(fixed.shape[1]):
+            if fixed[r, gg]:
+                s.setFixedSourceByFSR(r, gg + 1, fixed[r, gg])
+    s.setConvergenceThreshold(1e-7)
+    s.computeSource(1000, 0.9, om.TOTAL_SOURCE)
+    op = orc.OracleProblem.from_track_generator(tg, fixed_source=fixed)
+    sol = op.source(0.9, max_iters=1000, tol=1e-7, res_type=orc.TOTAL_SOURCE)
+    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
+    assert abs(s.getNumIterations() - sol.num_iterations) <= 2
+
+
+def test_compute_flux_matches_oracle(tracks):
+    tg = tracks("c5g7_pin_cell")
+    fixed = _fixed_source_problem(tg)
+    s = solver_for(tg)
+    for r in range(fixed.shape[0]):
+        for gg in range(fixed.shape[1]):
+            if fixed[r, gg]:
+                s.setFixedSourceByFSR(r, gg + 1, fixed[r, gg])
+    s.setConvergenceThreshold(1e-7)
+    s.computeFlux(1000)
+    sol = orc.OracleProblem.from_track_generator(tg, fixed_source=fixed).flux(
+        max_iters=1000, tol=1e-7)
+    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
+
+
+def test_repeat_solves_agree(tracks):
+    """Same result on repeated solves with one solver (reference
+    tests/test_multisim_simple): within tolerance (atomic order may differ)."""
+    tg = tracks("pin_cell_7g")
+    s = solver_for(tg)
+    res = []
+    for _ in range(3):
+        s.computeEigenvalue(1000)
+        res.append((s.getKeff(), s.getFluxes()))
+    for k, f in res[1:]:
+        assert abs(k - res[0][0]) < 1e-7
+        assert rel_l2(f, res[0][1]) < 1e-6
+
+
+def test_fission_rates_and_get_flux(tracks):
+    tg = tracks("pin_cell_7g")
+    s = solver_for(tg)
+    s.computeEigenvalue(1000)
+    phi = om.process.get_scalar_fluxes(s)
+    rates = s.computeFSRFissionRates()
+    exp = (tg.nu_sigma_f[tg.fsr_material] * phi).sum(axis=1) * tg.fsr_volume
+    np.testing.assert_allclose(rates, exp, rtol=1e-12)
+    assert s.getFlux(3, 2) == phi[3, 1]
+    with pytest.raises(RuntimeError):
+        s.getFlux(tg.getNumFSRs(), 1)
+    with pytest.raises(RuntimeError):
+        s.getFlux(0, 0)
+    with pytest.raises(RuntimeError):
+        s.getFluxes(5)
+
+
+def test_state_errors():
+    s = om.HipSolver()
+    with pytest.raises(RuntimeError, match="tracks"):
+        s.computeEigenvalue(10)
+    with pytest.raises(RuntimeError):
+        s.setConvergenceThreshold(-1.0)

@@ -1,0 +1,154 @@
+"""Pins the oracle (oracle/moc_oracle.cpp) against golden vectors, closed
+forms and the compiled reference's outputs.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+# Reference unit-test golden values of expF1_fractional
+# (tests/unit_tests/test_exponentials.py:11,75-78).
+TAU = [1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1, 10, 100]
+F1_GOLDEN = [0.9999999950003422, 0.9999999500034228, 0.9999995000343784,
+             0.9999950003587617, 0.9999500050851808, 0.9995002005718668,
+             0.9950169413610763, 0.9516272442309586, 0.6321211831479621,
+             0.09999547551656497, 0.010000005017389789]
+
+
+def test_exp_f1_golden_values():
+    for x, ref in zip(TAU, F1_GOLDEN):
+        assert abs(orc.exp_f1(x) - ref) < 1e-12
+
+
+def test_exp_f1_against_closed_form():
+    # the fit's stated accuracy: 2e-6 over [0, 1e5] (src/exponentials.h:150-152)
+    for x in np.concatenate([np.logspace(-8, 2, 400), [0.3, 3.7, 42.0]]):
+        exact = -math.expm1(-x) / x
+        assert abs(orc.exp_f1(x) - exact) < 2e-6, x
+
+
+def test_pairwise_sum_matches_exact_integer_sums():
+    rng = np.random.default_rng(0)
+    for n in [0, 1, 2, 3, 4, 5, 7, 8, 63, 64, 65, 1000, 12345]:
+        v = rng.integers(-1000, 1000, n).astype(np.float64)
+        assert orc.pairwise_sum(v) == float(v.sum())
+
+
+def test_single_segment_attenuation(tracks):
+    """One sweep on a 1-cell homogeneous problem: every track's outgoing flux
+    obeys psi_out = psi_in e^-tau + q (1 - e^-tau) per segment chain (with the
+    fit's error), checked on the CPU restatement."""
+    tg = tracks("hom_inf_medium")
+    op = orc.OracleProblem.from_track_generator(tg)
+    G, P = op.G, op.P
+    Q = np.full(op.N * G, 0.3)
+    psi = np.zeros(op.T * 2 * P * G, np.float32)
+    psi[:] = 1.0
+    tally, psi_out = op.transport_sweep(Q, psi, threads=1)
+    sig = tg.sigma_t[0]
+    st = tg.sin_theta[:P]
+    for t in [0, 5, 17]:
+        L = tg.seg_length[tg.seg_offset[t]:tg.seg_offset[t + 1]].sum()
+        nxt, nd = tg.next_fwd[t], 1 - tg.next_fwd_is_fwd[t]
+        got = psi_out[(nxt * 2 + nd) * P * G:(nxt * 2 + nd + 1) * P * G].reshape(P, G)
+        for p in range(P):
+            tau = sig * L / st[p]
+            qs = 0.3 / sig
+            exp = 1.0 * np.exp(-tau) + qs * (1 - np.exp(-tau))
+            np.testing.assert_allclose(got[p], exp, rtol=2e-5)
+
+
+def test_infinite_medium_k_matches_closed_form(tracks):
+    # k_inf = (nsf1 + nsf2 * s12 / sa2) / (sa1 + s12) for the 2-group medium
+    # of tests/input_set.py:32-92 (reference result 1.72307).
+    tg = tracks("hom_inf_medium")
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=500, tol=1e-10, threads=2)
+    s11, s12, s22 = 0.1, 0.117, 1.42
+    sa1 = 0.2208 - s11 - s12
+    sa2 = 1.604 - s22
+    k_inf = (0.0015 + 0.325 * s12 / sa2) / (sa1 + s12)
+    assert abs(sol.keff - k_inf) < 2e-5
+    # flat flux with ratio phi1/phi2 = sa2 / s12
+    ratio = sol.flux[:, 0] / sol.flux[:, 1]
+    np.testing.assert_allclose(ratio, sa2 / s12, rtol=1e-4)
+
+
+def test_reference_fixture_hom_inf_medium(tracks, golden):
+    # tests/test_forward_hom_inf_medium/results_true.dat: 108 iterations, 1.72307
+    g = golden("hom_inf_medium")
+    tg = tracks("hom_inf_medium")
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=500, tol=1e-5, threads=1)
+    assert sol.num_iterations == 108 == int(g["num_iterations"])
+    assert f"{sol.keff:12.5E}" == " 1.72307E+00"
+    np.testing.assert_allclose(sol.flux.ravel(), g["flux"], rtol=1e-9)
+
+
+def test_reference_fixture_c5g7_pin_cell(tracks, golden):
+    # tests/test_forward_pin_cell/results_true.dat (261 iterations, 1.04666,
+    # 14 fluxes printed with %12.6E)
+    truth = [6.428760E-01, 1.099273E+00, 5.710249E-01, 2.507108E-01, 1.962766E-01,
+             4.993489E-01, 1.287533E+00, 1.104038E+00, 1.411933E+00, 5.559559E-01,
+             2.268887E-01, 1.900581E-01, 4.444841E-01, 9.440460E-01]
+    tg = tracks("c5g7_pin_cell")
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=500, tol=1e-5, threads=1)
+    assert sol.num_iterations == 261
+    assert f"{sol.keff:12.5E}" == " 1.04666E+00"
+    assert [f"{v:12.6E}" for v in sol.flux.ravel()] == [f"{v:12.6E}" for v in truth]
+    np.testing.assert_allclose(sol.flux.ravel(), golden("c5g7_pin_cell")["flux"], rtol=1e-9)
+
+
+def test_reference_fixture_pin_cell_7g(tracks, golden):
+    g = golden("pin_cell_7g")
+    tg = tracks("pin_cell_7g")
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=1000, tol=1e-5, threads=2)
+    assert sol.num_iterations == int(g["num_iterations"])
+    assert abs(sol.keff - float(g["keff"])) < 1e-11
+    np.testing.assert_allclose(sol.flux.ravel(), g["flux"], rtol=1e-9)
+
+
+def test_oracle_on_reference_tracks(golden):
+    """The oracle run on the reference's own tracks (the golden fixture)
+    reproduces the reference CPUSolver's k and fluxes."""
+    from openmoc_amd import workloads
+    g = golden("pin_cell_7g")
+    p = workloads.get("pin_cell_7g")
+    d = {"trk_azim": g["trk_azim"], "trk_num_segments": g["trk_num_segments"],
+         "seg_length": g["seg_length"], "seg_fsr": g["seg_fsr"],
+         "trk_next_fwd": g["trk_next_fwd"], "trk_next_bwd": g["trk_next_bwd"],
+         "trk_next_fwd_is_fwd": g["trk_next_fwd_is_fwd"],
+         "trk_next_bwd_is_fwd": g["trk_next_bwd_is_fwd"], "trk_bc_fwd": g["trk_bc_fwd"],
+         "trk_bc_bwd": g["trk_bc_bwd"], "num_azim": 32, "num_polar": 6,
+         "quad_total_weight": g["quad_total_weight"],
+         "quad_sin_theta": np.tile([math.sin(math.asin(v)) for v in
+                                    (0.166648, 0.537707, 0.932954, 0.932954, 0.537707, 0.166648)],
+                                   16),
+         "fsr_material": g["fsr_material"], "fsr_volume": g["fsr_volume"]}
+    sol = orc.from_ref_dump(d, p).eigenvalue(max_iters=1000, tol=1e-5, threads=2)
+    assert sol.num_iterations == int(g["num_iterations"])
+    assert abs(sol.keff - float(g["keff"])) < 1e-12
+    np.testing.assert_allclose(sol.flux.ravel(), g["flux"], rtol=1e-11)
+
+
+def test_particle_balance_reflective(tracks):
+    """Reflective pin cell, fixed source only, no fission: absorption equals
+    the total source (no leakage) once converged (computeSource with k=1 and a
+    non-fissile copy)."""
+    tg = tracks("c5g7_pin_cell")
+    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
+    fixed = np.zeros((N, G))
+    fixed[:, 0] = 1.0
+    op = orc.OracleProblem(
+        num_azim=tg.info.num_azim, num_polar=tg.info.num_polar, seg_offset=tg.seg_offset,
+        seg_length=tg.seg_length, seg_fsr=tg.seg_fsr, azim=tg.azim, next_fwd=tg.next_fwd,
+        next_bwd=tg.next_bwd, next_fwd_is_fwd=tg.next_fwd_is_fwd,
+        next_bwd_is_fwd=tg.next_bwd_is_fwd, bc_fwd=tg.bc_fwd, bc_bwd=tg.bc_bwd,
+        total_weight=tg.total_weight, sin_theta=tg.sin_theta, sigma_t=tg.sigma_t,
+        sigma_s=tg.sigma_s, nu_sigma_f=np.zeros_like(tg.nu_sigma_f), chi=tg.chi,
+        fsr_material=tg.fsr_material, fsr_volume=tg.fsr_volume, fixed_source=fixed)
+    sol = op.source(1.0, max_iters=2000, tol=1e-9, res_type=orc.SCALAR_FLUX, threads=2)
+    sa = tg.sigma_t - tg.sigma_s.reshape(-1, G, G).sum(axis=1)
+    absorption = sum(tg.fsr_volume[r] * (sa[tg.fsr_material[r]] * sol.flux[r]).sum()
+                     for r in range(N))
+    source = (fixed * tg.fsr_volume[:, None]).sum()
+    assert abs(absorption / source - 1) < 2e-3
