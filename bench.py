@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +111,12 @@ def main():
 
     solver = om.HipSolver(tg, device=0)
     solver.setExponential(om.EXP_HW if args.exp == "hw" else om.EXP_RATIONAL)
+    shape = solver.getLaunchShape()
+    if not args.no_autotune:
+        from openmoc_amd.autotune import autotune
+        solver.computeEigenvalue(3)
+        shape, t_best = autotune(solver, repeats=5)
+        log(f"autotune: {shape} -> {t_best * 1e6:.1f} us/sweep")
 
     solve = None
     if not args.no_solve:
@@ -143,7 +150,8 @@ def main():
     achieved_tflops = FLOPS_PER_INTEGRATION * w_sweep / sweep_s / 1e12
     roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "sweep2d_kernel", "algorithmic_bytes_per_launch": bytes_sweep,
+                "kernel": "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel",
+                "algorithmic_bytes_per_launch": bytes_sweep,
                 "launch_seconds": sweep_s}
     roofline_valu = {"achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
@@ -165,6 +173,7 @@ def main():
         "config": {"workload": args.workload, "tracks": T, "segments": S, "fsrs": N,
                    "groups": G, "polar_half": P, "num_azim": problem.num_azim,
                    "azim_spacing": problem.azim_spacing, "exponential": args.exp,
+                   "launch_shape": shape,
                    "step": "one source iteration (sweep + per-FSR FP64 steps)"},
         "sweep_integrations_per_s": w_sweep / sweep_s,
         "roofline": roofline, "roofline_valu": roofline_valu,

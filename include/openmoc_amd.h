@@ -188,10 +188,28 @@ int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
 int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
                                double* tally);
 
+/* Sweep launch shape, shape9 = {groups_per_lane, lanes_per_track,
+ * tracks_per_wave, block, tally_copies, lds_groups, lds_block,
+ * lds_blocks_per_sub, lds_q} (chosen by default from the device's LDS size,
+ * or by the autotuner, openmoc_amd/autotune.py).
+ *   lds_groups > 0: LDS-privatised tally; each block holds the [N][lds_groups]
+ *     FP32 tally slice of one group subset and sigma_t by material in LDS,
+ *     plus the subset's Q/sigma_t slice when lds_q = 1
+ *     (lds_groups * ((1 + lds_q) * N + M) * 4 bytes must fit); lds_block
+ *     threads (multiple of 64, <= 1024), lds_blocks_per_sub blocks per
+ *     subset; partial slices are reduced in a fixed order.
+ *   lds_groups = 0: device-scope float atomics; groups_per_lane in
+ *     {1,2,4,9,16}, lanes_per_track = ceil(G/groups_per_lane),
+ *     tracks_per_wave * lanes_per_track <= 64, block a multiple of 64
+ *     (<= 256), tally_copies private tallies (block b uses copy b % copies). */
+int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape9);
+int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape9);
+
 /* Benchmark form of the eigenvalue iteration: runs num_iterations source
  * iterations (sweep + all per-FSR steps) from the current state with the
  * stopping rule disabled, synchronises, and returns the mean device time of
- * the sweep kernel measured with HIP events on the solver's stream. */
+ * the transport-sweep kernel (the launch itself, without the partial-tally
+ * reduction) measured with HIP events on the solver's stream. */
 int moc_solver_run_iterations(moc_solver_t* solver, int32_t num_iterations,
                               double* sweep_seconds_avg);
 

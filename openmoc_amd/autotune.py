@@ -1,0 +1,80 @@
+"""Sweep launch-shape autotuner.
+
+Times a set of candidate launch shapes of the transport sweep on the solver's
+own problem (HIP events around repeated sweeps, moc_solver_time_sweeps) and
+keeps the fastest.  Candidates cover both tally paths:
+
+  * LDS sweep (lds_groups > 0): groups per block subset (= lanes per track),
+    whether the subset's Q/sigma_t slice is staged in LDS (lds_q), threads per
+    block and blocks per subset;
+  * global-atomic sweep (lds_groups = 0): groups per lane, block size and the
+    number of private tally copies.
+
+Shapes the device rejects (LDS size) are skipped.  Results are cached per
+problem signature for the life of the process.
+"""
+from __future__ import annotations
+
+import itertools
+from typing import Dict, List, Optional, Tuple
+
+_CACHE: Dict[Tuple, Tuple[dict, float]] = {}
+
+
+def _signature(solver) -> Tuple:
+    tg = solver._tg
+    if tg is None:
+        return (solver._num_fsrs, solver._num_groups, id(solver))
+    i = tg.info
+    return (i.num_tracks, i.num_segments, i.num_fsrs, i.num_groups, i.num_materials,
+            i.num_azim, i.num_polar)
+
+
+def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
+    out = []
+    for gb in sorted({1, 2, 3, 4, 7, min(num_groups, 64)}):
+        if gb > num_groups:
+            continue
+        nsub = -(-num_groups // gb)
+        for q, block, per_cu in itertools.product((1, 0), (512, 1024), (1, 2, 4)):
+            out.append(dict(lds_groups=gb, lds_q=q, lds_block=block,
+                            lds_blocks_per_sub=max(1, (num_cus * per_cu) // nsub)))
+    for gpl in (1, 2, 4, 9):
+        lpt = -(-num_groups // gpl)
+        if lpt > 64:
+            continue
+        for block, copies in itertools.product((64, 256), (1, 8)):
+            out.append(dict(lds_groups=0, groups_per_lane=gpl, lanes_per_track=lpt,
+                            tracks_per_wave=64 // lpt, block=block, tally_copies=copies))
+    return out
+
+
+def autotune(solver, repeats: int = 5, num_cus: int = 256, verbose: bool = False,
+             cands: Optional[List[dict]] = None) -> Tuple[dict, float]:
+    """Pick the fastest launch shape for `solver`'s problem and apply it.
+    Returns (shape, seconds per sweep)."""
+    key = _signature(solver)
+    if key in _CACHE:
+        shape, t = _CACHE[key]
+        solver.setLaunchShape(**shape)
+        return shape, t
+    best: Optional[Tuple[dict, float]] = None
+    start = solver.getLaunchShape()
+    for c in (cands or candidates(solver._num_groups, num_cus)):
+        shape = dict(start)
+        shape.update(c)
+        try:
+            solver.setLaunchShape(**shape)
+        except RuntimeError:
+            continue
+        t = solver.timeSweeps(repeats)
+        if verbose:
+            print(f"autotune {t * 1e6:9.1f} us {c}", flush=True)
+        if best is None or t < best[1]:
+            best = (solver.getLaunchShape(), t)
+    if best is None:
+        solver.setLaunchShape(**start)
+        return start, float("nan")
+    solver.setLaunchShape(**best[0])
+    _CACHE[key] = best
+    return best

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -36,6 +37,14 @@ struct IterState {
   int reported_iters;   // Solver::_num_iterations semantics
   int64_t negative_fluxes;
   int64_t negative_sources;
+  // Fixed-point LDS tally (see sweep2d_lds_kernel): contributions / V are
+  // accumulated as int64 multiples of 1/tally_scale, with tally_scale set from
+  // a bound on |psi| and |Q/sigma_t| before every sweep.
+  double tally_scale;
+  uint32_t psi_max_bits;    // max |outgoing psi| of the running sweep (float bits)
+  uint32_t qmax_bits;       // max |Q/sigma_t| of the sources being computed (float bits)
+  float qmax;               // max |Q/sigma_t| of the sources in use
+  float psi_max;            // max |start psi| of the sweep in use (before psi_scale)
 };
 
 struct LaunchShape {
@@ -43,6 +52,13 @@ struct LaunchShape {
   int lanes_per_track = 1;   // ceil(G / groups_per_lane)
   int tracks_per_wave = 1;   // 64 / lanes_per_track
   int block = 256;           // threads per block (multiple of 64)
+  int tally_copies = 8;      // private FP32 tallies (one per XCD)
+  // LDS-privatised tally (used when lds_groups > 0): groups per subset
+  // (= lanes per track), threads per block, blocks per group subset.
+  int lds_groups = 0;
+  int lds_block = 512;
+  int lds_blocks_per_sub = 0;
+  int lds_q = 0;             // 1: the subset's Q/sigma_t slice is staged in LDS too
 };
 
 class DeviceError : public std::exception {
@@ -98,12 +114,16 @@ class HipSolver {
   void requireReady() const;
   void allocateIterationArrays();
   void runSolve(Mode mode, int max_iters, int res_type, double k_fixed);
-  void launchSweep(const float* psi_in, float* psi_out, hipStream_t s);
+  void launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
+                   hipEvent_t ev_after_sweep = nullptr);
   void launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   void launchInit(Mode mode, double k0);
   void pickShape();
   void countFissionable();
+  void buildTileLayout(int tracks_per_wave);
+  int sweepTracksPerWave() const;
+  size_t ldsBytes(int groups, int with_q) const;
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
@@ -121,8 +141,9 @@ class HipSolver {
 
   // device arrays (raw pointers owned by this object)
   float* d_seg_len_ = nullptr;
-  int32_t* d_seg_fsr_ = nullptr;
-  int32_t* d_work_ = nullptr;          // directed tracks, longest first
+  int32_t* d_seg_key_ = nullptr;       // tile-interleaved FSR | material << 24
+  int32_t* d_work_ = nullptr;          // [ntiles*tpw] directed tracks, longest first (-1 pad)
+  int32_t* d_tile_ = nullptr;          // [ntiles][2] first row, rows
   int32_t* d_trk_ = nullptr;           // [T][4] seg_begin, seg_count, out_fwd, out_bwd
   int32_t* d_trk_azim_ = nullptr;
   float* d_weights_ = nullptr;         // [num_azim/2][P]
@@ -141,7 +162,15 @@ class HipSolver {
   int32_t* d_mat_fissionable_ = nullptr;  // [M]
   int32_t* d_fsr_mat_ = nullptr;       // [N]
   double* d_fsr_vol_ = nullptr;        // [N]
+  float* d_inv_vol_ = nullptr;         // [N] 1/V (0 for zero volume)
+  double sigma_t_max_ = 0.0;
   double* d_partials_ = nullptr;       // [max blocks]
+  unsigned long long* d_lds_partials_ = nullptr;  // [nsub][blocks_per_sub][N*GB] fixed point
+  float* d_qsub_ = nullptr;            // [nsub][N][GB] Q/sigma_t for the LDS sweep
+  double* d_tally64_ = nullptr;        // [N*G] reduced LDS-path tally
+  int num_cus_ = 0;
+  size_t lds_max_ = 0;                 // max dynamic LDS per block (bytes)
+  size_t lds_per_cu_ = 0;
   IterState* d_state_ = nullptr;
   IterState* h_state_ = nullptr;       // pinned
   int num_partials_ = 0;
@@ -149,8 +178,13 @@ class HipSolver {
   bool state_valid_ = false;           // an iteration state exists on the device
 
   int32_t max_seg_fsr_ = -1;
+  std::vector<float> h_len_;             // host copies for the stream layout
+  std::vector<int32_t> h_fsr_, h_trk_, h_order_;
+  int layout_tpw_ = 0, layout_ntiles_ = 0;
+  int64_t layout_rows_ = 0;
   std::vector<int32_t> mat_fissionable_;  // host copy, per material
   std::vector<int32_t> h_fsr_mat_;        // host copy, per FSR
+  std::vector<double> h_fsr_vol_;         // host copy, per FSR
   std::vector<double> h_fixed_;        // host copy of fixed sources
   bool fixed_on_ = false;
 

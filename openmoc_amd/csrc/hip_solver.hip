@@ -28,6 +28,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "hip_solver.h"
 
@@ -48,6 +49,7 @@ constexpr double kFluxEpsilon = 1.0e-25;
 constexpr double kFltEpsilon = 1.0e-12;
 constexpr int kMaxPolarHalf = 3;
 constexpr int kReduceBlock = 256;
+constexpr int kSweepBlockMax = 256;  // __launch_bounds__ of the sweep kernel
 
 template <typename T>
 void dev_alloc(T** p, size_t n) {
@@ -99,108 +101,343 @@ __device__ __forceinline__ float attenuation_hw(float y) {
 // Transport sweep
 // ---------------------------------------------------------------------------
 struct SweepArgs {
-  const int32_t* work;      // directed track ids (t*2+d)
-  int num_work;
+  const int32_t* work;      // [ntiles*tpw] directed track ids (t*2+d) or -1, longest first
+  int ntiles;               // wave tiles of tpw directed tracks
+  int tpw;                  // directed tracks per tile (= per wave)
+  const int2* tile;         // [ntiles] (first row, rows) of the interleaved streams
   const int4* trk;          // seg_begin, seg_count, out slot fwd, out slot bwd
   const int32_t* trk_azim;
-  const float* seg_len;
-  const int32_t* seg_fsr;
+  const float* seg_len;     // tile-interleaved: [row][tpw], traversal order, 0-padded
+  const int32_t* seg_key;   // tile-interleaved: [row][tpw], FSR | material << 24
   const float2* qs;         // [N*G] (Q/sigma_t, sigma_t)
   const float* psi_in;      // [2T*fpt]
   float* psi_out;           // [2T*fpt]
-  float* tally;             // [N*G]
+  float* tally;             // global path: [tally_copies][N*G]
+  int tally_copies;         // global path: private copies (block b -> copy b % copies)
+  size_t tally_stride;      // N*G
   const float* weights;     // [num_azim/2][P]
   float pf0, pf1, pf2;      // polar factors: 1/sin(theta_p) (times log2 e for HW exp)
   const IterState* state;
+  unsigned* psi_max_bits;   // &state->psi_max_bits
   int G, P, fpt, lanes_per_track, tracks_per_wave;
+  int N;
 };
 
-template <int GPL, int EXPM>
-__global__ __launch_bounds__(256) void sweep2d_kernel(SweepArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int slot = lane / a.lanes_per_track;
-  const int gl = lane - slot * a.lanes_per_track;
-  if (slot >= a.tracks_per_wave) return;
-  const int widx = wave * a.tracks_per_wave + slot;
-  if (widx >= a.num_work) return;
-  if (a.state->converged) return;
+// Segment streams carry a packed key: FSR id in the low 24 bits, material in
+// the high 8 bits (N < 2^24, M <= 256 are checked on the host).
+constexpr int kFsrBits = 24;
+constexpr int kFsrMask = (1 << kFsrBits) - 1;
 
-  const int dt = a.work[widx];
+// Source policies: (Q/sigma_t, sigma_t) of group g of the segment's FSR.
+struct GatherSource {  // global FP32 table qs[N*G]
+  static constexpr bool kLocal = false;
+  const float2* qs;
+  int G;
+  __device__ float2 load(int key, int g, int /*loc*/) const {
+    return qs[(size_t)(key & kFsrMask) * G + g];
+  }
+};
+struct LdsSource {     // LDS slices: Q/sigma_t [N][GB] and sigma_t by material [M][GB]
+  static constexpr bool kLocal = true;
+  const float* q;
+  const float* sig;
+  int GB;
+  __device__ float2 load(int key, int /*g*/, int loc) const {
+    return make_float2(q[(key & kFsrMask) * GB + loc],
+                       sig[(int)((unsigned)key >> kFsrBits) * GB + loc]);
+  }
+};
+struct SubsetSource {  // global subset-major Q/sigma_t [N][GB] + LDS sigma_t by material
+  static constexpr bool kLocal = false;
+  const float* q;
+  const float* sig;
+  int GB;
+  __device__ float2 load(int key, int /*g*/, int loc) const {
+    return make_float2(q[(size_t)(key & kFsrMask) * GB + loc],
+                       sig[(int)((unsigned)key >> kFsrBits) * GB + loc]);
+  }
+};
+
+// Tally policies: where a lane's weighted delta-psi of one segment goes.
+struct GlobalTally {   // device-scope float atomics into this block's copy
+  float* t;
+  int G;
+  __device__ void add(int fsr, int g, int /*loc*/, float v) const {
+    unsafeAtomicAdd(&t[(size_t)fsr * G + g], v);
+  }
+};
+// Fixed-point LDS tally: the block's group slice [N][GB] of int64 sums of
+// contribution / V_fsr in units of 1/scale.  Integer LDS atomics run at full
+// rate (float LDS atomics do not), and integer sums are exact and
+// order-independent, so the tally is bitwise reproducible.
+struct LdsFixedTally {
+  unsigned long long* t;
+  const float* inv_vol;   // [N] 1/V (0 for zero-volume FSRs)
+  double scale;
+  int GB;
+  __device__ void add(int fsr, int /*g*/, int loc, float v) const {
+    const double x = (double)(v * inv_vol[fsr]) * scale;
+    atomicAdd(&t[fsr * GB + loc], (unsigned long long)(long long)x);
+  }
+};
+
+// Integrates directed track dt (tile slot `slot`) for the lane's groups
+// g_begin + gl + k*lpt (k < GPL, valid while < g_begin + g_count), taking
+// sources from `src`, tallying through `tally`, and writes the outgoing
+// angular flux to the linked track's start slot.
+template <int GPL, int EXPM, int NP, class Src, class Tally>
+__device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int slot, int dt, int gl,
+                                            int lpt, int g_begin, int g_count, const Src& src,
+                                            const Tally& tally) {
   const int t = dt >> 1;
   const int dir = dt & 1;
   const int4 tr = a.trk[t];
+  const int2 ti = a.tile[tile];
   const int azim = a.trk_azim[t];
-  const int G = a.G, P = a.P, fpt = a.fpt;
+  const int G = a.G, fpt = a.fpt;
+  constexpr int P = NP;  // polar half-angles, compile time: the segment body is straight-line
   const float scale = (float)a.state->psi_scale;
   const float pf[kMaxPolarHalf] = {a.pf0, a.pf1, a.pf2};
-  float w[kMaxPolarHalf];
+  float w[P];
 #pragma unroll
-  for (int p = 0; p < kMaxPolarHalf; p++) w[p] = p < P ? a.weights[azim * P + p] : 0.f;
+  for (int p = 0; p < P; p++) w[p] = a.weights[azim * P + p];
 
-  int g[GPL];
+  // Lanes past the last group work on a clamped group and add exactly zero,
+  // so the loop below has no per-lane branches.
+  int g[GPL], loc[GPL];
   bool ok[GPL];
-  float psi[GPL][kMaxPolarHalf];
+  float psi[GPL][P];
   const float* pin = a.psi_in + (size_t)dt * fpt;
 #pragma unroll
   for (int k = 0; k < GPL; k++) {
-    g[k] = gl + k * a.lanes_per_track;
-    ok[k] = g[k] < G;
+    const int l = gl + k * lpt;
+    ok[k] = l < g_count;
+    loc[k] = ok[k] ? l : g_count - 1;
+    g[k] = g_begin + loc[k];
 #pragma unroll
-    for (int p = 0; p < kMaxPolarHalf; p++)
-      psi[k][p] = (ok[k] && p < P) ? pin[p * G + g[k]] * scale : 0.f;
+    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p * G + g[k]] * scale : 0.f;
   }
 
-  const int n = tr.y;
-  const int step = dir ? -1 : 1;
-  int s = dir ? tr.x + n - 1 : tr.x;
-  float len = 0.f;
-  int fsr = 0;
-  if (n > 0) {
-    len = a.seg_len[s];
-    fsr = a.seg_fsr[s];
-  }
-  for (int i = 0; i < n; i++) {
-    const int sn = s + step;
-    float len_n = 0.f;
-    int fsr_n = 0;
-    if (i + 1 < n) {  // software prefetch of the next segment
-      len_n = a.seg_len[sn];
-      fsr_n = a.seg_fsr[sn];
+  // Deep software pipeline.  The tile's rows are processed in chunks of D
+  // rows held in three rotating register sets, so the stream loads of a chunk
+  // are issued two chunks (2*D segments) before it is integrated.  Sources
+  // that live in LDS are read one segment ahead; sources gathered from global
+  // memory are fetched for the whole next chunk before the current one is
+  // integrated.  The waits the compiler inserts therefore always leave the
+  // most recent loads (and the tally's memory operations) in flight.  The
+  // streams are tile-interleaved ([row][tpw], one coalesced line per row and
+  // wave) and padded to the tile's longest track with length-0 segments,
+  // which integrate as exact no-ops (tau = 0 -> attenuation 0 -> delta 0);
+  // rows past the tile are clamped to its last row, so every lane runs the
+  // same trip count with unconditional loads.
+  constexpr int D = GPL >= 4 ? 2 : 4;
+  const int n = ti.y;
+  const int last = n - 1;
+  const size_t row0 = (size_t)ti.x * a.tpw + slot;
+  auto load_chunk = [&](int c, float* L, int* K) {
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const int r = c * D + j;
+      const size_t idx = row0 + (size_t)min(r, last) * a.tpw;
+      L[j] = a.seg_len[idx] * (r < n ? 1.f : 0.f);
+      K[j] = a.seg_key[idx];
     }
-    const size_t base = (size_t)fsr * G;
+  };
+  auto fetch = [&](int key, float2* q) {
+#pragma unroll
+    for (int k = 0; k < GPL; k++) q[k] = src.load(key, g[k], loc[k]);
+  };
+  auto integrate = [&](float len, int key, const float2* q) {
+    const int fsr = key & kFsrMask;
 #pragma unroll
     for (int k = 0; k < GPL; k++) {
-      if (!ok[k]) continue;
-      const float2 q = a.qs[base + g[k]];
-      const float tau = q.y * len;
+      const float tau = q[k].y * len;
       float acc = 0.f;
 #pragma unroll
-      for (int p = 0; p < kMaxPolarHalf; p++) {
-        if (p < P) {
-          const float x = tau * pf[p];
-          const float A = EXPM == MOC_EXP_RATIONAL ? attenuation_rational(x) : attenuation_hw(x);
-          const float d = (psi[k][p] - q.x) * A;
-          psi[k][p] -= d;
-          acc = fmaf(w[p], d, acc);
-        }
+      for (int p = 0; p < P; p++) {
+        const float x = tau * pf[p];
+        const float A = EXPM == MOC_EXP_RATIONAL ? attenuation_rational(x) : attenuation_hw(x);
+        const float d = (psi[k][p] - q[k].x) * A;
+        psi[k][p] -= d;
+        acc = fmaf(w[p], d, acc);
       }
-      unsafeAtomicAdd(&a.tally[base + g[k]], acc);
+      if (!ok[k]) acc = 0.f;
+      tally.add(fsr, g[k], loc[k], acc);
     }
-    len = len_n;
-    fsr = fsr_n;
-    s = sn;
+  };
+  // Integrates one chunk.  Local sources: read one segment ahead.  Global
+  // sources: `cur` holds this chunk's sources (fetched one chunk earlier) and
+  // the next chunk's sources are fetched into `nxt` first.
+  auto process = [&](const float* L, const int* K, const int* Knext, float2 (*cur)[GPL],
+                     float2 (*nxt)[GPL]) {
+    if constexpr (Src::kLocal) {
+      float2 qa[GPL], qb[GPL];
+      fetch(K[0], qa);
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        if (j + 1 < D) fetch(K[j + 1], (j & 1) ? qa : qb);
+        integrate(L[j], K[j], (j & 1) ? qb : qa);
+      }
+      (void)Knext;
+      (void)cur;
+      (void)nxt;
+    } else {
+#pragma unroll
+      for (int j = 0; j < D; j++) fetch(Knext[j], nxt[j]);
+#pragma unroll
+      for (int j = 0; j < D; j++) integrate(L[j], K[j], cur[j]);
+    }
+  };
+
+  if (n > 0) {
+    const int nch = (n + D - 1) / D;
+    // three rotating sets: chunk c lives in set c % 3 (streams and sources)
+    float L0[D], L1[D], L2[D];
+    int K0[D], K1[D], K2[D];
+    float2 Q0[Src::kLocal ? 1 : D][GPL], Q1[Src::kLocal ? 1 : D][GPL], Q2[Src::kLocal ? 1 : D][GPL];
+    load_chunk(0, L0, K0);
+    load_chunk(1, L1, K1);
+    load_chunk(2, L2, K2);
+    if constexpr (!Src::kLocal) {
+#pragma unroll
+      for (int j = 0; j < D; j++) fetch(K0[j], Q0[j]);
+    }
+    for (int c = 0; c < nch; c += 3) {
+      process(L0, K0, K1, Q0, Q1);
+      load_chunk(c + 3, L0, K0);
+      if (c + 1 < nch) {
+        process(L1, K1, K2, Q1, Q2);
+        load_chunk(c + 4, L1, K1);
+      }
+      if (c + 2 < nch) {
+        process(L2, K2, K0, Q2, Q0);
+        load_chunk(c + 5, L2, K2);
+      }
+    }
   }
 
   const int out = dir ? tr.w : tr.z;
+  float amax = 0.f;
   if (out >= 0) {
     float* po = a.psi_out + (size_t)out * fpt;
 #pragma unroll
     for (int k = 0; k < GPL; k++)
 #pragma unroll
-      for (int p = 0; p < kMaxPolarHalf; p++)
-        if (ok[k] && p < P) po[p * G + g[k]] = psi[k][p];
+      for (int p = 0; p < P; p++)
+        if (ok[k]) {
+          po[p * G + g[k]] = psi[k][p];
+          amax = fmaxf(amax, fabsf(psi[k][p]));
+        }
   }
+  // running max of |start psi| for the next sweep's fixed-point scale
+  // (non-negative floats order like their bit patterns)
+  if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
+}
+
+// Global-atomic sweep: one wave per tile of tpw directed tracks,
+// lanes_per_track lanes per track, GPL groups per lane.
+template <int GPL, int EXPM, int NP>
+__global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int tile = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int slot = lane / a.lanes_per_track;
+  const int gl = lane - slot * a.lanes_per_track;
+  if (slot >= a.tpw || tile >= a.ntiles) return;
+  if (a.state->converged) return;
+  const int dt = a.work[tile * a.tpw + slot];
+  if (dt < 0) return;
+  GatherSource src{a.qs, a.G};
+  GlobalTally tally{a.tally + (size_t)(blockIdx.x % a.tally_copies) * a.tally_stride, a.G};
+  sweep_track<GPL, EXPM, NP>(a, tile, slot, dt, gl, a.lanes_per_track, 0, a.G, src, tally);
+}
+
+// LDS-privatised sweep.  Block b owns group subset b % nsub (GB groups, one
+// lane per group) and block slot b / nsub.  LDS holds the FP32 tally slice
+// [N][GB] of the subset, sigma_t of the subset by material [M][GB] and, when
+// QLDS, the subset's Q/sigma_t slice [N][GB] (otherwise gathered from the
+// subset-major global table).  The block processes the tiles
+// slot*wpb + wave, + blocks_per_sub*wpb, ... (longest first, so the strided
+// assignment balances) and finally writes its tally slice to its own partial
+// buffer; reduce_partials_kernel sums the partials in a fixed order.
+struct LdsParams {
+  unsigned long long* partials;  // [nsub][blocks_per_sub][N*GB] fixed-point sums
+  const float* qsub;      // [nsub][N][GB] Q/sigma_t, subset-major
+  const float* inv_vol;   // [N]
+  const double* sigma_t;  // [M][G]
+  int M;
+  int GB;                 // groups per subset (= lanes per track; tpw = 64 / GB)
+  int nsub;               // ceil(G / GB)
+  int blocks_per_sub;
+};
+
+template <int EXPM, int QLDS, int NP>
+__global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParams L) {
+  extern __shared__ unsigned long long lds64[];
+  if (a.state->converged) return;  // uniform: read by every thread of the block
+  const int sub = blockIdx.x % L.nsub;
+  const int bslot = blockIdx.x / L.nsub;
+  const int g0 = sub * L.GB;
+  const int gb = min(L.GB, a.G - g0);
+  const int nl = a.N * L.GB;
+  // LDS: int64 tally [N*GB] | float 1/V [N] | sigma_t [M*GB] | Q/sigma_t [N*GB] (QLDS)
+  unsigned long long* lds_tally = lds64;
+  float* lds_ivol = reinterpret_cast<float*>(lds64 + nl);
+  float* lds_sig = lds_ivol + a.N;
+  float* lds_q = lds_sig + L.M * L.GB;
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) lds_tally[i] = 0ull;
+  for (int i = threadIdx.x; i < a.N; i += blockDim.x) lds_ivol[i] = L.inv_vol[i];
+  for (int i = threadIdx.x; i < L.M * L.GB; i += blockDim.x) {
+    const int m = i / L.GB, l = i - m * L.GB;
+    lds_sig[i] = (float)L.sigma_t[(size_t)m * a.G + g0 + min(l, gb - 1)];
+  }
+  const float* qg = L.qsub + (size_t)sub * nl;
+  if (QLDS)
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) lds_q[i] = qg[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int slot = lane / L.GB;
+  const int gl = lane - slot * L.GB;
+  if (slot < a.tpw) {
+    LdsFixedTally tally{lds_tally, lds_ivol, a.state->tally_scale, L.GB};
+    for (int tile = bslot * wpb + wib; tile < a.ntiles; tile += L.blocks_per_sub * wpb) {
+      const int dt = a.work[tile * a.tpw + slot];
+      if (dt < 0) continue;
+      if (QLDS) {
+        LdsSource src{lds_q, lds_sig, L.GB};
+        sweep_track<1, EXPM, NP>(a, tile, slot, dt, gl, L.GB, g0, gb, src, tally);
+      } else {
+        SubsetSource src{qg, lds_sig, L.GB};
+        sweep_track<1, EXPM, NP>(a, tile, slot, dt, gl, L.GB, g0, gb, src, tally);
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = L.partials + ((size_t)sub * L.blocks_per_sub + bslot) * nl;
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) out[i] = lds_tally[i];
+}
+
+// tally64[r*G + g] = sum over the subset's blocks of partials (fixed order).
+__global__ void reduce_partials_kernel(const unsigned long long* partials, double* tally64,
+                                       int64_t N, int G, int GB, int nsub, int blocks_per_sub,
+                                       const IterState* state) {
+  if (state->converged) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nsub * N * GB
+  const int64_t nl = N * GB;
+  if (i >= nl * nsub) return;
+  const int sub = (int)(i / nl);
+  const int64_t j = i - sub * nl;
+  const int64_t r = j / GB;
+  const int l = (int)(j - r * GB);
+  const int g = sub * GB + l;
+  if (g >= G) return;
+  const unsigned long long* p = partials + (size_t)sub * blocks_per_sub * nl + j;
+  unsigned long long t = 0ull;
+  for (int b = 0; b < blocks_per_sub; b++) t += p[(size_t)b * nl];  // exact (two's complement)
+  // tally64 = sum of contribution / V (the volume division of
+  // addSourceToScalarFlux is folded into the fixed-point tally)
+  tally64[r * G + g] = (double)(long long)t / state->tally_scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -220,8 +457,12 @@ struct FsrArgs {
   double* phi;
   double* phi_old;
   double* Q;
-  float2* qs;
-  float* tally;
+  float2* qs;                 // [N*G] (global-atomic sweep), or null
+  float* qsub;                // [nsub][N][GB] Q/sigma_t (LDS sweep), or null
+  int qsub_groups;            // GB
+  float* tally;               // [tally_copies][N*G] (global-atomic sweep)
+  int tally_copies;
+  const double* tally64;      // [N*G] (LDS sweep), or null
   double* partials;
   IterState* state;
   int allow_negative;
@@ -260,13 +501,24 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
       const size_t i = (size_t)r * G + g;
       double f;
       if (from_tally) {
-        f = (double)a.tally[i] / (st[g] * vol) + kFourPi * a.Q[i] / st[g];
+        if (a.tally64) {
+          // fixed-point tally already holds sum(contribution) / V
+          f = a.tally64[i] / st[g] + kFourPi * a.Q[i] / st[g];
+        } else {
+          // sum of the private tally copies in a fixed order
+          double t = 0.0;
+          const size_t stride = (size_t)a.N * G;
+          for (int c = 0; c < a.tally_copies; c++) {
+            t += (double)a.tally[c * stride + i];
+            a.tally[c * stride + i] = 0.f;
+          }
+          f = t / (st[g] * vol) + kFourPi * a.Q[i] / st[g];
+        }
         if (f < 0.0 && !a.allow_negative) {
           f = kFluxEpsilon;
           neg++;
         }
         a.phi[i] = f;
-        a.tally[i] = 0.f;
       } else {
         f = a.phi[i];
       }
@@ -305,6 +557,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
   if (a.state->converged) return;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0;
+  double qmax = 0.0;
   int64_t negsrc = 0;
   if (r < a.N) {
     const int G = a.G;
@@ -372,7 +625,12 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
         }
         const size_t i = (size_t)r * G + gd;
         a.Q[i] = q;
-        a.qs[i] = make_float2((float)(q / st[gd]), (float)st[gd]);
+        qmax = fmax(qmax, fabs(q / st[gd]));
+        if (a.qs) a.qs[i] = make_float2((float)(q / st[gd]), (float)st[gd]);
+        if (a.qsub) {
+          const int sb = gd / a.qsub_groups, l = gd - sb * a.qsub_groups;
+          a.qsub[((size_t)sb * a.N + r) * a.qsub_groups + l] = (float)(q / st[gd]);
+        }
       }
     }
     (void)old_fis;
@@ -380,13 +638,41 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
   const double s = block_sum(res);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
   if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
+  if (qmax > 0.0) {
+    // round up so the float bound is not below the double value
+    const float qf = __uint_as_float(__float_as_uint((float)qmax) + 1u);
+    atomicMax(&a.state->qmax_bits, __float_as_uint(qf));
+  }
+}
+
+// Fixed-point scale of the next sweep's LDS tally.  Over one sweep,
+// sum |w delta| / V over an FSR's segments <= 4 pi sigma_t (|psi|max +
+// |Q/sigma_t|max), because sum over its segments (both directions, all polar
+// angles) of w l / sin(theta) = 4 pi V and |delta| <= (1 - e^-tau)|psi - q|
+// <= tau / sin(theta) |psi - q|, with psi a convex mix of its inflow and the
+// sources.  The scale keeps that bound at 2^59 (int64 range 2^63).
+__device__ void set_tally_scale(IterState* st, double sigma_t_max, int take_qmax) {
+  if (take_qmax) {
+    st->qmax = __uint_as_float(st->qmax_bits);
+    st->qmax_bits = 0u;
+  }
+  st->psi_max = __uint_as_float(st->psi_max_bits);
+  st->psi_max_bits = 0u;
+  const double bound =
+      2.0 * kFourPi * sigma_t_max * ((double)st->psi_max * st->psi_scale + (double)st->qmax);
+  st->tally_scale = bound > 1e-200 ? 0x1p60 / bound : 0x1p60;
+}
+
+__global__ void tally_scale_kernel(IterState* state, double sigma_t_max, int take_qmax) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) set_tally_scale(state, sigma_t_max, take_qmax);
 }
 
 // mode 0 eigen: stop when res < tol && |int(1e5 dk)| < 1 (src/Solver.cpp:1643-1681)
 // mode 1/2 flux/source: stop when i > 1 && res < tol, _num_iterations = i.
 __global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* partials, int nparts,
                                                                IterState* state, double norm,
-                                                               double tol, int mode) {
+                                                               double tol, int mode,
+                                                               double sigma_t_max, int take_qmax) {
   if (state->converged) return;
   double v = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
@@ -411,6 +697,7 @@ __global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* pa
         state->reported_iters = i;
       }
     }
+    set_tally_scale(state, sigma_t_max, take_qmax);
   }
 }
 
@@ -425,13 +712,17 @@ __global__ void zero_f32_kernel(float* p, int64_t n) {
 }
 
 __global__ void make_qs_kernel(const double* Q, const int32_t* fsr_mat, const double* sigma_t,
-                               float2* qs, int64_t N, int G) {
+                               float2* qs, float* qsub, int GB, int64_t N, int G) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < N * G) {
     const int64_t r = i / G;
     const int g = (int)(i - r * G);
     const double st = sigma_t[(size_t)fsr_mat[r] * G + g];
-    qs[i] = make_float2((float)(Q[i] / st), (float)st);
+    if (qs) qs[i] = make_float2((float)(Q[i] / st), (float)st);
+    if (qsub) {
+      const int sb = g / GB, l = g - sb * GB;
+      qsub[((size_t)sb * N + r) * GB + l] = (float)(Q[i] / st);
+    }
   }
 }
 
@@ -466,14 +757,26 @@ HipSolver::HipSolver(int device) {
   HIP_CHECK(hipMalloc((void**)&d_state_, sizeof(IterState)));
   HIP_CHECK(hipHostMalloc((void**)&h_state_, sizeof(IterState), hipHostMallocDefault));
   std::memset(h_state_, 0, sizeof(IterState));
+  int v = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device_));
+  num_cus_ = v;
+  HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, device_));
+  lds_max_ = (size_t)v;
+  if (lds_max_ == 0) {
+    HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+    lds_max_ = (size_t)v;
+  }
+  HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerMultiprocessor, device_));
+  lds_per_cu_ = v > 0 ? (size_t)v : lds_max_;
 }
 
 HipSolver::~HipSolver() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   dev_free(d_seg_len_);
-  dev_free(d_seg_fsr_);
+  dev_free(d_seg_key_);
   dev_free(d_work_);
+  dev_free(d_tile_);
   dev_free(d_trk_);
   dev_free(d_trk_azim_);
   dev_free(d_weights_);
@@ -492,7 +795,11 @@ HipSolver::~HipSolver() {
   dev_free(d_mat_fissionable_);
   dev_free(d_fsr_mat_);
   dev_free(d_fsr_vol_);
+  dev_free(d_inv_vol_);
   dev_free(d_partials_);
+  dev_free(d_lds_partials_);
+  dev_free(d_qsub_);
+  dev_free(d_tally64_);
   dev_free(d_state_);
   if (h_state_) (void)hipHostFree(h_state_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -513,7 +820,36 @@ void HipSolver::pickShape() {
   s.lanes_per_track = (G_ + s.groups_per_lane - 1) / s.groups_per_lane;
   s.tracks_per_wave = 64 / s.lanes_per_track;
   s.block = 256;
+  // LDS-privatised tally when at least one group's slice fits in LDS: the
+  // largest subset that fits, evened out over the subsets.
+  // LDS-privatised tally when a group's slice fits in LDS: with the source
+  // slice resident too (lds_q = 1) if that fits, else gathered.  The largest
+  // subset that fits, evened out over the subsets.
+  s.lds_groups = 0;
+  s.lds_q = 0;
+  if (N_ > 0 && lds_max_ > 0) {
+    for (int q = 1; q >= 0 && s.lds_groups == 0; q--) {
+      int gb = 0;
+      while (gb < std::min(G_, 64) && ldsBytes(gb + 1, q) <= lds_max_) gb++;
+      if (gb < 1) continue;
+      const int nsub = (G_ + gb - 1) / gb;
+      gb = (G_ + nsub - 1) / nsub;
+      s.lds_groups = gb;
+      s.lds_q = q;
+      s.lds_block = 1024;
+      // about two blocks per CU: the strided tile assignment then balances
+      // the longest-first tiles across blocks (measured on the 17x17 lattice)
+      s.lds_blocks_per_sub = std::max(1, (2 * num_cus_ + nsub - 1) / nsub);
+    }
+  }
   shape_ = s;
+}
+
+// LDS of the LDS sweep: int64 tally [N*gb] | 1/V [N] | sigma_t [M*gb] | Q/sigma_t [N*gb]
+size_t HipSolver::ldsBytes(int gb, int with_q) const {
+  return (size_t)N_ * gb * sizeof(uint64_t) + (size_t)N_ * sizeof(float) +
+         (size_t)std::max(M_, 1) * gb * sizeof(float) +
+         (with_q ? (size_t)N_ * gb * sizeof(float) : 0);
 }
 
 void HipSolver::setShape(const LaunchShape& s) {
@@ -523,10 +859,20 @@ void HipSolver::setShape(const LaunchShape& s) {
   if (!okg || s.lanes_per_track <= 0 || s.lanes_per_track > 64 ||
       s.lanes_per_track * s.groups_per_lane < G_ ||
       s.tracks_per_wave <= 0 || s.tracks_per_wave * s.lanes_per_track > 64 ||
-      s.block <= 0 || s.block % 64 != 0 || s.block > 1024)
+      s.block <= 0 || s.block % 64 != 0 || s.block > kSweepBlockMax || s.tally_copies < 1 ||
+      s.tally_copies > 64)
     throw DeviceError(MOC_ERR_INVALID, "invalid sweep launch shape");
+  if (s.lds_groups < 0 || s.lds_groups > 64 || s.lds_groups > G_ || s.lds_q < 0 || s.lds_q > 1 ||
+      (s.lds_groups > 0 &&
+       (ldsBytes(s.lds_groups, s.lds_q) > lds_max_ || s.lds_block <= 0 ||
+        s.lds_block % 64 != 0 || s.lds_block > 1024 || s.lds_blocks_per_sub <= 0)))
+    throw DeviceError(MOC_ERR_INVALID, "invalid LDS sweep shape");
+  const bool realloc = s.tally_copies != shape_.tally_copies || s.lds_groups != shape_.lds_groups ||
+                       s.lds_blocks_per_sub != shape_.lds_blocks_per_sub;
   shape_ = s;
   shape_forced_ = true;
+  if (realloc) allocateIterationArrays();
+  buildTileLayout(sweepTracksPerWave());
 }
 
 void HipSolver::setTracks(const moc_track_data_t& td) {
@@ -592,10 +938,10 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int32_t x, int32_t y) { return trk[4 * x + 1] > trk[4 * y + 1]; });
-  std::vector<int32_t> work(2 * T);
+  h_order_.resize(2 * T);
   for (int64_t i = 0; i < T; i++) {
-    work[2 * i] = 2 * order[i];
-    work[2 * i + 1] = 2 * order[i] + 1;
+    h_order_[2 * i] = 2 * order[i];
+    h_order_[2 * i + 1] = 2 * order[i] + 1;
   }
   std::vector<float> w((size_t)na2 * P);
   for (int a = 0; a < na2; a++)
@@ -603,18 +949,16 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   for (int p = 0; p < kMaxPolarHalf; p++)
     inv_sin_[p] = p < P ? (float)(1.0 / td.sin_theta[p]) : 0.f;
 
-  dev_alloc(&d_seg_len_, S);
-  dev_alloc(&d_seg_fsr_, S);
   dev_alloc(&d_trk_, 4 * T);
   dev_alloc(&d_trk_azim_, T);
-  dev_alloc(&d_work_, 2 * T);
   dev_alloc(&d_weights_, w.size());
-  HIP_CHECK(hipMemcpy(d_seg_len_, len.data(), S * sizeof(float), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(d_seg_fsr_, fsr.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_trk_, trk.data(), trk.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_trk_azim_, azim.data(), T * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(d_work_, work.data(), 2 * T * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_weights_, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+  h_len_.swap(len);
+  h_fsr_.swap(fsr);
+  h_trk_.swap(trk);
+  layout_tpw_ = 0;  // streams are laid out for the launch shape once it is known
 
   T_ = T;
   S_ = S;
@@ -628,6 +972,80 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   }
   max_seg_fsr_ = max_fsr;
   allocateIterationArrays();
+}
+
+// Tile-interleaved segment streams for tiles of `tpw` directed tracks taken
+// in work order (longest first): row i of a tile holds the i-th segment, in
+// traversal order, of each of its tracks; shorter tracks are padded with
+// length-0 segments that repeat their last FSR.
+void HipSolver::buildTileLayout(int tpw) {
+  if (tpw == layout_tpw_) return;
+  if (!(tracks_set_ && fsrs_set_)) return;
+  if (N_ >= ((int64_t)1 << kFsrBits) || M_ > 256)
+    throw DeviceError(MOC_ERR_INVALID, "at most 2^24 FSRs and 256 materials");
+  const int64_t W = 2 * T_;
+  const int64_t ntiles = (W + tpw - 1) / tpw;
+  if (ntiles >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "too many tiles");
+  std::vector<int32_t> work(ntiles * tpw, -1);
+  std::vector<int32_t> tile(2 * ntiles);
+  int64_t rows = 0;
+  for (int64_t tl = 0; tl < ntiles; tl++) {
+    int32_t nmax = 0;
+    for (int sl = 0; sl < tpw; sl++) {
+      const int64_t w = tl * tpw + sl;
+      if (w >= W) break;
+      const int32_t dt = h_order_[w];
+      work[tl * tpw + sl] = dt;
+      nmax = std::max(nmax, h_trk_[4 * (dt >> 1) + 1]);
+    }
+    tile[2 * tl] = (int32_t)rows;
+    tile[2 * tl + 1] = nmax;
+    rows += nmax;
+    if (rows >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "segment streams too large");
+  }
+  const size_t total = (size_t)rows * tpw;
+  std::vector<float> ilen(total, 0.f);
+  std::vector<int32_t> ifsr(total, 0);
+  auto key = [&](int32_t f) {
+    return (int32_t)((uint32_t)f | ((uint32_t)h_fsr_mat_[f] << kFsrBits));
+  };
+  for (int64_t tl = 0; tl < ntiles; tl++) {
+    const int64_t r0 = tile[2 * tl];
+    const int32_t nr = tile[2 * tl + 1];
+    for (int sl = 0; sl < tpw; sl++) {
+      const int32_t dt = work[tl * tpw + sl];
+      if (dt < 0) continue;
+      const int32_t t = dt >> 1, dir = dt & 1;
+      const int32_t b = h_trk_[4 * t], n = h_trk_[4 * t + 1];
+      int32_t lastf = n > 0 ? h_fsr_[dir ? b : b + n - 1] : 0;
+      for (int32_t i = 0; i < nr; i++) {
+        const size_t o = (size_t)(r0 + i) * tpw + sl;
+        if (i < n) {
+          const int32_t sidx = dir ? b + n - 1 - i : b + i;
+          ilen[o] = h_len_[sidx];
+          ifsr[o] = key(h_fsr_[sidx]);
+        } else {
+          ilen[o] = 0.f;
+          ifsr[o] = key(lastf);
+        }
+      }
+    }
+  }
+  dev_alloc(&d_seg_len_, total);
+  dev_alloc(&d_seg_key_, total);
+  dev_alloc(&d_work_, work.size());
+  dev_alloc(&d_tile_, tile.size());
+  HIP_CHECK(hipMemcpy(d_seg_len_, ilen.data(), total * sizeof(float), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_seg_key_, ifsr.data(), total * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_work_, work.data(), work.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_tile_, tile.data(), tile.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  layout_tpw_ = tpw;
+  layout_ntiles_ = (int)ntiles;
+  layout_rows_ = rows;
+}
+
+int HipSolver::sweepTracksPerWave() const {
+  return shape_.lds_groups > 0 ? 64 / shape_.lds_groups : shape_.tracks_per_wave;
 }
 
 void HipSolver::setMaterials(int num_materials, int num_groups, const double* sigma_t,
@@ -656,6 +1074,8 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
   HIP_CHECK(hipMemcpy(d_chi_, chi, (size_t)M * G * sizeof(double), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_mat_fissionable_, fis.data(), M * sizeof(int32_t), hipMemcpyHostToDevice));
   mat_fissionable_ = fis;
+  sigma_t_max_ = 0.0;
+  for (size_t i = 0; i < (size_t)M * G; i++) sigma_t_max_ = std::max(sigma_t_max_, sigma_t[i]);
   if (G != G_) {
     G_ = G;
     shape_forced_ = false;
@@ -698,7 +1118,18 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
   dev_alloc(&d_fsr_vol_, num_fsrs);
   HIP_CHECK(hipMemcpy(d_fsr_mat_, material, num_fsrs * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_fsr_vol_, volume, num_fsrs * sizeof(double), hipMemcpyHostToDevice));
+  {
+    // 1/V for the fixed-point tally; zero-volume FSRs get 0, as the reference
+    // divides their tally by 1e30 (src/CPUSolver.cpp:2621-2622)
+    std::vector<float> iv(num_fsrs);
+    for (int64_t r = 0; r < num_fsrs; r++)
+      iv[r] = volume[r] < kFltEpsilon ? 0.f : (float)(1.0 / volume[r]);
+    dev_alloc(&d_inv_vol_, num_fsrs);
+    HIP_CHECK(hipMemcpy(d_inv_vol_, iv.data(), num_fsrs * sizeof(float), hipMemcpyHostToDevice));
+  }
   h_fsr_mat_.assign(material, material + num_fsrs);
+  h_fsr_vol_.assign(volume, volume + num_fsrs);
+  layout_tpw_ = 0;
   if (num_fsrs != N_) {
     fixed_on_ = false;
     h_fixed_.clear();
@@ -715,16 +1146,26 @@ void HipSolver::allocateIterationArrays() {
   const size_t psi_n = (size_t)2 * T_ * P_ * G_;
   dev_alloc(&d_psi_[0], psi_n);
   dev_alloc(&d_psi_[1], psi_n);
-  dev_alloc(&d_tally_, NG);
+  dev_alloc(&d_tally_, NG * shape_.tally_copies);
   dev_alloc(&d_qs_, NG);
   dev_alloc(&d_phi_, NG);
   dev_alloc(&d_phi_old_, NG);
   dev_alloc(&d_Q_, NG);
+  if (!shape_forced_) pickShape();
+  if (shape_.lds_groups > 0) {
+    const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
+    dev_alloc(&d_lds_partials_, (size_t)nsub * shape_.lds_blocks_per_sub * N_ * shape_.lds_groups);
+    dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
+    HIP_CHECK(hipMemset(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float)));
+    dev_alloc(&d_tally64_, NG);
+    HIP_CHECK(hipMemset(d_tally64_, 0, NG * sizeof(double)));
+  }
   num_partials_ = (int)blocks_for(N_, kReduceBlock);
   dev_alloc(&d_partials_, num_partials_);
+  buildTileLayout(sweepTracksPerWave());
   HIP_CHECK(hipMemset(d_psi_[0], 0, psi_n * sizeof(float)));
   HIP_CHECK(hipMemset(d_psi_[1], 0, psi_n * sizeof(float)));
-  HIP_CHECK(hipMemset(d_tally_, 0, NG * sizeof(float)));
+  HIP_CHECK(hipMemset(d_tally_, 0, NG * shape_.tally_copies * sizeof(float)));
   HIP_CHECK(hipMemset(d_phi_, 0, NG * sizeof(double)));
   HIP_CHECK(hipMemset(d_phi_old_, 0, NG * sizeof(double)));
   HIP_CHECK(hipMemset(d_Q_, 0, NG * sizeof(double)));
@@ -766,58 +1207,130 @@ void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
   HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
 }
 
-void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s) {
+void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
+                            hipEvent_t ev_after_sweep) {
   SweepArgs a;
+  if (layout_tpw_ != sweepTracksPerWave()) buildTileLayout(sweepTracksPerWave());
   a.work = d_work_;
-  a.num_work = (int)(2 * T_);
+  a.ntiles = layout_ntiles_;
+  a.tpw = layout_tpw_;
+  a.tile = reinterpret_cast<const int2*>(d_tile_);
   a.trk = reinterpret_cast<const int4*>(d_trk_);
   a.trk_azim = d_trk_azim_;
   a.seg_len = d_seg_len_;
-  a.seg_fsr = d_seg_fsr_;
+  a.seg_key = d_seg_key_;
   a.qs = d_qs_;
   a.psi_in = psi_in;
   a.psi_out = psi_out;
   a.tally = d_tally_;
+  a.tally_copies = shape_.tally_copies;
+  a.tally_stride = (size_t)N_ * G_;
   a.weights = d_weights_;
   const float l2e = exp_mode_ == MOC_EXP_HW ? 1.4426950408889634f : 1.0f;
   a.pf0 = inv_sin_[0] * l2e;
   a.pf1 = inv_sin_[1] * l2e;
   a.pf2 = inv_sin_[2] * l2e;
   a.state = d_state_;
+  a.psi_max_bits = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(d_state_) +
+                                               offsetof(IterState, psi_max_bits));
   a.G = G_;
   a.P = P_;
   a.fpt = P_ * G_;
+  a.N = (int)N_;
   a.lanes_per_track = shape_.lanes_per_track;
   a.tracks_per_wave = shape_.tracks_per_wave;
-  const int64_t waves = (a.num_work + a.tracks_per_wave - 1) / a.tracks_per_wave;
+  if (shape_.lds_groups > 0) {
+    LdsParams L;
+    L.GB = shape_.lds_groups;
+    L.nsub = (G_ + L.GB - 1) / L.GB;
+    L.blocks_per_sub = shape_.lds_blocks_per_sub;
+    L.partials = d_lds_partials_;
+    L.qsub = d_qsub_;
+    L.inv_vol = d_inv_vol_;
+    L.sigma_t = d_sigma_t_;
+    L.M = M_;
+    const size_t lds_bytes = ldsBytes(L.GB, shape_.lds_q);
+    if (lds_bytes > lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
+    const unsigned grid = (unsigned)(L.nsub * L.blocks_per_sub);
+    auto launch = [&](auto kernel) {
+      HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_bytes));
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(shape_.lds_block), lds_bytes, s, a, L);
+    };
+    auto by_np = [&](auto e, auto q) {
+      constexpr int E = decltype(e)::value, Q = decltype(q)::value;
+      switch (P_) {
+        case 1: launch(sweep2d_lds_kernel<E, Q, 1>); break;
+        case 2: launch(sweep2d_lds_kernel<E, Q, 2>); break;
+        default: launch(sweep2d_lds_kernel<E, Q, 3>); break;
+      }
+    };
+    using EH = std::integral_constant<int, MOC_EXP_HW>;
+    using ER = std::integral_constant<int, MOC_EXP_RATIONAL>;
+    using Q1 = std::integral_constant<int, 1>;
+    using Q0 = std::integral_constant<int, 0>;
+    if (exp_mode_ == MOC_EXP_HW)
+      shape_.lds_q ? by_np(EH{}, Q1{}) : by_np(EH{}, Q0{});
+    else
+      shape_.lds_q ? by_np(ER{}, Q1{}) : by_np(ER{}, Q0{});
+    HIP_CHECK(hipGetLastError());
+    if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
+    const int64_t nred = (int64_t)L.nsub * N_ * L.GB;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(nred, 256)), dim3(256), 0, s,
+                       (const unsigned long long*)d_lds_partials_, d_tally64_, (int64_t)N_, G_,
+                       L.GB, L.nsub,
+                       L.blocks_per_sub, (const IterState*)d_state_);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const int64_t waves = a.ntiles;
   const int wpb = shape_.block / 64;
   const unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
   dim3 blk(shape_.block);
-#define MOC_SWEEP_CASE(GPL)                                                                  \
-  case GPL:                                                                                  \
-    if (exp_mode_ == MOC_EXP_HW)                                                             \
-      hipLaunchKernelGGL((sweep2d_kernel<GPL, MOC_EXP_HW>), dim3(grid), blk, 0, s, a);       \
-    else                                                                                     \
-      hipLaunchKernelGGL((sweep2d_kernel<GPL, MOC_EXP_RATIONAL>), dim3(grid), blk, 0, s, a); \
-    break;
+  auto go = [&](auto gpl) {
+    constexpr int GPL = decltype(gpl)::value;
+    auto by_np = [&](auto kernel1, auto kernel2, auto kernel3) {
+      switch (P_) {
+        case 1: hipLaunchKernelGGL(kernel1, dim3(grid), blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(kernel2, dim3(grid), blk, 0, s, a); break;
+        default: hipLaunchKernelGGL(kernel3, dim3(grid), blk, 0, s, a); break;
+      }
+    };
+    if (exp_mode_ == MOC_EXP_HW)
+      by_np(sweep2d_kernel<GPL, MOC_EXP_HW, 1>, sweep2d_kernel<GPL, MOC_EXP_HW, 2>,
+            sweep2d_kernel<GPL, MOC_EXP_HW, 3>);
+    else
+      by_np(sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 1>, sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 2>,
+            sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 3>);
+  };
   switch (shape_.groups_per_lane) {
-    MOC_SWEEP_CASE(1)
-    MOC_SWEEP_CASE(2)
-    MOC_SWEEP_CASE(4)
-    MOC_SWEEP_CASE(9)
-    MOC_SWEEP_CASE(16)
+    case 1: go(std::integral_constant<int, 1>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 9: go(std::integral_constant<int, 9>{}); break;
+    case 16: go(std::integral_constant<int, 16>{}); break;
     default:
       throw DeviceError(MOC_ERR_INVALID, "unsupported groups per lane");
   }
-#undef MOC_SWEEP_CASE
   HIP_CHECK(hipGetLastError());
+  if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
 }
 
 namespace {
+// Points the per-FSR kernels at the source/tally buffers of the LDS sweep.
+void apply_lds_args(FsrArgs& fa, const LaunchShape& sh, double* tally64, float* qsub) {
+  if (sh.lds_groups > 0) {
+    fa.tally64 = tally64;
+    fa.qsub = qsub;
+    fa.qsub_groups = sh.lds_groups;
+    fa.qs = nullptr;
+  }
+}
+
 FsrArgs make_fsr_args(int64_t N, int G, int32_t* fsr_mat, double* vol, double* st, double* ss,
                       double* nsf, double* chi, int32_t* fis, double* fixed, double* phi,
-                      double* old, double* Q, float2* qs, float* tally, double* partials,
-                      IterState* state, bool allow_neg) {
+                      double* old, double* Q, float2* qs, float* tally, int copies,
+                      double* partials, IterState* state, bool allow_neg) {
   FsrArgs a;
   a.N = N;
   a.G = G;
@@ -834,6 +1347,10 @@ FsrArgs make_fsr_args(int64_t N, int G, int32_t* fsr_mat, double* vol, double* s
   a.Q = Q;
   a.qs = qs;
   a.tally = tally;
+  a.tally_copies = copies;
+  a.tally64 = nullptr;
+  a.qsub = nullptr;
+  a.qsub_groups = 1;
   a.partials = partials;
   a.state = state;
   a.allow_negative = allow_neg ? 1 : 0;
@@ -846,12 +1363,12 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
                                 hipEvent_t ev0, hipEvent_t ev1) {
   FsrArgs fa = make_fsr_args(N_, G_, d_fsr_mat_, d_fsr_vol_, d_sigma_t_, d_sigma_s_, d_nsf_,
                              d_chi_, d_mat_fissionable_, fixed_on_ ? d_fixed_ : nullptr, d_phi_,
-                             d_phi_old_, d_Q_, d_qs_, d_tally_, d_partials_, d_state_,
+                             d_phi_old_, d_Q_, d_qs_, d_tally_, shape_.tally_copies, d_partials_, d_state_,
                              allow_negative_);
+  apply_lds_args(fa, shape_, d_tally64_, d_qsub_);
   const unsigned nb = blocks_for(N_, kReduceBlock);
   if (ev0) HIP_CHECK(hipEventRecord(ev0, stream_));
-  launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
-  if (ev1) HIP_CHECK(hipEventRecord(ev1, stream_));
+  launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_, ev1);
   psi_cur_ ^= 1;
   hipLaunchKernelGGL(post_sweep_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa, 1);
   if (mode == MODE_EIGEN)
@@ -865,7 +1382,7 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
                      mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1);
   hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
                      num_partials_, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
-                     tol, mode == MODE_EIGEN ? 0 : 1);
+                     tol, mode == MODE_EIGEN ? 0 : 1, sigma_t_max_, recompute_sources ? 1 : 0);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -884,13 +1401,14 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // fresh angular fluxes and tally (initializeFluxArrays, src/CPUSolver.cpp:281-380)
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[1], (int64_t)psi_n);
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)NG);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   psi_cur_ = 0;
 
   FsrArgs fa = make_fsr_args(N_, G_, d_fsr_mat_, d_fsr_vol_, d_sigma_t_, d_sigma_s_, d_nsf_,
                              d_chi_, d_mat_fissionable_, fixed_on_ ? d_fixed_ : nullptr, d_phi_,
-                             d_phi_old_, d_Q_, d_qs_, d_tally_, d_partials_, d_state_,
+                             d_phi_old_, d_Q_, d_qs_, d_tally_, shape_.tally_copies, d_partials_, d_state_,
                              allow_negative_);
+  apply_lds_args(fa, shape_, d_tally64_, d_qsub_);
   const unsigned nb = blocks_for(N_, kReduceBlock);
   // initial guess (computeInitialFluxGuess, src/Solver.cpp:1710-1731; computeFlux
   // starts from phi = 0, src/Solver.cpp:1383-1390)
@@ -904,7 +1422,8 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // normalise (eigen), store old flux, sources of iteration 0
   hipLaunchKernelGGL(update_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa,
                      mode == MODE_EIGEN ? 1 : 0, 0, 0, 1, 0);
-  // start fluxes are zero: the scale of the first sweep is irrelevant
+  // fixed-point scale of the first sweep (start fluxes are zero)
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1);
   HIP_CHECK(hipGetLastError());
 
   state_valid_ = true;
@@ -1028,22 +1547,55 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   std::memset(&st, 0, sizeof(st));
   st.psi_scale = 1.0;
   st.k = 1.0;
+  {
+    // fixed-point scale from the inputs' bounds (see set_tally_scale)
+    std::vector<double> st_h((size_t)M_ * G_);
+    HIP_CHECK(hipMemcpy(st_h.data(), d_sigma_t_, st_h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double qmax = 0.0, pmax = 0.0;
+    for (size_t i = 0; i < NG; i++)
+      qmax = std::max(qmax, std::fabs(Q[i] / st_h[(size_t)h_fsr_mat_[i / G_] * G_ + i % G_]));
+    for (size_t i = 0; i < psi_n; i++) pmax = std::max(pmax, (double)std::fabs(psi[i]));
+    st.qmax = (float)(qmax * (1 + 1e-6));
+    st.psi_max = (float)(pmax * (1 + 1e-6));
+    const double bound = 2.0 * kFourPi * sigma_t_max_ * ((double)st.psi_max + (double)st.qmax);
+    st.tally_scale = bound > 1e-200 ? 0x1p60 / bound : 0x1p60;
+  }
   HIP_CHECK(hipMemcpyAsync(d_state_, &st, sizeof(st), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipMemcpyAsync(d_Q_, Q, NG * sizeof(double), hipMemcpyHostToDevice, stream_));
+  const bool lds = shape_.lds_groups > 0;
   hipLaunchKernelGGL(make_qs_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_Q_,
-                     d_fsr_mat_, d_sigma_t_, d_qs_, N_, G_);
+                     d_fsr_mat_, d_sigma_t_, lds ? nullptr : d_qs_, lds ? d_qsub_ : nullptr,
+                     std::max(1, shape_.lds_groups), N_, G_);
   HIP_CHECK(hipMemcpyAsync(d_psi_[0], psi, psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
   // slots that no track writes keep their value (vacuum inflow), as in
   // CPUSolver::_start_flux
   HIP_CHECK(hipMemcpyAsync(d_psi_[1], psi, psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)NG);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   launchSweep(d_psi_[0], d_psi_[1], stream_);
-  std::vector<float> t32(NG);
-  HIP_CHECK(hipMemcpyAsync(t32.data(), d_tally_, NG * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  std::vector<float> t32;
+  if (shape_.lds_groups > 0) {
+    // the fixed-point tally holds sum / V; the raw tally is sum
+    HIP_CHECK(hipMemcpyAsync(tally, d_tally64_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  } else {
+    t32.resize(NG * shape_.tally_copies);
+    HIP_CHECK(hipMemcpyAsync(t32.data(), d_tally_, NG * shape_.tally_copies * sizeof(float),
+                             hipMemcpyDeviceToHost, stream_));
+  }
   HIP_CHECK(hipMemcpyAsync(psi, d_psi_[1], psi_n * sizeof(float), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
-  for (size_t i = 0; i < NG; i++) tally[i] = t32[i];
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)NG);
+  if (shape_.lds_groups == 0) {
+    for (size_t i = 0; i < NG; i++) {
+      double t = 0.0;
+      for (int c = 0; c < shape_.tally_copies; c++) t += t32[c * NG + i];
+      tally[i] = t;
+    }
+  } else {
+    for (size_t i = 0; i < NG; i++) {
+      const double v = h_fsr_vol_[i / G_];
+      tally[i] *= v < kFltEpsilon ? 0.0 : v;
+    }
+  }
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   HIP_CHECK(hipStreamSynchronize(stream_));
   psi_cur_ = 0;
 }
@@ -1081,11 +1633,17 @@ double HipSolver::timeSweeps(int repeats) {
   requireReady();
   if (repeats <= 0) throw DeviceError(MOC_ERR_INVALID, "repeats must be > 0");
   HIP_CHECK(hipSetDevice(device_));
-  IterState st;
-  std::memset(&st, 0, sizeof(st));
-  st.psi_scale = 1.0;
-  st.k = keff_;
-  HIP_CHECK(hipMemcpyAsync(d_state_, &st, sizeof(st), hipMemcpyHostToDevice, stream_));
+  if (!state_valid_) launchInit(MODE_EIGEN, 1.0);
+  // repeated sweeps on the current sources and start fluxes (psi_scale 1, so
+  // |psi| stays within the bound the fixed-point scale is computed from)
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  h_state_->converged = 0;
+  h_state_->psi_scale = 1.0;
+  h_state_->psi_max_bits = __builtin_bit_cast(uint32_t, std::max(h_state_->psi_max,
+                                                   __builtin_bit_cast(float, h_state_->psi_max_bits)));
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 0);
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -1102,7 +1660,7 @@ double HipSolver::timeSweeps(int repeats) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   const size_t NG = (size_t)N_ * G_;
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)NG);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   HIP_CHECK(hipStreamSynchronize(stream_));
   return ms * 1e-3 / repeats;
 }

@@ -303,6 +303,24 @@ class HipSolver:
         check(lib().moc_solver_transport_sweep(self._h, _ptr(q), _ptr(p), _ptr(tally)))
         return tally, p
 
+    SHAPE_KEYS = ("groups_per_lane", "lanes_per_track", "tracks_per_wave", "block",
+                  "tally_copies", "lds_groups", "lds_block", "lds_blocks_per_sub", "lds_q")
+
+    def setLaunchShape(self, **shape) -> None:
+        """Override the sweep launch shape (keys of getLaunchShape())."""
+        cur = self.getLaunchShape()
+        unknown = set(shape) - set(self.SHAPE_KEYS)
+        if unknown:
+            raise MOCError(-1, f"unknown launch shape keys {sorted(unknown)}")
+        cur.update(shape)
+        a = np.array([cur[k] for k in self.SHAPE_KEYS], np.int32)
+        check(lib().moc_solver_set_launch_shape(self._h, _ptr(a)))
+
+    def getLaunchShape(self) -> dict:
+        a = np.zeros(9, np.int32)
+        check(lib().moc_solver_get_launch_shape(self._h, _ptr(a)))
+        return dict(zip(self.SHAPE_KEYS, (int(x) for x in a)))
+
     def runIterations(self, num_iterations: int) -> float:
         """Benchmark form: num_iterations full source iterations from the
         current state, no stopping rule; returns mean sweep-kernel seconds."""
