@@ -193,11 +193,12 @@ int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi
  * lds_blocks_per_sub, lds_q} (chosen by default from the device's LDS size,
  * or by the autotuner, openmoc_amd/autotune.py).
  *   lds_groups > 0: LDS-privatised tally; each block holds the [N][lds_groups]
- *     FP32 tally slice of one group subset and sigma_t by material in LDS,
- *     plus the subset's Q/sigma_t slice when lds_q = 1
- *     (lds_groups * ((1 + lds_q) * N + M) * 4 bytes must fit); lds_block
+ *     int64 fixed-point tally slice of one group subset, 1/V per FSR and
+ *     sigma_t by material in LDS, plus the subset's FP32 Q/sigma_t slice when
+ *     lds_q = 1 (8*N*lds_groups + 4*N + 4*M*lds_groups
+ *     + 4*N*lds_groups*lds_q bytes must fit the device's LDS); lds_block
  *     threads (multiple of 64, <= 1024), lds_blocks_per_sub blocks per
- *     subset; partial slices are reduced in a fixed order.
+ *     subset; the integer partial slices are summed exactly.
  *   lds_groups = 0: device-scope float atomics; groups_per_lane in
  *     {1,2,4,9,16}, lanes_per_track = ceil(G/groups_per_lane),
  *     tracks_per_wave * lanes_per_track <= 64, block a multiple of 64
