@@ -77,6 +77,109 @@ def cpu_baseline(problem_text: str, G: int, P: int, S: int, sweeps: int, threads
                           f"({threads} OpenMP threads)", "seconds": sol.sweep_seconds}
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a plain device copy of known size), or None."""
+    f = ROOT / "profiles" / f"pmc_{kernel}_{workload}.json"
+    if not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    return float(d["traffic_bytes_per_launch"]), f"{f.relative_to(ROOT)}"
+
+
+def copy_calibration(mib: int):
+    """The library's plain device copy (copy_kernel) of `mib` MiB, three
+    times: rocprofv3 counters of that kernel calibrate FETCH_SIZE/WRITE_SIZE
+    (it reads and writes exactly mib MiB per launch)."""
+    import ctypes as C
+    from openmoc_amd._lib import check, lib
+    t = C.c_double()
+    check(lib().moc_device_copy_calibration(0, mib << 20, 3, C.byref(t)))
+    log(f"copy calibration: 3 copies of {mib << 20} bytes, {t.value * 1e6:.1f} us each "
+        f"({2 * (mib << 20) / t.value / 1e9:.0f} GB/s)")
+
+
+def serving_bench(n_requests: int, seed: int = 0, max_active: int = 16):
+    """Serving stream: eigenvalue solves of pin cells, 3x3 leaking lattices and
+    17x17 lattices (7 groups), each with per-request seeded +-10% per-group
+    cross-section perturbations.  Measures (1) back-to-back plain solves,
+    (2) the engine's sustained solves/s with all requests queued, (3) p50/p99
+    latency under Poisson arrivals at 70% of that rate, and compares every
+    served result with its plain solve."""
+    import openmoc_amd as om
+    from openmoc_amd import serve, workloads
+    rng = np.random.default_rng(seed)
+    kinds = [("pin_cell_7g", 0.6), ("lattice_3x3_vac_7g", 0.3), ("lattice_17x17_7g", 0.1)]
+    tgs = {}
+    for name, _ in kinds:
+        tg = om.TrackGenerator(workloads.get(name))
+        tg.generateTracks()
+        tgs[name] = tg
+    names = rng.choice([k for k, _ in kinds], size=n_requests, p=[w for _, w in kinds])
+    reqs = []
+    for i, name in enumerate(names):
+        p = workloads.get(name)
+        r = np.random.default_rng(1000 + i)
+        p.materials = [workloads.perturb_material(m, r) for m in p.materials]
+        reqs.append((name, serve.material_arrays(p)))
+
+    # (1) back-to-back plain solves (one solver per geometry, re-armed per request)
+    solvers = {n: om.HipSolver(tg) for n, tg in tgs.items()}
+    for n, s in solvers.items():       # warm (code objects, graphs)
+        s.computeEigenvalue(1000)
+    plain = []
+    t0 = time.perf_counter()
+    for name, xs in reqs:
+        s = solvers[name]
+        s.setMaterials(*xs)
+        s.computeEigenvalue(1000)
+        plain.append((s.getKeff(), s.getFluxes()))
+    t_plain = time.perf_counter() - t0
+
+    def run_engine(arrivals):
+        with serve.ServeEngine(max_active=max_active) as eng:
+            gids = {n: eng.add_geometry(tg) for n, tg in tgs.items()}
+            for n in gids:  # warm each geometry once
+                eng.wait(eng.submit(gids[n], serve.material_arrays(workloads.get(n))), 300)
+            ids = []
+            t0 = time.perf_counter()
+            for (name, xs), ta in zip(reqs, arrivals):
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids.append(eng.submit(gids[name], xs, want_flux=True))
+            res = [eng.wait(i, 600) for i in ids]
+            return time.perf_counter() - t0, res
+
+    # (2) saturation: everything queued at t = 0
+    t_sat, res = run_engine(np.zeros(n_requests))
+    rate_sat = n_requests / t_sat
+    # (3) Poisson arrivals at 70% of the sustained rate
+    gaps = rng.exponential(1.0 / (0.7 * rate_sat), size=n_requests)
+    t_poi, res_p = run_engine(np.cumsum(gaps) - gaps[0])
+    lat = np.array([r.latency_s for r in res_p])
+
+    dk = max(abs(r.keff - k) for r, (k, _) in zip(res + res_p, plain + plain))
+    dphi = max(float(np.linalg.norm(r.flux.ravel() - f) / np.linalg.norm(f))
+               for r, (_, f) in zip(res + res_p, plain + plain))
+    counts = {k: int(np.sum(names == k)) for k, _ in kinds}
+    return {
+        "requests": n_requests, "mix": counts, "max_active": max_active,
+        "plain_solves_per_s": n_requests / t_plain,
+        "served_solves_per_s": rate_sat,
+        "speedup_vs_plain": rate_sat / (n_requests / t_plain),
+        "poisson_rate_per_s": 0.7 * rate_sat,
+        "poisson_solves_per_s": n_requests / t_poi,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "max_dk_pcm_vs_plain": dk * 1e5, "max_flux_rel_l2_vs_plain": dphi,
+        "note": "each served result compared with the plain single solve of the same "
+                "inputs (bitwise equal expected); parity of plain solves vs the "
+                "compiled reference is in the solve object and the test suite",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +191,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--serving", type=int, default=48, metavar="N",
+                    help="requests in the serving-stream measurement (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy kernel "
+                         "of known size, the reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,7 +224,7 @@ def main():
         from openmoc_amd.autotune import autotune
         solver.computeEigenvalue(3)
         shape, t_best = autotune(solver, repeats=5)
-        log(f"autotune: {shape} -> {t_best * 1e6:.1f} us/sweep")
+        log(f"autotune: {shape} -> {t_best * 1e6:.1f} us/iteration")
 
     solve = None
     if not args.no_solve:
@@ -138,24 +246,37 @@ def main():
         log(f"solve: {solve}")
 
     # steps: full source iterations, stopping rule disabled
-    solver.runIterations(max(args.warmup, 1))
+    solver.runIterations(max(args.warmup, 1), time_sweep=False)
+    # exactly `steps` iterations; runIterations synchronises the device on
+    # entry (state read) and before returning
     t0 = time.perf_counter()
-    sweep_s = solver.runIterations(args.steps)   # synchronises before returning
+    step_dev_s, _ = solver.runIterations(args.steps, time_sweep=False)
     elapsed = time.perf_counter() - t0
+    # the sweep kernel alone (HIP events on its stream), for the roofline
+    _, sweep_s = solver.runIterations(8)
     value = w_sweep * args.steps / elapsed
     ms_per_step = elapsed / args.steps * 1e3
 
     bytes_sweep = 8 * S + 8 * N * G + 16 * T * G * P
     achieved_gbs = bytes_sweep / sweep_s / 1e9
     achieved_tflops = FLOPS_PER_INTEGRATION * w_sweep / sweep_s / 1e12
+    kernel = "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel"
+    traffic, traffic_src = measured_traffic(kernel, args.workload)
     roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel",
+                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": traffic_src,
+                "kernel": kernel,
                 "algorithmic_bytes_per_launch": bytes_sweep,
                 "launch_seconds": sweep_s}
     roofline_valu = {"achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                      "flops_per_integration": FLOPS_PER_INTEGRATION}
+
+    serving = None
+    if args.serving > 0:
+        log(f"serving: {args.serving} requests ...")
+        serving = serving_bench(args.serving)
+        log(f"serving: {serving}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -177,11 +298,13 @@ def main():
                    "step": "one source iteration (sweep + per-FSR FP64 steps)"},
         "sweep_integrations_per_s": w_sweep / sweep_s,
         "roofline": roofline, "roofline_valu": roofline_valu,
-        "cpu_baseline": cpu, "solve": solve,
+        "cpu_baseline": cpu, "solve": solve, "serving": serving,
     }
     if cpu:
         out["speedup_vs_cpu_sweep"] = (w_sweep / sweep_s) / cpu["value"]
     print(json.dumps(out), flush=True)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     return 0
 
 

@@ -152,6 +152,30 @@ int moc_solver_allow_negative_fluxes(moc_solver_t* solver, int allow);
 int moc_solver_set_fixed_source_by_fsr(moc_solver_t* solver, int64_t fsr, int32_t group,
                                        double source);
 
+/* stabilizationType (src/Solver.h:103-113) */
+#define MOC_STAB_NONE (-1)
+#define MOC_STAB_DIAGONAL 0
+#define MOC_STAB_YAMAMOTO 1
+#define MOC_STAB_GLOBAL 2
+/* Solver::stabilizeTransport(factor, type) (src/Solver.cpp:676-681); applied
+ * by computeEigenvalue from its second iteration on (src/Solver.cpp:1616-1635,
+ * CPUSolver::computeStabilizingFlux / stabilizeFlux, src/CPUSolver.cpp:
+ * 2665-2805).  type MOC_STAB_NONE switches it off. */
+int moc_solver_stabilize_transport(moc_solver_t* solver, double factor, int32_t type);
+/* Solver::setKeffFromNeutronBalance (src/Solver.cpp:2047-2049): keff =
+ * fission / (absorption + vacuum leakage) instead of the fission-rate ratio
+ * (CPUSolver::computeKeff, src/CPUSolver.cpp:2258-2328).  on = 0 restores the
+ * default. */
+int moc_solver_set_keff_from_neutron_balance(moc_solver_t* solver, int on);
+/* Krylov operator sweeps (Solver::fissionTransportSweep / scatterTransportSweep,
+ * src/Solver.cpp:1282-1297): sources from the current flux (the last solve's,
+ * or moc_solver_set_fluxes'), fission-only (CPUSolver::computeFSRFissionSources,
+ * src/CPUSolver.cpp:2030-2066) or scatter-only (computeFSRScatterSources,
+ * :2072-2104); one sweep from the start fluxes the previous operator sweep
+ * left (zero at first); addSourceToScalarFlux into the flux. */
+int moc_solver_fission_transport_sweep(moc_solver_t* solver);
+int moc_solver_scatter_transport_sweep(moc_solver_t* solver);
+
 /* Solver::computeEigenvalue(max_iters, res_type) (src/Solver.cpp:1542-1689). */
 int moc_solver_compute_eigenvalue(moc_solver_t* solver, int32_t max_iters, int32_t res_type);
 /* Solver::computeFlux(max_iters, only_fixed_source) (src/Solver.cpp:1352-1420). */
@@ -166,8 +190,10 @@ int moc_solver_get_num_iterations(const moc_solver_t* solver, int32_t* iters);
 int moc_solver_get_flux(moc_solver_t* solver, int64_t fsr, int32_t group, double* flux);
 /* CPUSolver::getFluxes(out, N*G) (src/CPUSolver.cpp:66-130). */
 int moc_solver_get_fluxes(moc_solver_t* solver, double* out, int64_t num_fluxes);
-/* CPUSolver::setFluxes: copied in, used as the initial guess of the next
- * computeEigenvalue / computeSource (src/CPUSolver.cpp:166-192). */
+/* CPUSolver::setFluxes (src/CPUSolver.cpp:166-192): copied in; the operand
+ * of the next Krylov operator sweep and what get_fluxes returns until then.
+ * A compute_* call starts from its own flat guess, as the reference does
+ * without a restart (Solver::computeInitialFluxGuess, src/Solver.cpp:1710-1731). */
 int moc_solver_set_fluxes(moc_solver_t* solver, const double* in, int64_t num_fluxes);
 /* CPUSolver::computeFSRFissionRates(rates, N) (src/CPUSolver.cpp:2850-2890). */
 int moc_solver_compute_fsr_fission_rates(moc_solver_t* solver, double* rates, int64_t num_fsrs);
@@ -206,18 +232,72 @@ int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi
 int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape9);
 int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape9);
 
-/* Benchmark form of the eigenvalue iteration: runs num_iterations source
- * iterations (sweep + all per-FSR steps) from the current state with the
- * stopping rule disabled, synchronises, and returns the mean device time of
- * the transport-sweep kernel (the launch itself, without the partial-tally
- * reduction) measured with HIP events on the solver's stream. */
+/* Benchmark form of the eigenvalue iteration (no counterpart in the
+ * reference): runs num_iterations source iterations (sweep + all per-FSR
+ * steps, in the graph batches a solve uses) from the current state with the
+ * stopping rule disabled, synchronises, and returns the mean device time per
+ * iteration; sweep_seconds_avg (nullable) receives the mean time of the
+ * transport-sweep kernel alone, from HIP events around it on the solver's
+ * stream in up to 8 further iterations. */
 int moc_solver_run_iterations(moc_solver_t* solver, int32_t num_iterations,
-                              double* sweep_seconds_avg);
+                              double* step_seconds_avg, double* sweep_seconds_avg);
 
 /* Device-side sweep timing: runs `repeats` sweeps back to back on the
  * solver's current state and returns the mean kernel time (s) measured with
  * HIP events on the sweep's stream. */
 int moc_solver_time_sweeps(moc_solver_t* solver, int32_t repeats, double* seconds_per_sweep);
+
+/* ========================================================================
+ * Serving: continuous batching of eigenvalue solves on one GPU.  No
+ * counterpart in the reference (its GPUSolver runs one synchronous solve,
+ * src/accel/cuda/GPUSolver.cu:1664); each solve follows
+ * Solver::computeEigenvalue (src/Solver.cpp:1542-1689).
+ * A geometry (tracks, FSRs; materials fix G, M and the launch shape) is
+ * uploaded once; requests bring their own cross sections.  Requests are
+ * admitted as solve slots and device memory (mem_budget_bytes <= 0: 80% of
+ * the free memory) allow; their iterations overlap on the device.  A
+ * result does not depend on what else ran (bitwise the same as the solve
+ * alone).
+ * ====================================================================== */
+typedef struct moc_engine moc_engine_t;
+
+typedef struct {
+  double keff;
+  int32_t num_iterations;
+  double latency_s;   /* submit -> result */
+  double queue_s;     /* submit -> admitted */
+  double solve_s;     /* admitted -> result */
+} moc_request_result_t;
+
+int moc_engine_create(int device, int32_t max_active, int64_t mem_budget_bytes,
+                      moc_engine_t** out);
+/* Finishes the solves in flight, then frees everything. */
+void moc_engine_destroy(moc_engine_t* engine);
+int moc_engine_add_geometry(moc_engine_t* engine, const moc_track_data_t* tracks,
+                            int64_t num_fsrs, const int32_t* fsr_material,
+                            const double* fsr_volume, int32_t num_materials, int32_t num_groups,
+                            const double* sigma_t, const double* sigma_s,
+                            const double* nu_sigma_f, const double* chi, int32_t* geo_id);
+/* Queues computeEigenvalue(max_iters, res_type) with convergence threshold
+ * `tolerance` on geometry geo_id with these cross sections (layout as in
+ * moc_solver_set_materials; counts must match the geometry's). */
+int moc_engine_submit(moc_engine_t* engine, int32_t geo_id, int32_t num_materials,
+                      int32_t num_groups, const double* sigma_t, const double* sigma_s,
+                      const double* nu_sigma_f, const double* chi, int32_t max_iters,
+                      int32_t res_type, double tolerance, int want_flux, int64_t* request_id);
+/* Waits up to timeout_s (< 0: no limit) for a request; MOC_ERR_STATE if it
+ * is still running.  flux (nullable, N*G) receives the normalised flux when
+ * want_flux was set.  A returned request is forgotten. */
+int moc_engine_wait(moc_engine_t* engine, int64_t request_id, double timeout_s,
+                    moc_request_result_t* result, double* flux, int64_t num_fluxes);
+int moc_engine_stats(moc_engine_t* engine, int64_t* completed, int64_t* failed,
+                     int64_t* iterations, int32_t* pending);
+
+/* Diagnostic (no counterpart in the reference): a plain device-to-device copy
+ * of `bytes` bytes (a multiple of 16) run `repeats` times on `device`; the mean
+ * kernel time goes to *seconds.  The copy kernel reads and writes exactly
+ * `bytes` each time, which calibrates profiler HBM byte counters. */
+int moc_device_copy_calibration(int device, int64_t bytes, int32_t repeats, double* seconds);
 
 #ifdef __cplusplus
 }

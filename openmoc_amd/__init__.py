@@ -7,6 +7,7 @@ The HIP library is built in-tree (``python -m openmoc_amd.build``); there is
 no CPU fallback for the solver.
 """
 from .workloads import Problem, get as get_workload  # noqa: F401
-from .solver import (EXP_HW, EXP_RATIONAL, FISSION_SOURCE, SCALAR_FLUX,  # noqa: F401
-                     TOTAL_SOURCE, GPUSolver, HipSolver, MOCError, TrackGenerator)
+from .solver import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL,  # noqa: F401
+                     SCALAR_FLUX, TOTAL_SOURCE, YAMAMOTO, GPUSolver, HipSolver, MOCError,
+                     TrackGenerator)
 from . import process  # noqa: F401

@@ -11,6 +11,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libopenmoc_amd.so"
 MOC_OK = 0
 SCALAR_FLUX, FISSION_SOURCE, TOTAL_SOURCE = 0, 1, 2
 EXP_RATIONAL, EXP_HW = 0, 1
+# stabilizationType (src/Solver.h:103-113)
+DIAGONAL, YAMAMOTO, GLOBAL = 0, 1, 2
 
 
 class MOCError(RuntimeError):
@@ -42,6 +44,11 @@ class TrackData(C.Structure):
                 ("total_weight", C.c_void_p), ("sin_theta", C.c_void_p)]
 
 
+class RequestResult(C.Structure):
+    _fields_ = [("keff", C.c_double), ("num_iterations", C.c_int32),
+                ("latency_s", C.c_double), ("queue_s", C.c_double), ("solve_s", C.c_double)]
+
+
 # (name, restype, argtypes) of every exported function
 _P = C.c_void_p
 _SIGS = [
@@ -64,6 +71,24 @@ _SIGS = [
     ("moc_solver_set_exponential", C.c_int, [_P, C.c_int]),
     ("moc_solver_allow_negative_fluxes", C.c_int, [_P, C.c_int]),
     ("moc_solver_set_fixed_source_by_fsr", C.c_int, [_P, C.c_int64, C.c_int32, C.c_double]),
+    ("moc_device_copy_calibration", C.c_int, [C.c_int, C.c_int64, C.c_int32,
+                                              C.POINTER(C.c_double)]),
+    ("moc_engine_create", C.c_int, [C.c_int, C.c_int32, C.c_int64, C.POINTER(_P)]),
+    ("moc_engine_destroy", None, [_P]),
+    ("moc_engine_add_geometry", C.c_int, [_P, C.POINTER(TrackData), C.c_int64, _P, _P,
+                                          C.c_int32, C.c_int32, _P, _P, _P, _P,
+                                          C.POINTER(C.c_int32)]),
+    ("moc_engine_submit", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P,
+                                    C.c_int32, C.c_int32, C.c_double, C.c_int,
+                                    C.POINTER(C.c_int64)]),
+    ("moc_engine_wait", C.c_int, [_P, C.c_int64, C.c_double, C.POINTER(RequestResult), _P,
+                                  C.c_int64]),
+    ("moc_engine_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    ("moc_solver_stabilize_transport", C.c_int, [_P, C.c_double, C.c_int32]),
+    ("moc_solver_set_keff_from_neutron_balance", C.c_int, [_P, C.c_int]),
+    ("moc_solver_fission_transport_sweep", C.c_int, [_P]),
+    ("moc_solver_scatter_transport_sweep", C.c_int, [_P]),
     ("moc_solver_compute_eigenvalue", C.c_int, [_P, C.c_int32, C.c_int32]),
     ("moc_solver_compute_flux", C.c_int, [_P, C.c_int32, C.c_int32]),
     ("moc_solver_compute_source", C.c_int, [_P, C.c_int32, C.c_double, C.c_int32]),
@@ -78,7 +103,8 @@ _SIGS = [
     ("moc_solver_transport_sweep", C.c_int, [_P, _P, _P, _P]),
     ("moc_solver_set_launch_shape", C.c_int, [_P, _P]),
     ("moc_solver_get_launch_shape", C.c_int, [_P, _P]),
-    ("moc_solver_run_iterations", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    ("moc_solver_run_iterations", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]),
     ("moc_solver_time_sweeps", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
 ]
 

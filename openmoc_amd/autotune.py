@@ -1,7 +1,8 @@
 """Sweep launch-shape autotuner.
 
 Times a set of candidate launch shapes of the transport sweep on the solver's
-own problem (HIP events around repeated sweeps, moc_solver_time_sweeps) and
+own problem (HIP events around whole source iterations, which include the
+reduction of the sweep's partial tallies, moc_solver_run_iterations) and
 keeps the fastest.  Candidates cover both tally paths:
 
   * LDS sweep (lds_groups > 0): groups per block subset (= lanes per track),
@@ -52,7 +53,7 @@ def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
 def autotune(solver, repeats: int = 5, num_cus: int = 256, verbose: bool = False,
              cands: Optional[List[dict]] = None) -> Tuple[dict, float]:
     """Pick the fastest launch shape for `solver`'s problem and apply it.
-    Returns (shape, seconds per sweep)."""
+    Returns (shape, seconds per source iteration)."""
     key = _signature(solver)
     if key in _CACHE:
         shape, t = _CACHE[key]
@@ -67,7 +68,10 @@ def autotune(solver, repeats: int = 5, num_cus: int = 256, verbose: bool = False
             solver.setLaunchShape(**shape)
         except RuntimeError:
             continue
-        t = solver.timeSweeps(repeats)
+        # a whole source iteration (the sweep, the reduction of its partial
+        # tallies and the per-FSR steps), as a solve runs it
+        solver.runIterations(2, time_sweep=False)
+        t, _ = solver.runIterations(max(8, repeats), time_sweep=False)
         if verbose:
             print(f"autotune {t * 1e6:9.1f} us {c}", flush=True)
         if best is None or t < best[1]:

@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libopenmoc_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tracks.cpp", "capi.cpp", "hip_solver.hip"]
+SOURCES = ["tracks.cpp", "capi.cpp", "serve.cpp", "hip_solver.hip"]
 
 # gfx950 only; packed FP32 instructions disabled (-packed-fp32-ops); hardware
 # float atomics for the sweep tally (-munsafe-fp-atomics).
@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
+HOST_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall"]
 
 
 def _run(cmd):

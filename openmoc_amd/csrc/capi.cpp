@@ -7,6 +7,7 @@
 
 #include "../../include/openmoc_amd.h"
 #include "hip_solver.h"
+#include "serve.h"
 #include "tracks.h"
 
 struct moc_tracks {
@@ -16,6 +17,10 @@ struct moc_tracks {
 
 struct moc_solver {
   moc::HipSolver* impl;
+};
+
+struct moc_engine {
+  moc::ServeEngine* impl;
 };
 
 namespace {
@@ -238,6 +243,111 @@ int moc_solver_set_fixed_source_by_fsr(moc_solver_t* s, int64_t fsr, int32_t gro
   MOC_SOLVER_CALL(s, s->impl->setFixedSourceByFSR(fsr, group, source));
 }
 
+int moc_engine_create(int device, int32_t max_active, int64_t mem_budget_bytes,
+                      moc_engine_t** out) {
+  if (!out) return fail(MOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto* e = new moc_engine;
+    e->impl = nullptr;
+    try {
+      e->impl = new moc::ServeEngine(device, max_active, mem_budget_bytes);
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+void moc_engine_destroy(moc_engine_t* e) {
+  if (!e) return;
+  try {
+    delete e->impl;
+  } catch (...) {
+  }
+  delete e;
+}
+
+#define MOC_ENGINE_CALL(e, body)                                            \
+  do {                                                                      \
+    if (!(e) || !(e)->impl) return fail(MOC_ERR_INVALID, "null engine");    \
+    return guarded([&] { body; });                                          \
+  } while (0)
+
+int moc_engine_add_geometry(moc_engine_t* e, const moc_track_data_t* tracks, int64_t num_fsrs,
+                            const int32_t* fsr_material, const double* fsr_volume,
+                            int32_t num_materials, int32_t num_groups, const double* sigma_t,
+                            const double* sigma_s, const double* nu_sigma_f, const double* chi,
+                            int32_t* geo_id) {
+  if (!tracks || !geo_id) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_ENGINE_CALL(e, *geo_id = e->impl->addGeometry(*tracks, num_fsrs, fsr_material, fsr_volume,
+                                                    num_materials, num_groups, sigma_t, sigma_s,
+                                                    nu_sigma_f, chi));
+}
+
+int moc_engine_submit(moc_engine_t* e, int32_t geo_id, int32_t num_materials, int32_t num_groups,
+                      const double* sigma_t, const double* sigma_s, const double* nu_sigma_f,
+                      const double* chi, int32_t max_iters, int32_t res_type, double tolerance,
+                      int want_flux, int64_t* request_id) {
+  if (!request_id) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_ENGINE_CALL(e, *request_id = e->impl->submit(geo_id, num_materials, num_groups, sigma_t,
+                                                   sigma_s, nu_sigma_f, chi, max_iters, res_type,
+                                                   tolerance, want_flux != 0));
+}
+
+int moc_engine_wait(moc_engine_t* e, int64_t request_id, double timeout_s,
+                    moc_request_result_t* result, double* flux, int64_t num_fluxes) {
+  if (!e || !e->impl) return fail(MOC_ERR_INVALID, "null engine");
+  bool done = false;
+  moc::RequestResult r;
+  const int rc = guarded([&] { done = e->impl->wait(request_id, timeout_s, &r, flux, num_fluxes); });
+  if (rc != MOC_OK) return rc;
+  if (!done) return fail(MOC_ERR_STATE, "request not finished within the timeout");
+  if (result) {
+    result->keff = r.keff;
+    result->num_iterations = r.num_iterations;
+    result->latency_s = r.latency_s;
+    result->queue_s = r.queue_s;
+    result->solve_s = r.solve_s;
+  }
+  return MOC_OK;
+}
+
+int moc_engine_stats(moc_engine_t* e, int64_t* completed, int64_t* failed, int64_t* iterations,
+                     int32_t* pending) {
+  MOC_ENGINE_CALL(e, {
+    const moc::ServeStats s = e->impl->stats();
+    if (completed) *completed = s.completed;
+    if (failed) *failed = s.failed;
+    if (iterations) *iterations = s.iterations;
+    if (pending) *pending = s.pending;
+  });
+}
+
+int moc_device_copy_calibration(int device, int64_t bytes, int32_t repeats, double* seconds) {
+  return guarded([&] {
+    const double t = moc::deviceCopyCalibration(device, bytes, repeats);
+    if (seconds) *seconds = t;
+  });
+}
+
+int moc_solver_stabilize_transport(moc_solver_t* s, double factor, int32_t type) {
+  MOC_SOLVER_CALL(s, s->impl->stabilizeTransport(factor, type));
+}
+
+int moc_solver_set_keff_from_neutron_balance(moc_solver_t* s, int on) {
+  MOC_SOLVER_CALL(s, s->impl->setKeffFromNeutronBalance(on != 0));
+}
+
+int moc_solver_fission_transport_sweep(moc_solver_t* s) {
+  MOC_SOLVER_CALL(s, s->impl->krylovSweep(1));
+}
+
+int moc_solver_scatter_transport_sweep(moc_solver_t* s) {
+  MOC_SOLVER_CALL(s, s->impl->krylovSweep(2));
+}
+
 int moc_solver_compute_eigenvalue(moc_solver_t* s, int32_t max_iters, int32_t res_type) {
   MOC_SOLVER_CALL(s, s->impl->computeEigenvalue(max_iters, res_type));
 }
@@ -326,9 +436,9 @@ int moc_solver_get_launch_shape(const moc_solver_t* s, int32_t* shape9) {
 }
 
 int moc_solver_run_iterations(moc_solver_t* s, int32_t num_iterations,
-                              double* sweep_seconds_avg) {
-  if (!sweep_seconds_avg) return fail(MOC_ERR_INVALID, "null argument");
-  MOC_SOLVER_CALL(s, *sweep_seconds_avg = s->impl->runIterations(num_iterations));
+                              double* step_seconds_avg, double* sweep_seconds_avg) {
+  if (!step_seconds_avg) return fail(MOC_ERR_INVALID, "null argument");
+  MOC_SOLVER_CALL(s, *step_seconds_avg = s->impl->runIterations(num_iterations, sweep_seconds_avg));
 }
 
 int moc_solver_time_sweeps(moc_solver_t* s, int32_t repeats, double* seconds_per_sweep) {

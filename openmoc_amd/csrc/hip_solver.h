@@ -14,8 +14,10 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -45,6 +47,8 @@ struct IterState {
   uint32_t qmax_bits;       // max |Q/sigma_t| of the sources being computed (float bits)
   float qmax;               // max |Q/sigma_t| of the sources in use
   float psi_max;            // max |start psi| of the sweep in use (before psi_scale)
+  double scale_cap;         // upper limit of the FP32 scale s32 (1e38 / max 1/V)
+  int old_sel;              // which old-flux buffer holds the current normalised flux
 };
 
 struct LaunchShape {
@@ -72,6 +76,12 @@ class DeviceError : public std::exception {
   std::string msg_;
 };
 
+// Plain device-to-device copy of `bytes` bytes, `repeats` times; returns the
+// mean kernel time (s).  Calibrates HBM byte counters against a known size.
+double deviceCopyCalibration(int device, int64_t bytes, int repeats);
+
+struct FsrArgs;  // arguments of the per-FSR kernels
+
 class HipSolver {
  public:
   explicit HipSolver(int device);
@@ -85,7 +95,38 @@ class HipSolver {
   void setFSRs(int64_t num_fsrs, const int32_t* material, const double* volume);
   void setConvergenceThreshold(double t);
   void setExponential(int mode);
-  void allowNegativeFluxes(bool allow) { allow_negative_ = allow; }
+  void allowNegativeFluxes(bool allow) {
+    allow_negative_ = allow;
+    graph_dirty_ = true;
+  }
+  // Solver::stabilizeTransport(factor, type) (src/Solver.cpp:676-681);
+  // type < 0 switches stabilisation off.
+  void stabilizeTransport(double factor, int type);
+  // Solver::setKeffFromNeutronBalance (src/Solver.cpp:2047-2049).
+  void setKeffFromNeutronBalance(bool on) {
+    balance_ = on;
+    graph_dirty_ = true;
+  }
+  // Solver::fissionTransportSweep / scatterTransportSweep (src/Solver.cpp:
+  // 1282-1297): sources from the current flux, one sweep from the current
+  // start fluxes, addSourceToScalarFlux.
+  void krylovSweep(int source_kind);
+
+  // Uses the tracks, FSRs and launch shape of `src` (device buffers shared,
+  // not copied) in this fresh solver; materials stay per solver.  For many
+  // solves of one geometry with different cross sections (serving).
+  void shareGeometry(HipSolver& src);
+  // Asynchronous solve: startSolve launches the first batch of iterations;
+  // pollSolve returns true once the solve has finished (results ready),
+  // launching the next batch when the previous one completed.  mode 0
+  // eigenvalue, 1 flux, 2 source (k_fixed).
+  void startSolve(int mode, int max_iters, int res_type, double k_fixed);
+  bool pollSolve();
+  bool solveActive() const { return async_active_; }
+  // Device bytes of one solve's own arrays (shared geometry excluded).
+  size_t solveBytes() const;
+  void useGraphs(bool on) { use_graphs_ = on; }
+  int device() const { return device_; }
   void setFixedSourceByFSR(int64_t fsr, int group, double source);
 
   void computeEigenvalue(int max_iters, int res_type);
@@ -102,8 +143,11 @@ class HipSolver {
 
   void transportSweepOnce(const double* Q, float* psi, double* tally);
   // Runs n eigenvalue source iterations with the stopping rule disabled
-  // (benchmark form).  Returns the mean device time of the sweep kernel.
-  double runIterations(int n);
+  // (benchmark form), as a solve runs them (graph batches).  Returns the mean
+  // device time per iteration; *sweep_seconds (nullable) receives the mean
+  // time of the sweep kernel alone, from events around it in up to 8 more
+  // iterations.
+  double runIterations(int n, double* sweep_seconds);
   double timeSweeps(int repeats);
 
   const LaunchShape& shape() const { return shape_; }
@@ -115,10 +159,17 @@ class HipSolver {
   void allocateIterationArrays();
   void runSolve(Mode mode, int max_iters, int res_type, double k_fixed);
   void launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
-                   hipEvent_t ev_after_sweep = nullptr);
+                   hipEvent_t ev_after_sweep = nullptr, float* leak = nullptr);
   void launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   void launchInit(Mode mode, double k0);
+  void ensureGraph(int mode, int res_type);
+  void launchIterations(int mode, int res_type, int n);
+  void launchAsyncBatch();
+  void finishSolve(int max_iters);
+  FsrArgs fsrArgs();
+  void refreshStabilization();
+  const double* currentFlux() const;
   void pickShape();
   void countFissionable();
   void buildTileLayout(int tracks_per_wave);
@@ -152,7 +203,7 @@ class HipSolver {
   float* d_tally_ = nullptr;           // [N*G]
   float2* d_qs_ = nullptr;             // [N*G] (Q/sigma_t, sigma_t)
   double* d_phi_ = nullptr;            // [N*G]
-  double* d_phi_old_ = nullptr;        // [N*G]
+  double* d_phi_old_[2] = {nullptr, nullptr};  // [N*G] normalised fluxes, ping-pong
   double* d_Q_ = nullptr;              // [N*G]
   double* d_fixed_ = nullptr;          // [N*G]
   double* d_sigma_t_ = nullptr;        // [M*G]
@@ -164,9 +215,34 @@ class HipSolver {
   double* d_fsr_vol_ = nullptr;        // [N]
   float* d_inv_vol_ = nullptr;         // [N] 1/V (0 for zero volume)
   double sigma_t_max_ = 0.0;
+  double scale_cap_ = 1e38;       // 1e38 / max(1/V): keeps (1/V) * s32 finite in FP32
   double* d_partials_ = nullptr;       // [max blocks]
   unsigned long long* d_lds_partials_ = nullptr;  // [nsub][blocks_per_sub][N*GB] fixed point
   float* d_qsub_ = nullptr;            // [nsub][N][GB] Q/sigma_t for the LDS sweep
+  double* d_sigma_a_ = nullptr;        // [M*G] absorption
+  float* d_leak_ = nullptr;            // [2T*G] vacuum leakage (neutron-balance keff)
+  double* d_stab_ratio_ = nullptr;     // [G] YAMAMOTO ratios
+  std::vector<double> h_sigma_t_, h_sigma_s_;  // host copies (dest-major scattering)
+  int stab_type_ = -1;
+  double stab_factor_ = 0.0;
+  bool balance_ = false;
+  bool phi_is_old_ = false;            // current flux lives in d_phi_old_[old_sel] (after a solve)
+  int old_sel_ = 0;                    // host copy of IterState::old_sel after a solve
+  bool krylov_ready_ = false;          // start fluxes initialised for Krylov sweeps
+  int pstride_ = 0;                    // stride of the per-block partial arrays
+  struct GeoHold;
+  std::shared_ptr<GeoHold> geo_hold_;  // set when the geometry buffers are shared
+  // captured iteration batch
+  hipGraphExec_t graph_exec_ = nullptr;
+  int graph_mode_ = -1, graph_res_type_ = -1;
+  bool graph_dirty_ = true;
+  bool use_graphs_ = true;
+  // asynchronous solve
+  bool async_active_ = false;
+  hipEvent_t async_ev_ = nullptr, async_ev0_ = nullptr;
+  int async_mode_ = 0, async_res_type_ = 0, async_max_ = 0, async_done_ = 0;
+  double async_ms_ = 0.0;
+  std::chrono::steady_clock::time_point async_t0_;
   double* d_tally64_ = nullptr;        // [N*G] reduced LDS-path tally
   int num_cus_ = 0;
   size_t lds_max_ = 0;                 // max dynamic LDS per block (bytes)

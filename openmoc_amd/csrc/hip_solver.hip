@@ -25,6 +25,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -50,24 +52,66 @@ constexpr double kFltEpsilon = 1.0e-12;
 constexpr int kMaxPolarHalf = 3;
 constexpr int kReduceBlock = 256;
 constexpr int kSweepBlockMax = 256;  // __launch_bounds__ of the sweep kernel
+// Segment-stream rows: every tile's row count is a multiple of kRowPad (the
+// pipeline chunk sizes divide it), and kTailRows rows follow the last tile,
+// so the pipeline's look-ahead loads (at most 3 chunks of 4 rows past a
+// tile's end) stay inside the stream without clamping.
+constexpr int kRowPad = 4;
+constexpr int kTailRows = 12;
+// Source iterations captured once into a graph and relaunched as a batch:
+// one launch call per batch instead of six per iteration.  The batch length
+// is even, so the ping-pong start-flux buffers return to their places.
+constexpr int kGraphIters = 8;
+
+// Fixed-point tally scale for a bound B on |sum of contribution / V| of any
+// FSR over one sweep: contributions / V are multiplied by the float
+// s32 = 2^28 / B and split exactly into an int64 in units of 2^-32, so B maps
+// to 2^60 (int64 range 2^63).  Returns the effective scale s32 * 2^32.
+// `cap` keeps (1/V) * s32 finite in FP32 for the smallest FSR volume.
+__host__ __device__ inline double tally_scale_for(double bound, double cap) {
+  const double s = 0x1p28 / (bound > 1e-300 ? bound : 1e-300);
+  const float s32 = (float)(s < cap ? s : cap);
+  return (double)s32 * 0x1p32;
+}
+
+// Device allocations remember their size, so a buffer that is big enough is
+// kept when it is "allocated" again (a solver re-armed with new cross
+// sections then makes no hipMalloc/hipFree, which would stall other streams).
+std::mutex g_alloc_mu;
+std::map<void*, size_t> g_alloc_bytes;
 
 template <typename T>
 void dev_alloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  const size_t bytes = n * sizeof(T);
   if (*p) {
+    {
+      std::lock_guard<std::mutex> lk(g_alloc_mu);
+      auto it = g_alloc_bytes.find((void*)*p);
+      if (it != g_alloc_bytes.end() && it->second >= bytes) return;
+      if (it != g_alloc_bytes.end()) g_alloc_bytes.erase(it);
+    }
     HIP_CHECK(hipFree(*p));
     *p = nullptr;
   }
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  hipError_t e = hipMalloc((void**)p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     throw DeviceError(MOC_ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   }
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_alloc_bytes[(void*)*p] = bytes;
 }
 
 template <typename T>
 void dev_free(T*& p) {
-  if (p) (void)hipFree(p);
+  if (p) {
+    {
+      std::lock_guard<std::mutex> lk(g_alloc_mu);
+      g_alloc_bytes.erase((void*)p);
+    }
+    (void)hipFree(p);
+  }
   p = nullptr;
 }
 
@@ -109,9 +153,11 @@ struct SweepArgs {
   const int32_t* trk_azim;
   const float* seg_len;     // tile-interleaved: [row][tpw], traversal order, 0-padded
   const int32_t* seg_key;   // tile-interleaved: [row][tpw], FSR | material << 24
+  unsigned stream_bytes;    // bytes of each stream (rows + tail) * tpw * 4
   const float2* qs;         // [N*G] (Q/sigma_t, sigma_t)
   const float* psi_in;      // [2T*fpt]
   float* psi_out;           // [2T*fpt]
+  float* leak;              // [2T*G] vacuum leakage w(a,0) * sum_p psi_out, or null
   float* tally;             // global path: [tally_copies][N*G]
   int tally_copies;         // global path: private copies (block b -> copy b % copies)
   size_t tally_stride;      // N*G
@@ -128,33 +174,42 @@ struct SweepArgs {
 constexpr int kFsrBits = 24;
 constexpr int kFsrMask = (1 << kFsrBits) - 1;
 
-// Source policies: (Q/sigma_t, sigma_t) of group g of the segment's FSR.
+// Source policies: Q/sigma_t and sigma_t of group g of the segment's FSR,
+// plus the FSR's fixed-point tally factor (1/V times the tally scale) where
+// the tally needs it.
+struct SrcVal {
+  float q;    // Q / sigma_t
+  float sig;  // sigma_t
+  float iv;   // (1/V) * s32 (fixed-point tally only)
+};
 struct GatherSource {  // global FP32 table qs[N*G]
   static constexpr bool kLocal = false;
   const float2* qs;
   int G;
-  __device__ float2 load(int key, int g, int /*loc*/) const {
-    return qs[(size_t)(key & kFsrMask) * G + g];
+  __device__ SrcVal load(int key, int g, int /*loc*/) const {
+    const float2 v = qs[(size_t)(key & kFsrMask) * G + g];
+    return {v.x, v.y, 0.f};
   }
 };
-struct LdsSource {     // LDS slices: Q/sigma_t [N][GB] and sigma_t by material [M][GB]
+struct LdsSource {     // LDS: {Q/sigma_t, iv} [N][GB] and sigma_t by material [M][GB]
   static constexpr bool kLocal = true;
-  const float* q;
+  const float2* qi;
   const float* sig;
   int GB;
-  __device__ float2 load(int key, int /*g*/, int loc) const {
-    return make_float2(q[(key & kFsrMask) * GB + loc],
-                       sig[(int)((unsigned)key >> kFsrBits) * GB + loc]);
+  __device__ SrcVal load(int key, int /*g*/, int loc) const {
+    const float2 v = qi[__umul24(key & kFsrMask, GB) + loc];
+    return {v.x, sig[__umul24((unsigned)key >> kFsrBits, GB) + loc], v.y};
   }
 };
-struct SubsetSource {  // global subset-major Q/sigma_t [N][GB] + LDS sigma_t by material
+struct SubsetSource {  // global subset-major Q/sigma_t [N][GB]; LDS iv [N], sigma_t [M][GB]
   static constexpr bool kLocal = false;
   const float* q;
+  const float* iv;
   const float* sig;
   int GB;
-  __device__ float2 load(int key, int /*g*/, int loc) const {
-    return make_float2(q[(size_t)(key & kFsrMask) * GB + loc],
-                       sig[(int)((unsigned)key >> kFsrBits) * GB + loc]);
+  __device__ SrcVal load(int key, int /*g*/, int loc) const {
+    const int f = key & kFsrMask;
+    return {q[__umul24(f, GB) + loc], sig[__umul24((unsigned)key >> kFsrBits, GB) + loc], iv[f]};
   }
 };
 
@@ -162,22 +217,28 @@ struct SubsetSource {  // global subset-major Q/sigma_t [N][GB] + LDS sigma_t by
 struct GlobalTally {   // device-scope float atomics into this block's copy
   float* t;
   int G;
-  __device__ void add(int fsr, int g, int /*loc*/, float v) const {
+  __device__ void add(int fsr, int g, int /*loc*/, float v, float /*iv*/) const {
     unsafeAtomicAdd(&t[(size_t)fsr * G + g], v);
   }
 };
 // Fixed-point LDS tally: the block's group slice [N][GB] of int64 sums of
-// contribution / V_fsr in units of 1/scale.  Integer LDS atomics run at full
-// rate (float LDS atomics do not), and integer sums are exact and
-// order-independent, so the tally is bitwise reproducible.
+// contribution / V_fsr in units of 2^-32 / s32 (see tally_scale_for).
+// Integer LDS atomics run at full rate (float LDS atomics do not), and
+// integer sums are exact and order-independent, so the tally is bitwise
+// reproducible.
 struct LdsFixedTally {
   unsigned long long* t;
-  const float* inv_vol;   // [N] 1/V (0 for zero-volume FSRs)
-  double scale;
   int GB;
-  __device__ void add(int fsr, int /*g*/, int loc, float v) const {
-    const double x = (double)(v * inv_vol[fsr]) * scale;
-    atomicAdd(&t[fsr * GB + loc], (unsigned long long)(long long)x);
+  __device__ void add(int fsr, int /*g*/, int loc, float v, float iv) const {
+    // y = contribution * (1/V) * s32, |y| <= 2^28; the int64 floor(y * 2^32)
+    // is built from floor(y) (high word) and (y - floor(y)) * 2^32 (low
+    // word), both exact in FP32 down to the last bit of y
+    const float y = v * iv;
+    const float hi = floorf(y);
+    const unsigned lo = (unsigned)((y - hi) * 4294967296.f);
+    const unsigned long long x =
+        ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)lo;
+    atomicAdd(&t[__umul24(fsr, GB) + loc], x);
   }
 };
 
@@ -201,6 +262,11 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   float w[P];
 #pragma unroll
   for (int p = 0; p < P; p++) w[p] = a.weights[azim * P + p];
+  if (GPL == 1 && gl >= g_count) {
+    // a lane past the last group tallies exactly zero
+#pragma unroll
+    for (int p = 0; p < P; p++) w[p] = 0.f;
+  }
 
   // Lanes past the last group work on a clamped group and add exactly zero,
   // so the loop below has no per-lane branches.
@@ -231,47 +297,53 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   // rows past the tile are clamped to its last row, so every lane runs the
   // same trip count with unconditional loads.
   constexpr int D = GPL >= 4 ? 2 : 4;
-  const int n = ti.y;
-  const int last = n - 1;
-  const size_t row0 = (size_t)ti.x * a.tpw + slot;
+  static_assert(kRowPad % D == 0 && kTailRows >= 3 * D, "stream padding");
+  const int n = ti.y;  // multiple of kRowPad; tile is wave-uniform
+  // Buffer loads: 32-bit lane offset plus a scalar row offset, bounds-checked
+  // by the hardware against the stream size (out-of-range reads give 0).
+  const __amdgpu_buffer_rsrc_t rs_len =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.seg_len, (short)0, (int)a.stream_bytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_key =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.seg_key, (short)0, (int)a.stream_bytes, 0x00020000);
+  const unsigned row_bytes = (unsigned)a.tpw * 4u;
+  const unsigned lane_off = (unsigned)ti.x * row_bytes + (unsigned)slot * 4u;
   auto load_chunk = [&](int c, float* L, int* K) {
+    const unsigned vo = lane_off + (unsigned)(c * D) * row_bytes;
 #pragma unroll
     for (int j = 0; j < D; j++) {
-      const int r = c * D + j;
-      const size_t idx = row0 + (size_t)min(r, last) * a.tpw;
-      L[j] = a.seg_len[idx] * (r < n ? 1.f : 0.f);
-      K[j] = a.seg_key[idx];
+      L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, j * row_bytes, 0));
+      K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, j * row_bytes, 0);
     }
   };
-  auto fetch = [&](int key, float2* q) {
+  auto fetch = [&](int key, SrcVal* q) {
 #pragma unroll
     for (int k = 0; k < GPL; k++) q[k] = src.load(key, g[k], loc[k]);
   };
-  auto integrate = [&](float len, int key, const float2* q) {
+  auto integrate = [&](float len, int key, const SrcVal* q) {
     const int fsr = key & kFsrMask;
 #pragma unroll
     for (int k = 0; k < GPL; k++) {
-      const float tau = q[k].y * len;
+      const float tau = q[k].sig * len;
       float acc = 0.f;
 #pragma unroll
       for (int p = 0; p < P; p++) {
         const float x = tau * pf[p];
         const float A = EXPM == MOC_EXP_RATIONAL ? attenuation_rational(x) : attenuation_hw(x);
-        const float d = (psi[k][p] - q[k].x) * A;
+        const float d = (psi[k][p] - q[k].q) * A;
         psi[k][p] -= d;
         acc = fmaf(w[p], d, acc);
       }
-      if (!ok[k]) acc = 0.f;
-      tally.add(fsr, g[k], loc[k], acc);
+      if (GPL > 1 && !ok[k]) acc = 0.f;
+      tally.add(fsr, g[k], loc[k], acc, q[k].iv);
     }
   };
   // Integrates one chunk.  Local sources: read one segment ahead.  Global
   // sources: `cur` holds this chunk's sources (fetched one chunk earlier) and
   // the next chunk's sources are fetched into `nxt` first.
-  auto process = [&](const float* L, const int* K, const int* Knext, float2 (*cur)[GPL],
-                     float2 (*nxt)[GPL]) {
+  auto process = [&](const float* L, const int* K, const int* Knext, SrcVal (*cur)[GPL],
+                     SrcVal (*nxt)[GPL]) {
     if constexpr (Src::kLocal) {
-      float2 qa[GPL], qb[GPL];
+      SrcVal qa[GPL], qb[GPL];
       fetch(K[0], qa);
 #pragma unroll
       for (int j = 0; j < D; j++) {
@@ -290,11 +362,11 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   };
 
   if (n > 0) {
-    const int nch = (n + D - 1) / D;
+    const int nch = n / D;
     // three rotating sets: chunk c lives in set c % 3 (streams and sources)
     float L0[D], L1[D], L2[D];
     int K0[D], K1[D], K2[D];
-    float2 Q0[Src::kLocal ? 1 : D][GPL], Q1[Src::kLocal ? 1 : D][GPL], Q2[Src::kLocal ? 1 : D][GPL];
+    SrcVal Q0[Src::kLocal ? 1 : D][GPL], Q1[Src::kLocal ? 1 : D][GPL], Q2[Src::kLocal ? 1 : D][GPL];
     load_chunk(0, L0, K0);
     load_chunk(1, L1, K1);
     load_chunk(2, L2, K2);
@@ -302,18 +374,26 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
 #pragma unroll
       for (int j = 0; j < D; j++) fetch(K0[j], Q0[j]);
     }
-    for (int c = 0; c < nch; c += 3) {
+    // The steady-state loop has no conditional sections, so the compiler's
+    // wait counts at its head see all three sets in flight; the last one or
+    // two chunks are integrated after it.
+    // Scheduling barriers keep each chunk's work in its own section: without
+    // them the scheduler hoists later chunks' address arithmetic to the loop
+    // head, where it has to wait for the newest loads.
+    int c = 0;
+    for (; c + 3 <= nch; c += 3) {
       process(L0, K0, K1, Q0, Q1);
       load_chunk(c + 3, L0, K0);
-      if (c + 1 < nch) {
-        process(L1, K1, K2, Q1, Q2);
-        load_chunk(c + 4, L1, K1);
-      }
-      if (c + 2 < nch) {
-        process(L2, K2, K0, Q2, Q0);
-        load_chunk(c + 5, L2, K2);
-      }
+      __builtin_amdgcn_sched_barrier(0);
+      process(L1, K1, K2, Q1, Q2);
+      load_chunk(c + 4, L1, K1);
+      __builtin_amdgcn_sched_barrier(0);
+      process(L2, K2, K0, Q2, Q0);
+      load_chunk(c + 5, L2, K2);
+      __builtin_amdgcn_sched_barrier(0);
     }
+    if (c < nch) process(L0, K0, K1, Q0, Q1);
+    if (c + 1 < nch) process(L1, K1, K2, Q1, Q2);
   }
 
   const int out = dir ? tr.w : tr.z;
@@ -329,6 +409,21 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
           amax = fmaxf(amax, fabsf(psi[k][p]));
         }
   }
+  if (a.leak) {
+    // vacuum leakage of the neutron-balance keff (transferBoundaryFlux,
+    // src/CPUSolver.cpp:2593-2600: the 2D sweep passes polar index 0, so
+    // every polar angle of the track is weighted by w(a, 0)); every directed
+    // track writes its groups each sweep
+    const float w0 = a.weights[azim * P];
+#pragma unroll
+    for (int k = 0; k < GPL; k++)
+      if (ok[k]) {
+        float v = 0.f;
+#pragma unroll
+        for (int p = 0; p < P; p++) v += psi[k][p];
+        a.leak[(size_t)dt * G + g[k]] = out < 0 ? w0 * v : 0.f;
+      }
+  }
   // running max of |start psi| for the next sweep's fixed-point scale
   // (non-negative floats order like their bit patterns)
   if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
@@ -339,7 +434,7 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
 template <int GPL, int EXPM, int NP>
 __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
   const int lane = threadIdx.x & 63;
-  const int tile = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int tile = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int slot = lane / a.lanes_per_track;
   const int gl = lane - slot * a.lanes_per_track;
   if (slot >= a.tpw || tile >= a.ntiles) return;
@@ -352,8 +447,8 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
 }
 
 // LDS-privatised sweep.  Block b owns group subset b % nsub (GB groups, one
-// lane per group) and block slot b / nsub.  LDS holds the FP32 tally slice
-// [N][GB] of the subset, sigma_t of the subset by material [M][GB] and, when
+// lane per group) and block slot b / nsub.  LDS holds the int64 fixed-point
+// tally slice [N][GB] of the subset, 1/V per FSR, sigma_t of the subset by material [M][GB] and, when
 // QLDS, the subset's Q/sigma_t slice [N][GB] (otherwise gathered from the
 // subset-major global table).  The block processes the tiles
 // slot*wpb + wave, + blocks_per_sub*wpb, ... (longest first, so the strided
@@ -379,20 +474,25 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
   const int g0 = sub * L.GB;
   const int gb = min(L.GB, a.G - g0);
   const int nl = a.N * L.GB;
-  // LDS: int64 tally [N*GB] | float 1/V [N] | sigma_t [M*GB] | Q/sigma_t [N*GB] (QLDS)
+  // LDS: int64 tally [N*GB] | then QLDS: {Q/sigma_t, iv} [N*GB], else iv [N] |
+  // sigma_t [M*GB]; iv = (1/V) * s32 is the fixed-point factor of the FSR
+  const float s32 = (float)(a.state->tally_scale * 0x1p-32);
   unsigned long long* lds_tally = lds64;
-  float* lds_ivol = reinterpret_cast<float*>(lds64 + nl);
-  float* lds_sig = lds_ivol + a.N;
-  float* lds_q = lds_sig + L.M * L.GB;
+  float2* lds_qi = reinterpret_cast<float2*>(lds64 + nl);
+  float* lds_iv = reinterpret_cast<float*>(lds64 + nl);
+  float* lds_sig = QLDS ? reinterpret_cast<float*>(lds_qi + nl) : lds_iv + a.N;
+  const float* qg = L.qsub + (size_t)sub * nl;
   for (int i = threadIdx.x; i < nl; i += blockDim.x) lds_tally[i] = 0ull;
-  for (int i = threadIdx.x; i < a.N; i += blockDim.x) lds_ivol[i] = L.inv_vol[i];
+  if (QLDS) {
+    for (int i = threadIdx.x; i < nl; i += blockDim.x)
+      lds_qi[i] = make_float2(qg[i], L.inv_vol[i / L.GB] * s32);
+  } else {
+    for (int i = threadIdx.x; i < a.N; i += blockDim.x) lds_iv[i] = L.inv_vol[i] * s32;
+  }
   for (int i = threadIdx.x; i < L.M * L.GB; i += blockDim.x) {
     const int m = i / L.GB, l = i - m * L.GB;
     lds_sig[i] = (float)L.sigma_t[(size_t)m * a.G + g0 + min(l, gb - 1)];
   }
-  const float* qg = L.qsub + (size_t)sub * nl;
-  if (QLDS)
-    for (int i = threadIdx.x; i < nl; i += blockDim.x) lds_q[i] = qg[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
@@ -400,15 +500,16 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
   const int slot = lane / L.GB;
   const int gl = lane - slot * L.GB;
   if (slot < a.tpw) {
-    LdsFixedTally tally{lds_tally, lds_ivol, a.state->tally_scale, L.GB};
-    for (int tile = bslot * wpb + wib; tile < a.ntiles; tile += L.blocks_per_sub * wpb) {
+    LdsFixedTally tally{lds_tally, L.GB};
+    for (int tile = __builtin_amdgcn_readfirstlane(bslot * wpb + wib); tile < a.ntiles;
+         tile += L.blocks_per_sub * wpb) {
       const int dt = a.work[tile * a.tpw + slot];
       if (dt < 0) continue;
       if (QLDS) {
-        LdsSource src{lds_q, lds_sig, L.GB};
+        LdsSource src{lds_qi, lds_sig, L.GB};
         sweep_track<1, EXPM, NP>(a, tile, slot, dt, gl, L.GB, g0, gb, src, tally);
       } else {
-        SubsetSource src{qg, lds_sig, L.GB};
+        SubsetSource src{qg, lds_iv, lds_sig, L.GB};
         sweep_track<1, EXPM, NP>(a, tile, slot, dt, gl, L.GB, g0, gb, src, tally);
       }
     }
@@ -443,6 +544,9 @@ __global__ void reduce_partials_kernel(const unsigned long long* partials, doubl
 // ---------------------------------------------------------------------------
 // FP64 per-FSR kernels
 // ---------------------------------------------------------------------------
+}  // namespace
+
+// Arguments of the per-FSR kernels (declared in hip_solver.h).
 struct FsrArgs {
   int64_t N;
   int G;
@@ -452,21 +556,43 @@ struct FsrArgs {
   const double* sigma_s;      // dest-major
   const double* nsf;
   const double* chi;
+  const double* sigma_a;      // [M*G] absorption (neutron-balance keff)
   const int32_t* fissionable;
   const double* fixed;        // nullable
-  double* phi;
-  double* phi_old;
+  double* phi;                // [N*G] scalar flux of the current sweep
+  double* phi_old[2];         // normalised fluxes, ping-pong (state->old_sel is current)
   double* Q;
   float2* qs;                 // [N*G] (global-atomic sweep), or null
   float* qsub;                // [nsub][N][GB] Q/sigma_t (LDS sweep), or null
   int qsub_groups;            // GB
   float* tally;               // [tally_copies][N*G] (global-atomic sweep)
   int tally_copies;
-  const double* tally64;      // [N*G] (LDS sweep), or null
-  double* partials;
+  const unsigned long long* lds_partials;  // [nsub][bps][N*GB] fixed point (LDS sweep), or null
+  int lds_bps;
+  const float* leak;          // [2T*G] vacuum leakage per directed track and group, or null
+  int64_t leak_n;
+  double* partials;           // [kNumPartials][pstride] per-block sums
+  int pstride;
   IterState* state;
   int allow_negative;
+  int stab_type;              // -1 off, else DIAGONAL / YAMAMOTO / GLOBAL
+  double stab_factor;
+  const double* stab_ratio;   // [G] YAMAMOTO: factor * max |sigma_s(g->g) / sigma_t(g)|
 };
+
+namespace {
+
+// per-block partial sums written by post_sweep_kernel / update_kernel
+constexpr int kPartFission = 0;     // sum V nu-sigma-f phi (unstabilised) / residual terms
+constexpr int kPartAbsorption = 1;  // sum V sigma_a phi
+constexpr int kPartNormFission = 2; // sum V nu-sigma-f phi after stabilisation
+constexpr int kPartLeakage = 3;     // sum of vacuum leakage
+constexpr int kNumPartials = 4;
+
+// stabilizationType (src/Solver.h:103-113)
+constexpr int kStabDiagonal = 0;
+constexpr int kStabYamamoto = 1;
+constexpr int kStabGlobal = 2;
 
 // Fixed-order block sum (deterministic): wave shuffles, then LDS.
 __device__ double block_sum(double v) {
@@ -482,110 +608,215 @@ __device__ double block_sum(double v) {
   return r;  // valid in thread 0
 }
 
-// from_tally=1: phi = tally/(sigma_t V) + 4 pi Q / sigma_t (addSourceToScalarFlux,
-// src/CPUSolver.cpp:2608-2659) and the tally is cleared for the next sweep.
-// Always: partials[block] = sum over the block's FSRs of V * sum_g nsf*phi.
-__global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int from_tally) {
+__device__ double block_max(double v) {
+  __shared__ double sh[kReduceBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmax(r, sh[i]);
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// One thread per (FSR, group), in the LDS sweep's subset-major order
+// (sub, r, l) so the fixed-point partials are read coalesced.
+//   from_tally=1: the tally of the sweep becomes the scalar flux
+//     phi = tally/(sigma_t V) + 4 pi Q / sigma_t with the negative-flux clamp
+//     (addSourceToScalarFlux, src/CPUSolver.cpp:2608-2659); the fixed-point
+//     partials are summed exactly here (the global-atomic path sums and clears
+//     its FP32 tally copies), then transport stabilisation
+//     (computeStabilizingFlux / stabilizeFlux, src/CPUSolver.cpp:2665-2805)
+//     is applied from iteration 1 on, from the previous normalised flux.
+//   from_tally=0: phi is the current array (initial guess).
+// Per-block partials: fission rate before and after stabilisation,
+// absorption rate and vacuum leakage (computeKeff, src/CPUSolver.cpp:2258-2328).
+__global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int from_tally,
+                                                                 int balance) {
   if (a.state->converged) return;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rate = 0.0;
+  const int G = a.G;
+  const int GB = a.qsub_groups;
+  const int64_t NG = a.N * G;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rate = 0.0, rate_s = 0.0, absr = 0.0, leak = 0.0;
   int64_t neg = 0;
-  if (r < a.N) {
-    const int G = a.G;
+  // (sub, r, l) -> (r, g)
+  const int64_t nl = a.N * GB;
+  const int64_t sub = i / nl;
+  const int64_t j = i - sub * nl;
+  const int64_t r = j / GB;
+  const int g = (int)(sub * GB + (j - r * GB));
+  if (i < (int64_t)((G + GB - 1) / GB) * nl && g < G) {
     const int m = a.fsr_mat[r];
     const double vol_true = a.fsr_vol[r];
     const double vol = vol_true < kFltEpsilon ? 1e30 : vol_true;
-    const double* st = a.sigma_t + (size_t)m * G;
-    const double* nsf = a.nsf + (size_t)m * G;
-    for (int g = 0; g < G; g++) {
-      const size_t i = (size_t)r * G + g;
-      double f;
-      if (from_tally) {
-        if (a.tally64) {
-          // fixed-point tally already holds sum(contribution) / V
-          f = a.tally64[i] / st[g] + kFourPi * a.Q[i] / st[g];
-        } else {
-          // sum of the private tally copies in a fixed order
-          double t = 0.0;
-          const size_t stride = (size_t)a.N * G;
-          for (int c = 0; c < a.tally_copies; c++) {
-            t += (double)a.tally[c * stride + i];
-            a.tally[c * stride + i] = 0.f;
-          }
-          f = t / (st[g] * vol) + kFourPi * a.Q[i] / st[g];
+    const double st = a.sigma_t[(size_t)m * G + g];
+    const size_t k = (size_t)r * G + g;
+    double f;
+    if (from_tally) {
+      if (a.lds_partials) {
+        const unsigned long long* p = a.lds_partials + (size_t)sub * a.lds_bps * nl + j;
+        // exact (two's complement) sums, so any order gives the same bits;
+        // eight independent accumulators keep eight loads in flight
+        unsigned long long acc[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+        int b = 0;
+        for (; b + 8 <= a.lds_bps; b += 8) {
+#pragma unroll
+          for (int u = 0; u < 8; u++) acc[u] += p[(size_t)(b + u) * nl];
         }
-        if (f < 0.0 && !a.allow_negative) {
-          f = kFluxEpsilon;
-          neg++;
-        }
-        a.phi[i] = f;
+        for (; b < a.lds_bps; b++) acc[0] += p[(size_t)b * nl];
+        const unsigned long long t =
+            ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        // the fixed-point tally holds sum(contribution) / V
+        f = ((double)(long long)t / a.state->tally_scale) / st + kFourPi * a.Q[k] / st;
       } else {
-        f = a.phi[i];
+        double t = 0.0;
+        for (int c = 0; c < a.tally_copies; c++) {
+          t += (double)a.tally[c * NG + k];
+          a.tally[c * NG + k] = 0.f;
+        }
+        f = t / (st * vol) + kFourPi * a.Q[k] / st;
       }
-      rate += nsf[g] * f;
+      if (f < 0.0 && !a.allow_negative) {
+        f = kFluxEpsilon;
+        neg++;
+      }
+    } else {
+      f = a.phi[k];
     }
-    rate *= vol_true;
+    const double nsf = a.nsf[(size_t)m * G + g];
+    rate = nsf * f * vol_true;
+    if (balance) absr = a.sigma_a[(size_t)m * G + g] * f * vol_true;
+    if (from_tally && a.stab_type >= 0 && a.state->iteration > 0) {
+      const double old = a.phi_old[a.state->old_sel][k];
+      const double rho = a.stab_factor;
+      if (a.stab_type == kStabDiagonal) {
+        const double ss = a.sigma_s[((size_t)m * G + g) * G + g];
+        if (ss < 0.0) {
+          const double s = -old * rho * ss / st;
+          f += s;
+          f /= (1.0 - rho * ss / st);
+        }
+      } else if (a.stab_type == kStabYamamoto) {
+        const double mr = a.stab_ratio[g];
+        const double s = old * mr;
+        f += s;
+        f /= (1 + mr);
+      } else if (a.stab_type == kStabGlobal) {
+        const double s = (1.0 / rho - 1.0) * old;
+        f += s;
+        f *= rho;
+      }
+    }
+    rate_s = nsf * f * vol_true;
+    if (from_tally) a.phi[k] = f;
   }
-  const double s = block_sum(rate);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+  if (balance && a.leak)
+    for (int64_t q = i; q < a.leak_n; q += (int64_t)gridDim.x * blockDim.x) leak += (double)a.leak[q];
+  const double s0 = block_sum(rate);
+  const double s2 = block_sum(rate_s);
+  double s1 = 0.0, s3 = 0.0;
+  if (balance) {
+    s1 = block_sum(absr);
+    s3 = block_sum(leak);
+  }
+  if (threadIdx.x == 0) {
+    a.partials[kPartFission * a.pstride + blockIdx.x] = s0;
+    a.partials[kPartNormFission * a.pstride + blockIdx.x] = s2;
+    a.partials[kPartAbsorption * a.pstride + blockIdx.x] = s1;
+    a.partials[kPartLeakage * a.pstride + blockIdx.x] = s3;
+  }
   if (neg) atomicAdd((unsigned long long*)&a.state->negative_fluxes, (unsigned long long)neg);
 }
 
 // phase 0: initial guess normalisation (k unchanged); phase 1: computeKeff
-// (k *= F/N, src/CPUSolver.cpp:2258-2328) then normalizeFluxes' factor N/F
-// (src/CPUSolver.cpp:1860-1931).
+// (k *= F/N, or k = F / (A + L) from the neutron balance; src/CPUSolver.cpp:
+// 2258-2328) then normalizeFluxes' factor N/F (src/CPUSolver.cpp:1860-1931)
+// from the (stabilised) fluxes.
 __global__ __launch_bounds__(kReduceBlock) void keff_kernel(const double* partials, int nparts,
-                                                           IterState* state, double num_fsrs,
-                                                           int phase) {
+                                                           int pstride, IterState* state,
+                                                           double num_fsrs, int phase,
+                                                           int balance) {
   if (state->converged) return;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
-  const double total = block_sum(v);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    v0 += partials[kPartFission * pstride + i];
+    v2 += partials[kPartNormFission * pstride + i];
+    if (balance) {
+      v1 += partials[kPartAbsorption * pstride + i];
+      v3 += partials[kPartLeakage * pstride + i];
+    }
+  }
+  const double fiss = block_sum(v0);
+  const double fiss_norm = block_sum(v2);
+  double absr = 0.0, leak = 0.0;
+  if (balance) {
+    absr = block_sum(v1);
+    leak = block_sum(v3);
+  }
   if (threadIdx.x == 0) {
-    if (phase == 1) state->k = state->k * total / num_fsrs;
-    state->fission_total = total;
-    state->norm_factor = num_fsrs / total;
-    state->psi_scale = num_fsrs / total;
+    if (phase == 1) {
+      if (balance)
+        state->k = fiss / (absr + leak);
+      else
+        state->k = state->k * fiss / num_fsrs;
+    }
+    state->fission_total = fiss_norm;
+    state->norm_factor = num_fsrs / fiss_norm;
+    state->psi_scale = num_fsrs / fiss_norm;
   }
 }
 
-// normalise (optional), residual partials, store old flux, next sources.
+// Source kinds of update_kernel
+constexpr int kSrcTotal = 0;    // computeFSRSources (src/CPUSolver.cpp:1939-2023)
+constexpr int kSrcFission = 1;  // computeFSRFissionSources (src/CPUSolver.cpp:2030-2066)
+constexpr int kSrcScatter = 2;  // computeFSRScatterSources (src/CPUSolver.cpp:2072-2104)
+
+// One thread per (FSR, destination group): normalise (optional), residual
+// partials, store the normalised flux as the next "old" flux, sources of the
+// next sweep.  Reads phi and the current old flux only, writes the other
+// old buffer, so threads of one FSR never see each other's writes.
 __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int normalize,
                                                              int res_type, int do_residual,
                                                              int compute_sources,
-                                                             int source_iteration_offset) {
+                                                             int source_iteration_offset,
+                                                             int source_kind) {
   if (a.state->converged) return;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int G = a.G;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0;
   double qmax = 0.0;
   int64_t negsrc = 0;
-  if (r < a.N) {
-    const int G = a.G;
+  if (i < a.N * G) {
+    const int64_t r = i / G;
+    const int gd = (int)(i - r * G);
     const int m = a.fsr_mat[r];
     const double f = normalize ? a.state->norm_factor : 1.0;
     const double k = a.state->k;
-    const double* st = a.sigma_t + (size_t)m * G;
+    const double st = a.sigma_t[(size_t)m * G + gd];
     const double* ss = a.sigma_s + (size_t)m * G * G;
     const double* nsf = a.nsf + (size_t)m * G;
-    const double* chi = a.chi + (size_t)m * G;
     const bool fis = a.fissionable[m] != 0;
-    double* phi = a.phi + (size_t)r * G;
-    double* old = a.phi_old + (size_t)r * G;
-    double new_fis = 0.0, old_fis = 0.0;
-    if (normalize)
-      for (int g = 0; g < G; g++) phi[g] *= f;
+    const double* phi = a.phi + (size_t)r * G;
+    const int sel = a.state->old_sel;
+    const double* old = a.phi_old[sel] + (size_t)r * G;
+    double* nxt = a.phi_old[sel ^ 1] + (size_t)r * G;
+    const double phin = normalize ? phi[gd] * f : phi[gd];
+    nxt[gd] = phin;
     if (do_residual) {
+      // computeResidual (src/CPUSolver.cpp:2113-2252)
       if (res_type == MOC_SCALAR_FLUX) {
-        for (int g = 0; g < G; g++)
-          if (old[g] > 0.0) {
-            const double d = (phi[g] - old[g]) / old[g];
-            res += d * d;
-          }
-      } else {
+        if (old[gd] > 0.0) {
+          const double d = (phin - old[gd]) / old[gd];
+          res = d * d;
+        }
+      } else if (gd == 0) {
         double nw = 0.0, od = 0.0;
         if (fis) {
           for (int g = 0; g < G; g++) {
-            nw += phi[g] * nsf[g];
+            nw += (normalize ? phi[g] * f : phi[g]) * nsf[g];
             od += old[g] * nsf[g];
           }
           if (res_type == MOC_TOTAL_SOURCE) {
@@ -594,10 +825,10 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
           }
         }
         if (res_type == MOC_TOTAL_SOURCE) {
-          for (int gd = 0; gd < G; gd++)
+          for (int d = 0; d < G; d++)
             for (int g = 0; g < G; g++) {
-              nw += ss[gd * G + g] * phi[g];
-              od += ss[gd * G + g] * old[g];
+              nw += ss[d * G + g] * (normalize ? phi[g] * f : phi[g]);
+              od += ss[d * G + g] * old[g];
             }
         }
         if ((res_type == MOC_TOTAL_SOURCE || fis) && od > 0.0) {
@@ -606,43 +837,54 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
         }
       }
     }
-    for (int g = 0; g < G; g++) old[g] = phi[g];
     if (compute_sources) {
-      // computeFSRSources (src/CPUSolver.cpp:1939-2023)
       const int it = a.state->iteration + source_iteration_offset;
-      if (fis)
-        for (int g = 0; g < G; g++) new_fis += nsf[g] * phi[g];
-      for (int gd = 0; gd < G; gd++) {
+      double q;
+      if (source_kind == kSrcFission) {
+        // chi(gd) nu-sigma-f(g) phi(g), without 1/k or fixed sources
+        double fsrc = 0.0;
+        if (fis)
+          for (int g = 0; g < G; g++) fsrc += a.chi[(size_t)m * G + gd] * nsf[g] * phi[g];
+        q = fsrc * kOneOverFourPi;
+      } else if (source_kind == kSrcScatter) {
+        double sc = 0.0;
+        for (int g = 0; g < G; g++) sc += ss[(size_t)gd * G + g] * phi[g];
+        q = sc * kOneOverFourPi;
+      } else {
+        double new_fis = 0.0;
+        if (fis)
+          for (int g = 0; g < G; g++) new_fis += nsf[g] * (normalize ? phi[g] * f : phi[g]);
         double sc = 0.0;
         const double* row = ss + (size_t)gd * G;
-        for (int g = 0; g < G; g++) sc += row[g] * phi[g];
-        double q = (fis ? chi[gd] * new_fis : 0.0) / k + sc;
+        for (int g = 0; g < G; g++) sc += row[g] * (normalize ? phi[g] * f : phi[g]);
+        q = (fis ? a.chi[(size_t)m * G + gd] * new_fis : 0.0) / k + sc;
         if (a.fixed) q += a.fixed[(size_t)r * G + gd];
         q *= kOneOverFourPi;
         if (q < 0.0) {
           negsrc++;
           if (it < 30 && !a.allow_negative) q = kFluxEpsilon;
         }
-        const size_t i = (size_t)r * G + gd;
-        a.Q[i] = q;
-        qmax = fmax(qmax, fabs(q / st[gd]));
-        if (a.qs) a.qs[i] = make_float2((float)(q / st[gd]), (float)st[gd]);
-        if (a.qsub) {
-          const int sb = gd / a.qsub_groups, l = gd - sb * a.qsub_groups;
-          a.qsub[((size_t)sb * a.N + r) * a.qsub_groups + l] = (float)(q / st[gd]);
-        }
+      }
+      a.Q[i] = q;
+      qmax = fabs(q / st);
+      if (a.qs) a.qs[i] = make_float2((float)(q / st), (float)st);
+      if (a.qsub) {
+        const int sb = gd / a.qsub_groups, l = gd - sb * a.qsub_groups;
+        a.qsub[((size_t)sb * a.N + r) * a.qsub_groups + l] = (float)(q / st);
       }
     }
-    (void)old_fis;
   }
   const double s = block_sum(res);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
-  if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
-  if (qmax > 0.0) {
-    // round up so the float bound is not below the double value
-    const float qf = __uint_as_float(__float_as_uint((float)qmax) + 1u);
-    atomicMax(&a.state->qmax_bits, __float_as_uint(qf));
+  const double qm = block_max(qmax);
+  if (threadIdx.x == 0) {
+    a.partials[blockIdx.x] = s;
+    if (qm > 0.0) {
+      // round up so the float bound is not below the double value
+      const float qf = __uint_as_float(__float_as_uint((float)qm) + 1u);
+      atomicMax(&a.state->qmax_bits, __float_as_uint(qf));
+    }
   }
+  if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
 }
 
 // Fixed-point scale of the next sweep's LDS tally.  Over one sweep,
@@ -650,7 +892,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
 // |Q/sigma_t|max), because sum over its segments (both directions, all polar
 // angles) of w l / sin(theta) = 4 pi V and |delta| <= (1 - e^-tau)|psi - q|
 // <= tau / sin(theta) |psi - q|, with psi a convex mix of its inflow and the
-// sources.  The scale keeps that bound at 2^59 (int64 range 2^63).
+// sources.  Twice that bound is mapped to 2^60 (int64 range 2^63).
 __device__ void set_tally_scale(IterState* st, double sigma_t_max, int take_qmax) {
   if (take_qmax) {
     st->qmax = __uint_as_float(st->qmax_bits);
@@ -660,11 +902,15 @@ __device__ void set_tally_scale(IterState* st, double sigma_t_max, int take_qmax
   st->psi_max_bits = 0u;
   const double bound =
       2.0 * kFourPi * sigma_t_max * ((double)st->psi_max * st->psi_scale + (double)st->qmax);
-  st->tally_scale = bound > 1e-200 ? 0x1p60 / bound : 0x1p60;
+  st->tally_scale = tally_scale_for(bound, st->scale_cap);
 }
 
-__global__ void tally_scale_kernel(IterState* state, double sigma_t_max, int take_qmax) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) set_tally_scale(state, sigma_t_max, take_qmax);
+// flip: 1 makes the old-flux buffer written by update_kernel the current one
+__global__ void tally_scale_kernel(IterState* state, double sigma_t_max, int take_qmax, int flip) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    set_tally_scale(state, sigma_t_max, take_qmax);
+    if (flip) state->old_sel ^= 1;
+  }
 }
 
 // mode 0 eigen: stop when res < tol && |int(1e5 dk)| < 1 (src/Solver.cpp:1643-1681)
@@ -681,6 +927,7 @@ __global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* pa
     double s = sum < 0.0 ? 0.0 : sum;
     const double res = sqrt(s / (norm <= 0.0 ? 1.0 : norm));
     state->residual = res;
+    state->old_sel ^= 1;
     const int i = state->iteration;
     if (mode == 0) {
       const int dk = (int)(1e5 * (state->k - state->k_prev));
@@ -699,6 +946,14 @@ __global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* pa
     }
     set_tally_scale(state, sigma_t_max, take_qmax);
   }
+}
+
+// Plain device copy (16-byte vector loads and stores), the reference kernel
+// for calibrating HBM byte counters.
+__global__ void copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void fill_kernel(double* p, int64_t n, double v) {
@@ -740,6 +995,19 @@ __global__ void fission_rates_kernel(const double* phi, const int32_t* fsr_mat,
 
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Dynamic LDS above the default needs the opt-in attribute; it is set once
+// per kernel and device (before any graph capture that launches the kernel).
+void ensure_lds_attribute(const void* kernel, int device, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> done;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = done[{kernel, device}];
+  if (cur < bytes) {
+    HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -773,6 +1041,22 @@ HipSolver::HipSolver(int device) {
 HipSolver::~HipSolver() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (async_ev_) (void)hipEventDestroy(async_ev_);
+  if (async_ev0_) (void)hipEventDestroy(async_ev0_);
+  if (geo_hold_) {
+    // geometry buffers belong to the shared holder
+    d_seg_len_ = nullptr;
+    d_seg_key_ = nullptr;
+    d_work_ = nullptr;
+    d_tile_ = nullptr;
+    d_trk_ = nullptr;
+    d_trk_azim_ = nullptr;
+    d_weights_ = nullptr;
+    d_fsr_mat_ = nullptr;
+    d_fsr_vol_ = nullptr;
+    d_inv_vol_ = nullptr;
+  }
   dev_free(d_seg_len_);
   dev_free(d_seg_key_);
   dev_free(d_work_);
@@ -785,7 +1069,11 @@ HipSolver::~HipSolver() {
   dev_free(d_tally_);
   dev_free(d_qs_);
   dev_free(d_phi_);
-  dev_free(d_phi_old_);
+  dev_free(d_phi_old_[0]);
+  dev_free(d_phi_old_[1]);
+  dev_free(d_sigma_a_);
+  dev_free(d_leak_);
+  dev_free(d_stab_ratio_);
   dev_free(d_Q_);
   dev_free(d_fixed_);
   dev_free(d_sigma_t_);
@@ -847,12 +1135,13 @@ void HipSolver::pickShape() {
 
 // LDS of the LDS sweep: int64 tally [N*gb] | 1/V [N] | sigma_t [M*gb] | Q/sigma_t [N*gb]
 size_t HipSolver::ldsBytes(int gb, int with_q) const {
-  return (size_t)N_ * gb * sizeof(uint64_t) + (size_t)N_ * sizeof(float) +
-         (size_t)std::max(M_, 1) * gb * sizeof(float) +
-         (with_q ? (size_t)N_ * gb * sizeof(float) : 0);
+  return (size_t)N_ * gb * sizeof(uint64_t) +
+         (with_q ? (size_t)N_ * gb * 2 * sizeof(float) : (size_t)N_ * sizeof(float)) +
+         (size_t)std::max(M_, 1) * gb * sizeof(float);
 }
 
 void HipSolver::setShape(const LaunchShape& s) {
+  graph_dirty_ = true;
   static const int kGpl[] = {1, 2, 4, 9, 16};
   bool okg = false;
   for (int gpl : kGpl) okg |= gpl == s.groups_per_lane;
@@ -877,6 +1166,7 @@ void HipSolver::setShape(const LaunchShape& s) {
 
 void HipSolver::setTracks(const moc_track_data_t& td) {
   HIP_CHECK(hipSetDevice(device_));
+  if (geo_hold_) throw DeviceError(MOC_ERR_STATE, "this solver shares its geometry; use a new solver");
   if (td.num_tracks <= 0 || td.num_segments <= 0 || !td.seg_offset || !td.seg_length ||
       !td.seg_fsr || !td.azim || !td.next_fwd || !td.next_bwd || !td.next_fwd_is_fwd ||
       !td.next_bwd_is_fwd || !td.bc_fwd || !td.bc_bwd || !td.total_weight || !td.sin_theta)
@@ -980,6 +1270,9 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
 // length-0 segments that repeat their last FSR.
 void HipSolver::buildTileLayout(int tpw) {
   if (tpw == layout_tpw_) return;
+  if (geo_hold_)
+    throw DeviceError(MOC_ERR_INVALID,
+                      "shared geometry is laid out for another launch shape (same group count needed)");
   if (!(tracks_set_ && fsrs_set_)) return;
   if (N_ >= ((int64_t)1 << kFsrBits) || M_ > 256)
     throw DeviceError(MOC_ERR_INVALID, "at most 2^24 FSRs and 256 materials");
@@ -998,12 +1291,16 @@ void HipSolver::buildTileLayout(int tpw) {
       work[tl * tpw + sl] = dt;
       nmax = std::max(nmax, h_trk_[4 * (dt >> 1) + 1]);
     }
+    nmax = (nmax + kRowPad - 1) / kRowPad * kRowPad;
     tile[2 * tl] = (int32_t)rows;
     tile[2 * tl + 1] = nmax;
     rows += nmax;
     if (rows >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "segment streams too large");
   }
-  const size_t total = (size_t)rows * tpw;
+  // zero-length tail rows keyed to FSR 0 (read by the look-ahead, never integrated)
+  const size_t total = (size_t)(rows + kTailRows) * tpw;
+  if (total * sizeof(float) >= ((size_t)1 << 31))
+    throw DeviceError(MOC_ERR_INVALID, "segment streams exceed 2 GiB");
   std::vector<float> ilen(total, 0.f);
   std::vector<int32_t> ifsr(total, 0);
   auto key = [&](int32_t f) {
@@ -1050,6 +1347,7 @@ int HipSolver::sweepTracksPerWave() const {
 
 void HipSolver::setMaterials(int num_materials, int num_groups, const double* sigma_t,
                              const double* sigma_s, const double* nu_sigma_f, const double* chi) {
+  graph_dirty_ = true;
   HIP_CHECK(hipSetDevice(device_));
   if (num_materials <= 0 || num_groups <= 0 || !sigma_t || !sigma_s || !nu_sigma_f || !chi)
     throw DeviceError(MOC_ERR_INVALID, "invalid material arrays");
@@ -1074,6 +1372,21 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
   HIP_CHECK(hipMemcpy(d_chi_, chi, (size_t)M * G * sizeof(double), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_mat_fissionable_, fis.data(), M * sizeof(int32_t), hipMemcpyHostToDevice));
   mat_fissionable_ = fis;
+  h_sigma_t_.assign(sigma_t, sigma_t + (size_t)M * G);
+  h_sigma_s_.assign(sigma_s, sigma_s + (size_t)M * G * G);
+  {
+    // Material::getSigmaA: sigma_t minus the out-scattering of the group
+    // (src/Material.cpp:228-257)
+    std::vector<double> sa((size_t)M * G);
+    for (int m = 0; m < M; m++)
+      for (int g = 0; g < G; g++) {
+        double v = sigma_t[(size_t)m * G + g];
+        for (int d = 0; d < G; d++) v -= sigma_s[((size_t)m * G + d) * G + g];
+        sa[(size_t)m * G + g] = v;
+      }
+    dev_alloc(&d_sigma_a_, (size_t)M * G);
+    HIP_CHECK(hipMemcpy(d_sigma_a_, sa.data(), sa.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   sigma_t_max_ = 0.0;
   for (size_t i = 0; i < (size_t)M * G; i++) sigma_t_max_ = std::max(sigma_t_max_, sigma_t[i]);
   if (G != G_) {
@@ -1105,6 +1418,7 @@ void HipSolver::countFissionable() {
 
 void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double* volume) {
   HIP_CHECK(hipSetDevice(device_));
+  if (geo_hold_) throw DeviceError(MOC_ERR_STATE, "this solver shares its geometry; use a new solver");
   if (num_fsrs <= 0 || !material || !volume) throw DeviceError(MOC_ERR_INVALID, "invalid FSR arrays");
   if (num_fsrs >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "too many FSRs");
   for (int64_t r = 0; r < num_fsrs; r++) {
@@ -1124,6 +1438,9 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
     std::vector<float> iv(num_fsrs);
     for (int64_t r = 0; r < num_fsrs; r++)
       iv[r] = volume[r] < kFltEpsilon ? 0.f : (float)(1.0 / volume[r]);
+    float iv_max = 0.f;
+    for (float v : iv) iv_max = std::max(iv_max, v);
+    scale_cap_ = iv_max > 0.f ? 1e38 / iv_max : 1e38;
     dev_alloc(&d_inv_vol_, num_fsrs);
     HIP_CHECK(hipMemcpy(d_inv_vol_, iv.data(), num_fsrs * sizeof(float), hipMemcpyHostToDevice));
   }
@@ -1141,6 +1458,8 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
 }
 
 void HipSolver::allocateIterationArrays() {
+  graph_dirty_ = true;
+  state_valid_ = false;
   if (!(tracks_set_ && materials_set_ && fsrs_set_)) return;
   const size_t NG = (size_t)N_ * G_;
   const size_t psi_n = (size_t)2 * T_ * P_ * G_;
@@ -1149,7 +1468,9 @@ void HipSolver::allocateIterationArrays() {
   dev_alloc(&d_tally_, NG * shape_.tally_copies);
   dev_alloc(&d_qs_, NG);
   dev_alloc(&d_phi_, NG);
-  dev_alloc(&d_phi_old_, NG);
+  dev_alloc(&d_phi_old_[0], NG);
+  dev_alloc(&d_phi_old_[1], NG);
+  dev_alloc(&d_leak_, (size_t)2 * T_ * G_);
   dev_alloc(&d_Q_, NG);
   if (!shape_forced_) pickShape();
   if (shape_.lds_groups > 0) {
@@ -1160,14 +1481,25 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_tally64_, NG);
     HIP_CHECK(hipMemset(d_tally64_, 0, NG * sizeof(double)));
   }
-  num_partials_ = (int)blocks_for(N_, kReduceBlock);
-  dev_alloc(&d_partials_, num_partials_);
+  {
+    // per-block partials: post_sweep_kernel runs over nsub*N*GB threads,
+    // update_kernel over N*G
+    const int gb = std::max(1, shape_.lds_groups);
+    const int64_t npost = (int64_t)((G_ + gb - 1) / gb) * N_ * gb;
+    num_partials_ = (int)blocks_for(npost, kReduceBlock);
+    pstride_ = std::max(num_partials_, (int)blocks_for(NG, kReduceBlock));
+    dev_alloc(&d_partials_, (size_t)kNumPartials * pstride_);
+  }
   buildTileLayout(sweepTracksPerWave());
   HIP_CHECK(hipMemset(d_psi_[0], 0, psi_n * sizeof(float)));
   HIP_CHECK(hipMemset(d_psi_[1], 0, psi_n * sizeof(float)));
   HIP_CHECK(hipMemset(d_tally_, 0, NG * shape_.tally_copies * sizeof(float)));
   HIP_CHECK(hipMemset(d_phi_, 0, NG * sizeof(double)));
-  HIP_CHECK(hipMemset(d_phi_old_, 0, NG * sizeof(double)));
+  HIP_CHECK(hipMemset(d_phi_old_[0], 0, NG * sizeof(double)));
+  HIP_CHECK(hipMemset(d_phi_old_[1], 0, NG * sizeof(double)));
+  HIP_CHECK(hipMemset(d_leak_, 0, (size_t)2 * T_ * G_ * sizeof(float)));
+  phi_is_old_ = false;
+  krylov_ready_ = false;
   HIP_CHECK(hipMemset(d_Q_, 0, NG * sizeof(double)));
   HIP_CHECK(hipMemset(d_qs_, 0, NG * sizeof(float2)));
   if (fixed_on_ && h_fixed_.size() == NG) {
@@ -1181,20 +1513,24 @@ void HipSolver::requireReady() const {
   if (!tracks_set_) throw DeviceError(MOC_ERR_STATE, "tracks have not been set");
   if (!materials_set_) throw DeviceError(MOC_ERR_STATE, "materials have not been set");
   if (!fsrs_set_) throw DeviceError(MOC_ERR_STATE, "FSRs have not been set");
+  if (async_active_) throw DeviceError(MOC_ERR_STATE, "a solve is still running on this solver");
 }
 
 void HipSolver::setConvergenceThreshold(double t) {
+  graph_dirty_ = true;
   if (!(t > 0.0)) throw DeviceError(MOC_ERR_INVALID, "convergence threshold must be > 0");
   threshold_ = t;
 }
 
 void HipSolver::setExponential(int mode) {
+  graph_dirty_ = true;
   if (mode != MOC_EXP_RATIONAL && mode != MOC_EXP_HW)
     throw DeviceError(MOC_ERR_INVALID, "unknown exponential mode");
   exp_mode_ = mode;
 }
 
 void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
+  graph_dirty_ = true;
   requireReady();
   if (fsr < 0 || fsr >= N_) throw DeviceError(MOC_ERR_INVALID, "FSR id out of range");
   if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
@@ -1208,7 +1544,7 @@ void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
 }
 
 void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
-                            hipEvent_t ev_after_sweep) {
+                            hipEvent_t ev_after_sweep, float* leak) {
   SweepArgs a;
   if (layout_tpw_ != sweepTracksPerWave()) buildTileLayout(sweepTracksPerWave());
   a.work = d_work_;
@@ -1219,9 +1555,11 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   a.trk_azim = d_trk_azim_;
   a.seg_len = d_seg_len_;
   a.seg_key = d_seg_key_;
+  a.stream_bytes = (unsigned)(((size_t)layout_rows_ + kTailRows) * layout_tpw_ * sizeof(float));
   a.qs = d_qs_;
   a.psi_in = psi_in;
   a.psi_out = psi_out;
+  a.leak = leak;
   a.tally = d_tally_;
   a.tally_copies = shape_.tally_copies;
   a.tally_stride = (size_t)N_ * G_;
@@ -1253,8 +1591,7 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     if (lds_bytes > lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
     const unsigned grid = (unsigned)(L.nsub * L.blocks_per_sub);
     auto launch = [&](auto kernel) {
-      HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_bytes));
+      ensure_lds_attribute((const void*)kernel, device_, lds_max_);
       hipLaunchKernelGGL(kernel, dim3(grid), dim3(shape_.lds_block), lds_bytes, s, a, L);
     };
     auto by_np = [&](auto e, auto q) {
@@ -1275,12 +1612,6 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
       shape_.lds_q ? by_np(ER{}, Q1{}) : by_np(ER{}, Q0{});
     HIP_CHECK(hipGetLastError());
     if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
-    const int64_t nred = (int64_t)L.nsub * N_ * L.GB;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(nred, 256)), dim3(256), 0, s,
-                       (const unsigned long long*)d_lds_partials_, d_tally64_, (int64_t)N_, G_,
-                       L.GB, L.nsub,
-                       L.blocks_per_sub, (const IterState*)d_state_);
-    HIP_CHECK(hipGetLastError());
     return;
   }
   const int64_t waves = a.ntiles;
@@ -1316,72 +1647,72 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
 }
 
-namespace {
-// Points the per-FSR kernels at the source/tally buffers of the LDS sweep.
-void apply_lds_args(FsrArgs& fa, const LaunchShape& sh, double* tally64, float* qsub) {
-  if (sh.lds_groups > 0) {
-    fa.tally64 = tally64;
-    fa.qsub = qsub;
-    fa.qsub_groups = sh.lds_groups;
-    fa.qs = nullptr;
-  }
-}
-
-FsrArgs make_fsr_args(int64_t N, int G, int32_t* fsr_mat, double* vol, double* st, double* ss,
-                      double* nsf, double* chi, int32_t* fis, double* fixed, double* phi,
-                      double* old, double* Q, float2* qs, float* tally, int copies,
-                      double* partials, IterState* state, bool allow_neg) {
+FsrArgs HipSolver::fsrArgs() {
   FsrArgs a;
-  a.N = N;
-  a.G = G;
-  a.fsr_mat = fsr_mat;
-  a.fsr_vol = vol;
-  a.sigma_t = st;
-  a.sigma_s = ss;
-  a.nsf = nsf;
-  a.chi = chi;
-  a.fissionable = fis;
-  a.fixed = fixed;
-  a.phi = phi;
-  a.phi_old = old;
-  a.Q = Q;
-  a.qs = qs;
-  a.tally = tally;
-  a.tally_copies = copies;
-  a.tally64 = nullptr;
-  a.qsub = nullptr;
-  a.qsub_groups = 1;
-  a.partials = partials;
-  a.state = state;
-  a.allow_negative = allow_neg ? 1 : 0;
+  a.N = N_;
+  a.G = G_;
+  a.fsr_mat = d_fsr_mat_;
+  a.fsr_vol = d_fsr_vol_;
+  a.sigma_t = d_sigma_t_;
+  a.sigma_s = d_sigma_s_;
+  a.nsf = d_nsf_;
+  a.chi = d_chi_;
+  a.sigma_a = d_sigma_a_;
+  a.fissionable = d_mat_fissionable_;
+  a.fixed = fixed_on_ ? d_fixed_ : nullptr;
+  a.phi = d_phi_;
+  a.phi_old[0] = d_phi_old_[0];
+  a.phi_old[1] = d_phi_old_[1];
+  a.Q = d_Q_;
+  a.tally = d_tally_;
+  a.tally_copies = shape_.tally_copies;
+  a.leak = balance_ ? d_leak_ : nullptr;
+  a.leak_n = (int64_t)2 * T_ * G_;
+  a.partials = d_partials_;
+  a.pstride = pstride_;
+  a.state = d_state_;
+  a.allow_negative = allow_negative_ ? 1 : 0;
+  a.stab_type = stab_type_;
+  a.stab_factor = stab_factor_;
+  a.stab_ratio = d_stab_ratio_;
+  if (shape_.lds_groups > 0) {
+    a.qs = nullptr;
+    a.qsub = d_qsub_;
+    a.qsub_groups = shape_.lds_groups;
+    a.lds_partials = d_lds_partials_;
+    a.lds_bps = shape_.lds_blocks_per_sub;
+  } else {
+    a.qs = d_qs_;
+    a.qsub = nullptr;
+    a.qsub_groups = 1;
+    a.lds_partials = nullptr;
+    a.lds_bps = 0;
+  }
   return a;
 }
-}  // namespace
 
 // One source iteration; all launches on stream_, no host synchronisation.
 void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
                                 hipEvent_t ev0, hipEvent_t ev1) {
-  FsrArgs fa = make_fsr_args(N_, G_, d_fsr_mat_, d_fsr_vol_, d_sigma_t_, d_sigma_s_, d_nsf_,
-                             d_chi_, d_mat_fissionable_, fixed_on_ ? d_fixed_ : nullptr, d_phi_,
-                             d_phi_old_, d_Q_, d_qs_, d_tally_, shape_.tally_copies, d_partials_, d_state_,
-                             allow_negative_);
-  apply_lds_args(fa, shape_, d_tally64_, d_qsub_);
-  const unsigned nb = blocks_for(N_, kReduceBlock);
+  const FsrArgs fa = fsrArgs();
+  const bool balance = balance_ && mode == MODE_EIGEN;
   if (ev0) HIP_CHECK(hipEventRecord(ev0, stream_));
-  launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_, ev1);
+  launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_, ev1, balance ? d_leak_ : nullptr);
   psi_cur_ ^= 1;
-  hipLaunchKernelGGL(post_sweep_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa, 1);
+  hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1,
+                     balance ? 1 : 0);
   if (mode == MODE_EIGEN)
     hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
-                       num_partials_, d_state_, (double)N_, 1);
+                       num_partials_, pstride_, d_state_, (double)N_, 1, balance ? 1 : 0);
   const double norm = (mode == MODE_EIGEN && res_type == MOC_FISSION_SOURCE)
                           ? (double)num_fissionable_
                           : (double)N_;
   const int rt = mode == MODE_FLUX ? MOC_SCALAR_FLUX : res_type;
-  hipLaunchKernelGGL(update_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa,
-                     mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1);
+  const unsigned nb_upd = blocks_for((int64_t)N_ * G_, kReduceBlock);
+  hipLaunchKernelGGL(update_kernel, dim3(nb_upd), dim3(kReduceBlock), 0, stream_, fa,
+                     mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1, kSrcTotal);
   hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
-                     num_partials_, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
+                     (int)nb_upd, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
                      tol, mode == MODE_EIGEN ? 0 : 1, sigma_t_max_, recompute_sources ? 1 : 0);
   HIP_CHECK(hipGetLastError());
 }
@@ -1397,6 +1728,7 @@ void HipSolver::launchInit(Mode mode, double k0) {
   st.k_prev = st.k;
   st.norm_factor = 1.0;
   st.psi_scale = 1.0;
+  st.scale_cap = scale_cap_;
   HIP_CHECK(hipMemcpyAsync(d_state_, &st, sizeof(st), hipMemcpyHostToDevice, stream_));
   // fresh angular fluxes and tally (initializeFluxArrays, src/CPUSolver.cpp:281-380)
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
@@ -1404,74 +1736,125 @@ void HipSolver::launchInit(Mode mode, double k0) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   psi_cur_ = 0;
 
-  FsrArgs fa = make_fsr_args(N_, G_, d_fsr_mat_, d_fsr_vol_, d_sigma_t_, d_sigma_s_, d_nsf_,
-                             d_chi_, d_mat_fissionable_, fixed_on_ ? d_fixed_ : nullptr, d_phi_,
-                             d_phi_old_, d_Q_, d_qs_, d_tally_, shape_.tally_copies, d_partials_, d_state_,
-                             allow_negative_);
-  apply_lds_args(fa, shape_, d_tally64_, d_qsub_);
-  const unsigned nb = blocks_for(N_, kReduceBlock);
+  refreshStabilization();
+  const FsrArgs fa = fsrArgs();
   // initial guess (computeInitialFluxGuess, src/Solver.cpp:1710-1731; computeFlux
   // starts from phi = 0, src/Solver.cpp:1383-1390)
   hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_phi_,
                      (int64_t)NG, mode == MODE_FLUX ? 0.0 : 1.0);
   if (mode == MODE_EIGEN) {
-    hipLaunchKernelGGL(post_sweep_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa, 0);
+    hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 0, 0);
     hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
-                       num_partials_, d_state_, (double)N_, 0);
+                       num_partials_, pstride_, d_state_, (double)N_, 0, 0);
   }
   // normalise (eigen), store old flux, sources of iteration 0
-  hipLaunchKernelGGL(update_kernel, dim3(nb), dim3(kReduceBlock), 0, stream_, fa,
-                     mode == MODE_EIGEN ? 1 : 0, 0, 0, 1, 0);
-  // fixed-point scale of the first sweep (start fluxes are zero)
-  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1);
+  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+                     stream_, fa, mode == MODE_EIGEN ? 1 : 0, 0, 0, 1, 0, kSrcTotal);
+  // fixed-point scale of the first sweep (start fluxes are zero); the old
+  // flux just written becomes the current one
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
   HIP_CHECK(hipGetLastError());
 
   state_valid_ = true;
+  phi_is_old_ = true;
+  krylov_ready_ = false;
+}
+
+// YAMAMOTO: per group, the factor times the largest |sigma_s(g->g) / sigma_t(g)|
+// over the FSRs' materials (src/CPUSolver.cpp:2692-2710).
+void HipSolver::refreshStabilization() {
+  if (stab_type_ != 1) return;
+  std::vector<int> used(M_, 0);
+  for (int64_t r = 0; r < N_; r++) used[h_fsr_mat_[r]] = 1;
+  std::vector<double> ratio(G_, 0.0);
+  for (int g = 0; g < G_; g++) {
+    double mx = 0.0;
+    for (int m = 0; m < M_; m++)
+      if (used[m]) {
+        const double v = std::fabs(h_sigma_s_[((size_t)m * G_ + g) * G_ + g] / h_sigma_t_[(size_t)m * G_ + g]);
+        if (v > mx) mx = v;
+      }
+    ratio[g] = mx * stab_factor_;
+  }
+  dev_alloc(&d_stab_ratio_, G_);
+  HIP_CHECK(hipMemcpy(d_stab_ratio_, ratio.data(), G_ * sizeof(double), hipMemcpyHostToDevice));
+}
+
+void HipSolver::stabilizeTransport(double factor, int type) {
+  graph_dirty_ = true;
+  if (type < 0) {
+    stab_type_ = -1;
+    return;
+  }
+  if (type > 2) throw DeviceError(MOC_ERR_INVALID, "unknown stabilization type");
+  if (!(factor > 0.0) && type != 1) throw DeviceError(MOC_ERR_INVALID, "stabilization factor must be positive");
+  stab_type_ = type;
+  stab_factor_ = factor;
+}
+
+const double* HipSolver::currentFlux() const {
+  return phi_is_old_ ? d_phi_old_[old_sel_] : d_phi_;
+}
+
+// One Krylov operator sweep (Solver::fissionTransportSweep /
+// scatterTransportSweep): sources from the current flux, the sweep from the
+// start fluxes left by the previous operator sweep (zero at first, as after
+// initializeFluxArrays), then addSourceToScalarFlux into the flux.
+void HipSolver::krylovSweep(int source_kind) {
+  requireReady();
+  if (source_kind != kSrcFission && source_kind != kSrcScatter)
+    throw DeviceError(MOC_ERR_INVALID, "unknown Krylov source kind");
+  HIP_CHECK(hipSetDevice(device_));
+  const size_t NG = (size_t)N_ * G_;
+  const size_t psi_n = (size_t)2 * T_ * P_ * G_;
+  if (phi_is_old_) {
+    // the flux of the last solve becomes the operand
+    HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[old_sel_], NG * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+    phi_is_old_ = false;
+  }
+  if (!krylov_ready_) {
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[1], (int64_t)psi_n);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
+    psi_cur_ = 0;
+    krylov_ready_ = true;
+  }
+  IterState st;
+  std::memset(&st, 0, sizeof(st));
+  st.k = 1.0;
+  st.k_prev = 1.0;
+  st.norm_factor = 1.0;
+  st.psi_scale = 1.0;
+  st.scale_cap = scale_cap_;
+  st.iteration = 30;  // no negative-source clamp
+  // the start fluxes' bound for the fixed-point scale: the largest outgoing
+  // flux of the previous operator sweep
+  st.psi_max_bits = h_state_ ? h_state_->psi_max_bits : 0u;
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  *h_state_ = st;
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  FsrArgs fa = fsrArgs();
+  fa.leak = nullptr;
+  fa.stab_type = -1;
+  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+                     stream_, fa, 0, 0, 0, 1, 0, source_kind);
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
+  launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
+  psi_cur_ ^= 1;
+  hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1, 0);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  state_valid_ = false;
 }
 
 void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed) {
-  requireReady();
-  HIP_CHECK(hipSetDevice(device_));
-  if (max_iters <= 0) throw DeviceError(MOC_ERR_INVALID, "max_iters must be > 0");
-  if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
-    throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
-  if (mode != MODE_FLUX && res_type == MOC_FISSION_SOURCE && num_fissionable_ == 0)
-    throw DeviceError(MOC_ERR_INVALID,
-                      "cannot compute a FISSION_SOURCE residual without fissionable FSRs");
-  if (mode == MODE_EIGEN && num_fissionable_ == 0)
-    throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
-  const auto t0 = std::chrono::steady_clock::now();
-  launchInit(mode, mode == MODE_SOURCE ? k_fixed : 1.0);
-
-  const bool recompute = mode != MODE_FLUX;
-  hipEvent_t e0, e1;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
-  int done = 0;
-  int batch = 4;
-  float sweep_ms_total = 0.f;
-  while (done < max_iters) {
-    const int nb_it = std::min(batch, max_iters - done);
-    HIP_CHECK(hipEventRecord(e0, stream_));
-    for (int i = 0; i < nb_it; i++) launchIteration(mode, res_type, recompute, threshold_);
-    HIP_CHECK(hipEventRecord(e1, stream_));
-    HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
-    HIP_CHECK(hipStreamSynchronize(stream_));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    sweep_ms_total += ms;
-    done += nb_it;
-    if (h_state_->converged) break;
-    batch = std::min(batch * 2, 32);
+  startSolve(mode, max_iters, res_type, k_fixed);
+  while (true) {
+    HIP_CHECK(hipEventSynchronize(async_ev_));
+    if (pollSolve()) break;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  keff_ = h_state_->k;
-  num_iterations_ = h_state_->converged ? h_state_->reported_iters : max_iters;
-  total_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  // device time of the iteration batches (sweeps plus the per-FSR steps)
-  sweep_seconds_ = sweep_ms_total * 1e-3;
-  integrations_ = 2LL * G_ * P_ * S_ * (int64_t)h_state_->iteration;
 }
 
 void HipSolver::computeEigenvalue(int max_iters, int res_type) {
@@ -1495,7 +1878,7 @@ double HipSolver::getFlux(int64_t fsr, int group) {
   if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
   double v = 0;
   HIP_CHECK(hipSetDevice(device_));
-  HIP_CHECK(hipMemcpyAsync(&v, d_phi_ + (size_t)fsr * G_ + group - 1, sizeof(double),
+  HIP_CHECK(hipMemcpyAsync(&v, currentFlux() + (size_t)fsr * G_ + group - 1, sizeof(double),
                            hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   return v;
@@ -1505,7 +1888,7 @@ void HipSolver::getFluxes(double* out, int64_t n) {
   requireReady();
   if (!out || n != N_ * G_) throw DeviceError(MOC_ERR_INVALID, "getFluxes size must be N*G");
   HIP_CHECK(hipSetDevice(device_));
-  HIP_CHECK(hipMemcpyAsync(out, d_phi_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(out, currentFlux(), n * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
@@ -1515,6 +1898,7 @@ void HipSolver::setFluxes(const double* in, int64_t n) {
   HIP_CHECK(hipSetDevice(device_));
   HIP_CHECK(hipMemcpyAsync(d_phi_, in, n * sizeof(double), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
+  phi_is_old_ = false;
 }
 
 void HipSolver::computeFSRFissionRates(double* out, int64_t n) {
@@ -1524,7 +1908,7 @@ void HipSolver::computeFSRFissionRates(double* out, int64_t n) {
   double* d_out = nullptr;
   dev_alloc(&d_out, N_);
   hipLaunchKernelGGL(fission_rates_kernel, dim3(blocks_for(N_, 256)), dim3(256), 0, stream_,
-                     d_phi_, d_fsr_mat_, d_nsf_, d_fsr_vol_, d_out, N_, G_);
+                     currentFlux(), d_fsr_mat_, d_nsf_, d_fsr_vol_, d_out, N_, G_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(out, d_out, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1558,7 +1942,8 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
     st.qmax = (float)(qmax * (1 + 1e-6));
     st.psi_max = (float)(pmax * (1 + 1e-6));
     const double bound = 2.0 * kFourPi * sigma_t_max_ * ((double)st.psi_max + (double)st.qmax);
-    st.tally_scale = bound > 1e-200 ? 0x1p60 / bound : 0x1p60;
+    st.scale_cap = scale_cap_;
+    st.tally_scale = tally_scale_for(bound, scale_cap_);
   }
   HIP_CHECK(hipMemcpyAsync(d_state_, &st, sizeof(st), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipMemcpyAsync(d_Q_, Q, NG * sizeof(double), hipMemcpyHostToDevice, stream_));
@@ -1574,6 +1959,12 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   launchSweep(d_psi_[0], d_psi_[1], stream_);
   std::vector<float> t32;
   if (shape_.lds_groups > 0) {
+    const int gb = shape_.lds_groups, nsub = (G_ + gb - 1) / gb;
+    const int64_t nred = (int64_t)nsub * N_ * gb;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(nred, 256)), dim3(256), 0, stream_,
+                       (const unsigned long long*)d_lds_partials_, d_tally64_, (int64_t)N_, G_, gb,
+                       nsub, shape_.lds_blocks_per_sub, (const IterState*)d_state_);
+    HIP_CHECK(hipGetLastError());
     // the fixed-point tally holds sum / V; the raw tally is sum
     HIP_CHECK(hipMemcpyAsync(tally, d_tally64_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
   } else {
@@ -1600,7 +1991,7 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   psi_cur_ = 0;
 }
 
-double HipSolver::runIterations(int n) {
+double HipSolver::runIterations(int n, double* sweep_seconds) {
   requireReady();
   if (n <= 0) throw DeviceError(MOC_ERR_INVALID, "number of iterations must be > 0");
   if (num_fissionable_ == 0) throw DeviceError(MOC_ERR_INVALID, "no fissionable material");
@@ -1612,21 +2003,55 @@ double HipSolver::runIterations(int n) {
   HIP_CHECK(hipStreamSynchronize(stream_));
   h_state_->converged = 0;
   HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
-  std::vector<hipEvent_t> ev(2 * n);
-  for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-  for (int i = 0; i < n; i++)
-    launchIteration(MODE_EIGEN, MOC_FISSION_SOURCE, true, 0.0, ev[2 * i], ev[2 * i + 1]);
+  const double tol = threshold_;
+  threshold_ = 0.0;
+  graph_dirty_ = true;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  double step_ms = 0.0;
+  try {
+    // the iterations as a solve runs them (graph batches)
+    if (use_graphs_ && n >= kGraphIters) ensureGraph(MODE_EIGEN, MOC_FISSION_SOURCE);
+    HIP_CHECK(hipEventRecord(e0, stream_));
+    launchIterations(MODE_EIGEN, MOC_FISSION_SOURCE, n);
+    HIP_CHECK(hipEventRecord(e1, stream_));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    step_ms = ms / n;
+    if (sweep_seconds) {
+      // the sweep kernel alone: events around it in a few more iterations
+      const int m = std::min(n, 8);
+      std::vector<hipEvent_t> ev(2 * m);
+      for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+      for (int i = 0; i < m; i++)
+        launchIteration(MODE_EIGEN, MOC_FISSION_SOURCE, true, 0.0, ev[2 * i], ev[2 * i + 1]);
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      double sw = 0.0;
+      for (int i = 0; i < m; i++) {
+        float x = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&x, ev[2 * i], ev[2 * i + 1]));
+        sw += x;
+      }
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      *sweep_seconds = sw * 1e-3 / m;
+    }
+  } catch (...) {
+    threshold_ = tol;
+    graph_dirty_ = true;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  threshold_ = tol;
+  graph_dirty_ = true;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
-  double total_ms = 0;
-  for (int i = 0; i < n; i++) {
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-    total_ms += ms;
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   keff_ = h_state_->k;
-  return total_ms * 1e-3 / n;
+  return step_ms * 1e-3;
 }
 
 double HipSolver::timeSweeps(int repeats) {
@@ -1643,7 +2068,7 @@ double HipSolver::timeSweeps(int repeats) {
   h_state_->psi_max_bits = __builtin_bit_cast(uint32_t, std::max(h_state_->psi_max,
                                                    __builtin_bit_cast(float, h_state_->psi_max_bits)));
   HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
-  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 0);
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 0, 0);
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
@@ -1662,6 +2087,239 @@ double HipSolver::timeSweeps(int repeats) {
   const size_t NG = (size_t)N_ * G_;
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   HIP_CHECK(hipStreamSynchronize(stream_));
+  return ms * 1e-3 / repeats;
+}
+
+// Geometry buffers shared by the solvers of one track set (serving): freed
+// with the last solver that uses them.
+struct HipSolver::GeoHold {
+  int device = 0;
+  std::vector<void*> bufs;
+  ~GeoHold() {
+    (void)hipSetDevice(device);
+    for (void* p : bufs)
+      if (p) {
+        {
+          std::lock_guard<std::mutex> lk(g_alloc_mu);
+          g_alloc_bytes.erase(p);
+        }
+        (void)hipFree(p);
+      }
+  }
+};
+
+void HipSolver::shareGeometry(HipSolver& src) {
+  if (&src == this) return;
+  if (!(src.tracks_set_ && src.fsrs_set_ && src.layout_tpw_ > 0))
+    throw DeviceError(MOC_ERR_STATE, "the source solver has no complete geometry (tracks, FSRs, materials)");
+  if (src.device_ != device_) throw DeviceError(MOC_ERR_INVALID, "solvers on different devices");
+  if (tracks_set_ || fsrs_set_ || geo_hold_)
+    throw DeviceError(MOC_ERR_STATE, "geometry can only be shared into a fresh solver");
+  if (!src.geo_hold_) {
+    src.geo_hold_ = std::make_shared<GeoHold>();
+    src.geo_hold_->device = src.device_;
+    src.geo_hold_->bufs = {src.d_seg_len_, src.d_seg_key_, src.d_work_,  src.d_tile_,
+                           src.d_trk_,     src.d_trk_azim_, src.d_weights_, src.d_fsr_mat_,
+                           src.d_fsr_vol_, src.d_inv_vol_};
+  }
+  geo_hold_ = src.geo_hold_;
+  d_seg_len_ = src.d_seg_len_;
+  d_seg_key_ = src.d_seg_key_;
+  d_work_ = src.d_work_;
+  d_tile_ = src.d_tile_;
+  d_trk_ = src.d_trk_;
+  d_trk_azim_ = src.d_trk_azim_;
+  d_weights_ = src.d_weights_;
+  d_fsr_mat_ = src.d_fsr_mat_;
+  d_fsr_vol_ = src.d_fsr_vol_;
+  d_inv_vol_ = src.d_inv_vol_;
+  T_ = src.T_;
+  S_ = src.S_;
+  P_ = src.P_;
+  N_ = src.N_;
+  G_ = src.G_;
+  num_azim_ = src.num_azim_;
+  num_polar_ = src.num_polar_;
+  for (int p = 0; p < kMaxPolarHalf; p++) inv_sin_[p] = src.inv_sin_[p];
+  max_seg_fsr_ = src.max_seg_fsr_;
+  layout_tpw_ = src.layout_tpw_;
+  layout_ntiles_ = src.layout_ntiles_;
+  layout_rows_ = src.layout_rows_;
+  h_fsr_mat_ = src.h_fsr_mat_;
+  h_fsr_vol_ = src.h_fsr_vol_;
+  scale_cap_ = src.scale_cap_;
+  shape_ = src.shape_;
+  shape_forced_ = true;
+  exp_mode_ = src.exp_mode_;
+  tracks_set_ = true;
+  fsrs_set_ = true;
+}
+
+size_t HipSolver::solveBytes() const {
+  const size_t NG = (size_t)N_ * G_;
+  const size_t psi = (size_t)2 * T_ * P_ * G_ * sizeof(float);
+  const int gb = std::max(1, shape_.lds_groups);
+  const size_t nsub = (size_t)(G_ + gb - 1) / gb;
+  size_t b = 2 * psi + NG * (6 * sizeof(double) + sizeof(float2)) +
+             NG * shape_.tally_copies * sizeof(float) + (size_t)2 * T_ * G_ * sizeof(float) +
+             (size_t)M_ * G_ * (G_ + 4) * sizeof(double);
+  if (shape_.lds_groups > 0)
+    b += nsub * shape_.lds_blocks_per_sub * N_ * gb * sizeof(uint64_t) + nsub * N_ * gb * sizeof(float);
+  return b;
+}
+
+
+void HipSolver::ensureGraph(int mode, int res_type) {
+  const bool recompute = mode != MODE_FLUX;
+  if (graph_exec_ && graph_mode_ == mode && graph_res_type_ == res_type && !graph_dirty_) return;
+  if (graph_exec_) {
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+  if (layout_tpw_ != sweepTracksPerWave()) buildTileLayout(sweepTracksPerWave());
+  // kernels that need the LDS opt-in get it before the capture
+  const int psi0 = psi_cur_;
+  {
+    const void* ks[] = {
+        (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 0, 1>, (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 0, 2>,
+        (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 0, 3>, (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 1, 1>,
+        (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 1, 2>, (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 1, 3>,
+        (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 1>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 2>,
+        (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 3>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 1>,
+        (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 2>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 3>};
+    for (const void* k : ks) ensure_lds_attribute(k, device_, lds_max_);
+  }
+  hipGraph_t g = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int i = 0; i < kGraphIters; i++)
+      launchIteration((Mode)mode, res_type, recompute, threshold_);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    psi_cur_ = psi0;
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(stream_, &g));
+  hipError_t e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    graph_exec_ = nullptr;
+    throw DeviceError(MOC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  psi_cur_ = psi0;  // an even number of iterations was recorded
+  graph_mode_ = mode;
+  graph_res_type_ = res_type;
+  graph_dirty_ = false;
+}
+
+// Launches `n` iterations: whole graphs while at least kGraphIters remain and
+// psi_cur_ is where the graph expects it, single iterations otherwise.
+void HipSolver::launchIterations(int mode, int res_type, int n) {
+  const bool recompute = mode != MODE_FLUX;
+  while (n > 0) {
+    if (use_graphs_ && n >= kGraphIters) {
+      ensureGraph(mode, res_type);
+      HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+      n -= kGraphIters;
+    } else {
+      launchIteration((Mode)mode, res_type, recompute, threshold_);
+      n--;
+    }
+  }
+}
+
+void HipSolver::startSolve(int mode, int max_iters, int res_type, double k_fixed) {
+  requireReady();
+  HIP_CHECK(hipSetDevice(device_));
+  if (mode < MODE_EIGEN || mode > MODE_SOURCE) throw DeviceError(MOC_ERR_INVALID, "unknown solve mode");
+  if (max_iters <= 0) throw DeviceError(MOC_ERR_INVALID, "max_iters must be > 0");
+  if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
+    throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
+  if (mode != MODE_FLUX && res_type == MOC_FISSION_SOURCE && num_fissionable_ == 0)
+    throw DeviceError(MOC_ERR_INVALID,
+                      "cannot compute a FISSION_SOURCE residual without fissionable FSRs");
+  if (mode == MODE_EIGEN && num_fissionable_ == 0)
+    throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
+  if (mode == MODE_FLUX && !fixed_on_) throw DeviceError(MOC_ERR_STATE, "computeFlux requires fixed sources");
+  if (!async_ev_) HIP_CHECK(hipEventCreate(&async_ev_));
+  if (!async_ev0_) HIP_CHECK(hipEventCreate(&async_ev0_));
+  async_ms_ = 0.0;
+  async_t0_ = std::chrono::steady_clock::now();
+  launchInit((Mode)mode, mode == MODE_SOURCE ? k_fixed : 1.0);
+  if (use_graphs_ && max_iters >= kGraphIters) ensureGraph(mode, res_type);
+  async_mode_ = mode;
+  async_res_type_ = res_type;
+  async_max_ = max_iters;
+  async_done_ = 0;
+  async_active_ = true;
+  launchAsyncBatch();
+}
+
+void HipSolver::launchAsyncBatch() {
+  const int n = std::min(kGraphIters, async_max_ - async_done_);
+  HIP_CHECK(hipEventRecord(async_ev0_, stream_));
+  launchIterations(async_mode_, async_res_type_, n);
+  async_done_ += n;
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipEventRecord(async_ev_, stream_));
+}
+
+bool HipSolver::pollSolve() {
+  if (!async_active_) return true;
+  const hipError_t q = hipEventQuery(async_ev_);
+  if (q == hipErrorNotReady) return false;
+  HIP_CHECK(q);
+  {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, async_ev0_, async_ev_) == hipSuccess) async_ms_ += ms;
+  }
+  if (!(h_state_->converged || async_done_ >= async_max_)) {
+    launchAsyncBatch();
+    return false;
+  }
+  finishSolve(async_max_);
+  async_active_ = false;
+  return true;
+}
+
+void HipSolver::finishSolve(int max_iters) {
+  keff_ = h_state_->k;
+  old_sel_ = h_state_->old_sel;
+  phi_is_old_ = true;
+  num_iterations_ = h_state_->converged ? h_state_->reported_iters : max_iters;
+  total_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - async_t0_).count();
+  // device time of the iteration batches (sweeps plus the per-FSR steps)
+  sweep_seconds_ = async_ms_ * 1e-3;
+  integrations_ = 2LL * G_ * P_ * S_ * (int64_t)h_state_->iteration;
+}
+
+double deviceCopyCalibration(int device, int64_t bytes, int repeats) {
+  if (bytes <= 0 || bytes % 16 != 0 || repeats <= 0)
+    throw DeviceError(MOC_ERR_INVALID, "copy size must be a positive multiple of 16 bytes");
+  HIP_CHECK(hipSetDevice(device));
+  int4 *a = nullptr, *b = nullptr;
+  const int64_t n = bytes / 16;
+  dev_alloc(&a, (size_t)n);
+  dev_alloc(&b, (size_t)n);
+  HIP_CHECK(hipMemset(a, 1, (size_t)bytes));
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  int cus = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)cus * 16));
+  HIP_CHECK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < repeats; r++) hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, nullptr, a, b, n);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(e1, nullptr));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(a);
+  dev_free(b);
   return ms * 1e-3 / repeats;
 }
 

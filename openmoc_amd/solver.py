@@ -24,12 +24,13 @@ from typing import Optional, Union
 import numpy as np
 
 from . import _lib
-from ._lib import (EXP_HW, EXP_RATIONAL, FISSION_SOURCE, SCALAR_FLUX, TOTAL_SOURCE,
-                   MOCError, check, lib)
+from ._lib import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL, SCALAR_FLUX,
+                   TOTAL_SOURCE, YAMAMOTO, MOCError, check, lib)
 from .workloads import Problem
 
 __all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "SCALAR_FLUX", "FISSION_SOURCE",
-           "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "MOCError"]
+           "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "DIAGONAL", "YAMAMOTO", "GLOBAL",
+           "MOCError"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -216,6 +217,17 @@ class HipSolver:
         self._tg = None
         self._psi_size = 2 * len(arrs["azim"]) * (int(num_polar) // 2) * G
 
+    def setMaterials(self, sigma_t, sigma_s, nu_sigma_f, chi) -> None:
+        """Replace the cross sections (sigma_t/nu_sigma_f/chi [M, G], sigma_s
+        [M, G*G] dest-major), e.g. a perturbed copy of the geometry's materials
+        (Material::setSigma*, src/Material.cpp:628-979)."""
+        c = np.ascontiguousarray
+        st, ss, nsf, ch = (c(x, np.float64) for x in (sigma_t, sigma_s, nu_sigma_f, chi))
+        M, G = st.shape
+        check(lib().moc_solver_set_materials(self._h, M, G, _ptr(st), _ptr(ss), _ptr(nsf),
+                                             _ptr(ch)))
+        self._num_groups = int(G)
+
     def setConvergenceThreshold(self, threshold: float) -> None:
         check(lib().moc_solver_set_convergence_threshold(self._h, float(threshold)))
 
@@ -224,6 +236,24 @@ class HipSolver:
 
     def allowNegativeFluxes(self, allow: bool) -> None:
         check(lib().moc_solver_allow_negative_fluxes(self._h, int(bool(allow))))
+
+    def stabilizeTransport(self, stabilization_factor: float,
+                           stabilization_type: int = DIAGONAL) -> None:
+        """Solver::stabilizeTransport (src/Solver.cpp:676-681)."""
+        check(lib().moc_solver_stabilize_transport(self._h, float(stabilization_factor),
+                                                   int(stabilization_type)))
+
+    def setKeffFromNeutronBalance(self) -> None:
+        """Solver::setKeffFromNeutronBalance (src/Solver.cpp:2047-2049)."""
+        check(lib().moc_solver_set_keff_from_neutron_balance(self._h, 1))
+
+    def fissionTransportSweep(self) -> None:
+        """Solver::fissionTransportSweep (src/Solver.cpp:1282-1286)."""
+        check(lib().moc_solver_fission_transport_sweep(self._h))
+
+    def scatterTransportSweep(self) -> None:
+        """Solver::scatterTransportSweep (src/Solver.cpp:1293-1297)."""
+        check(lib().moc_solver_scatter_transport_sweep(self._h))
 
     def setFixedSourceByFSR(self, fsr_id: int, group: int, source: float) -> None:
         check(lib().moc_solver_set_fixed_source_by_fsr(self._h, int(fsr_id), int(group),
@@ -321,12 +351,15 @@ class HipSolver:
         check(lib().moc_solver_get_launch_shape(self._h, _ptr(a)))
         return dict(zip(self.SHAPE_KEYS, (int(x) for x in a)))
 
-    def runIterations(self, num_iterations: int) -> float:
+    def runIterations(self, num_iterations: int, time_sweep: bool = True):
         """Benchmark form: num_iterations full source iterations from the
-        current state, no stopping rule; returns mean sweep-kernel seconds."""
-        v = C.c_double()
-        check(lib().moc_solver_run_iterations(self._h, int(num_iterations), C.byref(v)))
-        return v.value
+        current state, no stopping rule.  Returns (mean device seconds per
+        iteration, mean seconds of the sweep kernel alone or None), the latter
+        from up to 8 further iterations with events around the sweep."""
+        v, w = C.c_double(), C.c_double()
+        check(lib().moc_solver_run_iterations(self._h, int(num_iterations), C.byref(v),
+                                              C.byref(w) if time_sweep else None))
+        return v.value, (w.value if time_sweep else None)
 
     def timeSweeps(self, repeats: int = 10) -> float:
         v = C.c_double()

@@ -363,7 +363,41 @@ def minicore_70g(seed: int = 3, num_azim: int = 32, spacing: float = 0.05) -> Pr
                    tolerance=1e-5, max_iters=1000)
 
 
+def lattice_3x3_vac_7g(seed: int = 4) -> Problem:
+    """3x3 pin lattice (centre position moderator-filled), vacuum on all four
+    sides, 7 synthetic groups; 16 azimuthal angles, 0.05 cm.  Leaks
+    neutrons, so it exercises the neutron-balance keff and the Krylov
+    operator sweeps (which the reference requires to have vacuum
+    boundaries, openmoc/krylov.py:98-108)."""
+    rng = np.random.default_rng(seed)
+    fuel = synthetic_material(rng, 7, "fuel", True)
+    mod = synthetic_material(rng, 7, "moderator", False)
+    cells = np.zeros((3, 3), np.int32)
+    cells[1, 1] = 1
+    return Problem("lattice_3x3_vac_7g", [fuel, mod], [_pin(0, 1), _pin(1, 1)], 3, 3,
+                   PITCH, PITCH, cells, bc=("V", "V", "V", "V"), num_azim=16,
+                   azim_spacing=0.05, tolerance=1e-5, max_iters=1000)
+
+
+def pin_cell_7g_tc(seed: int = 5) -> Problem:
+    """pin_cell_7g with transport-corrected-style moderator data: the
+    in-group scattering of the two thermal groups is negative (what the
+    reference's DIAGONAL transport stabilisation is for)."""
+    rng = np.random.default_rng(seed)
+    fuel = synthetic_material(rng, 7, "fuel", True)
+    mod = synthetic_material(rng, 7, "moderator", False)
+    ss = mod.sigma_s.reshape(7, 7).copy()
+    for g in (5, 6):
+        ss[g, g] = -0.1 * mod.sigma_t[g]
+    mod = MaterialXS(mod.name, mod.sigma_t, ss.ravel(), mod.nu_sigma_f, mod.chi)
+    return Problem("pin_cell_7g_tc", [fuel, mod], [_pin(0, 1)], 1, 1, PITCH, PITCH,
+                   np.zeros((1, 1), np.int32), num_azim=32, azim_spacing=0.05,
+                   tolerance=1e-5, max_iters=1000)
+
+
 WORKLOADS = {
+    "lattice_3x3_vac_7g": lattice_3x3_vac_7g,
+    "pin_cell_7g_tc": pin_cell_7g_tc,
     "hom_inf_medium": hom_inf_medium,
     "c5g7_pin_cell": c5g7_pin_cell,
     "pin_cell_7g": pin_cell_7g,

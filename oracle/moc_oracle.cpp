@@ -30,6 +30,21 @@
  *   oracle_compute_flux           src/Solver.cpp:1352-1420
  *       pinned: infinite-medium fixed-source closed form phi = 4pi q / sigma_a
  *       (q isotropic source per steradian, Q = fixed/4pi).
+ *   neutron-balance keff          src/CPUSolver.cpp:2258-2328 (rates of
+ *                                 fission, absorption and the vacuum leakage
+ *                                 tallied by transferBoundaryFlux :2593-2600)
+ *       pinned: the compiled reference with setKeffFromNeutronBalance, and
+ *       the reflective infinite medium where it equals the fission-rate k.
+ *   transport stabilisation       src/Solver.cpp:1616-1635,
+ *                                 src/CPUSolver.cpp:2665-2805
+ *       pinned: the compiled reference with stabilizeTransport (each type),
+ *       and the fixed point: a converged flux is unchanged by it.
+ *   oracle_krylov_sweep           src/Solver.cpp:1282-1297,
+ *                                 src/CPUSolver.cpp:2030-2104
+ *       pinned: the compiled reference's fissionTransportSweep /
+ *       scatterTransportSweep outputs, and linearity (the two operator
+ *       sweeps from zero start fluxes add up to one sweep on the sum of
+ *       their sources).
  */
 #include "moc_oracle.h"
 
@@ -134,7 +149,8 @@ void fsr_sources(const View& v, const double* phi, double k, int it, bool allow_
 }
 
 template <typename PsiT>
-void sweep(const View& v, const double* Q, PsiT* start, double* tally, int threads) {
+void sweep(const View& v, const double* Q, PsiT* start, double* tally, int threads,
+           float* leakage = nullptr /* [T] per track, or null */) {
   const OracleProblem* p = v.p;
   const int G = v.G, P = v.P, fpt = v.fpt;
   const int64_t T = v.T;
@@ -190,6 +206,11 @@ void sweep(const View& v, const double* Q, PsiT* start, double* tally, int threa
           int nd = d == 0 ? !p->next_fwd_is_fwd[t] : !p->next_bwd_is_fwd[t];
           PsiT* out = start + (nt * 2 + nd) * fpt;
           for (int e = 0; e < fpt; e++) out[e] = psi[e];
+        } else if (leakage) {
+          // vacuum leakage with the weight of polar index 0 (the 2D sweep
+          // passes polar_index = 0, src/TrackTraversingAlgorithms.cpp:902)
+          const double w0 = w[0];
+          for (int e = 0; e < fpt; e++) leakage[t] += (float)(w0 * (double)psi[e]);
         }
       }
     }
@@ -309,16 +330,88 @@ int solve(const OracleProblem* p, const OracleOptions* o, OracleResult* out, dou
 
   double k_prev = k;
   bool converged = false;
+  const bool balance = mode == MODE_EIGEN && o->keff_from_fission_rates == 0;
+  const bool stab = mode == MODE_EIGEN && o->stab_type >= 0;
+  std::vector<float> leakage(balance ? T : 0);
+  std::vector<double> stab_flux(stab ? N * G : 0), stab_ratio(stab ? G : 0);
+  if (stab && o->stab_type == 1) {
+    // YAMAMOTO: largest |sigma_s(g->g) / sigma_t(g)| over the FSRs, times the factor
+    for (int g = 0; g < G; g++) {
+      double mx = 0.0;
+      for (int64_t r = 0; r < N; r++) {
+        double ratio = std::abs(v.sig_s(r)[g * G + g] / v.sig_t(r)[g]);
+        if (ratio > mx) mx = ratio;
+      }
+      stab_ratio[g] = mx * o->stab_factor;
+    }
+  }
   for (int i = 0; i < o->max_iters; i++) {
+    if (stab && i > 0) {
+      // computeStabilizingFlux (src/CPUSolver.cpp:2665-2723) from the
+      // normalised flux of the previous iteration
+      for (int64_t r = 0; r < N; r++)
+        for (int g = 0; g < G; g++) {
+          const int64_t q = r * G + g;
+          if (o->stab_type == 0) {
+            const double ss = v.sig_s(r)[g * G + g];
+            if (ss < 0.0) stab_flux[q] = -phi[q] * o->stab_factor * ss / v.sig_t(r)[g];
+          } else if (o->stab_type == 1) {
+            stab_flux[q] = phi[q] * stab_ratio[g];
+          } else {
+            stab_flux[q] = (1.0 / o->stab_factor - 1.0) * phi[q];
+          }
+        }
+    }
     if (mode != MODE_FLUX) fsr_sources(v, phi.data(), k, i, allow_neg, Q.data(), 0);
+    if (balance) std::fill(leakage.begin(), leakage.end(), 0.f);
     auto t0 = std::chrono::steady_clock::now();
-    sweep<PsiT>(v, Q.data(), start.data(), tally.data(), o->threads);
+    sweep<PsiT>(v, Q.data(), start.data(), tally.data(), o->threads,
+                balance ? leakage.data() : nullptr);
     sweep_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     phi = tally;
     add_source(v, Q.data(), allow_neg, phi.data());
     if (mode == MODE_EIGEN) {
       double rate = fission_rate_total(v, phi.data(), scratch);
-      k *= rate / (double)N;  // computeKeff with fission rates (:2258-2328)
+      if (balance) {
+        // k = fission / (absorption + leakage) (:2264-2325), sigma_a from
+        // Material::getSigmaA (src/Material.cpp:228-257)
+        for (int64_t r = 0; r < N; r++) {
+          std::vector<double> gr(G);
+          const double* st = v.sig_t(r);
+          const double* ss = v.sig_s(r);
+          for (int g = 0; g < G; g++) {
+            double sa = st[g];
+            for (int d = 0; d < G; d++) sa -= ss[d * G + g];
+            gr[g] = sa * phi[r * G + g];
+          }
+          scratch[r] = pairwise(gr.data(), G) * p->fsr_volume[r];
+        }
+        double absorption = pairwise(scratch.data(), N);
+        double leak = pairwise(leakage.data(), T);
+        k = rate / (absorption + leak);
+      } else {
+        k *= rate / (double)N;  // computeKeff with fission rates (:2258-2328)
+      }
+      if (stab && i > 0) {
+        // stabilizeFlux (src/CPUSolver.cpp:2730-2805)
+        for (int64_t r = 0; r < N; r++)
+          for (int g = 0; g < G; g++) {
+            const int64_t q = r * G + g;
+            if (o->stab_type == 0) {
+              const double ss = v.sig_s(r)[g * G + g];
+              if (ss < 0.0) {
+                phi[q] += stab_flux[q];
+                phi[q] /= (1.0 - o->stab_factor * ss / v.sig_t(r)[g]);
+              }
+            } else if (o->stab_type == 1) {
+              phi[q] += stab_flux[q];
+              phi[q] /= (1 + stab_ratio[g]);
+            } else {
+              phi[q] += stab_flux[q];
+              phi[q] *= o->stab_factor;
+            }
+          }
+      }
       normalize();
       res = residual(v, phi.data(), old.data(), o->res_type, k, scratch);
       int dk = (int)(1e5 * (k - k_prev));
@@ -376,9 +469,20 @@ int oracle_transport_sweep(const OracleProblem* p, const double* Q, float* psi_s
 int oracle_compute_eigenvalue(const OracleProblem* p, const OracleOptions* o,
                               OracleResult* r, double* flux_out) {
   if (!p || !o || !r || !flux_out) return -1;
-  if (o->keff_from_fission_rates == 0) return -2;  // balance mode not restated
   return o->psi_fp64 ? solve<double>(p, o, r, flux_out, MODE_EIGEN, 1.0)
                      : solve<float>(p, o, r, flux_out, MODE_EIGEN, 1.0);
+}
+
+int oracle_krylov_sweep(const OracleProblem* p, int kind, int allow_negative, double* phi,
+                        float* psi_start, int threads) {
+  if (!p || !phi || !psi_start || (kind != 1 && kind != 2)) return -1;
+  View v = make_view(p);
+  std::vector<double> Q(v.N * v.G), tally(v.N * v.G);
+  fsr_sources(v, phi, 1.0, 0, true, Q.data(), kind);
+  sweep<float>(v, Q.data(), psi_start, tally.data(), threads);
+  std::memcpy(phi, tally.data(), sizeof(double) * v.N * v.G);
+  add_source(v, Q.data(), allow_negative != 0, phi);
+  return 0;
 }
 
 int oracle_compute_flux(const OracleProblem* p, const OracleOptions* o, OracleResult* r,

@@ -59,8 +59,10 @@ typedef struct {
   int res_type;               /* ORACLE_* */
   int psi_fp64;               /* 0: float angular fluxes (reference), 1: double */
   int threads;                /* <=0: OpenMP default */
-  int keff_from_fission_rates;/* 1: reference default */
+  int keff_from_fission_rates;/* 1: reference default; 0: neutron balance */
   int allow_negative_fluxes;  /* 0: reference default */
+  int stab_type;              /* -1 off; 0 DIAGONAL, 1 YAMAMOTO, 2 GLOBAL */
+  double stab_factor;
 } OracleOptions;
 
 typedef struct {
@@ -92,6 +94,13 @@ int oracle_compute_source(const OracleProblem* p, const OracleOptions* o, double
  * transferred outgoing fluxes. */
 int oracle_transport_sweep(const OracleProblem* p, const double* Q, float* psi_start,
                            double* tally_out, int threads);
+
+/* Solver::fissionTransportSweep (kind 1) / scatterTransportSweep (kind 2),
+ * src/Solver.cpp:1282-1297: sources from phi (computeFSRFissionSources /
+ * computeFSRScatterSources, src/CPUSolver.cpp:2030-2104), one sweep from
+ * psi_start (updated in place), addSourceToScalarFlux into phi (in place). */
+int oracle_krylov_sweep(const OracleProblem* p, int kind, int allow_negative, double* phi,
+                        float* psi_start, int threads);
 
 /* CPUSolver::computeFSRSources (src/CPUSolver.cpp:1939-2023). */
 int oracle_compute_fsr_sources(const OracleProblem* p, const double* phi, double keff,

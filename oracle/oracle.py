@@ -35,7 +35,8 @@ class OracleProblemC(C.Structure):
 class OracleOptionsC(C.Structure):
     _fields_ = [("max_iters", C.c_int), ("tolerance", C.c_double), ("res_type", C.c_int),
                 ("psi_fp64", C.c_int), ("threads", C.c_int),
-                ("keff_from_fission_rates", C.c_int), ("allow_negative_fluxes", C.c_int)]
+                ("keff_from_fission_rates", C.c_int), ("allow_negative_fluxes", C.c_int),
+                ("stab_type", C.c_int), ("stab_factor", C.c_double)]
 
 
 class OracleResultC(C.Structure):
@@ -68,6 +69,9 @@ def lib():
         h.oracle_compute_fsr_sources.restype = C.c_int
         h.oracle_add_source_to_scalar_flux.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         h.oracle_add_source_to_scalar_flux.restype = C.c_int
+        h.oracle_krylov_sweep.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_int]
+        h.oracle_krylov_sweep.restype = C.c_int
         h.oracle_exp_f1.argtypes = [C.c_double]
         h.oracle_exp_f1.restype = C.c_double
         h.oracle_pairwise_sum.argtypes = [C.c_void_p, C.c_int64]
@@ -136,16 +140,20 @@ class OracleProblem:
                    nu_sigma_f=tg.nu_sigma_f, chi=tg.chi, fsr_material=tg.fsr_material,
                    fsr_volume=tg.fsr_volume, fixed_source=fixed_source)
 
-    def _opts(self, max_iters, tol, res_type, psi_fp64, threads):
+    def _opts(self, max_iters, tol, res_type, psi_fp64, threads, balance=False,
+              stabilization=None):
         o = OracleOptionsC()
         o.max_iters, o.tolerance, o.res_type = int(max_iters), float(tol), int(res_type)
         o.psi_fp64, o.threads = int(psi_fp64), int(threads)
-        o.keff_from_fission_rates, o.allow_negative_fluxes = 1, 0
+        o.keff_from_fission_rates, o.allow_negative_fluxes = 0 if balance else 1, 0
+        o.stab_type, o.stab_factor = -1, 0.0
+        if stabilization is not None:
+            o.stab_factor, o.stab_type = float(stabilization[0]), int(stabilization[1])
         return o
 
     def _run(self, fn, *extra, max_iters=1000, tol=1e-5, res_type=FISSION_SOURCE,
-             psi_fp64=False, threads=0) -> OracleSolution:
-        o = self._opts(max_iters, tol, res_type, psi_fp64, threads)
+             psi_fp64=False, threads=0, balance=False, stabilization=None) -> OracleSolution:
+        o = self._opts(max_iters, tol, res_type, psi_fp64, threads, balance, stabilization)
         r = OracleResultC()
         flux = np.zeros(self.N * self.G)
         rc = fn(C.byref(self.c), C.byref(o), *extra, C.byref(r), flux.ctypes.data)
@@ -174,6 +182,17 @@ class OracleProblem:
         if rc != 0:
             raise RuntimeError("oracle sweep failed")
         return tally, p
+
+    def krylov_sweep(self, kind: int, phi: np.ndarray, psi: np.ndarray, threads: int = 0):
+        """kind 1: fissionTransportSweep, 2: scatterTransportSweep.  Returns
+        (new flux, start fluxes after the sweep)."""
+        f = np.array(phi, np.float64, copy=True).ravel()
+        p = np.array(psi, np.float32, copy=True).ravel()
+        rc = lib().oracle_krylov_sweep(C.byref(self.c), int(kind), 0, f.ctypes.data,
+                                       p.ctypes.data, int(threads))
+        if rc != 0:
+            raise RuntimeError("oracle Krylov sweep failed")
+        return f, p
 
     def fsr_sources(self, phi: np.ndarray, keff: float, iteration: int = 100) -> np.ndarray:
         f = np.ascontiguousarray(phi, np.float64).ravel()

@@ -150,7 +150,8 @@ class TimedCPUSolver : public CPUSolver {
 int main(int argc, char** argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: %s problem.txt out.bin [--threads N] [--no-solve] "
-                    "[--max-iters N] [--tol X] [--sweeps N]\n", argv[0]);
+                    "[--max-iters N] [--tol X] [--sweeps N] [--balance] "
+                    "[--stabilize FACTOR TYPE] [--krylov]\n", argv[0]);
     return 2;
   }
   int threads = 1;
@@ -158,12 +159,21 @@ int main(int argc, char** argv) {
   int max_iters_override = -1;
   double tol_override = -1;
   int timed_sweeps = 0;
+  bool balance = false, krylov = false;
+  double stab_factor = 0.0;
+  int stab_type = -1;
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
     else if (!strcmp(argv[a], "--max-iters")) max_iters_override = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--tol")) tol_override = atof(argv[++a]);
     else if (!strcmp(argv[a], "--sweeps")) timed_sweeps = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--balance")) balance = true;
+    else if (!strcmp(argv[a], "--stabilize")) {
+      stab_factor = atof(argv[++a]);
+      stab_type = atoi(argv[++a]);
+    }
+    else if (!strcmp(argv[a], "--krylov")) krylov = true;
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -322,6 +332,8 @@ int main(int argc, char** argv) {
   TimedCPUSolver solver(tg);
   solver.setNumThreads(threads);
   solver.setConvergenceThreshold(s.tol);
+  if (balance) solver.setKeffFromNeutronBalance();
+  if (stab_type >= 0) solver.stabilizeTransport(stab_factor, (stabilizationType)stab_type);
 
   long N = geometry->getNumFSRs();
   w.l1("num_fsrs", N);
@@ -372,6 +384,34 @@ int main(int argc, char** argv) {
     w.d("fsr_centroid", cen);
     w.d("fsr_point", pt);
     w.i("fsr_material", fm);
+  }
+
+  /* Optional: Krylov operator sweeps on the converged flux (the pattern of
+   * openmoc/krylov.py: initializeSolver zeroes the angular fluxes, then
+   * setFluxes + fission/scatterTransportSweep + getFluxes). */
+  if (solve && krylov) {
+    std::vector<double> phi0(N * G);
+    for (long r = 0; r < N; r++)
+      for (int g = 0; g < G; g++) phi0[r * G + g] = solver.getFlux(r, g + 1);
+    solver.initializeSolver(FORWARD);
+    std::vector<FP_PRECISION> b1(phi0.begin(), phi0.end()), b2(b1), b3(b1), out(N * G);
+    auto dump = [&](const char* name) {
+      std::vector<double> v(out.begin(), out.end());
+      w.d(name, v);
+    };
+    w.d("krylov_phi0", phi0);
+    solver.setFluxes(b1.data(), (int)(N * G));
+    solver.fissionTransportSweep();
+    solver.getFluxes(out.data(), (int)(N * G));
+    dump("krylov_fission_1");
+    solver.setFluxes(b2.data(), (int)(N * G));
+    solver.scatterTransportSweep();
+    solver.getFluxes(out.data(), (int)(N * G));
+    dump("krylov_scatter");
+    solver.setFluxes(b3.data(), (int)(N * G));
+    solver.fissionTransportSweep();
+    solver.getFluxes(out.data(), (int)(N * G));
+    dump("krylov_fission_2");
   }
 
   /* Optional: time a fixed number of extra sweeps on the converged state

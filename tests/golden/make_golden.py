@@ -10,6 +10,11 @@ Outputs (data only: inputs and the reference's outputs):
   c5g7_pin_cell_ref.npz     keff/flux/iterations (matches the reference's
                             tests/test_forward_pin_cell/results_true.dat)
   hom_inf_medium_ref.npz    keff/flux/iterations
+  lattice_3x3_vac_7g_ref.npz  default and neutron-balance keff/flux/iterations,
+                            Krylov operator sweeps (fission, scatter, fission)
+                            from the converged flux
+  pin_cell_7g_tc_ref.npz    keff/flux/iterations without and with each
+                            transport stabilisation type
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 """
 import sys
@@ -27,13 +32,13 @@ from oracle import mocdump, oracle  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
-def run(name: str, threads: int = 8):
+def run(name: str, threads: int = 8, extra=()):
     p = workloads.get(name)
     txt = OUT / f"{name}.txt"
     txt.write_text(p.to_text())
     with tempfile.TemporaryDirectory() as td:
         dump = Path(td) / "ref.bin"
-        oracle.run_reference(str(txt), str(dump), threads=threads, fp="f64")
+        oracle.run_reference(str(txt), str(dump), threads=threads, fp="f64", extra=extra)
         return p, mocdump.read(str(dump))
 
 
@@ -69,6 +74,25 @@ def main():
         trk_num_segments=ns.astype(np.int32), trk_length_sum=len_sum, trk_fsr_sum=fsr_sum,
         trk_next_fwd=d["trk_next_fwd"], trk_next_bwd=d["trk_next_bwd"],
         fsr_volume=d["fsr_volume"], fsr_material=d["fsr_material"], **base(d))
+    _, d = run("lattice_3x3_vac_7g", threads=1, extra=("--krylov",))
+    _, db = run("lattice_3x3_vac_7g", threads=1, extra=("--balance",))
+    np.savez_compressed(
+        OUT / "lattice_3x3_vac_7g_ref.npz", flux=d["flux"],
+        trk_num_segments=d["trk_num_segments"].astype(np.int32),
+        balance_keff=np.float64(db["keff"]), balance_flux=db["flux"],
+        balance_num_iterations=np.int64(db["num_iterations"]),
+        krylov_phi0=d["krylov_phi0"], krylov_fission_1=d["krylov_fission_1"],
+        krylov_scatter=d["krylov_scatter"], krylov_fission_2=d["krylov_fission_2"], **base(d))
+
+    _, d = run("pin_cell_7g_tc", threads=1)
+    out = dict(flux=d["flux"], **base(d))
+    for tag, factor, typ in (("diagonal", 1.0, 0), ("yamamoto", 0.5, 1), ("global", 0.5, 2)):
+        _, ds = run("pin_cell_7g_tc", threads=1, extra=("--stabilize", str(factor), str(typ)))
+        out[f"{tag}_keff"] = np.float64(ds["keff"])
+        out[f"{tag}_flux"] = ds["flux"]
+        out[f"{tag}_num_iterations"] = np.int64(ds["num_iterations"])
+        out[f"{tag}_factor"] = np.float64(factor)
+    np.savez_compressed(OUT / "pin_cell_7g_tc_ref.npz", **out)
     print("golden fixtures written to", OUT)
 
 

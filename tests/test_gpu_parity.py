@@ -195,3 +195,67 @@ def test_state_errors():
         s.computeEigenvalue(10)
     with pytest.raises(RuntimeError):
         s.setConvergenceThreshold(-1.0)
+
+
+# --- neutron-balance keff, transport stabilisation, Krylov operator sweeps ---
+
+def test_balance_keff_matches_reference(tracks, golden):
+    g = golden("lattice_3x3_vac_7g")
+    s = solver_for(tracks("lattice_3x3_vac_7g"))
+    s.setKeffFromNeutronBalance()
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["balance_keff"])) < K_TOL
+    assert abs(s.getNumIterations() - int(g["balance_num_iterations"])) <= 1
+    assert rel_l2(s.getFluxes(), g["balance_flux"]) < FLUX_RL2
+
+
+def test_default_keff_on_leaking_lattice(tracks, golden):
+    g = golden("lattice_3x3_vac_7g")
+    s = solver_for(tracks("lattice_3x3_vac_7g"))
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+
+
+@pytest.mark.parametrize("tag,typ", [("diagonal", om.DIAGONAL), ("yamamoto", om.YAMAMOTO),
+                                     ("global", om.GLOBAL)])
+def test_stabilization_matches_reference(tracks, golden, tag, typ):
+    g = golden("pin_cell_7g_tc")
+    s = solver_for(tracks("pin_cell_7g_tc"))
+    s.stabilizeTransport(float(g[f"{tag}_factor"]), typ)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g[f"{tag}_keff"])) < K_TOL
+    assert abs(s.getNumIterations() - int(g[f"{tag}_num_iterations"])) <= 2
+    assert rel_l2(s.getFluxes(), g[f"{tag}_flux"]) < FLUX_RL2
+
+
+def test_krylov_sweeps_match_reference(tracks, golden):
+    g = golden("lattice_3x3_vac_7g")
+    s = solver_for(tracks("lattice_3x3_vac_7g"))
+    s.computeEigenvalue(1000)
+    phi0 = g["krylov_phi0"]
+    s.setFluxes(phi0)
+    s.fissionTransportSweep()
+    assert rel_l2(s.getFluxes(), g["krylov_fission_1"]) < 2e-5
+    s.setFluxes(phi0)
+    s.scatterTransportSweep()
+    assert rel_l2(s.getFluxes(), g["krylov_scatter"]) < 2e-5
+    s.setFluxes(phi0)
+    s.fissionTransportSweep()
+    assert rel_l2(s.getFluxes(), g["krylov_fission_2"]) < 2e-5
+
+
+def test_krylov_sweeps_keep_start_fluxes_reflective(tracks):
+    # reflective pin cell: the second operator sweep starts from the angular
+    # fluxes the first one left (as CPUSolver::_start_flux does)
+    tg = tracks("pin_cell_7g")
+    op = orc.OracleProblem.from_track_generator(tg)
+    rng = np.random.default_rng(11)
+    phi0 = rng.uniform(0.5, 1.5, tg.getNumFSRs() * tg.getNumEnergyGroups())
+    s = solver_for(tg)
+    psi = np.zeros(2 * op.T * op.P * op.G, np.float32)
+    for kind in (1, 2, 1, 1):
+        s.setFluxes(phi0)
+        (s.fissionTransportSweep if kind == 1 else s.scatterTransportSweep)()
+        ref, psi = op.krylov_sweep(kind, phi0, psi)
+        assert rel_l2(s.getFluxes(), ref) < 2e-5

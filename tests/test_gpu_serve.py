@@ -1,0 +1,110 @@
+"""Serving engine (continuous batching) through the C ABI: each result equals
+the plain single solve of the same inputs, alone and in a mix, and stays
+within the parity bar of the oracle."""
+import numpy as np
+import pytest
+
+import openmoc_amd as om
+from openmoc_amd import serve, workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def perturbed(name, seed):
+    p = workloads.get(name)
+    rng = np.random.default_rng(seed)
+    p.materials = [workloads.perturb_material(m, rng) for m in p.materials]
+    return p
+
+
+def plain_solve(tg, problem, tol=1e-5):
+    s = om.HipSolver(tg)
+    st, ss, nsf, chi = serve.material_arrays(problem)
+    s.setMaterials(st, ss, nsf, chi)
+    s.setConvergenceThreshold(tol)
+    s.computeEigenvalue(1000)
+    return s.getKeff(), s.getNumIterations(), s.getFluxes()
+
+
+def test_served_results_equal_plain_solves(tracks):
+    tg = tracks("pin_cell_7g")
+    probs = [perturbed("pin_cell_7g", seed) for seed in range(6)]
+    with serve.ServeEngine(max_active=4) as eng:
+        gid = eng.add_geometry(tg)
+        ids = [eng.submit(gid, p, want_flux=True) for p in probs]
+        res = [eng.wait(i, timeout=120) for i in ids]
+    for p, r in zip(probs, res):
+        k, it, flux = plain_solve(tg, p)
+        assert r.keff == k
+        assert r.num_iterations == it
+        np.testing.assert_array_equal(r.flux.ravel(), flux)
+        assert r.latency_s >= r.solve_s > 0
+
+
+def test_result_same_alone_and_in_the_mix(tracks):
+    tg_pin = tracks("pin_cell_7g")
+    tg_lat = tracks("lattice_3x3_vac_7g")
+    probe = perturbed("pin_cell_7g", 100)
+    with serve.ServeEngine(max_active=1) as eng:
+        g = eng.add_geometry(tg_pin)
+        alone = eng.wait(eng.submit(g, probe, want_flux=True), timeout=120)
+    with serve.ServeEngine(max_active=8) as eng:
+        g_pin = eng.add_geometry(tg_pin)
+        g_lat = eng.add_geometry(tg_lat)
+        ids = []
+        for seed in range(12):
+            if seed == 5:
+                probe_id = eng.submit(g_pin, probe, want_flux=True)
+            ids.append(eng.submit(g_lat if seed % 2 else g_pin,
+                                  perturbed("lattice_3x3_vac_7g" if seed % 2 else "pin_cell_7g",
+                                            seed)))
+        mixed = eng.wait(probe_id, timeout=300)
+        for i in ids:
+            eng.wait(i, timeout=300)
+        st = eng.stats()
+    assert mixed.keff == alone.keff
+    assert mixed.num_iterations == alone.num_iterations
+    np.testing.assert_array_equal(mixed.flux, alone.flux)
+    assert st["failed"] == 0
+
+
+def test_served_result_matches_oracle(tracks):
+    tg = tracks("pin_cell_7g")
+    p = perturbed("pin_cell_7g", 7)
+    with serve.ServeEngine() as eng:
+        gid = eng.add_geometry(tg)
+        r = eng.wait(eng.submit(gid, p, want_flux=True), timeout=120)
+    st, ss, nsf, chi = serve.material_arrays(p)
+    op = orc.OracleProblem(
+        num_azim=tg.info.num_azim, num_polar=tg.info.num_polar, seg_offset=tg.seg_offset,
+        seg_length=tg.seg_length, seg_fsr=tg.seg_fsr, azim=tg.azim, next_fwd=tg.next_fwd,
+        next_bwd=tg.next_bwd, next_fwd_is_fwd=tg.next_fwd_is_fwd,
+        next_bwd_is_fwd=tg.next_bwd_is_fwd, bc_fwd=tg.bc_fwd, bc_bwd=tg.bc_bwd,
+        total_weight=tg.total_weight, sin_theta=tg.sin_theta, sigma_t=st, sigma_s=ss,
+        nu_sigma_f=nsf, chi=chi, fsr_material=tg.fsr_material, fsr_volume=tg.fsr_volume)
+    ref = op.eigenvalue(max_iters=1000, tol=1e-5)
+    assert abs(r.keff - ref.keff) < 1e-5
+    assert np.linalg.norm(r.flux - ref.flux) / np.linalg.norm(ref.flux) < 1e-4
+
+
+def test_memory_admission_serialises_when_budget_is_one_solve(tracks):
+    tg = tracks("pin_cell_7g")
+    probs = [perturbed("pin_cell_7g", 20 + s) for s in range(4)]
+    # a budget below two solves: requests are admitted one at a time
+    with serve.ServeEngine(max_active=8, mem_budget_bytes=1) as eng:
+        gid = eng.add_geometry(tg)
+        res = [eng.wait(eng.submit(gid, p), timeout=120) for p in probs]
+    for p, r in zip(probs, res):
+        assert r.keff == plain_solve(tg, p)[0]
+
+
+def test_submit_validation(tracks):
+    tg = tracks("pin_cell_7g")
+    with serve.ServeEngine() as eng:
+        gid = eng.add_geometry(tg)
+        st, ss, nsf, chi = serve.material_arrays(workloads.get("pin_cell_7g"))
+        with pytest.raises(RuntimeError):
+            eng.submit(gid + 5, (st, ss, nsf, chi))
+        with pytest.raises(RuntimeError):
+            eng.submit(gid, (st[:, :3], ss, nsf, chi))

@@ -152,3 +152,91 @@ def test_particle_balance_reflective(tracks):
                      for r in range(N))
     source = (fixed * tg.fsr_volume[:, None]).sum()
     assert abs(absorption / source - 1) < 2e-3
+
+
+# --- neutron-balance keff, transport stabilisation, Krylov operator sweeps ---
+
+def test_tracks_of_new_fixtures_match_reference(tracks, golden):
+    for name in ("lattice_3x3_vac_7g", "pin_cell_7g_tc"):
+        g = golden(name)
+        tg = tracks(name)
+        assert tg.getNumTracks() == int(g["num_tracks"])
+        assert tg.getNumSegments() == int(g["num_segments"])
+        assert tg.getNumFSRs() == int(g["num_fsrs"])
+    g = golden("lattice_3x3_vac_7g")
+    tg = tracks("lattice_3x3_vac_7g")
+    np.testing.assert_array_equal(np.diff(tg.seg_offset), g["trk_num_segments"])
+
+
+def test_oracle_balance_keff_matches_reference(tracks, golden):
+    g = golden("lattice_3x3_vac_7g")
+    op = orc.OracleProblem.from_track_generator(tracks("lattice_3x3_vac_7g"))
+    sol = op.eigenvalue(max_iters=1000, tol=1e-5, balance=True)
+    assert abs(sol.keff - float(g["balance_keff"])) < 1e-8
+    assert sol.num_iterations == int(g["balance_num_iterations"])
+    assert np.linalg.norm(sol.flux.ravel() - g["balance_flux"]) / np.linalg.norm(g["balance_flux"]) < 1e-7
+    # the vacuum leakage is what separates it from the fission-rate keff
+    assert abs(float(g["balance_keff"]) - float(g["keff"])) > 0.1
+
+
+def test_oracle_balance_keff_equals_fission_rate_keff_without_leakage(tracks):
+    # reflective infinite medium: no leakage, so at convergence
+    # fission / absorption equals the fission-rate eigenvalue k_inf
+    # (FP64 angular fluxes, so that only the iteration error remains)
+    op = orc.OracleProblem.from_track_generator(tracks("hom_inf_medium"))
+    kb = op.eigenvalue(max_iters=5000, tol=1e-12, balance=True, psi_fp64=True).keff
+    kf = op.eigenvalue(max_iters=5000, tol=1e-12, psi_fp64=True).keff
+    s11, s12, s22 = 0.1, 0.117, 1.42
+    k_inf = (0.0015 + 0.325 * s12 / (1.604 - s22)) / (0.2208 - s11 - s12 + s12)
+    assert abs(kb - k_inf) < 1e-9
+    assert abs(kf - k_inf) < 1e-8
+
+
+@pytest.mark.parametrize("tag,typ", [("diagonal", 0), ("yamamoto", 1), ("global", 2)])
+def test_oracle_stabilization_matches_reference(tracks, golden, tag, typ):
+    g = golden("pin_cell_7g_tc")
+    op = orc.OracleProblem.from_track_generator(tracks("pin_cell_7g_tc"))
+    sol = op.eigenvalue(max_iters=1000, tol=1e-5,
+                        stabilization=(float(g[f"{tag}_factor"]), typ))
+    assert abs(sol.keff - float(g[f"{tag}_keff"])) < 1e-9
+    assert sol.num_iterations == int(g[f"{tag}_num_iterations"])
+    ref = g[f"{tag}_flux"]
+    assert np.linalg.norm(sol.flux.ravel() - ref) / np.linalg.norm(ref) < 1e-9
+
+
+def test_oracle_stabilization_changes_the_iteration(golden):
+    # damping changes the path (iteration counts differ from the plain solve),
+    # DIAGONAL only acts through the negative in-group scattering
+    g = golden("pin_cell_7g_tc")
+    assert int(g["yamamoto_num_iterations"]) != int(g["num_iterations"])
+    assert int(g["global_num_iterations"]) != int(g["num_iterations"])
+    assert abs(float(g["diagonal_keff"]) - float(g["keff"])) < 1e-5
+
+
+def test_oracle_krylov_sweeps_match_reference(tracks, golden):
+    g = golden("lattice_3x3_vac_7g")
+    op = orc.OracleProblem.from_track_generator(tracks("lattice_3x3_vac_7g"))
+    psi = np.zeros(2 * op.T * op.P * op.G, np.float32)
+    phi0 = g["krylov_phi0"]
+    out = {}
+    out["krylov_fission_1"], psi = op.krylov_sweep(1, phi0, psi)
+    out["krylov_scatter"], psi = op.krylov_sweep(2, phi0, psi)
+    out["krylov_fission_2"], psi = op.krylov_sweep(1, phi0, psi)
+    for k, v in out.items():
+        assert np.linalg.norm(v - g[k]) / np.linalg.norm(g[k]) < 1e-12, k
+
+
+def test_oracle_krylov_sweeps_are_linear(tracks, golden):
+    # all-vacuum problem: start fluxes stay zero, so the sweep is linear in
+    # the source and fission-only + scatter-only equals one sweep on the
+    # total (k = 1, no fixed source) source
+    g = golden("lattice_3x3_vac_7g")
+    op = orc.OracleProblem.from_track_generator(tracks("lattice_3x3_vac_7g"))
+    phi0 = g["krylov_phi0"]
+    zero = np.zeros(2 * op.T * op.P * op.G, np.float32)
+    f, _ = op.krylov_sweep(1, phi0, zero)
+    s, _ = op.krylov_sweep(2, phi0, zero)
+    Q = op.fsr_sources(phi0, 1.0)
+    tally, _ = op.transport_sweep(Q, zero)
+    total = op.add_source(tally, Q)
+    np.testing.assert_allclose(f + s, total, rtol=1e-5)
