@@ -91,6 +91,10 @@ int moc_tracks_export_tracks(const moc_tracks_t* tracks, int64_t* seg_offset /*T
                              uint8_t* bc_fwd, uint8_t* bc_bwd);
 int moc_tracks_export_segments(const moc_tracks_t* tracks, double* length /*S*/,
                                int32_t* fsr /*S*/);
+/* Start point (x, y) of every segment [2S] (Geometry::segmentize2D records
+ * it, src/Geometry.cpp:2196-2210; the linear source re-centres it on the
+ * FSR centroid). */
+int moc_tracks_export_segment_starts(const moc_tracks_t* tracks, double* start_xy /*2S*/);
 int moc_tracks_export_fsrs(const moc_tracks_t* tracks, int32_t* material /*N*/,
                            double* volume /*N*/, double* centroid /*2N*/, int64_t* key /*N*/);
 /* Quadrature after laydown: phi/spacing/azim_weight [num_azim/2],
@@ -128,6 +132,11 @@ typedef struct {
   const uint8_t* bc_bwd;            /* [T] */
   const double* total_weight;       /* [num_azim/2 * num_polar] Quadrature w(a,p) */
   const double* sin_theta;          /* [num_polar] */
+  /* Needed only for the linear source (may be NULL otherwise): */
+  const double* azim_phi;           /* [num_azim/2] track angle of each azimuthal index */
+  const double* azim_weight;        /* [num_azim/2] azimuthal spacing * azimuthal weight */
+  const double* polar_weight;       /* [num_polar] Quadrature::getPolarWeight */
+  const double* seg_start;          /* [2S] start point (x, y) of each segment */
 } moc_track_data_t;
 
 /* device < 0: current device. */

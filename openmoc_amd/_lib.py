@@ -41,7 +41,9 @@ class TrackData(C.Structure):
                 ("next_fwd", C.c_void_p), ("next_bwd", C.c_void_p),
                 ("next_fwd_is_fwd", C.c_void_p), ("next_bwd_is_fwd", C.c_void_p),
                 ("bc_fwd", C.c_void_p), ("bc_bwd", C.c_void_p),
-                ("total_weight", C.c_void_p), ("sin_theta", C.c_void_p)]
+                ("total_weight", C.c_void_p), ("sin_theta", C.c_void_p),
+                ("azim_phi", C.c_void_p), ("azim_weight", C.c_void_p),
+                ("polar_weight", C.c_void_p), ("seg_start", C.c_void_p)]
 
 
 class RequestResult(C.Structure):
@@ -59,6 +61,7 @@ _SIGS = [
     ("moc_tracks_info", C.c_int, [_P, C.POINTER(TracksInfo)]),
     ("moc_tracks_export_tracks", C.c_int, [_P] + [_P] * 11),
     ("moc_tracks_export_segments", C.c_int, [_P, _P, _P]),
+    ("moc_tracks_export_segment_starts", C.c_int, [_P, _P]),
     ("moc_tracks_export_fsrs", C.c_int, [_P, _P, _P, _P, _P]),
     ("moc_tracks_export_quadrature", C.c_int, [_P] * 7),
     ("moc_tracks_export_materials", C.c_int, [_P] * 5),

@@ -141,6 +141,12 @@ int moc_tracks_export_segments(const moc_tracks_t* t, double* length, int32_t* f
   return MOC_OK;
 }
 
+int moc_tracks_export_segment_starts(const moc_tracks_t* t, double* start_xy) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  copy_out(start_xy, t->ts.seg_start);
+  return MOC_OK;
+}
+
 int moc_tracks_export_fsrs(const moc_tracks_t* t, int32_t* material, double* volume,
                            double* centroid, int64_t* key) {
   if (!t) return fail(MOC_ERR_INVALID, "null argument");

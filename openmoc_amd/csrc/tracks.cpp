@@ -235,7 +235,7 @@ inline int sector_of(int ns, double dx, double dy) {
   return std::min(std::max(s, 0), ns - 1);
 }
 
-struct RawSeg { double length; int64_t key; int material; double mx, my; };
+struct RawSeg { double length; int64_t key; int material; double mx, my, sx, sy; };
 
 struct Tracer {
   const Problem& p;
@@ -358,6 +358,10 @@ struct Tracer {
       RawSeg s;
       classify(xs, ys, s.key, s.material);
       s.length = (u == kTinyMove) ? next - kTinyMove : next - u;
+      // segment start point: the marching point (Geometry::segmentize2D
+      // records the nudged start, src/Geometry.cpp:2196-2202)
+      s.sx = xs;
+      s.sy = ys;
       double um = 0.5 * (u + std::min(next, L));
       s.mx = x0 + um * ux;
       s.my = y0 + um * uy;
@@ -541,6 +545,7 @@ TrackSet generate_tracks(const Problem& p, int threads) {
   for (long t = 0; t < T; t++) S += (long)per_track[t].size();
   ts.seg_length.resize(S);
   ts.seg_fsr.resize(S);
+  ts.seg_start.resize(2 * S);
   long s = 0;
   for (long t = 0; t < T; t++) {
     ts.seg_offset[t] = s;
@@ -553,6 +558,8 @@ TrackSet generate_tracks(const Problem& p, int threads) {
       }
       ts.seg_length[s] = r.length;
       ts.seg_fsr[s] = id;
+      ts.seg_start[2 * s] = r.sx;
+      ts.seg_start[2 * s + 1] = r.sy;
       s++;
     }
   }
@@ -597,16 +604,25 @@ void split_segments(TrackSet& ts, const Problem& p, double max_tau) {
   for (size_t m = 0; m < p.materials.size(); m++)
     for (double v : p.materials[m].sigma_t) max_sig[m] = std::max(max_sig[m], v);
   std::vector<int64_t> off(ts.num_tracks + 1, 0);
-  std::vector<double> len;
+  std::vector<double> len, start;
   std::vector<int32_t> fsr;
   for (long t = 0; t < ts.num_tracks; t++) {
     off[t] = (int64_t)len.size();
+    const double ux = std::cos(ts.phi[t]), uy = std::sin(ts.phi[t]);
     for (int64_t k = ts.seg_offset[t]; k < ts.seg_offset[t + 1]; k++) {
       double tau = ts.seg_length[k] * max_sig[ts.fsr_material[ts.seg_fsr[k]]];
       int cuts = std::max(1, (int)std::ceil(tau / max_tau));
+      // sub-segments start where the previous one ended
+      // (src/TrackTraversingAlgorithms.cpp:238-262)
+      double x = ts.seg_start[2 * k], y = ts.seg_start[2 * k + 1];
       for (int c = 0; c < cuts; c++) {
-        len.push_back(ts.seg_length[k] / cuts);
+        const double l = ts.seg_length[k] / cuts;
+        len.push_back(l);
         fsr.push_back(ts.seg_fsr[k]);
+        start.push_back(x);
+        start.push_back(y);
+        x += l * ux;
+        y += l * uy;
       }
     }
   }
@@ -614,6 +630,7 @@ void split_segments(TrackSet& ts, const Problem& p, double max_tau) {
   ts.seg_offset.swap(off);
   ts.seg_length.swap(len);
   ts.seg_fsr.swap(fsr);
+  ts.seg_start.swap(start);
   ts.max_optical_length = std::min(ts.max_optical_length, max_tau);
 }
 

@@ -91,6 +91,7 @@ struct TrackSet {
   std::vector<int64_t> seg_offset;          // [T+1]
   std::vector<double> seg_length;           // [S]
   std::vector<int32_t> seg_fsr;             // [S]
+  std::vector<double> seg_start;            // [2S] start point (x, y) of each segment
   long num_fsrs = 0;
   std::vector<int32_t> fsr_material;        // [N]
   std::vector<double> fsr_volume;           // [N]

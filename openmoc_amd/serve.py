@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import FISSION_SOURCE, check, lib
-from .solver import TrackGenerator, _ptr
+from .solver import TrackGenerator, _ptr, _track_data
 from .workloads import Problem
 
 ZERO_SIGMA_T = 1e-6      # reference ZERO_SIGMA_T (src/Material.cpp:628-650)
@@ -88,15 +88,7 @@ class ServeEngine:
         """Uploads the generator's tracks and FSRs (its materials fix the group
         and material counts).  Returns the geometry id."""
         i = tg.info
-        td = _lib.TrackData()
-        td.num_azim, td.num_polar = i.num_azim, i.num_polar
-        td.num_tracks, td.num_segments = i.num_tracks, i.num_segments
-        keep = [tg.seg_offset, tg.seg_length, tg.seg_fsr, tg.azim, tg.next_fwd, tg.next_bwd,
-                tg.next_fwd_is_fwd, tg.next_bwd_is_fwd, tg.bc_fwd, tg.bc_bwd,
-                np.ascontiguousarray(tg.total_weight), tg.sin_theta]
-        (td.seg_offset, td.seg_length, td.seg_fsr, td.azim, td.next_fwd, td.next_bwd,
-         td.next_fwd_is_fwd, td.next_bwd_is_fwd, td.bc_fwd, td.bc_bwd, td.total_weight,
-         td.sin_theta) = [a.ctypes.data for a in keep]
+        td, keep = _track_data(tg)
         gid = C.c_int32()
         check(lib().moc_engine_add_geometry(
             self._h, C.byref(td), i.num_fsrs, _ptr(tg.fsr_material), _ptr(tg.fsr_volume),

@@ -33,6 +33,25 @@ __all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "SCALAR_FLUX", "FISSION_S
            "MOCError"]
 
 
+def _track_data(tg: "TrackGenerator"):
+    """moc_track_data_t of a generator's tracks, and the arrays it points
+    into (keep them alive while the struct is in use)."""
+    i = tg.info
+    td = _lib.TrackData()
+    td.num_azim, td.num_polar = i.num_azim, i.num_polar
+    td.num_tracks, td.num_segments = i.num_tracks, i.num_segments
+    keep = [tg.seg_offset, tg.seg_length, tg.seg_fsr, tg.azim, tg.next_fwd, tg.next_bwd,
+            tg.next_fwd_is_fwd, tg.next_bwd_is_fwd, tg.bc_fwd, tg.bc_bwd,
+            np.ascontiguousarray(tg.total_weight), tg.sin_theta, tg.phi,
+            np.ascontiguousarray(tg.azim_spacing * tg.azim_weight), tg.polar_weight,
+            np.ascontiguousarray(tg.seg_start)]
+    (td.seg_offset, td.seg_length, td.seg_fsr, td.azim, td.next_fwd, td.next_bwd,
+     td.next_fwd_is_fwd, td.next_bwd_is_fwd, td.bc_fwd, td.bc_bwd, td.total_weight,
+     td.sin_theta, td.azim_phi, td.azim_weight, td.polar_weight,
+     td.seg_start) = [a.ctypes.data for a in keep]
+    return td, keep
+
+
 def _ptr(a: Optional[np.ndarray]):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
@@ -89,6 +108,8 @@ class TrackGenerator:
         self.seg_length = np.zeros(S)
         self.seg_fsr = np.zeros(S, np.int32)
         check(lib().moc_tracks_export_segments(self._h, _ptr(self.seg_length), _ptr(self.seg_fsr)))
+        self.seg_start = np.zeros((S, 2))
+        check(lib().moc_tracks_export_segment_starts(self._h, _ptr(self.seg_start)))
         self.fsr_material = np.zeros(N, np.int32)
         self.fsr_volume = np.zeros(N)
         self.fsr_centroid = np.zeros((N, 2))
@@ -166,15 +187,7 @@ class HipSolver:
             raise MOCError(-2, "Unable to set the TrackGenerator for the Solver since the "
                                "TrackGenerator has not yet generated tracks")
         i = tg.info
-        td = _lib.TrackData()
-        td.num_azim, td.num_polar = i.num_azim, i.num_polar
-        td.num_tracks, td.num_segments = i.num_tracks, i.num_segments
-        keep = [tg.seg_offset, tg.seg_length, tg.seg_fsr, tg.azim, tg.next_fwd, tg.next_bwd,
-                tg.next_fwd_is_fwd, tg.next_bwd_is_fwd, tg.bc_fwd, tg.bc_bwd,
-                np.ascontiguousarray(tg.total_weight), tg.sin_theta]
-        (td.seg_offset, td.seg_length, td.seg_fsr, td.azim, td.next_fwd, td.next_bwd,
-         td.next_fwd_is_fwd, td.next_bwd_is_fwd, td.bc_fwd, td.bc_bwd, td.total_weight,
-         td.sin_theta) = [a.ctypes.data for a in keep]
+        td, keep = _track_data(tg)
         check(lib().moc_solver_set_materials(
             self._h, i.num_materials, i.num_groups, _ptr(tg.sigma_t), _ptr(tg.sigma_s),
             _ptr(tg.nu_sigma_f), _ptr(tg.chi)))

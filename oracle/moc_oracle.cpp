@@ -437,6 +437,361 @@ int solve(const OracleProblem* p, const OracleOptions* o, OracleResult* out, dou
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Linear source (2D), CPULSSolver
+// ---------------------------------------------------------------------------
+
+// 5/6 rational fit of G(x) = 1/x - (1 - e^-x)/x^2 (src/exponentials.h:100-145)
+inline double exp_g(double x) {
+  const double p0 = 0.5, p1 = 1.76558112351595e-1, p2 = 4.041584305811143e-2,
+               p3 = 6.178333902037397e-3, p4 = 6.429894635552992e-4,
+               p5 = 6.064409107557148e-5;
+  const double d1 = 6.864462055546078e-1, d2 = 2.263358514260129e-1,
+               d3 = 4.721469893686252e-2, d4 = 6.883236664917246e-3,
+               d5 = 7.036272419147752e-4, d6 = 6.064409107557148e-5;
+  double den = d6 * x + d5;
+  den = den * x + d4;
+  den = den * x + d3;
+  den = den * x + d2;
+  den = den * x + d1;
+  den = den * x + 1.0;
+  den = 1.0 / den;
+  double num = p5 * x + p4;
+  num = num * x + p3;
+  num = num * x + p2;
+  num = num * x + p1;
+  num = num * x + p0;
+  return num * den;
+}
+
+// 5/5 rational fit of G2(x) (src/exponentials.h:280-323)
+inline double exp_g2(double x) {
+  const double a1 = -8.335775885589858e-2, a2 = -3.603942303847604e-3,
+               a3 = 3.7673183263550827e-3, a4 = 1.124183494990467e-5,
+               a5 = 1.6837426505799449e-4;
+  const double b1 = 7.454048371823628e-1, b2 = 2.3794300531408347e-1,
+               b3 = 5.367250964303789e-2, b4 = 6.125197988351906e-3,
+               b5 = 1.0102514456857377e-3;
+  double num = a5 * x + a4;
+  num = num * x + a3;
+  num = num * x + a2;
+  num = num * x + a1;
+  num *= x;
+  double den = b5 * x + b4;
+  den = den * x + b3;
+  den = den * x + b2;
+  den = den * x + b1;
+  den = den * x + 1.0;
+  return num / den;
+}
+
+constexpr double kMinDet = 1e-10;  // MIN_DET (src/constants.h:70)
+
+struct LinearSource {
+  std::vector<double> centroid;  // [2N]
+  std::vector<double> pos;       // [2S] segment start relative to its FSR centroid
+  std::vector<double> ilem;      // [3N] inverse linear-expansion matrix
+  std::vector<double> src_c;     // [N][3][G] source constants
+};
+
+// FSR centroids (CentroidGenerator, src/TrackTraversingAlgorithms.cpp:377-460,
+// from the track start and the segment lengths), the segments re-centred on
+// them (RecenterSegments, :1322-1332), and the LinearExpansionGenerator
+// (:470-831): moments of the segments about the centroid, their inverse and
+// the source constants.
+LinearSource ls_setup(const View& v) {
+  const OracleProblem* p = v.p;
+  const int G = v.G, P = v.P;
+  const int64_t N = v.N, T = v.T;
+  LinearSource ls;
+  ls.centroid.assign(2 * N, 0.0);
+  for (int64_t t = 0; t < T; t++) {
+    const int a = p->trk_azim[t];
+    const double w = p->azim_weight[a];
+    const double cp = std::cos(p->azim_phi[a]), sp = std::sin(p->azim_phi[a]);
+    const int64_t s0 = p->seg_offset[t], s1 = p->seg_offset[t + 1];
+    if (s1 == s0) continue;
+    double x = p->seg_start[2 * s0], y = p->seg_start[2 * s0 + 1];
+    for (int64_t k = s0; k < s1; k++) {
+      const int32_t r = p->seg_fsr[k];
+      const double l = p->seg_length[k];
+      ls.centroid[2 * r] += w * (x + cp * l / 2.0) * l / p->fsr_volume[r];
+      ls.centroid[2 * r + 1] += w * (y + sp * l / 2.0) * l / p->fsr_volume[r];
+      x += cp * l;
+      y += sp * l;
+    }
+  }
+  const int64_t S = p->seg_offset[T];
+  ls.pos.resize(2 * S);
+  for (int64_t k = 0; k < S; k++) {
+    const int32_t r = p->seg_fsr[k];
+    ls.pos[2 * k] = p->seg_start[2 * k] - ls.centroid[2 * r];
+    ls.pos[2 * k + 1] = p->seg_start[2 * k + 1] - ls.centroid[2 * r + 1];
+  }
+  std::vector<double> lem(3 * N, 0.0);
+  ls.src_c.assign(N * 3 * G, 0.0);
+  for (int64_t t = 0; t < T; t++) {
+    const int a = p->trk_azim[t];
+    const double wgt = p->azim_weight[a];
+    const double cp = std::cos(p->azim_phi[a]), sp = std::sin(p->azim_phi[a]);
+    for (int64_t k = p->seg_offset[t]; k < p->seg_offset[t + 1]; k++) {
+      const int32_t r = p->seg_fsr[k];
+      const double l = p->seg_length[k];
+      const double vol = p->fsr_volume[r];
+      const double xc = ls.pos[2 * k] + l * 0.5 * cp;
+      const double yc = ls.pos[2 * k + 1] + l * 0.5 * sp;
+      const double vol_impact = wgt * l / vol;
+      const double src_constant = vol_impact * l / 2.0;
+      const double* st = v.sig_t(r);
+      double* c = &ls.src_c[r * 3 * G];
+      for (int g = 0; g < G; g++) {
+        double c0 = vol_impact * xc * xc, c1 = vol_impact * yc * yc, c2 = vol_impact * xc * yc;
+        const double tau = l * st[g];
+        for (int q = 0; q < P; q++) {
+          const double sin_t = 1.0 / p->inv_sin_theta[q];
+          const double g2 = l * exp_g2(tau / sin_t) * src_constant * 2 * p->polar_weight[q] * sin_t;
+          c0 += cp * cp * g2;
+          c1 += sp * sp * g2;
+          c2 += sp * cp * g2;
+        }
+        c[g] += c0;
+        c[G + g] += c1;
+        c[2 * G + g] += c2;
+      }
+      lem[3 * r] += wgt * l / vol * (xc * xc + std::pow(cp * l, 2) / 12.0);
+      lem[3 * r + 1] += wgt * l / vol * (yc * yc + std::pow(sp * l, 2) / 12.0);
+      lem[3 * r + 2] += wgt * l / vol * (xc * yc + sp * cp * std::pow(l, 2) / 12.0);
+    }
+  }
+  ls.ilem.assign(3 * N, 0.0);
+  for (int64_t r = 0; r < N; r++) {
+    const double det = lem[3 * r] * lem[3 * r + 1] - lem[3 * r + 2] * lem[3 * r + 2];
+    if (std::abs(det) >= kMinDet) {
+      ls.ilem[3 * r] = lem[3 * r + 1] / det;
+      ls.ilem[3 * r + 1] = lem[3 * r] / det;
+      ls.ilem[3 * r + 2] = -lem[3 * r + 2] / det;
+    }
+  }
+  return ls;
+}
+
+// Source moments (CPULSSolver::computeFSRSources, src/CPULSSolver.cpp:386-524),
+// after the flat sources Q of the same iteration.
+void ls_sources(const View& v, const LinearSource& ls, const double* phi_xy, const double* Q,
+                double k, int it, bool allow_neg, double* Qxy) {
+  const int G = v.G;
+  for (int64_t r = 0; r < v.N; r++) {
+    const double* s = v.sig_s(r);
+    const double* chi = v.chi(r);
+    const double* nsf = v.nsf(r);
+    const bool fis = v.fissionable(r);
+    std::vector<double> tmp(G);
+    for (int g = 0; g < G; g++) {
+      double src[2];
+      for (int m = 0; m < 2; m++) {
+        double fission = 0.0;
+        if (fis) {
+          for (int gp = 0; gp < G; gp++) tmp[gp] = chi[g] * nsf[gp] * phi_xy[(r * G + gp) * 2 + m];
+          fission = pairwise(tmp.data(), G) / k;
+        }
+        for (int gp = 0; gp < G; gp++) tmp[gp] = s[g * G + gp] * phi_xy[(r * G + gp) * 2 + m];
+        src[m] = pairwise(tmp.data(), G) + fission;
+      }
+      double* q = &Qxy[(r * G + g) * 2];
+      if (allow_neg || Q[r * G + g] > 10 * kFluxEpsilon || it > 29) {
+        q[0] = kOneOverFourPi / 2 * (ls.ilem[3 * r] * src[0] + ls.ilem[3 * r + 2] * src[1]);
+        q[1] = kOneOverFourPi / 2 * (ls.ilem[3 * r + 2] * src[0] + ls.ilem[3 * r + 1] * src[1]);
+      } else {
+        q[0] = 0.0;
+        q[1] = 0.0;
+      }
+    }
+  }
+}
+
+// Linear-source transport sweep (tallyLSScalarFlux 2D, src/CPULSSolver.cpp:
+// 542-739, accumulateLinearFluxContribution :749-780, TransportSweep::onTrack).
+template <typename PsiT>
+void ls_sweep(const View& v, const LinearSource& ls, const double* Q, const double* Qxy,
+              PsiT* start, double* tally, double* tally_xy) {
+  const OracleProblem* p = v.p;
+  const int G = v.G, P = v.P, fpt = v.fpt;
+  const int64_t T = v.T;
+  std::memset(tally, 0, sizeof(double) * v.N * G);
+  std::memset(tally_xy, 0, sizeof(double) * v.N * G * 2);
+  std::vector<PsiT> boundary(start, start + 2 * T * fpt);
+#pragma omp parallel
+  {
+    std::vector<double> buf(G), bx(G), by(G);
+#pragma omp for schedule(dynamic)
+    for (int64_t t = 0; t < T; t++) {
+      const int a = p->trk_azim[t];
+      const double* w = p->total_weight + (int64_t)a * P;
+      const double cp = std::cos(p->azim_phi[a]), sp = std::sin(p->azim_phi[a]);
+      const int64_t s0 = p->seg_offset[t], s1 = p->seg_offset[t + 1];
+      const int64_t ns = s1 - s0;
+      // the FSR buffers carry over from the forward to the backward pass, as
+      // the reference flushes only when the FSR changes (and at the very end)
+      std::fill(buf.begin(), buf.end(), 0.0);
+      std::fill(bx.begin(), bx.end(), 0.0);
+      std::fill(by.begin(), by.end(), 0.0);
+      for (int d = 0; d < 2; d++) {
+        PsiT* psi = &boundary[(t * 2 + d) * fpt];
+        const double dx = d == 0 ? cp : -cp, dy = d == 0 ? sp : -sp;
+        for (int64_t kk = 0; kk < ns; kk++) {
+          const int64_t k = d == 0 ? s0 + kk : s1 - 1 - kk;
+          const int32_t r = p->seg_fsr[k];
+          const double l = p->seg_length[k];
+          const double* st = v.sig_t(r);
+          // forward: the segment start; backward: its end
+          double px = ls.pos[2 * k], py = ls.pos[2 * k + 1];
+          if (d == 1) {
+            px += cp * l;
+            py += sp * l;
+          }
+          const double cx = 2 * px + l * dx, cy = 2 * py + l * dy;
+          for (int g = 0; g < G; g++) {
+            const double* qxy = &Qxy[(r * G + g) * 2];
+            const double src_flat = Q[r * G + g] + qxy[0] * cx + qxy[1] * cy;
+            const double src_lin = dx * qxy[0] + dy * qxy[1];
+            const double tau = st[g] * l;
+            for (int q = 0; q < P; q++) {
+              const double isin = p->inv_sin_theta[q];
+              const double tq = std::max(1e-8, tau * isin);
+              const double eg = exp_g(tq);
+              const double f1 = (1.0 - tq * eg) * isin;
+              const double gq = eg * isin;
+              const double f2 = 2.0 * gq - f1;
+              double h = f1 - gq;
+              const double wgt = w[q];
+              const double ps = (double)psi[q * G + g];
+              h *= wgt * tau * l * ps;
+              double dl = (tau * ps - l * src_flat) * f1 - l * l * src_lin * f2;
+              psi[q * G + g] = (PsiT)(ps - dl);
+              dl *= wgt;
+              buf[g] += dl;
+              bx[g] += h * dx + dl * px;
+              by[g] += h * dy + dl * py;
+            }
+          }
+          const bool last = kk == ns - 1;
+          const int64_t knext = d == 0 ? k + 1 : k - 1;
+          const bool flush = (d == 0) ? (!last && p->seg_fsr[knext] != r)
+                                      : (last || p->seg_fsr[knext] != r);
+          if (flush) {
+            for (int g = 0; g < G; g++) {
+#pragma omp atomic
+              tally[r * G + g] += buf[g];
+#pragma omp atomic
+              tally_xy[(r * G + g) * 2] += bx[g];
+#pragma omp atomic
+              tally_xy[(r * G + g) * 2 + 1] += by[g];
+              buf[g] = bx[g] = by[g] = 0.0;
+            }
+          }
+        }
+        const uint8_t bc = d == 0 ? p->bc_fwd[t] : p->bc_bwd[t];
+        if (bc == 1 || bc == 2) {
+          const int64_t nt = d == 0 ? p->next_fwd[t] : p->next_bwd[t];
+          const int nd = d == 0 ? !p->next_fwd_is_fwd[t] : !p->next_bwd_is_fwd[t];
+          PsiT* out = start + (nt * 2 + nd) * fpt;
+          for (int e = 0; e < fpt; e++) out[e] = psi[e];
+        }
+      }
+    }
+  }
+}
+
+// CPULSSolver::addSourceToScalarFlux (src/CPULSSolver.cpp:787-882).
+void ls_add_source(const View& v, const LinearSource& ls, const double* Q, const double* Qxy,
+                   const double* old, bool allow_neg, double* phi, double* phi_xy) {
+  const int G = v.G;
+  for (int64_t r = 0; r < v.N; r++) {
+    double vol = v.p->fsr_volume[r];
+    if (vol < kFltEpsilon) vol = 1e30;
+    const double* st = v.sig_t(r);
+    const double* c = &ls.src_c[r * 3 * G];
+    for (int g = 0; g < G; g++) {
+      const int64_t i = r * G + g;
+      const double fc = kFourPi * 2;
+      phi[i] /= vol;
+      phi[i] += kFourPi * Q[i];
+      phi[i] /= st[g];
+      double* m = &phi_xy[i * 2];
+      const double* q = &Qxy[i * 2];
+      m[0] /= vol;
+      m[0] += fc * q[0] * c[g];
+      m[0] += fc * q[1] * c[2 * G + g];
+      m[1] /= vol;
+      m[1] += fc * q[0] * c[2 * G + g];
+      m[1] += fc * q[1] * c[G + g];
+      m[0] /= st[g];
+      m[1] /= st[g];
+      if (phi[i] < 0.0 && !allow_neg) {
+        phi[i] = std::max(old[i], kFluxEpsilon);
+        m[0] = 0.0;
+        m[1] = 0.0;
+      }
+    }
+  }
+}
+
+template <typename PsiT>
+int solve_ls(const OracleProblem* p, const OracleOptions* o, OracleResult* out,
+             double* flux_out, double* flux_xy_out, double* centroid_out) {
+  if (!p->azim_phi || !p->azim_weight || !p->polar_weight || !p->seg_start) return -1;
+  if (o->keff_from_fission_rates == 0 || o->stab_type >= 0) return -3;
+  View v = make_view(p);
+  const int G = v.G;
+  const int64_t N = v.N, T = v.T;
+  if (o->threads > 0) omp_set_num_threads(o->threads);
+  const LinearSource ls = ls_setup(v);
+  std::vector<double> phi(N * G, 1.0), old(N * G), Q(N * G), scratch(N), tally(N * G);
+  std::vector<double> phi_xy(N * G * 2, 0.0), Qxy(N * G * 2, 0.0), tally_xy(N * G * 2);
+  std::vector<PsiT> start(2 * T * v.fpt, (PsiT)0);
+  const bool allow_neg = o->allow_negative_fluxes != 0;
+  double k = 1.0, res = 0.0, sweep_s = 0.0;
+  int iters = 0;
+  auto normalize = [&]() {
+    const double tot = fission_rate_total(v, phi.data(), scratch);
+    const double f = (double)N / tot;
+    for (auto& x : phi) x *= f;
+    for (auto& x : start) x = (PsiT)(x * f);
+    for (auto& x : phi_xy) x *= f;
+  };
+  normalize();
+  old = phi;
+  double k_prev = k;
+  for (int i = 0; i < o->max_iters; i++) {
+    fsr_sources(v, phi.data(), k, i, allow_neg, Q.data(), 0);
+    ls_sources(v, ls, phi_xy.data(), Q.data(), k, i, allow_neg, Qxy.data());
+    auto t0 = std::chrono::steady_clock::now();
+    ls_sweep<PsiT>(v, ls, Q.data(), Qxy.data(), start.data(), tally.data(), tally_xy.data());
+    sweep_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    phi = tally;
+    phi_xy = tally_xy;
+    ls_add_source(v, ls, Q.data(), Qxy.data(), old.data(), allow_neg, phi.data(), phi_xy.data());
+    const double rate = fission_rate_total(v, phi.data(), scratch);
+    k *= rate / (double)N;
+    normalize();
+    res = residual(v, phi.data(), old.data(), o->res_type, k, scratch);
+    const int dk = (int)(1e5 * (k - k_prev));
+    k_prev = k;
+    old = phi;
+    iters++;
+    if (res < o->tolerance && std::abs(dk) < 1) break;
+  }
+  std::memcpy(flux_out, phi.data(), sizeof(double) * N * G);
+  if (flux_xy_out) std::memcpy(flux_xy_out, phi_xy.data(), sizeof(double) * N * G * 2);
+  if (centroid_out) std::memcpy(centroid_out, ls.centroid.data(), sizeof(double) * 2 * N);
+  out->keff = k;
+  out->num_iterations = iters;
+  out->residual = res;
+  out->sweep_seconds = sweep_s;
+  out->num_integrations = 2LL * G * v.P * p->seg_offset[T] * iters;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -483,6 +838,14 @@ int oracle_krylov_sweep(const OracleProblem* p, int kind, int allow_negative, do
   std::memcpy(phi, tally.data(), sizeof(double) * v.N * v.G);
   add_source(v, Q.data(), allow_negative != 0, phi);
   return 0;
+}
+
+int oracle_compute_eigenvalue_ls(const OracleProblem* p, const OracleOptions* o,
+                                 OracleResult* r, double* flux_out, double* flux_xy_out,
+                                 double* centroid_out) {
+  if (!p || !o || !r || !flux_out) return -1;
+  return o->psi_fp64 ? solve_ls<double>(p, o, r, flux_out, flux_xy_out, centroid_out)
+                     : solve_ls<float>(p, o, r, flux_out, flux_xy_out, centroid_out);
 }
 
 int oracle_compute_flux(const OracleProblem* p, const OracleOptions* o, OracleResult* r,

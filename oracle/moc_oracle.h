@@ -49,6 +49,11 @@ typedef struct {
   const int32_t* fsr_material;     /* [N] */
   const double* fsr_volume;        /* [N] */
   const double* fixed_source;      /* [N*G] or NULL */
+  /* linear source only (NULL otherwise) */
+  const double* azim_phi;          /* [num_azim_half] track angle */
+  const double* azim_weight;       /* [num_azim_half] azimuthal spacing * weight */
+  const double* polar_weight;      /* [P] polar weight of the quadrature */
+  const double* seg_start;         /* [2S] segment start points (x, y) */
 } OracleProblem;
 
 enum { ORACLE_SCALAR_FLUX = 0, ORACLE_FISSION_SOURCE = 1, ORACLE_TOTAL_SOURCE = 2 };
@@ -77,6 +82,16 @@ typedef struct {
  * flux_out: [N*G] converged, normalised scalar flux. Returns 0 on success. */
 int oracle_compute_eigenvalue(const OracleProblem* p, const OracleOptions* o,
                               OracleResult* r, double* flux_out);
+
+/* CPULSSolver::computeEigenvalue (linear source, 2D): src/CPULSSolver.cpp
+ * with the LinearExpansionGenerator (src/TrackTraversingAlgorithms.cpp:
+ * 470-831) and the FSR centroids of TrackGenerator::generateFSRCentroids
+ * (src/TrackGenerator.cpp:1681-1735).  flux_xy_out (nullable): [N*G*2]
+ * scalar-flux moments (x, y) per FSR and group.  Stabilisation and the
+ * neutron-balance keff are not restated for the linear source (-3). */
+int oracle_compute_eigenvalue_ls(const OracleProblem* p, const OracleOptions* o,
+                                 OracleResult* r, double* flux_out, double* flux_xy_out,
+                                 double* centroid_out);
 
 /* Solver::computeFlux(max_iters, only_fixed_source=true):
  * src/Solver.cpp:1352-1420 (sources computed once from phi = 0). */

@@ -24,6 +24,7 @@
  *   u32 name_len, name bytes, u8 dtype ('d' f64, 'f' f32, 'i' i32, 'l' i64),
  *   u64 count, count * sizeof(dtype) bytes.
  */
+#include "CPULSSolver.h"
 #include "CPUSolver.h"
 #include "Geometry.h"
 #include "Quadrature.h"
@@ -137,12 +138,33 @@ struct Writer {
   void l1(const char* n, int64_t x) { put(n, 'l', &x, 1); }
 };
 
-/* Subclass only to read the protected timer. */
+/* Subclasses only to read the protected timer and linear-source moments. */
 class TimedCPUSolver : public CPUSolver {
  public:
   explicit TimedCPUSolver(TrackGenerator* tg) : CPUSolver(tg) {}
   double split(const char* name) { return _timer->getSplit(name); }
   void sweep_once() { transportSweep(); }
+  void dump_ls(Writer&) {}
+};
+
+class TimedCPULSSolver : public CPULSSolver {
+ public:
+  explicit TimedCPULSSolver(TrackGenerator* tg) : CPULSSolver(tg) {}
+  double split(const char* name) { return _timer->getSplit(name); }
+  void sweep_once() { transportSweep(); }
+  /* scalar-flux moments [r][x|y|z][g] and the FSR centroids */
+  void dump_ls(Writer& w) {
+    long n = _num_FSRs * _NUM_GROUPS * 3;
+    std::vector<double> m(_scalar_flux_xyz, _scalar_flux_xyz + n);
+    w.d("flux_xyz", m);
+    std::vector<double> c(2 * _num_FSRs);
+    for (long r = 0; r < _num_FSRs; r++) {
+      Point* p = _geometry->getFSRCentroid(r);
+      c[2 * r] = p->getX();
+      c[2 * r + 1] = p->getY();
+    }
+    w.d("fsr_centroid_ls", c);
+  }
 };
 
 }  // namespace
@@ -151,7 +173,7 @@ int main(int argc, char** argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: %s problem.txt out.bin [--threads N] [--no-solve] "
                     "[--max-iters N] [--tol X] [--sweeps N] [--balance] "
-                    "[--stabilize FACTOR TYPE] [--krylov]\n", argv[0]);
+                    "[--stabilize FACTOR TYPE] [--krylov] [--linear-source]\n", argv[0]);
     return 2;
   }
   int threads = 1;
@@ -159,7 +181,7 @@ int main(int argc, char** argv) {
   int max_iters_override = -1;
   double tol_override = -1;
   int timed_sweeps = 0;
-  bool balance = false, krylov = false;
+  bool balance = false, krylov = false, linear_source = false;
   double stab_factor = 0.0;
   int stab_type = -1;
   for (int a = 3; a < argc; a++) {
@@ -174,6 +196,7 @@ int main(int argc, char** argv) {
       stab_type = atoi(argv[++a]);
     }
     else if (!strcmp(argv[a], "--krylov")) krylov = true;
+    else if (!strcmp(argv[a], "--linear-source")) linear_source = true;
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -329,7 +352,7 @@ int main(int argc, char** argv) {
   w.l("seg_fsr", s_fsr);
   w.i("seg_material", s_mat);
 
-  TimedCPUSolver solver(tg);
+  auto body = [&](auto& solver) -> int {
   solver.setNumThreads(threads);
   solver.setConvergenceThreshold(s.tol);
   if (balance) solver.setKeffFromNeutronBalance();
@@ -424,5 +447,13 @@ int main(int argc, char** argv) {
     w.l1("timed_sweeps", timed_sweeps);
     fprintf(stdout, "timed_sweeps %d seconds %.6f\n", timed_sweeps, dt);
   }
+  if (solve && linear_source) solver.dump_ls(w);
   return 0;
+  };
+  if (linear_source) {
+    TimedCPULSSolver solver(tg);
+    return body(solver);
+  }
+  TimedCPUSolver solver(tg);
+  return body(solver);
 }
