@@ -264,15 +264,18 @@ def from_ref_dump(d: dict, problem, polar_weight=None) -> OracleProblem:
     npol = int(d["num_polar"])
     sin_theta = d["quad_sin_theta"].reshape(na // 2, npol)[0]
     # segment start points from the track start and the segment lengths
-    phi = d["trk_phi"]
-    st0 = d["trk_start"].reshape(-1, 2)
-    lens = d["seg_length"]
-    seg_start = np.zeros((len(lens), 2))
-    for t in range(T):
-        a, b = off[t], off[t + 1]
-        cum = np.concatenate([[0.0], np.cumsum(lens[a:b])[:-1]])
-        seg_start[a:b, 0] = st0[t, 0] + cum * np.cos(phi[t])
-        seg_start[a:b, 1] = st0[t, 1] + cum * np.sin(phi[t])
+    # (linear-source data; absent when the dump has no track starts)
+    seg_start = None
+    if "trk_start" in d and "quad_phi" in d:
+        phi = np.asarray(d["quad_phi"])[np.asarray(d["trk_azim"])]
+        st0 = d["trk_start"].reshape(-1, 2)
+        lens = d["seg_length"]
+        seg_start = np.zeros((len(lens), 2))
+        for t in range(T):
+            a, b = off[t], off[t + 1]
+            cum = np.concatenate([[0.0], np.cumsum(lens[a:b])[:-1]])
+            seg_start[a:b, 0] = st0[t, 0] + cum * np.cos(phi[t])
+            seg_start[a:b, 1] = st0[t, 1] + cum * np.sin(phi[t])
     return OracleProblem(
         num_azim=na, num_polar=npol, seg_offset=off, seg_length=d["seg_length"],
         seg_fsr=d["seg_fsr"].astype(np.int32), azim=d["trk_azim"],
@@ -282,7 +285,9 @@ def from_ref_dump(d: dict, problem, polar_weight=None) -> OracleProblem:
         total_weight=d["quad_total_weight"].reshape(na // 2, npol), sin_theta=sin_theta,
         sigma_t=st, sigma_s=ss, nu_sigma_f=nsf, chi=chi,
         fsr_material=d["fsr_material"], fsr_volume=d["fsr_volume"],
-        azim_phi=d["quad_phi"], azim_weight=d["quad_azim_spacing"] * d["quad_azim_weight"],
+        azim_phi=d.get("quad_phi"),
+        azim_weight=(d["quad_azim_spacing"] * d["quad_azim_weight"]
+                     if "quad_azim_spacing" in d else None),
         polar_weight=polar_weight, seg_start=seg_start)
 
 
