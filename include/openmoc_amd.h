@@ -183,6 +183,22 @@ int moc_solver_set_keff_from_neutron_balance(moc_solver_t* solver, int on);
  * :2072-2104); one sweep from the start fluxes the previous operator sweep
  * left (zero at first); addSourceToScalarFlux into the flux. */
 int moc_solver_fission_transport_sweep(moc_solver_t* solver);
+
+/* Linear source (CPULSSolver, src/CPULSSolver.cpp; 2D): on = 1 makes
+ * compute_eigenvalue solve with linear sources in every FSR.  Needs the
+ * track data's azim_phi, azim_weight, polar_weight and seg_start.  The FSR
+ * centroids (TrackGenerator::generateFSRCentroids, src/TrackGenerator.cpp:
+ * 1681-1735) and the linear expansion coefficients
+ * (LinearExpansionGenerator, src/TrackTraversingAlgorithms.cpp:470-831) are
+ * computed at the first solve after tracks, FSRs or materials change.
+ * Stabilisation, the neutron-balance keff and the Krylov sweeps are
+ * flat-source only. */
+int moc_solver_set_linear_source(moc_solver_t* solver, int on);
+/* Normalised scalar-flux moments [N*G*2], (x, y) per FSR and group, layout
+ * [(fsr*G + g)*2 + m] (CPULSSolver::_scalar_flux_xyz). */
+int moc_solver_get_flux_moments(moc_solver_t* solver, double* out, int64_t num_values);
+/* FSR centroids [2N] (x, y) the linear source expands about. */
+int moc_solver_get_fsr_centroids(moc_solver_t* solver, double* out, int64_t num_values);
 int moc_solver_scatter_transport_sweep(moc_solver_t* solver);
 
 /* Solver::computeEigenvalue(max_iters, res_type) (src/Solver.cpp:1542-1689). */

@@ -91,6 +91,9 @@ _SIGS = [
     ("moc_solver_stabilize_transport", C.c_int, [_P, C.c_double, C.c_int32]),
     ("moc_solver_set_keff_from_neutron_balance", C.c_int, [_P, C.c_int]),
     ("moc_solver_fission_transport_sweep", C.c_int, [_P]),
+    ("moc_solver_set_linear_source", C.c_int, [_P, C.c_int]),
+    ("moc_solver_get_flux_moments", C.c_int, [_P, _P, C.c_int64]),
+    ("moc_solver_get_fsr_centroids", C.c_int, [_P, _P, C.c_int64]),
     ("moc_solver_scatter_transport_sweep", C.c_int, [_P]),
     ("moc_solver_compute_eigenvalue", C.c_int, [_P, C.c_int32, C.c_int32]),
     ("moc_solver_compute_flux", C.c_int, [_P, C.c_int32, C.c_int32]),

@@ -346,6 +346,18 @@ int moc_solver_set_keff_from_neutron_balance(moc_solver_t* s, int on) {
   MOC_SOLVER_CALL(s, s->impl->setKeffFromNeutronBalance(on != 0));
 }
 
+int moc_solver_set_linear_source(moc_solver_t* s, int on) {
+  MOC_SOLVER_CALL(s, s->impl->setLinearSource(on != 0));
+}
+
+int moc_solver_get_flux_moments(moc_solver_t* s, double* out, int64_t num_values) {
+  MOC_SOLVER_CALL(s, s->impl->getFluxMoments(out, num_values));
+}
+
+int moc_solver_get_fsr_centroids(moc_solver_t* s, double* out, int64_t num_values) {
+  MOC_SOLVER_CALL(s, s->impl->getCentroids(out, num_values));
+}
+
 int moc_solver_fission_transport_sweep(moc_solver_t* s) {
   MOC_SOLVER_CALL(s, s->impl->krylovSweep(1));
 }

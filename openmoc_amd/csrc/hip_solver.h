@@ -111,6 +111,14 @@ class HipSolver {
   // 1282-1297): sources from the current flux, one sweep from the current
   // start fluxes, addSourceToScalarFlux.
   void krylovSweep(int source_kind);
+  // Linear source (CPULSSolver): on/off; needs the track data's azimuthal
+  // angles and weights, polar weights and segment start points.
+  void setLinearSource(bool on);
+  bool linearSource() const { return linear_; }
+  // Normalised scalar-flux moments [N*G*2] (x, y) of the last linear-source solve.
+  void getFluxMoments(double* out, int64_t n);
+  // FSR centroids [2N] used by the linear source.
+  void getCentroids(double* out, int64_t n);
 
   // Uses the tracks, FSRs and launch shape of `src` (device buffers shared,
   // not copied) in this fresh solver; materials stay per solver.  For many
@@ -167,6 +175,8 @@ class HipSolver {
   void launchIterations(int mode, int res_type, int n);
   void launchAsyncBatch();
   void finishSolve(int max_iters);
+  void prepareLinearSource();
+  void buildPositionStream();
   FsrArgs fsrArgs();
   void refreshStabilization();
   const double* currentFlux() const;
@@ -230,6 +240,23 @@ class HipSolver {
   int old_sel_ = 0;                    // host copy of IterState::old_sel after a solve
   bool krylov_ready_ = false;          // start fluxes initialised for Krylov sweeps
   int pstride_ = 0;                    // stride of the per-block partial arrays
+  // linear source
+  bool linear_ = false;
+  bool ls_ready_ = false;              // constants and position stream are current
+  std::vector<double> h_seg_start_;    // [2S]
+  std::vector<double> h_azim_phi_, h_azim_w_, h_polar_w_;
+  std::vector<double> h_centroid_;     // [2N]
+  std::vector<int32_t> h_work_, h_tile_;  // host copies of the stream layout
+  float2* d_seg_pos_ = nullptr;        // stream positions (LS)
+  float* d_azim_cos_ = nullptr;
+  float* d_azim_sin_ = nullptr;
+  double* d_ilem_ = nullptr;           // [N*3]
+  double* d_src_c_ = nullptr;          // [N*3*G]
+  double* d_phi_xy_ = nullptr;         // [N*G*2]
+  double* d_phi_xy_n_ = nullptr;       // [N*G*2]
+  double* d_Qxy_ = nullptr;            // [N*G*2]
+  float4* d_qls_ = nullptr;            // [N*G]
+  float* d_tally_xy_ = nullptr;        // [copies][N*G*2]
   struct GeoHold;
   std::shared_ptr<GeoHold> geo_hold_;  // set when the geometry buffers are shared
   // captured iteration batch

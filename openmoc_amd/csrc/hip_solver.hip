@@ -30,6 +30,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 
 #include "hip_solver.h"
@@ -446,6 +447,143 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
   sweep_track<GPL, EXPM, NP>(a, tile, slot, dt, gl, a.lanes_per_track, 0, a.G, src, tally);
 }
 
+// ---------------------------------------------------------------------------
+// Linear-source sweep (CPULSSolver::tallyLSScalarFlux 2D,
+// src/CPULSSolver.cpp:542-739)
+// ---------------------------------------------------------------------------
+// 5/6 rational fit of G(x) = 1/x - (1 - e^-x)/x^2 the reference uses to form
+// F1, F2 and H (src/exponentials.h:100-145, src/ExpEvaluator.h:349-379).
+__device__ __forceinline__ float exp_g_rational(float x) {
+  float den = fmaf(6.064409107557148e-5f, x, 7.036272419147752e-4f);
+  den = fmaf(den, x, 6.883236664917246e-3f);
+  den = fmaf(den, x, 4.721469893686252e-2f);
+  den = fmaf(den, x, 2.263358514260129e-1f);
+  den = fmaf(den, x, 6.864462055546078e-1f);
+  den = fmaf(den, x, 1.0f);
+  float num = fmaf(6.064409107557148e-5f, x, 6.429894635552992e-4f);
+  num = fmaf(num, x, 6.178333902037397e-3f);
+  num = fmaf(num, x, 4.041584305811143e-2f);
+  num = fmaf(num, x, 1.76558112351595e-1f);
+  num = fmaf(num, x, 0.5f);
+  return num * __builtin_amdgcn_rcpf(den);
+}
+
+struct LsArgs {
+  const float2* seg_pos;    // tile-interleaved [row][tpw]: the segment's entry point
+                            // in traversal order, relative to its FSR centroid
+  const float4* qls;        // [N*G] (Q, Q_x, Q_y, sigma_t)
+  float* tally_xy;          // [tally_copies][N*G*2]
+  const float* azim_cos;    // [num_azim/2]
+  const float* azim_sin;
+};
+
+// One lane per (directed track, group subset of GPL groups); float atomics
+// into the block's private tally copy (flat tally and the x/y moments).
+template <int GPL, int NP>
+__global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a, LsArgs L) {
+  const int lane = threadIdx.x & 63;
+  const int tile = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int slot = lane / a.lanes_per_track;
+  const int gl = lane - slot * a.lanes_per_track;
+  if (slot >= a.tpw || tile >= a.ntiles) return;
+  if (a.state->converged) return;
+  const int dt = a.work[tile * a.tpw + slot];
+  if (dt < 0) return;
+  const int t = dt >> 1;
+  const int dir = dt & 1;
+  const int4 tr = a.trk[t];
+  const int2 ti = a.tile[tile];
+  const int azim = a.trk_azim[t];
+  const int G = a.G, fpt = a.fpt, lpt = a.lanes_per_track;
+  constexpr int P = NP;
+  const float scale = (float)a.state->psi_scale;
+  const float pf[kMaxPolarHalf] = {a.pf0, a.pf1, a.pf2};
+  float w[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) w[p] = a.weights[azim * P + p];
+  const float dx = dir ? -L.azim_cos[azim] : L.azim_cos[azim];
+  const float dy = dir ? -L.azim_sin[azim] : L.azim_sin[azim];
+  const size_t copy = (size_t)(blockIdx.x % a.tally_copies);
+  float* tf = a.tally + copy * a.tally_stride;
+  float* txy = L.tally_xy + copy * a.tally_stride * 2;
+
+  int g[GPL];
+  bool ok[GPL];
+  float psi[GPL][P];
+  const float* pin = a.psi_in + (size_t)dt * fpt;
+#pragma unroll
+  for (int k = 0; k < GPL; k++) {
+    const int l = gl + k * lpt;
+    ok[k] = l < G;
+    g[k] = ok[k] ? l : G - 1;
+#pragma unroll
+    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p * G + g[k]] * scale : 0.f;
+  }
+  const int n = ti.y;
+  const size_t row0 = (size_t)ti.x * a.tpw + slot;
+  // one segment of look-ahead on the stream loads
+  float len_n = a.seg_len[row0];
+  int key_n = a.seg_key[row0];
+  float2 pos_n = L.seg_pos[row0];
+  for (int r = 0; r < n; r++) {
+    const float len = len_n;
+    const int key = key_n;
+    const float2 pos = pos_n;
+    if (r + 1 < n) {
+      const size_t o = row0 + (size_t)(r + 1) * a.tpw;
+      len_n = a.seg_len[o];
+      key_n = a.seg_key[o];
+      pos_n = L.seg_pos[o];
+    }
+    const int fsr = key & kFsrMask;
+    const float cx = 2.f * pos.x + len * dx, cy = 2.f * pos.y + len * dy;
+#pragma unroll
+    for (int k = 0; k < GPL; k++) {
+      const float4 q = L.qls[(size_t)fsr * G + g[k]];
+      const float src_flat = q.x + q.y * cx + q.z * cy;
+      const float src_lin = dx * q.y + dy * q.z;
+      const float tau = q.w * len;
+      float acc = 0.f, ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        const float isin = pf[p];
+        const float tq = fmaxf(1e-8f, tau * isin);
+        const float eg = exp_g_rational(tq);
+        const float f1 = (1.f - tq * eg) * isin;
+        const float gq = eg * isin;
+        const float f2 = 2.f * gq - f1;
+        const float h = (f1 - gq) * w[p] * tau * len * psi[k][p];
+        float d = (tau * psi[k][p] - len * src_flat) * f1 - len * len * src_lin * f2;
+        psi[k][p] -= d;
+        d *= w[p];
+        acc += d;
+        ax += h * dx + d * pos.x;
+        ay += h * dy + d * pos.y;
+      }
+      if (ok[k] && len > 0.f) {
+        const size_t i = (size_t)fsr * G + g[k];
+        unsafeAtomicAdd(&tf[i], acc);
+        unsafeAtomicAdd(&txy[2 * i], ax);
+        unsafeAtomicAdd(&txy[2 * i + 1], ay);
+      }
+    }
+  }
+  const int out = dir ? tr.w : tr.z;
+  float amax = 0.f;
+  if (out >= 0) {
+    float* po = a.psi_out + (size_t)out * fpt;
+#pragma unroll
+    for (int k = 0; k < GPL; k++)
+#pragma unroll
+      for (int p = 0; p < P; p++)
+        if (ok[k]) {
+          po[p * G + g[k]] = psi[k][p];
+          amax = fmaxf(amax, fabsf(psi[k][p]));
+        }
+  }
+  if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
+}
+
 // LDS-privatised sweep.  Block b owns group subset b % nsub (GB groups, one
 // lane per group) and block slot b / nsub.  LDS holds the int64 fixed-point
 // tally slice [N][GB] of the subset, 1/V per FSR, sigma_t of the subset by material [M][GB] and, when
@@ -578,6 +716,15 @@ struct FsrArgs {
   int stab_type;              // -1 off, else DIAGONAL / YAMAMOTO / GLOBAL
   double stab_factor;
   const double* stab_ratio;   // [G] YAMAMOTO: factor * max |sigma_s(g->g) / sigma_t(g)|
+  // linear source (null / 0 for the flat source)
+  int linear;
+  const float* tally_xy;      // [tally_copies][N*G*2] sweep moment tallies
+  double* phi_xy;             // [N*G*2] flux moments of the current sweep
+  double* phi_xy_n;           // [N*G*2] normalised flux moments
+  double* Qxy;                // [N*G*2] source moments
+  float4* qls;                // [N*G] (Q, Q_x, Q_y, sigma_t) for the sweep
+  const double* ilem;         // [N*3] inverse linear expansion matrix
+  const double* src_c;        // [N][3][G] source constants
 };
 
 namespace {
@@ -671,6 +818,39 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
             ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         // the fixed-point tally holds sum(contribution) / V
         f = ((double)(long long)t / a.state->tally_scale) / st + kFourPi * a.Q[k] / st;
+      } else if (a.linear) {
+        // CPULSSolver::addSourceToScalarFlux (src/CPULSSolver.cpp:787-882)
+        double t = 0.0, tx = 0.0, ty = 0.0;
+        for (int c = 0; c < a.tally_copies; c++) {
+          t += (double)a.tally[c * NG + k];
+          tx += (double)a.tally_xy[(c * NG + k) * 2];
+          ty += (double)a.tally_xy[(c * NG + k) * 2 + 1];
+          a.tally[c * NG + k] = 0.f;
+          const_cast<float*>(a.tally_xy)[(c * NG + k) * 2] = 0.f;
+          const_cast<float*>(a.tally_xy)[(c * NG + k) * 2 + 1] = 0.f;
+        }
+        f = t / vol;
+        f += kFourPi * a.Q[k];
+        f /= st;
+        const double* c3 = a.src_c + (size_t)r * 3 * G;
+        const double fc = kFourPi * 2;
+        const double qx = a.Qxy[2 * k], qy = a.Qxy[2 * k + 1];
+        double mx = tx / vol;
+        mx += fc * qx * c3[g];
+        mx += fc * qy * c3[2 * G + g];
+        double my = ty / vol;
+        my += fc * qx * c3[2 * G + g];
+        my += fc * qy * c3[G + g];
+        mx /= st;
+        my /= st;
+        if (f < 0.0 && !a.allow_negative) {
+          f = fmax(a.phi_old[a.state->old_sel][k], kFluxEpsilon);
+          mx = 0.0;
+          my = 0.0;
+          neg++;
+        }
+        a.phi_xy[2 * k] = mx;
+        a.phi_xy[2 * k + 1] = my;
       } else {
         double t = 0.0;
         for (int c = 0; c < a.tally_copies; c++) {
@@ -868,6 +1048,35 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
       a.Q[i] = q;
       qmax = fabs(q / st);
       if (a.qs) a.qs[i] = make_float2((float)(q / st), (float)st);
+      if (a.linear) {
+        // source moments (CPULSSolver::computeFSRSources, src/CPULSSolver.cpp:
+        // 386-524) from the normalised flux moments
+        const double* mom = a.phi_xy + (size_t)r * G * 2;
+        const double fm = normalize ? f : 1.0;
+        double src[2];
+        for (int c = 0; c < 2; c++) {
+          double fis_c = 0.0, sc_c = 0.0;
+          if (fis) {
+            const double cg = a.chi[(size_t)m * G + gd];
+            for (int g = 0; g < G; g++) fis_c += cg * nsf[g] * (mom[2 * g + c] * fm);
+            fis_c /= k;
+          }
+          const double* row = ss + (size_t)gd * G;
+          for (int g = 0; g < G; g++) sc_c += row[g] * (mom[2 * g + c] * fm);
+          src[c] = sc_c + fis_c;
+        }
+        a.phi_xy_n[2 * i] = mom[2 * gd] * fm;
+        a.phi_xy_n[2 * i + 1] = mom[2 * gd + 1] * fm;
+        double qx = 0.0, qy = 0.0;
+        if (a.allow_negative || q > 10 * kFluxEpsilon || it > 29) {
+          const double* il = a.ilem + (size_t)r * 3;
+          qx = kOneOverFourPi / 2 * (il[0] * src[0] + il[2] * src[1]);
+          qy = kOneOverFourPi / 2 * (il[2] * src[0] + il[1] * src[1]);
+        }
+        a.Qxy[2 * i] = qx;
+        a.Qxy[2 * i + 1] = qy;
+        a.qls[i] = make_float4((float)q, (float)qx, (float)qy, (float)st);
+      }
       if (a.qsub) {
         const int sb = gd / a.qsub_groups, l = gd - sb * a.qsub_groups;
         a.qsub[((size_t)sb * a.N + r) * a.qsub_groups + l] = (float)(q / st);
@@ -1073,6 +1282,16 @@ HipSolver::~HipSolver() {
   dev_free(d_phi_old_[1]);
   dev_free(d_sigma_a_);
   dev_free(d_leak_);
+  dev_free(d_seg_pos_);
+  dev_free(d_azim_cos_);
+  dev_free(d_azim_sin_);
+  dev_free(d_ilem_);
+  dev_free(d_src_c_);
+  dev_free(d_phi_xy_);
+  dev_free(d_phi_xy_n_);
+  dev_free(d_Qxy_);
+  dev_free(d_qls_);
+  dev_free(d_tally_xy_);
   dev_free(d_stab_ratio_);
   dev_free(d_Q_);
   dev_free(d_fixed_);
@@ -1115,7 +1334,7 @@ void HipSolver::pickShape() {
   // subset that fits, evened out over the subsets.
   s.lds_groups = 0;
   s.lds_q = 0;
-  if (N_ > 0 && lds_max_ > 0) {
+  if (N_ > 0 && lds_max_ > 0 && !linear_) {
     for (int q = 1; q >= 0 && s.lds_groups == 0; q--) {
       int gb = 0;
       while (gb < std::min(G_, 64) && ldsBytes(gb + 1, q) <= lds_max_) gb++;
@@ -1249,6 +1468,18 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   h_fsr_.swap(fsr);
   h_trk_.swap(trk);
   layout_tpw_ = 0;  // streams are laid out for the launch shape once it is known
+  // linear-source inputs (optional)
+  h_seg_start_.clear();
+  h_azim_phi_.clear();
+  h_azim_w_.clear();
+  h_polar_w_.clear();
+  if (td.seg_start && td.azim_phi && td.azim_weight && td.polar_weight) {
+    h_seg_start_.assign(td.seg_start, td.seg_start + 2 * S);
+    h_azim_phi_.assign(td.azim_phi, td.azim_phi + na2);
+    h_azim_w_.assign(td.azim_weight, td.azim_weight + na2);
+    h_polar_w_.assign(td.polar_weight, td.polar_weight + P);
+  }
+  ls_ready_ = false;
 
   T_ = T;
   S_ = S;
@@ -1339,6 +1570,9 @@ void HipSolver::buildTileLayout(int tpw) {
   layout_tpw_ = tpw;
   layout_ntiles_ = (int)ntiles;
   layout_rows_ = rows;
+  h_work_.swap(work);
+  h_tile_.swap(tile);
+  ls_ready_ = false;
 }
 
 int HipSolver::sweepTracksPerWave() const {
@@ -1397,6 +1631,7 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
   }
   M_ = M;
   materials_set_ = true;
+  ls_ready_ = false;
   pickShape();
   if (fsrs_set_) {
     for (int64_t r = 0; r < N_; r++)
@@ -1453,6 +1688,7 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
   }
   N_ = num_fsrs;
   fsrs_set_ = true;
+  ls_ready_ = false;
   countFissionable();
   allocateIterationArrays();
 }
@@ -1471,6 +1707,16 @@ void HipSolver::allocateIterationArrays() {
   dev_alloc(&d_phi_old_[0], NG);
   dev_alloc(&d_phi_old_[1], NG);
   dev_alloc(&d_leak_, (size_t)2 * T_ * G_);
+  if (linear_) {
+    dev_alloc(&d_phi_xy_, NG * 2);
+    dev_alloc(&d_phi_xy_n_, NG * 2);
+    dev_alloc(&d_Qxy_, NG * 2);
+    dev_alloc(&d_qls_, NG);
+    dev_alloc(&d_tally_xy_, NG * 2 * shape_.tally_copies);
+    HIP_CHECK(hipMemset(d_tally_xy_, 0, NG * 2 * shape_.tally_copies * sizeof(float)));
+    HIP_CHECK(hipMemset(d_phi_xy_n_, 0, NG * 2 * sizeof(double)));
+    ls_ready_ = false;
+  }
   dev_alloc(&d_Q_, NG);
   if (!shape_forced_) pickShape();
   if (shape_.lds_groups > 0) {
@@ -1577,6 +1823,32 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   a.N = (int)N_;
   a.lanes_per_track = shape_.lanes_per_track;
   a.tracks_per_wave = shape_.tracks_per_wave;
+  if (linear_) {
+    if (!ls_ready_) throw DeviceError(MOC_ERR_STATE, "linear-source data not prepared");
+    LsArgs L{d_seg_pos_, d_qls_, d_tally_xy_, d_azim_cos_, d_azim_sin_};
+    // the reference's LS exponentials take plain 1/sin(theta)
+    a.pf0 = inv_sin_[0];
+    a.pf1 = inv_sin_[1];
+    a.pf2 = inv_sin_[2];
+    const unsigned grid = (unsigned)(((int64_t)layout_ntiles_ * 64 + shape_.block - 1) / shape_.block);
+    auto by_np = [&](auto k1, auto k2, auto k3) {
+      switch (P_) {
+        case 1: hipLaunchKernelGGL(k1, dim3(grid), dim3(shape_.block), 0, s, a, L); break;
+        case 2: hipLaunchKernelGGL(k2, dim3(grid), dim3(shape_.block), 0, s, a, L); break;
+        default: hipLaunchKernelGGL(k3, dim3(grid), dim3(shape_.block), 0, s, a, L); break;
+      }
+    };
+    switch (shape_.groups_per_lane) {
+      case 1: by_np(sweep2d_ls_kernel<1, 1>, sweep2d_ls_kernel<1, 2>, sweep2d_ls_kernel<1, 3>); break;
+      case 2: by_np(sweep2d_ls_kernel<2, 1>, sweep2d_ls_kernel<2, 2>, sweep2d_ls_kernel<2, 3>); break;
+      case 4: by_np(sweep2d_ls_kernel<4, 1>, sweep2d_ls_kernel<4, 2>, sweep2d_ls_kernel<4, 3>); break;
+      case 9: by_np(sweep2d_ls_kernel<9, 1>, sweep2d_ls_kernel<9, 2>, sweep2d_ls_kernel<9, 3>); break;
+      default: by_np(sweep2d_ls_kernel<16, 1>, sweep2d_ls_kernel<16, 2>, sweep2d_ls_kernel<16, 3>); break;
+    }
+    HIP_CHECK(hipGetLastError());
+    if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
+    return;
+  }
   if (shape_.lds_groups > 0) {
     LdsParams L;
     L.GB = shape_.lds_groups;
@@ -1675,6 +1947,14 @@ FsrArgs HipSolver::fsrArgs() {
   a.stab_type = stab_type_;
   a.stab_factor = stab_factor_;
   a.stab_ratio = d_stab_ratio_;
+  a.linear = linear_ ? 1 : 0;
+  a.tally_xy = d_tally_xy_;
+  a.phi_xy = d_phi_xy_;
+  a.phi_xy_n = d_phi_xy_n_;
+  a.Qxy = d_Qxy_;
+  a.qls = d_qls_;
+  a.ilem = d_ilem_;
+  a.src_c = d_src_c_;
   if (shape_.lds_groups > 0) {
     a.qs = nullptr;
     a.qsub = d_qsub_;
@@ -1742,6 +2022,13 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // starts from phi = 0, src/Solver.cpp:1383-1390)
   hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_phi_,
                      (int64_t)NG, mode == MODE_FLUX ? 0.0 : 1.0);
+  if (linear_) {
+    // flattenFSRFluxes zeroes the moments (src/CPULSSolver.cpp:343-356)
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
+                       d_phi_xy_, (int64_t)(2 * NG), 0.0);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG * shape_.tally_copies, 256)),
+                       dim3(256), 0, stream_, d_tally_xy_, (int64_t)(2 * NG * shape_.tally_copies));
+  }
   if (mode == MODE_EIGEN) {
     hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 0, 0);
     hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
@@ -1802,6 +2089,7 @@ const double* HipSolver::currentFlux() const {
 // initializeFluxArrays), then addSourceToScalarFlux into the flux.
 void HipSolver::krylovSweep(int source_kind) {
   requireReady();
+  if (linear_) throw DeviceError(MOC_ERR_INVALID, "Krylov operator sweeps are flat-source only");
   if (source_kind != kSrcFission && source_kind != kSrcScatter)
     throw DeviceError(MOC_ERR_INVALID, "unknown Krylov source kind");
   HIP_CHECK(hipSetDevice(device_));
@@ -1923,6 +2211,7 @@ void HipSolver::timing(double* sweep_s, double* total_s, int64_t* integrations) 
 
 void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   requireReady();
+  if (linear_) throw DeviceError(MOC_ERR_INVALID, "the single-sweep entry point is flat-source only");
   if (!Q || !psi || !tally) throw DeviceError(MOC_ERR_INVALID, "null argument");
   HIP_CHECK(hipSetDevice(device_));
   const size_t NG = (size_t)N_ * G_;
@@ -2242,6 +2531,13 @@ void HipSolver::startSolve(int mode, int max_iters, int res_type, double k_fixed
   if (mode == MODE_EIGEN && num_fissionable_ == 0)
     throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
   if (mode == MODE_FLUX && !fixed_on_) throw DeviceError(MOC_ERR_STATE, "computeFlux requires fixed sources");
+  if (linear_) {
+    if (mode != MODE_EIGEN) throw DeviceError(MOC_ERR_INVALID, "the linear source supports computeEigenvalue");
+    if (stab_type_ >= 0 || balance_)
+      throw DeviceError(MOC_ERR_INVALID,
+                        "stabilisation and the neutron-balance keff are not available with the linear source");
+    if (!ls_ready_) prepareLinearSource();
+  }
   if (!async_ev_) HIP_CHECK(hipEventCreate(&async_ev_));
   if (!async_ev0_) HIP_CHECK(hipEventCreate(&async_ev0_));
   async_ms_ = 0.0;
@@ -2292,6 +2588,223 @@ void HipSolver::finishSolve(int max_iters) {
   // device time of the iteration batches (sweeps plus the per-FSR steps)
   sweep_seconds_ = async_ms_ * 1e-3;
   integrations_ = 2LL * G_ * P_ * S_ * (int64_t)h_state_->iteration;
+}
+
+// ---------------------------------------------------------------------------
+// Linear source: host set-up
+// ---------------------------------------------------------------------------
+void HipSolver::setLinearSource(bool on) {
+  if (on == linear_) return;
+  if (on && geo_hold_) throw DeviceError(MOC_ERR_INVALID, "linear source on a shared geometry");
+  linear_ = on;
+  graph_dirty_ = true;
+  ls_ready_ = false;
+  // the linear source tallies through device-scope atomics
+  shape_forced_ = false;
+  pickShape();
+  layout_tpw_ = 0;
+  allocateIterationArrays();
+}
+
+namespace {
+// 5/5 rational fit of G2 (src/exponentials.h:280-323), for the source constants.
+double exp_g2_host(double x) {
+  double num = 1.6837426505799449e-4 * x + 1.124183494990467e-5;
+  num = num * x + 3.7673183263550827e-3;
+  num = num * x + -3.603942303847604e-3;
+  num = num * x + -8.335775885589858e-2;
+  num *= x;
+  double den = 1.0102514456857377e-3 * x + 6.125197988351906e-3;
+  den = den * x + 5.367250964303789e-2;
+  den = den * x + 2.3794300531408347e-1;
+  den = den * x + 7.454048371823628e-1;
+  den = den * x + 1.0;
+  return num / den;
+}
+}  // namespace
+
+// FSR centroids (TrackGenerator::generateFSRCentroids, src/TrackGenerator.cpp:
+// 1681-1735), the inverse linear expansion matrices and the source constants
+// (LinearExpansionGenerator, src/TrackTraversingAlgorithms.cpp:470-831), then
+// the stream of segment entry points relative to the centroids.  Host work,
+// once per geometry and cross-section set; tracks are split over threads and
+// the per-thread sums are added in thread order (deterministic).
+void HipSolver::prepareLinearSource() {
+  if (h_seg_start_.empty())
+    throw DeviceError(MOC_ERR_INVALID,
+                      "linear source needs azimuthal angles/weights, polar weights and segment "
+                      "start points in the track data");
+  const int G = G_, P = P_;
+  const int64_t N = N_, T = T_;
+  std::vector<double> cosa(h_azim_phi_.size()), sina(h_azim_phi_.size());
+  for (size_t i = 0; i < h_azim_phi_.size(); i++) {
+    cosa[i] = std::cos(h_azim_phi_[i]);
+    sina[i] = std::sin(h_azim_phi_[i]);
+  }
+  // track azimuthal indices and offsets from the host copies
+  std::vector<int64_t> off(T + 1);
+  std::vector<int32_t> az(T);
+  {
+    for (int64_t t = 0; t < T; t++) off[t] = h_trk_[4 * t];
+    off[T] = S_;
+    HIP_CHECK(hipMemcpy(az.data(), d_trk_azim_, T * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  auto parallel = [&](auto&& fn) {
+    std::vector<std::thread> th;
+    for (int i = 0; i < nth; i++) th.emplace_back(fn, i);
+    for (auto& x : th) x.join();
+  };
+  // centroids: sum of w l (x_start + l/2 dir) / V with x accumulated from the
+  // track's first segment
+  std::vector<std::vector<double>> cpart(nth, std::vector<double>(2 * N, 0.0));
+  parallel([&](int id) {
+    for (int64_t t = id; t < T; t += nth) {
+      const int a = az[t];
+      const double w = h_azim_w_[a];
+      if (off[t + 1] == off[t]) continue;
+      double x = h_seg_start_[2 * off[t]], y = h_seg_start_[2 * off[t] + 1];
+      for (int64_t k = off[t]; k < off[t + 1]; k++) {
+        const int32_t r = h_fsr_[k];
+        const double l = h_len_[k];
+        cpart[id][2 * r] += w * (x + cosa[a] * l / 2.0) * l / h_fsr_vol_[r];
+        cpart[id][2 * r + 1] += w * (y + sina[a] * l / 2.0) * l / h_fsr_vol_[r];
+        x += cosa[a] * l;
+        y += sina[a] * l;
+      }
+    }
+  });
+  h_centroid_.assign(2 * N, 0.0);
+  for (int id = 0; id < nth; id++)
+    for (int64_t i = 0; i < 2 * N; i++) h_centroid_[i] += cpart[id][i];
+  cpart.clear();
+  // expansion matrices and source constants
+  std::vector<double> st((size_t)M_ * G);
+  HIP_CHECK(hipMemcpy(st.data(), d_sigma_t_, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<std::vector<double>> lpart(nth, std::vector<double>(3 * N, 0.0));
+  std::vector<std::vector<double>> spart(nth, std::vector<double>((size_t)N * 3 * G, 0.0));
+  parallel([&](int id) {
+    std::vector<double>& lem = lpart[id];
+    std::vector<double>& sc = spart[id];
+    for (int64_t t = id; t < T; t += nth) {
+      const int a = az[t];
+      const double wgt = h_azim_w_[a], cp = cosa[a], sp = sina[a];
+      for (int64_t k = off[t]; k < off[t + 1]; k++) {
+        const int32_t r = h_fsr_[k];
+        const double l = h_len_[k];
+        const double vol = h_fsr_vol_[r];
+        const double xc = (h_seg_start_[2 * k] - h_centroid_[2 * r]) + l * 0.5 * cp;
+        const double yc = (h_seg_start_[2 * k + 1] - h_centroid_[2 * r + 1]) + l * 0.5 * sp;
+        const double vol_impact = wgt * l / vol;
+        const double src_constant = vol_impact * l / 2.0;
+        const double* s = st.data() + (size_t)h_fsr_mat_[r] * G;
+        double* c = sc.data() + (size_t)r * 3 * G;
+        for (int g = 0; g < G; g++) {
+          double c0 = vol_impact * xc * xc, c1 = vol_impact * yc * yc, c2 = vol_impact * xc * yc;
+          const double tau = l * s[g];
+          for (int p = 0; p < P; p++) {
+            const double sin_t = 1.0 / (double)inv_sin_[p];
+            const double g2 =
+                l * exp_g2_host(tau / sin_t) * src_constant * 2 * h_polar_w_[p] * sin_t;
+            c0 += cp * cp * g2;
+            c1 += sp * sp * g2;
+            c2 += sp * cp * g2;
+          }
+          c[g] += c0;
+          c[G + g] += c1;
+          c[2 * G + g] += c2;
+        }
+        lem[3 * r] += wgt * l / vol * (xc * xc + std::pow(cp * l, 2) / 12.0);
+        lem[3 * r + 1] += wgt * l / vol * (yc * yc + std::pow(sp * l, 2) / 12.0);
+        lem[3 * r + 2] += wgt * l / vol * (xc * yc + sp * cp * std::pow(l, 2) / 12.0);
+      }
+    }
+  });
+  std::vector<double> lem(3 * N, 0.0), src_c((size_t)N * 3 * G, 0.0);
+  for (int id = 0; id < nth; id++) {
+    for (int64_t i = 0; i < 3 * N; i++) lem[i] += lpart[id][i];
+    for (size_t i = 0; i < src_c.size(); i++) src_c[i] += spart[id][i];
+  }
+  lpart.clear();
+  spart.clear();
+  std::vector<double> ilem(3 * N, 0.0);
+  for (int64_t r = 0; r < N; r++) {
+    const double det = lem[3 * r] * lem[3 * r + 1] - lem[3 * r + 2] * lem[3 * r + 2];
+    if (std::fabs(det) >= 1e-10) {  // MIN_DET (src/constants.h:70)
+      ilem[3 * r] = lem[3 * r + 1] / det;
+      ilem[3 * r + 1] = lem[3 * r] / det;
+      ilem[3 * r + 2] = -lem[3 * r + 2] / det;
+    }
+  }
+  dev_alloc(&d_ilem_, 3 * N);
+  dev_alloc(&d_src_c_, src_c.size());
+  HIP_CHECK(hipMemcpy(d_ilem_, ilem.data(), ilem.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_src_c_, src_c.data(), src_c.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<float> fc(cosa.begin(), cosa.end()), fs(sina.begin(), sina.end());
+  dev_alloc(&d_azim_cos_, fc.size());
+  dev_alloc(&d_azim_sin_, fs.size());
+  HIP_CHECK(hipMemcpy(d_azim_cos_, fc.data(), fc.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_azim_sin_, fs.data(), fs.size() * sizeof(float), hipMemcpyHostToDevice));
+  buildPositionStream();
+  ls_ready_ = true;
+}
+
+// Entry point of each segment in traversal order, relative to its FSR
+// centroid, in the tile-interleaved layout of the length/key streams
+// (forward: the segment start; backward: its end, as the reference moves
+// the starting position to the end after the forward pass,
+// src/CPULSSolver.cpp:736-738).
+void HipSolver::buildPositionStream() {
+  const int tpw = layout_tpw_;
+  const int64_t ntiles = layout_ntiles_;
+  const size_t total = (size_t)(layout_rows_ + kTailRows) * tpw;
+  std::vector<float2> pos(total, make_float2(0.f, 0.f));
+  std::vector<double> cosa(h_azim_phi_.size()), sina(h_azim_phi_.size());
+  for (size_t i = 0; i < h_azim_phi_.size(); i++) {
+    cosa[i] = std::cos(h_azim_phi_[i]);
+    sina[i] = std::sin(h_azim_phi_[i]);
+  }
+  std::vector<int32_t> az(T_);
+  HIP_CHECK(hipMemcpy(az.data(), d_trk_azim_, T_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int64_t tl = 0; tl < ntiles; tl++) {
+    const int64_t r0 = h_tile_[2 * tl];
+    for (int sl = 0; sl < tpw; sl++) {
+      const int32_t dt = h_work_[tl * tpw + sl];
+      if (dt < 0) continue;
+      const int32_t t = dt >> 1, dir = dt & 1;
+      const int32_t b = h_trk_[4 * t], n = h_trk_[4 * t + 1];
+      for (int32_t i = 0; i < n; i++) {
+        const int32_t k = dir ? b + n - 1 - i : b + i;
+        const int32_t r = h_fsr_[k];
+        double x = h_seg_start_[2 * k] - h_centroid_[2 * r];
+        double y = h_seg_start_[2 * k + 1] - h_centroid_[2 * r + 1];
+        if (dir) {
+          x += cosa[az[t]] * h_len_[k];
+          y += sina[az[t]] * h_len_[k];
+        }
+        pos[(size_t)(r0 + i) * tpw + sl] = make_float2((float)x, (float)y);
+      }
+    }
+  }
+  dev_alloc(&d_seg_pos_, total);
+  HIP_CHECK(hipMemcpy(d_seg_pos_, pos.data(), total * sizeof(float2), hipMemcpyHostToDevice));
+}
+
+void HipSolver::getFluxMoments(double* out, int64_t n) {
+  requireReady();
+  if (!linear_) throw DeviceError(MOC_ERR_STATE, "the linear source is not enabled");
+  if (!out || n != 2 * N_ * G_) throw DeviceError(MOC_ERR_INVALID, "moments size must be 2*N*G");
+  HIP_CHECK(hipSetDevice(device_));
+  HIP_CHECK(hipMemcpyAsync(out, d_phi_xy_n_, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void HipSolver::getCentroids(double* out, int64_t n) {
+  requireReady();
+  if (!out || n != 2 * N_) throw DeviceError(MOC_ERR_INVALID, "centroid size must be 2*N");
+  if (linear_ && !ls_ready_) prepareLinearSource();
+  if (h_centroid_.size() != (size_t)(2 * N_)) throw DeviceError(MOC_ERR_STATE, "no centroids yet");
+  std::memcpy(out, h_centroid_.data(), 2 * N_ * sizeof(double));
 }
 
 double deviceCopyCalibration(int device, int64_t bytes, int repeats) {

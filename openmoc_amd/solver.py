@@ -28,7 +28,8 @@ from ._lib import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL, SCALA
                    TOTAL_SOURCE, YAMAMOTO, MOCError, check, lib)
 from .workloads import Problem
 
-__all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "SCALAR_FLUX", "FISSION_SOURCE",
+__all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "HipLSSolver", "SCALAR_FLUX",
+           "FISSION_SOURCE",
            "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "DIAGONAL", "YAMAMOTO", "GLOBAL",
            "MOCError"]
 
@@ -260,6 +261,21 @@ class HipSolver:
         """Solver::setKeffFromNeutronBalance (src/Solver.cpp:2047-2049)."""
         check(lib().moc_solver_set_keff_from_neutron_balance(self._h, 1))
 
+    def useLinearSource(self, on: bool = True) -> None:
+        """Linear source in every FSR (the reference's CPULSSolver)."""
+        check(lib().moc_solver_set_linear_source(self._h, int(bool(on))))
+
+    def getFluxMoments(self) -> np.ndarray:
+        """Normalised scalar-flux moments [N, G, 2] of the last linear-source solve."""
+        out = np.zeros(self._num_fsrs * self._num_groups * 2)
+        check(lib().moc_solver_get_flux_moments(self._h, _ptr(out), out.size))
+        return out.reshape(self._num_fsrs, self._num_groups, 2)
+
+    def getFSRCentroids(self) -> np.ndarray:
+        out = np.zeros(self._num_fsrs * 2)
+        check(lib().moc_solver_get_fsr_centroids(self._h, _ptr(out), out.size))
+        return out.reshape(self._num_fsrs, 2)
+
     def fissionTransportSweep(self) -> None:
         """Solver::fissionTransportSweep (src/Solver.cpp:1282-1286)."""
         check(lib().moc_solver_fission_transport_sweep(self._h))
@@ -389,3 +405,16 @@ class HipSolver:
 
 
 GPUSolver = HipSolver
+
+
+class HipLSSolver(HipSolver):
+    """Linear-source solver (the reference's CPULSSolver) on one MI355X."""
+
+    def __init__(self, track_generator: Optional[TrackGenerator] = None, device: int = -1):
+        super().__init__(track_generator, device)
+        if track_generator is not None:
+            self.useLinearSource(True)
+
+    def setTrackGenerator(self, tg: TrackGenerator) -> None:
+        super().setTrackGenerator(tg)
+        self.useLinearSource(True)

@@ -15,6 +15,9 @@ Outputs (data only: inputs and the reference's outputs):
                             from the converged flux
   pin_cell_7g_tc_ref.npz    keff/flux/iterations without and with each
                             transport stabilisation type
+  *_ls_ref.npz              linear-source (CPULSSolver) keff/flux/iterations,
+                            flux moments [N, G, 2] and FSR centroids for
+                            pin_cell_7g and lattice_3x3_vac_7g
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 """
 import sys
@@ -93,6 +96,13 @@ def main():
         out[f"{tag}_num_iterations"] = np.int64(ds["num_iterations"])
         out[f"{tag}_factor"] = np.float64(factor)
     np.savez_compressed(OUT / "pin_cell_7g_tc_ref.npz", **out)
+
+    for name in ("pin_cell_7g", "lattice_3x3_vac_7g"):
+        _, d = run(name, threads=1, extra=("--linear-source",))
+        N, G = int(d["num_fsrs"]), int(d["num_groups"])
+        mom = d["flux_xyz"].reshape(N, 3, G)[:, :2, :].transpose(0, 2, 1)
+        np.savez_compressed(OUT / f"{name}_ls_ref.npz", flux=d["flux"], flux_moments=mom,
+                            fsr_centroid=d["fsr_centroid_ls"].reshape(N, 2), **base(d))
     print("golden fixtures written to", OUT)
 
 

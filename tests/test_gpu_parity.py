@@ -259,3 +259,41 @@ def test_krylov_sweeps_keep_start_fluxes_reflective(tracks):
         (s.fissionTransportSweep if kind == 1 else s.scatterTransportSweep)()
         ref, psi = op.krylov_sweep(kind, phi0, psi)
         assert rel_l2(s.getFluxes(), ref) < 2e-5
+
+
+# --- linear source (CPULSSolver) ---
+
+@pytest.mark.parametrize("name", ["pin_cell_7g", "lattice_3x3_vac_7g"])
+def test_linear_source_matches_reference(tracks, golden, name):
+    g = golden(f"{name}_ls")
+    s = om.HipLSSolver(tracks(name))
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert abs(s.getNumIterations() - int(g["num_iterations"])) <= 2
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+    # moments: FP32 sweep arithmetic on the difference terms of F2 and H
+    assert rel_l2(s.getFluxMoments(), g["flux_moments"]) < 5e-3
+    np.testing.assert_allclose(s.getFSRCentroids(), g["fsr_centroid"], atol=1e-7)
+
+
+def test_linear_source_matches_oracle_on_same_tracks(tracks):
+    tg = tracks("pin_cell_7g")
+    sol, mom, cen = orc.OracleProblem.from_track_generator(tg).eigenvalue_ls(max_iters=1000,
+                                                                             tol=1e-5)
+    s = om.HipLSSolver(tg)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - sol.keff) < K_TOL
+    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
+    assert rel_l2(s.getFluxMoments(), mom) < 5e-3
+    np.testing.assert_allclose(s.getFSRCentroids(), cen, atol=1e-12)
+
+
+def test_linear_source_switches_back_to_flat(tracks, golden):
+    g = golden("pin_cell_7g")
+    s = om.HipLSSolver(tracks("pin_cell_7g"))
+    s.computeEigenvalue(1000)
+    s.useLinearSource(False)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    with pytest.raises(RuntimeError):
+        s.getFluxMoments()

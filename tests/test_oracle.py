@@ -240,3 +240,48 @@ def test_oracle_krylov_sweeps_are_linear(tracks, golden):
     tally, _ = op.transport_sweep(Q, zero)
     total = op.add_source(tally, Q)
     np.testing.assert_allclose(f + s, total, rtol=1e-5)
+
+
+# --- linear source (CPULSSolver) ---
+
+@pytest.mark.parametrize("name", ["pin_cell_7g", "lattice_3x3_vac_7g"])
+def test_oracle_linear_source_matches_reference(tracks, golden, name):
+    g = golden(f"{name}_ls")
+    tg = tracks(name)
+    op = orc.OracleProblem.from_track_generator(tg)
+    sol, mom, cen = op.eigenvalue_ls(max_iters=1000, tol=1e-5)
+    assert abs(sol.keff - float(g["keff"])) < 1e-8
+    assert sol.num_iterations == int(g["num_iterations"])
+    assert np.linalg.norm(sol.flux.ravel() - g["flux"]) / np.linalg.norm(g["flux"]) < 1e-7
+    m = g["flux_moments"]
+    # the reference's segment starts carry its 1e-8 cm surface nudges
+    assert np.linalg.norm(mom - m) / np.linalg.norm(m) < 1e-4
+    np.testing.assert_allclose(cen, g["fsr_centroid"], atol=1e-7)
+    # the linear source changes the answer (it is not the flat source)
+    flat = op.eigenvalue(max_iters=1000, tol=1e-5)
+    assert abs(flat.keff - sol.keff) > 1e-6
+
+
+def test_oracle_linear_source_reduces_to_flat_in_infinite_medium(tracks):
+    # homogeneous reflective medium: the flux has no gradient, so the
+    # moments vanish and the linear-source k equals the flat-source k
+    op = orc.OracleProblem.from_track_generator(tracks("hom_inf_medium"))
+    sol, mom, _ = op.eigenvalue_ls(max_iters=1000, tol=1e-8, psi_fp64=True)
+    flat = op.eigenvalue(max_iters=1000, tol=1e-8, psi_fp64=True)
+    s11, s12, s22 = 0.1, 0.117, 1.42
+    k_inf = (0.0015 + 0.325 * s12 / (1.604 - s22)) / (0.2208 - s11 - s12 + s12)
+    # both stop once |dk| < 1 pcm; within that they agree with k_inf
+    assert abs(sol.keff - flat.keff) < 1e-6
+    assert abs(sol.keff - k_inf) < 1e-5
+    assert np.abs(mom).max() < 1e-6 * np.abs(sol.flux).max()
+
+
+def test_oracle_centroids_are_volume_weighted_means(tracks):
+    # centroid of the whole pin-cell geometry (volume-weighted mean of the
+    # FSR centroids) is the cell centre, and every centroid lies in the cell
+    tg = tracks("pin_cell_7g")
+    op = orc.OracleProblem.from_track_generator(tg)
+    _, _, cen = op.eigenvalue_ls(max_iters=2, tol=1e-5)
+    v = tg.fsr_volume
+    np.testing.assert_allclose((cen * v[:, None]).sum(0) / v.sum(), [0.0, 0.0], atol=1e-7)
+    assert np.all(np.abs(cen) < 0.63)
