@@ -38,8 +38,13 @@ def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
             continue
         nsub = -(-num_groups // gb)
         for q, block, per_cu in itertools.product((1, 0), (512, 1024), (1, 2, 4)):
-            out.append(dict(lds_groups=gb, lds_q=q, lds_block=block,
-                            lds_blocks_per_sub=max(1, (num_cus * per_cu) // nsub)))
+            bps = max(1, (num_cus * per_cu) // nsub)
+            out.append(dict(lds_groups=gb, lds_q=q, lds_block=block, lds_blocks_per_sub=bps))
+            # multiples of 8 keep the subsets of a slot on one XCD (shared L2)
+            bps8 = (bps + 7) // 8 * 8
+            if bps8 != bps:
+                out.append(dict(lds_groups=gb, lds_q=q, lds_block=block,
+                                lds_blocks_per_sub=bps8))
     for gpl in (1, 2, 4, 9):
         lpt = -(-num_groups // gpl)
         if lpt > 64:

@@ -247,6 +247,10 @@ class HipSolver {
   std::vector<double> h_azim_phi_, h_azim_w_, h_polar_w_;
   std::vector<double> h_centroid_;     // [2N]
   std::vector<int32_t> h_work_, h_tile_;  // host copies of the stream layout
+  std::vector<int32_t> h_wpos_;        // [2T] work position of each directed track
+  int32_t* d_out_wp_ = nullptr;        // [2T] work position of the linked start slot
+  int64_t psi_w_ = 0;                  // start-flux slots per group (ntiles * tpw)
+  size_t psiCount() const { return (size_t)G_ * psi_w_ * P_; }
   float2* d_seg_pos_ = nullptr;        // stream positions (LS)
   float* d_azim_cos_ = nullptr;
   float* d_azim_sin_ = nullptr;

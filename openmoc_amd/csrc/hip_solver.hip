@@ -53,6 +53,7 @@ constexpr double kFltEpsilon = 1.0e-12;
 constexpr int kMaxPolarHalf = 3;
 constexpr int kReduceBlock = 256;
 constexpr int kSweepBlockMax = 256;  // __launch_bounds__ of the sweep kernel
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 // Segment-stream rows: every tile's row count is a multiple of kRowPad (the
 // pipeline chunk sizes divide it), and kTailRows rows follow the last tile,
 // so the pipeline's look-ahead loads (at most 3 chunks of 4 rows past a
@@ -156,8 +157,10 @@ struct SweepArgs {
   const int32_t* seg_key;   // tile-interleaved: [row][tpw], FSR | material << 24
   unsigned stream_bytes;    // bytes of each stream (rows + tail) * tpw * 4
   const float2* qs;         // [N*G] (Q/sigma_t, sigma_t)
-  const float* psi_in;      // [2T*fpt]
-  float* psi_out;           // [2T*fpt]
+  const float* psi_in;      // [G][W][P], W = ntiles * tpw: group-major by work position
+  float* psi_out;           // [G][W][P]
+  const int32_t* out_wp;    // [2T] work position of the linked start slot, -1 vacuum
+  int psi_w;                // W
   float* leak;              // [2T*G] vacuum leakage w(a,0) * sum_p psi_out, or null
   float* tally;             // global path: [tally_copies][N*G]
   int tally_copies;         // global path: private copies (block b -> copy b % copies)
@@ -252,12 +255,12 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
                                             int lpt, int g_begin, int g_count, const Src& src,
                                             const Tally& tally) {
   const int t = dt >> 1;
-  const int dir = dt & 1;
-  const int4 tr = a.trk[t];
   const int2 ti = a.tile[tile];
   const int azim = a.trk_azim[t];
-  const int G = a.G, fpt = a.fpt;
   constexpr int P = NP;  // polar half-angles, compile time: the segment body is straight-line
+  // start fluxes: group-major, the P polar values of a (group, directed
+  // track) contiguous, tracks in work order, so a wave's loads are one run
+  const size_t wp = (size_t)tile * a.tpw + slot;
   const float scale = (float)a.state->psi_scale;
   const float pf[kMaxPolarHalf] = {a.pf0, a.pf1, a.pf2};
   float w[P];
@@ -274,15 +277,15 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   int g[GPL], loc[GPL];
   bool ok[GPL];
   float psi[GPL][P];
-  const float* pin = a.psi_in + (size_t)dt * fpt;
 #pragma unroll
   for (int k = 0; k < GPL; k++) {
     const int l = gl + k * lpt;
     ok[k] = l < g_count;
     loc[k] = ok[k] ? l : g_count - 1;
     g[k] = g_begin + loc[k];
+    const float* pin = a.psi_in + ((size_t)g[k] * a.psi_w + wp) * P;
 #pragma unroll
-    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p * G + g[k]] * scale : 0.f;
+    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p] * scale : 0.f;
   }
 
   // Deep software pipeline.  The tile's rows are processed in chunks of D
@@ -397,18 +400,19 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
     if (c + 1 < nch) process(L1, K1, K2, Q1, Q2);
   }
 
-  const int out = dir ? tr.w : tr.z;
+  const int out = a.out_wp[dt];
   float amax = 0.f;
   if (out >= 0) {
-    float* po = a.psi_out + (size_t)out * fpt;
 #pragma unroll
-    for (int k = 0; k < GPL; k++)
+    for (int k = 0; k < GPL; k++) {
+      float* po = a.psi_out + ((size_t)g[k] * a.psi_w + out) * P;
 #pragma unroll
       for (int p = 0; p < P; p++)
         if (ok[k]) {
-          po[p * G + g[k]] = psi[k][p];
+          po[p] = psi[k][p];
           amax = fmaxf(amax, fabsf(psi[k][p]));
         }
+    }
   }
   if (a.leak) {
     // vacuum leakage of the neutron-balance keff (transferBoundaryFlux,
@@ -422,7 +426,7 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
         float v = 0.f;
 #pragma unroll
         for (int p = 0; p < P; p++) v += psi[k][p];
-        a.leak[(size_t)dt * G + g[k]] = out < 0 ? w0 * v : 0.f;
+        a.leak[(size_t)dt * a.G + g[k]] = out < 0 ? w0 * v : 0.f;
       }
   }
   // running max of |start psi| for the next sweep's fixed-point scale
@@ -491,10 +495,10 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
   if (dt < 0) return;
   const int t = dt >> 1;
   const int dir = dt & 1;
-  const int4 tr = a.trk[t];
   const int2 ti = a.tile[tile];
   const int azim = a.trk_azim[t];
-  const int G = a.G, fpt = a.fpt, lpt = a.lanes_per_track;
+  const int G = a.G, lpt = a.lanes_per_track;
+  const size_t wp = (size_t)tile * a.tpw + slot;
   constexpr int P = NP;
   const float scale = (float)a.state->psi_scale;
   const float pf[kMaxPolarHalf] = {a.pf0, a.pf1, a.pf2};
@@ -510,14 +514,14 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
   int g[GPL];
   bool ok[GPL];
   float psi[GPL][P];
-  const float* pin = a.psi_in + (size_t)dt * fpt;
 #pragma unroll
   for (int k = 0; k < GPL; k++) {
     const int l = gl + k * lpt;
     ok[k] = l < G;
     g[k] = ok[k] ? l : G - 1;
+    const float* pin = a.psi_in + ((size_t)g[k] * a.psi_w + wp) * P;
 #pragma unroll
-    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p * G + g[k]] * scale : 0.f;
+    for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p] * scale : 0.f;
   }
   const int n = ti.y;
   const size_t row0 = (size_t)ti.x * a.tpw + slot;
@@ -568,18 +572,19 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
       }
     }
   }
-  const int out = dir ? tr.w : tr.z;
+  const int out = a.out_wp[dt];
   float amax = 0.f;
   if (out >= 0) {
-    float* po = a.psi_out + (size_t)out * fpt;
 #pragma unroll
-    for (int k = 0; k < GPL; k++)
+    for (int k = 0; k < GPL; k++) {
+      float* po = a.psi_out + ((size_t)g[k] * a.psi_w + out) * P;
 #pragma unroll
       for (int p = 0; p < P; p++)
         if (ok[k]) {
-          po[p * G + g[k]] = psi[k][p];
+          po[p] = psi[k][p];
           amax = fmaxf(amax, fabsf(psi[k][p]));
         }
+    }
   }
   if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
 }
@@ -607,8 +612,19 @@ template <int EXPM, int QLDS, int NP>
 __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParams L) {
   extern __shared__ unsigned long long lds64[];
   if (a.state->converged) return;  // uniform: read by every thread of the block
-  const int sub = blockIdx.x % L.nsub;
-  const int bslot = blockIdx.x / L.nsub;
+  // Workgroups are dispatched to the 8 XCDs round-robin by block id.  With
+  // blocks_per_sub a multiple of 8, the nsub subset blocks of one slot (the
+  // same tiles, different groups) are placed on the same XCD one after the
+  // other, so the segment streams they all read come from that XCD's L2.
+  int sub, bslot;
+  if (L.blocks_per_sub % kNumXcd == 0) {
+    const int x = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd;
+    sub = j % L.nsub;
+    bslot = x + kNumXcd * (j / L.nsub);
+  } else {
+    sub = blockIdx.x % L.nsub;
+    bslot = blockIdx.x / L.nsub;
+  }
   const int g0 = sub * L.GB;
   const int gb = min(L.GB, a.G - g0);
   const int nl = a.N * L.GB;
@@ -1265,6 +1281,7 @@ HipSolver::~HipSolver() {
     d_fsr_mat_ = nullptr;
     d_fsr_vol_ = nullptr;
     d_inv_vol_ = nullptr;
+    d_out_wp_ = nullptr;
   }
   dev_free(d_seg_len_);
   dev_free(d_seg_key_);
@@ -1303,6 +1320,7 @@ HipSolver::~HipSolver() {
   dev_free(d_fsr_mat_);
   dev_free(d_fsr_vol_);
   dev_free(d_inv_vol_);
+  dev_free(d_out_wp_);
   dev_free(d_partials_);
   dev_free(d_lds_partials_);
   dev_free(d_qsub_);
@@ -1347,6 +1365,8 @@ void HipSolver::pickShape() {
       // about two blocks per CU: the strided tile assignment then balances
       // the longest-first tiles across blocks (measured on the 17x17 lattice)
       s.lds_blocks_per_sub = std::max(1, (2 * num_cus_ + nsub - 1) / nsub);
+      // a multiple of the XCD count keeps a slot's subset blocks on one XCD
+      s.lds_blocks_per_sub = (s.lds_blocks_per_sub + kNumXcd - 1) / kNumXcd * kNumXcd;
     }
   }
   shape_ = s;
@@ -1377,10 +1397,12 @@ void HipSolver::setShape(const LaunchShape& s) {
     throw DeviceError(MOC_ERR_INVALID, "invalid LDS sweep shape");
   const bool realloc = s.tally_copies != shape_.tally_copies || s.lds_groups != shape_.lds_groups ||
                        s.lds_blocks_per_sub != shape_.lds_blocks_per_sub;
+  (void)realloc;
   shape_ = s;
   shape_forced_ = true;
-  if (realloc) allocateIterationArrays();
-  buildTileLayout(sweepTracksPerWave());
+  // streams, start-flux slots and tallies follow the shape (buffers that are
+  // big enough are kept)
+  allocateIterationArrays();
 }
 
 void HipSolver::setTracks(const moc_track_data_t& td) {
@@ -1570,9 +1592,23 @@ void HipSolver::buildTileLayout(int tpw) {
   layout_tpw_ = tpw;
   layout_ntiles_ = (int)ntiles;
   layout_rows_ = rows;
+  // start-flux slots live at work positions: position of each directed
+  // track, and the position its outgoing flux is written to
+  h_wpos_.assign(2 * T_, -1);
+  for (size_t i = 0; i < work.size(); i++)
+    if (work[i] >= 0) h_wpos_[work[i]] = (int32_t)i;
+  std::vector<int32_t> out_wp(2 * T_, -1);
+  for (int64_t dt = 0; dt < 2 * T_; dt++) {
+    const int32_t link = h_trk_[4 * (dt >> 1) + 2 + (dt & 1)];
+    out_wp[dt] = link >= 0 ? h_wpos_[link] : -1;
+  }
+  dev_alloc(&d_out_wp_, out_wp.size());
+  HIP_CHECK(hipMemcpy(d_out_wp_, out_wp.data(), out_wp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  psi_w_ = (int64_t)ntiles * tpw;
   h_work_.swap(work);
   h_tile_.swap(tile);
   ls_ready_ = false;
+  state_valid_ = false;  // start fluxes are laid out by work position
 }
 
 int HipSolver::sweepTracksPerWave() const {
@@ -1698,7 +1734,9 @@ void HipSolver::allocateIterationArrays() {
   state_valid_ = false;
   if (!(tracks_set_ && materials_set_ && fsrs_set_)) return;
   const size_t NG = (size_t)N_ * G_;
-  const size_t psi_n = (size_t)2 * T_ * P_ * G_;
+  if (!shape_forced_) pickShape();
+  buildTileLayout(sweepTracksPerWave());
+  const size_t psi_n = psiCount();
   dev_alloc(&d_psi_[0], psi_n);
   dev_alloc(&d_psi_[1], psi_n);
   dev_alloc(&d_tally_, NG * shape_.tally_copies);
@@ -1792,7 +1830,8 @@ void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
 void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
                             hipEvent_t ev_after_sweep, float* leak) {
   SweepArgs a;
-  if (layout_tpw_ != sweepTracksPerWave()) buildTileLayout(sweepTracksPerWave());
+  if (layout_tpw_ != sweepTracksPerWave())
+    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
   a.work = d_work_;
   a.ntiles = layout_ntiles_;
   a.tpw = layout_tpw_;
@@ -1805,6 +1844,8 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   a.qs = d_qs_;
   a.psi_in = psi_in;
   a.psi_out = psi_out;
+  a.out_wp = d_out_wp_;
+  a.psi_w = (int)psi_w_;
   a.leak = leak;
   a.tally = d_tally_;
   a.tally_copies = shape_.tally_copies;
@@ -2001,7 +2042,7 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
 // its normalisation (eigen) and the sources of iteration 0.
 void HipSolver::launchInit(Mode mode, double k0) {
   const size_t NG = (size_t)N_ * G_;
-  const size_t psi_n = (size_t)2 * T_ * P_ * G_;
+  const size_t psi_n = psiCount();
   IterState st;
   std::memset(&st, 0, sizeof(st));
   st.k = k0;
@@ -2094,7 +2135,7 @@ void HipSolver::krylovSweep(int source_kind) {
     throw DeviceError(MOC_ERR_INVALID, "unknown Krylov source kind");
   HIP_CHECK(hipSetDevice(device_));
   const size_t NG = (size_t)N_ * G_;
-  const size_t psi_n = (size_t)2 * T_ * P_ * G_;
+  const size_t psi_n = psiCount();
   if (phi_is_old_) {
     // the flux of the last solve becomes the operand
     HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[old_sel_], NG * sizeof(double),
@@ -2215,7 +2256,8 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   if (!Q || !psi || !tally) throw DeviceError(MOC_ERR_INVALID, "null argument");
   HIP_CHECK(hipSetDevice(device_));
   const size_t NG = (size_t)N_ * G_;
-  const size_t psi_n = (size_t)2 * T_ * P_ * G_;
+  const size_t psi_n = psiCount();
+  const size_t ref_n = (size_t)2 * T_ * P_ * G_;  // reference layout (t*2+d)*fpt + p*G + g
   IterState st;
   std::memset(&st, 0, sizeof(st));
   st.psi_scale = 1.0;
@@ -2227,7 +2269,7 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
     double qmax = 0.0, pmax = 0.0;
     for (size_t i = 0; i < NG; i++)
       qmax = std::max(qmax, std::fabs(Q[i] / st_h[(size_t)h_fsr_mat_[i / G_] * G_ + i % G_]));
-    for (size_t i = 0; i < psi_n; i++) pmax = std::max(pmax, (double)std::fabs(psi[i]));
+    for (size_t i = 0; i < ref_n; i++) pmax = std::max(pmax, (double)std::fabs(psi[i]));
     st.qmax = (float)(qmax * (1 + 1e-6));
     st.psi_max = (float)(pmax * (1 + 1e-6));
     const double bound = 2.0 * kFourPi * sigma_t_max_ * ((double)st.psi_max + (double)st.qmax);
@@ -2240,10 +2282,17 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   hipLaunchKernelGGL(make_qs_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_Q_,
                      d_fsr_mat_, d_sigma_t_, lds ? nullptr : d_qs_, lds ? d_qsub_ : nullptr,
                      std::max(1, shape_.lds_groups), N_, G_);
-  HIP_CHECK(hipMemcpyAsync(d_psi_[0], psi, psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
+  // reference layout -> start-flux slots [G][W][P] by work position
+  const int fpt = P_ * G_;
+  std::vector<float> slots(psi_n, 0.f);
+  for (int64_t dt = 0; dt < 2 * T_; dt++)
+    for (int p = 0; p < P_; p++)
+      for (int g = 0; g < G_; g++)
+        slots[((size_t)g * psi_w_ + h_wpos_[dt]) * P_ + p] = psi[(size_t)dt * fpt + p * G_ + g];
+  HIP_CHECK(hipMemcpyAsync(d_psi_[0], slots.data(), psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
   // slots that no track writes keep their value (vacuum inflow), as in
   // CPUSolver::_start_flux
-  HIP_CHECK(hipMemcpyAsync(d_psi_[1], psi, psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
+  HIP_CHECK(hipMemcpyAsync(d_psi_[1], slots.data(), psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   launchSweep(d_psi_[0], d_psi_[1], stream_);
   std::vector<float> t32;
@@ -2261,7 +2310,7 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
     HIP_CHECK(hipMemcpyAsync(t32.data(), d_tally_, NG * shape_.tally_copies * sizeof(float),
                              hipMemcpyDeviceToHost, stream_));
   }
-  HIP_CHECK(hipMemcpyAsync(psi, d_psi_[1], psi_n * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipMemcpyAsync(slots.data(), d_psi_[1], psi_n * sizeof(float), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   if (shape_.lds_groups == 0) {
     for (size_t i = 0; i < NG; i++) {
@@ -2277,6 +2326,10 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   }
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
   HIP_CHECK(hipStreamSynchronize(stream_));
+  for (int64_t dt = 0; dt < 2 * T_; dt++)
+    for (int p = 0; p < P_; p++)
+      for (int g = 0; g < G_; g++)
+        psi[(size_t)dt * fpt + p * G_ + g] = slots[((size_t)g * psi_w_ + h_wpos_[dt]) * P_ + p];
   psi_cur_ = 0;
 }
 
@@ -2409,7 +2462,7 @@ void HipSolver::shareGeometry(HipSolver& src) {
     src.geo_hold_->device = src.device_;
     src.geo_hold_->bufs = {src.d_seg_len_, src.d_seg_key_, src.d_work_,  src.d_tile_,
                            src.d_trk_,     src.d_trk_azim_, src.d_weights_, src.d_fsr_mat_,
-                           src.d_fsr_vol_, src.d_inv_vol_};
+                           src.d_fsr_vol_, src.d_inv_vol_, src.d_out_wp_};
   }
   geo_hold_ = src.geo_hold_;
   d_seg_len_ = src.d_seg_len_;
@@ -2422,6 +2475,9 @@ void HipSolver::shareGeometry(HipSolver& src) {
   d_fsr_mat_ = src.d_fsr_mat_;
   d_fsr_vol_ = src.d_fsr_vol_;
   d_inv_vol_ = src.d_inv_vol_;
+  d_out_wp_ = src.d_out_wp_;
+  psi_w_ = src.psi_w_;
+  h_wpos_ = src.h_wpos_;
   T_ = src.T_;
   S_ = src.S_;
   P_ = src.P_;
@@ -2465,7 +2521,8 @@ void HipSolver::ensureGraph(int mode, int res_type) {
     (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
   }
-  if (layout_tpw_ != sweepTracksPerWave()) buildTileLayout(sweepTracksPerWave());
+  if (layout_tpw_ != sweepTracksPerWave())
+    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
   // kernels that need the LDS opt-in get it before the capture
   const int psi0 = psi_cur_;
   {
