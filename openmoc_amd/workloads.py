@@ -363,6 +363,17 @@ def minicore_70g(seed: int = 3, num_azim: int = 32, spacing: float = 0.05) -> Pr
                    tolerance=1e-5, max_iters=1000)
 
 
+def pin_cell_70g(seed: int = 6) -> Problem:
+    """pin_cell_7g geometry with 70 synthetic groups (dense lower-triangular
+    plus band scattering): the 70-group parity case."""
+    rng = np.random.default_rng(seed)
+    fuel = synthetic_material(rng, 70, "fuel", True)
+    mod = synthetic_material(rng, 70, "moderator", False)
+    return Problem("pin_cell_70g", [fuel, mod], [_pin(0, 1)], 1, 1, PITCH, PITCH,
+                   np.zeros((1, 1), np.int32), num_azim=16, azim_spacing=0.05,
+                   tolerance=1e-5, max_iters=1000)
+
+
 def lattice_3x3_vac_7g(seed: int = 4) -> Problem:
     """3x3 pin lattice (centre position moderator-filled), vacuum on all four
     sides, 7 synthetic groups; 16 azimuthal angles, 0.05 cm.  Leaks
@@ -396,6 +407,7 @@ def pin_cell_7g_tc(seed: int = 5) -> Problem:
 
 
 WORKLOADS = {
+    "pin_cell_70g": pin_cell_70g,
     "lattice_3x3_vac_7g": lattice_3x3_vac_7g,
     "pin_cell_7g_tc": pin_cell_7g_tc,
     "hom_inf_medium": hom_inf_medium,

@@ -17,7 +17,8 @@ Outputs (data only: inputs and the reference's outputs):
                             transport stabilisation type
   *_ls_ref.npz              linear-source (CPULSSolver) keff/flux/iterations,
                             flux moments [N, G, 2] and FSR centroids for
-                            pin_cell_7g and lattice_3x3_vac_7g
+                            pin_cell_7g, lattice_3x3_vac_7g and pin_cell_70g
+  pin_cell_70g_ref.npz      70-group keff/flux/iterations
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 """
 import sys
@@ -97,7 +98,10 @@ def main():
         out[f"{tag}_factor"] = np.float64(factor)
     np.savez_compressed(OUT / "pin_cell_7g_tc_ref.npz", **out)
 
-    for name in ("pin_cell_7g", "lattice_3x3_vac_7g"):
+    _, d = run("pin_cell_70g", threads=1)
+    np.savez_compressed(OUT / "pin_cell_70g_ref.npz", flux=d["flux"], **base(d))
+
+    for name in ("pin_cell_7g", "lattice_3x3_vac_7g", "pin_cell_70g"):
         _, d = run(name, threads=1, extra=("--linear-source",))
         N, G = int(d["num_fsrs"]), int(d["num_groups"])
         mom = d["flux_xyz"].reshape(N, 3, G)[:, :2, :].transpose(0, 2, 1)

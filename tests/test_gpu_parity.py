@@ -297,3 +297,22 @@ def test_linear_source_switches_back_to_flat(tracks, golden):
     assert abs(s.getKeff() - float(g["keff"])) < K_TOL
     with pytest.raises(RuntimeError):
         s.getFluxMoments()
+
+
+# --- 70 groups ---
+
+def test_70_groups_match_reference(tracks, golden):
+    g = golden("pin_cell_70g")
+    s = solver_for(tracks("pin_cell_70g"))
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert abs(s.getNumIterations() - int(g["num_iterations"])) <= 2
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+
+
+def test_70_groups_linear_source_match_reference(tracks, golden):
+    g = golden("pin_cell_70g_ls")
+    s = om.HipLSSolver(tracks("pin_cell_70g"))
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2

@@ -285,3 +285,12 @@ def test_oracle_centroids_are_volume_weighted_means(tracks):
     v = tg.fsr_volume
     np.testing.assert_allclose((cen * v[:, None]).sum(0) / v.sum(), [0.0, 0.0], atol=1e-7)
     assert np.all(np.abs(cen) < 0.63)
+
+
+def test_oracle_70_groups_matches_reference(tracks, golden):
+    g = golden("pin_cell_70g")
+    op = orc.OracleProblem.from_track_generator(tracks("pin_cell_70g"))
+    sol = op.eigenvalue(max_iters=1000, tol=1e-5)
+    assert abs(sol.keff - float(g["keff"])) < 1e-8
+    assert sol.num_iterations == int(g["num_iterations"])
+    assert np.linalg.norm(sol.flux.ravel() - g["flux"]) / np.linalg.norm(g["flux"]) < 1e-7
