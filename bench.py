@@ -187,6 +187,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="lattice_17x17_7g")
     ap.add_argument("--exp", choices=["rational", "hw"], default="rational")
+    ap.add_argument("--linear-source", action="store_true",
+                    help="linear-source solver (CPULSSolver form) instead of the flat source")
+    ap.add_argument("--scatter", choices=["auto", "fp64", "mfma", "fp32"], default="auto",
+                    help="scattering-source arithmetic (auto: matrix cores from 32 groups on)")
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
@@ -217,8 +221,13 @@ def main():
     S, T, N = tg.getNumSegments(), tg.getNumTracks(), tg.getNumFSRs()
     w_sweep = 2 * G * P * S
 
-    solver = om.HipSolver(tg, device=0)
+    solver = (om.HipLSSolver if args.linear_source else om.HipSolver)(tg, device=0)
     solver.setExponential(om.EXP_HW if args.exp == "hw" else om.EXP_RATIONAL)
+    scatter = solver.setScatterPrecision({"auto": om.SCATTER_AUTO, "fp64": om.SCATTER_FP64,
+                                          "mfma": om.SCATTER_MFMA_F16,
+                                          "fp32": om.SCATTER_FP32}[args.scatter])
+    scatter_name = {om.SCATTER_FP64: "fp64", om.SCATTER_MFMA_F16: "mfma_f16_split",
+                    om.SCATTER_FP32: "fp32"}[scatter]
     shape = solver.getLaunchShape()
     if not args.no_autotune:
         from openmoc_amd.autotune import autotune
@@ -235,7 +244,8 @@ def main():
         keff = solver.getKeff()
         solve = {"seconds": solve_s, "solves_per_s": 1.0 / solve_s,
                  "iterations": solver.getNumIterations(), "keff": keff}
-        golden = ROOT / "tests" / "golden" / f"{args.workload}_ref.npz"
+        golden = ROOT / "tests" / "golden" / (
+            f"{args.workload}_ls_ref.npz" if args.linear_source else f"{args.workload}_ref.npz")
         if golden.exists():
             g = np.load(golden)
             flux = solver.getFluxes()
@@ -260,7 +270,10 @@ def main():
     bytes_sweep = 8 * S + 8 * N * G + 16 * T * G * P
     achieved_gbs = bytes_sweep / sweep_s / 1e9
     achieved_tflops = FLOPS_PER_INTEGRATION * w_sweep / sweep_s / 1e12
-    kernel = "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel"
+    if args.linear_source:
+        kernel = "sweep2d_ls_kernel"
+    else:
+        kernel = "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel"
     traffic, traffic_src = measured_traffic(kernel, args.workload)
     roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -294,6 +307,8 @@ def main():
         "config": {"workload": args.workload, "tracks": T, "segments": S, "fsrs": N,
                    "groups": G, "polar_half": P, "num_azim": problem.num_azim,
                    "azim_spacing": problem.azim_spacing, "exponential": args.exp,
+                   "source": "linear" if args.linear_source else "flat",
+                   "scatter": scatter_name,
                    "launch_shape": shape,
                    "step": "one source iteration (sweep + per-FSR FP64 steps)"},
         "sweep_integrations_per_s": w_sweep / sweep_s,

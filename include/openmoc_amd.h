@@ -47,6 +47,13 @@ extern "C" {
 #define MOC_EXP_RATIONAL 0   /* the reference's 5/6 rational fit (default) */
 #define MOC_EXP_HW 1         /* hardware exp2: 1 - 2^(-x log2 e) */
 
+/* Arithmetic of the scattering-source contraction sum_g sigma_s(g->g') phi(g)
+ * (and the fission rate sum_g nu-sigma-f(g) phi(g)) in computeFSRSources. */
+#define MOC_SCATTER_AUTO 0     /* matrix cores from 32 groups on, FP64 below */
+#define MOC_SCATTER_FP64 1     /* FP64 loops (the reference's arithmetic) */
+#define MOC_SCATTER_MFMA_F16 2 /* fp16 MFMA on hi/lo split operands, FP32 accumulation */
+#define MOC_SCATTER_FP32 3     /* FP32 vector fallback */
+
 /* Thread-local text of the last error ("" if none). */
 const char* moc_last_error(void);
 
@@ -157,6 +164,20 @@ int moc_solver_set_fsrs(moc_solver_t* solver, int64_t num_fsrs, const int32_t* m
 int moc_solver_set_convergence_threshold(moc_solver_t* solver, double threshold);
 int moc_solver_set_exponential(moc_solver_t* solver, int mode /* MOC_EXP_* */);
 int moc_solver_allow_negative_fluxes(moc_solver_t* solver, int allow);
+/* Scattering-source arithmetic (MOC_SCATTER_*).  Replaces the FP64 loops of
+ * CPUSolver::computeFSRSources (src/CPUSolver.cpp:1939-2023) and the moment
+ * sources of CPULSSolver::computeFSRSources (src/CPULSSolver.cpp:386-524);
+ * the reference has no such switch (its GPUSolver uses FP32 loops,
+ * src/accel/cuda/GPUSolver.cu:288-338).  *in_use (nullable) receives the
+ * path MOC_SCATTER_AUTO resolves to for the current group count. */
+int moc_solver_set_scatter_precision(moc_solver_t* solver, int mode, int* in_use);
+/* CPUSolver::computeFSRSources(iteration) (src/CPUSolver.cpp:1939-2023; a
+ * protected method of Solver, src/Solver.h:396) from the current flux and
+ * keff: sources [N*G] per steradian, the reference's _reduced_sources, with
+ * negative sources set to 1e-25 below iteration 30 unless negative fluxes
+ * are allowed. */
+int moc_solver_compute_fsr_sources(moc_solver_t* solver, int32_t iteration, double* sources,
+                                   int64_t n);
 /* Solver::setFixedSourceByFSR (1-based group). */
 int moc_solver_set_fixed_source_by_fsr(moc_solver_t* solver, int64_t fsr, int32_t group,
                                        double source);

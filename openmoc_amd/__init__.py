@@ -8,6 +8,7 @@ no CPU fallback for the solver.
 """
 from .workloads import Problem, get as get_workload  # noqa: F401
 from .solver import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL,  # noqa: F401
-                     SCALAR_FLUX, TOTAL_SOURCE, YAMAMOTO, GPUSolver, HipLSSolver, HipSolver,
-                     MOCError, TrackGenerator)
+                     SCALAR_FLUX, SCATTER_AUTO, SCATTER_FP32, SCATTER_FP64, SCATTER_MFMA_F16,
+                     TOTAL_SOURCE, YAMAMOTO, GPUSolver, HipLSSolver, HipSolver, MOCError,
+                     TrackGenerator)
 from . import process  # noqa: F401

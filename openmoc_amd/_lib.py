@@ -11,6 +11,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libopenmoc_amd.so"
 MOC_OK = 0
 SCALAR_FLUX, FISSION_SOURCE, TOTAL_SOURCE = 0, 1, 2
 EXP_RATIONAL, EXP_HW = 0, 1
+# scattering-source arithmetic (moc_solver_set_scatter_precision)
+SCATTER_AUTO, SCATTER_FP64, SCATTER_MFMA_F16, SCATTER_FP32 = 0, 1, 2, 3
 # stabilizationType (src/Solver.h:103-113)
 DIAGONAL, YAMAMOTO, GLOBAL = 0, 1, 2
 
@@ -73,6 +75,8 @@ _SIGS = [
     ("moc_solver_set_convergence_threshold", C.c_int, [_P, C.c_double]),
     ("moc_solver_set_exponential", C.c_int, [_P, C.c_int]),
     ("moc_solver_allow_negative_fluxes", C.c_int, [_P, C.c_int]),
+    ("moc_solver_set_scatter_precision", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("moc_solver_compute_fsr_sources", C.c_int, [_P, C.c_int32, C.c_void_p, C.c_int64]),
     ("moc_solver_set_fixed_source_by_fsr", C.c_int, [_P, C.c_int64, C.c_int32, C.c_double]),
     ("moc_device_copy_calibration", C.c_int, [C.c_int, C.c_int64, C.c_int32,
                                               C.POINTER(C.c_double)]),

@@ -244,6 +244,17 @@ int moc_solver_allow_negative_fluxes(moc_solver_t* s, int allow) {
   MOC_SOLVER_CALL(s, s->impl->allowNegativeFluxes(allow != 0));
 }
 
+int moc_solver_compute_fsr_sources(moc_solver_t* s, int32_t iteration, double* sources, int64_t n) {
+  MOC_SOLVER_CALL(s, s->impl->computeFSRSources(iteration, sources, n));
+}
+
+int moc_solver_set_scatter_precision(moc_solver_t* s, int mode, int* in_use) {
+  MOC_SOLVER_CALL(s, {
+    s->impl->setScatterPrecision(mode);
+    if (in_use) *in_use = s->impl->scatterPrecisionInUse();
+  });
+}
+
 int moc_solver_set_fixed_source_by_fsr(moc_solver_t* s, int64_t fsr, int32_t group,
                                        double source) {
   MOC_SOLVER_CALL(s, s->impl->setFixedSourceByFSR(fsr, group, source));

@@ -136,6 +136,13 @@ class HipSolver {
   void useGraphs(bool on) { use_graphs_ = on; }
   int device() const { return device_; }
   void setFixedSourceByFSR(int64_t fsr, int group, double source);
+  // Arithmetic of the scattering-source contraction (MOC_SCATTER_*);
+  // scatterPrecisionInUse resolves MOC_SCATTER_AUTO for the current groups.
+  void setScatterPrecision(int mode);
+  // computeFSRSources(iteration) from the current flux and keff; out [N*G]
+  // receives the sources per steradian (the reference's _reduced_sources).
+  void computeFSRSources(int iteration, double* out, int64_t n);
+  int scatterPrecisionInUse() const;
 
   void computeEigenvalue(int max_iters, int res_type);
   void computeFlux(int max_iters, bool only_fixed_source);
@@ -176,6 +183,9 @@ class HipSolver {
   void launchAsyncBatch();
   void finishSolve(int max_iters);
   void prepareLinearSource();
+  void prepareScatter();
+  void buildScatterTiles();
+  void launchScatter(hipStream_t s);
   void buildPositionStream();
   FsrArgs fsrArgs();
   void refreshStabilization();
@@ -261,6 +271,16 @@ class HipSolver {
   double* d_Qxy_ = nullptr;            // [N*G*2]
   float4* d_qls_ = nullptr;            // [N*G]
   float* d_tally_xy_ = nullptr;        // [copies][N*G*2]
+  // scattering contraction on the matrix cores / FP32 fallback
+  int scatter_mode_ = MOC_SCATTER_AUTO;
+  float* d_scat_ = nullptr;            // [C][N][G+1]
+  int32_t* d_scat_rows_ = nullptr;     // [tiles*16] FSRs grouped by material (-1 pad)
+  int32_t* d_scat_tile_mat_ = nullptr; // [tiles]
+  uint32_t* d_scat_b_ = nullptr;       // [M][ks][nt][64] x 4 words of split fp16 fragments
+  float* d_scat_colmul_ = nullptr;     // [M][nt*16]
+  double* d_ss_colsum_ = nullptr;      // [M*G] sum over destinations of sigma_s
+  int64_t scat_ntiles_ = 0;
+  bool scat_tiles_ok_ = false, scat_frag_ok_ = false;
   struct GeoHold;
   std::shared_ptr<GeoHold> geo_hold_;  // set when the geometry buffers are shared
   // captured iteration batch

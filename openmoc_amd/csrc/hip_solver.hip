@@ -524,26 +524,36 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
     for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p] * scale : 0.f;
   }
   const int n = ti.y;
-  const size_t row0 = (size_t)ti.x * a.tpw + slot;
-  // one segment of look-ahead on the stream loads
-  float len_n = a.seg_len[row0];
-  int key_n = a.seg_key[row0];
-  float2 pos_n = L.seg_pos[row0];
+  // Two-deep pipeline: the streams of segment r + 2 and the sources of
+  // segment r + 1 are loaded before segment r is integrated.  Rows past the
+  // tile (the next tile's or the zero-length tail rows keyed to valid FSRs)
+  // are read but never integrated; at most two of them are read
+  // (kTailRows >= 2).
+  const float* lp = a.seg_len + (size_t)ti.x * a.tpw + slot;
+  const int* kp = a.seg_key + (size_t)ti.x * a.tpw + slot;
+  const float2* pp = L.seg_pos + (size_t)ti.x * a.tpw + slot;
+  const size_t tpw = (size_t)a.tpw;
+  float len_a = lp[0], len_b = lp[tpw];
+  int key_a = kp[0], key_b = kp[tpw];
+  float2 pos_a = pp[0], pos_b = pp[tpw];
+  float4 q_a[GPL], q_b[GPL];
+#pragma unroll
+  for (int k = 0; k < GPL; k++) q_a[k] = L.qls[(size_t)(key_a & kFsrMask) * G + g[k]];
   for (int r = 0; r < n; r++) {
-    const float len = len_n;
-    const int key = key_n;
-    const float2 pos = pos_n;
-    if (r + 1 < n) {
-      const size_t o = row0 + (size_t)(r + 1) * a.tpw;
-      len_n = a.seg_len[o];
-      key_n = a.seg_key[o];
-      pos_n = L.seg_pos[o];
-    }
+    const size_t o2 = (size_t)(r + 2) * tpw;
+    const float len_c = lp[o2];
+    const int key_c = kp[o2];
+    const float2 pos_c = pp[o2];
+#pragma unroll
+    for (int k = 0; k < GPL; k++) q_b[k] = L.qls[(size_t)(key_b & kFsrMask) * G + g[k]];
+    const float len = len_a;
+    const int key = key_a;
+    const float2 pos = pos_a;
     const int fsr = key & kFsrMask;
     const float cx = 2.f * pos.x + len * dx, cy = 2.f * pos.y + len * dy;
 #pragma unroll
     for (int k = 0; k < GPL; k++) {
-      const float4 q = L.qls[(size_t)fsr * G + g[k]];
+      const float4 q = q_a[k];
       const float src_flat = q.x + q.y * cx + q.z * cy;
       const float src_lin = dx * q.y + dy * q.z;
       const float tau = q.w * len;
@@ -571,6 +581,14 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
         unsafeAtomicAdd(&txy[2 * i + 1], ay);
       }
     }
+    len_a = len_b;
+    key_a = key_b;
+    pos_a = pos_b;
+    len_b = len_c;
+    key_b = key_c;
+    pos_b = pos_c;
+#pragma unroll
+    for (int k = 0; k < GPL; k++) q_a[k] = q_b[k];
   }
   const int out = a.out_wp[dt];
   float amax = 0.f;
@@ -741,6 +759,12 @@ struct FsrArgs {
   float4* qls;                // [N*G] (Q, Q_x, Q_y, sigma_t) for the sweep
   const double* ilem;         // [N*3] inverse linear expansion matrix
   const double* src_c;        // [N][3][G] source constants
+  // scattering contraction computed beforehand (scatter_mfma_kernel /
+  // scatter_fp32_kernel), or null for the FP64 loops of update_kernel:
+  // [C][N][G+1], column gd = sum_g sigma_s(g->gd) x(g), column G =
+  // sum_g nu-sigma-f(g) x(g), of x = phi (c = 0) and the moments (c = 1, 2)
+  const float* scat;
+  const double* ss_colsum;    // [M*G] sum over destinations of sigma_s(g->d)
 };
 
 namespace {
@@ -805,13 +829,24 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rate = 0.0, rate_s = 0.0, absr = 0.0, leak = 0.0;
   int64_t neg = 0;
-  // (sub, r, l) -> (r, g)
+  // LDS sweep: (sub, r, l) -> (r, g), the order of its partials; otherwise
+  // FSR-major (r, g), the order of the tally and flux arrays
   const int64_t nl = a.N * GB;
-  const int64_t sub = i / nl;
-  const int64_t j = i - sub * nl;
-  const int64_t r = j / GB;
-  const int g = (int)(sub * GB + (j - r * GB));
-  if (i < (int64_t)((G + GB - 1) / GB) * nl && g < G) {
+  int64_t sub = 0, j = 0, r;
+  int g;
+  bool valid;
+  if (a.lds_partials) {
+    sub = i / nl;
+    j = i - sub * nl;
+    r = j / GB;
+    g = (int)(sub * GB + (j - r * GB));
+    valid = i < (int64_t)((G + GB - 1) / GB) * nl && g < G;
+  } else {
+    r = i / G;
+    g = (int)(i - r * G);
+    valid = i < NG;
+  }
+  if (valid) {
     const int m = a.fsr_mat[r];
     const double vol_true = a.fsr_vol[r];
     const double vol = vol_true < kFltEpsilon ? 1e30 : vol_true;
@@ -965,6 +1000,62 @@ __global__ __launch_bounds__(kReduceBlock) void keff_kernel(const double* partia
   }
 }
 
+// From this many groups on, the per-FSR source residual terms are summed by
+// fsr_residual_kernel (one lane per FSR) instead of one lane of every FSR's
+// G lanes in update_kernel.
+constexpr int kFsrResidualGroups = 32;
+
+// FISSION_SOURCE / TOTAL_SOURCE residual term of FSR r (computeResidual,
+// src/CPUSolver.cpp:2113-2252), from the new flux phi * f and the old flux.
+__device__ double source_residual_term(const FsrArgs& a, int64_t r, int res_type, int normalize,
+                                       double f, double k) {
+  const int G = a.G;
+  const int m = a.fsr_mat[r];
+  const bool fis = a.fissionable[m] != 0;
+  const double* nsf = a.nsf + (size_t)m * G;
+  const double* phi = a.phi + (size_t)r * G;
+  const double* old = a.phi_old[a.state->old_sel] + (size_t)r * G;
+  double nw = 0.0, od = 0.0;
+  if (fis) {
+    for (int g = 0; g < G; g++) {
+      nw += (normalize ? phi[g] * f : phi[g]) * nsf[g];
+      od += old[g] * nsf[g];
+    }
+    if (res_type == MOC_TOTAL_SOURCE) {
+      nw /= k;
+      od /= k;
+    }
+  }
+  if (res_type == MOC_TOTAL_SOURCE) {
+    // sum over d and g of sigma_s(g->d) x(g) = sum_g colsum(g) x(g)
+    const double* cs = a.ss_colsum + (size_t)m * G;
+    for (int g = 0; g < G; g++) {
+      nw += cs[g] * (normalize ? phi[g] * f : phi[g]);
+      od += cs[g] * old[g];
+    }
+  }
+  if ((res_type == MOC_TOTAL_SOURCE || fis) && od > 0.0) {
+    const double d = (nw - od) / od;
+    return d * d;
+  }
+  return 0.0;
+}
+
+// Source residual terms for G >= kFsrResidualGroups, one thread per FSR,
+// after update_kernel (which leaves phi and the current old flux as they
+// were and writes zero partials); per-block sums replace its partials.
+__global__ __launch_bounds__(kReduceBlock) void fsr_residual_kernel(FsrArgs a, int normalize,
+                                                                   int res_type) {
+  if (a.state->converged) return;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0;
+  if (r < a.N)
+    res = source_residual_term(a, r, res_type, normalize, normalize ? a.state->norm_factor : 1.0,
+                               a.state->k);
+  const double s = block_sum(res);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+}
+
 // Source kinds of update_kernel
 constexpr int kSrcTotal = 0;    // computeFSRSources (src/CPUSolver.cpp:1939-2023)
 constexpr int kSrcFission = 1;  // computeFSRFissionSources (src/CPUSolver.cpp:2030-2066)
@@ -1008,51 +1099,43 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
           const double d = (phin - old[gd]) / old[gd];
           res = d * d;
         }
-      } else if (gd == 0) {
-        double nw = 0.0, od = 0.0;
-        if (fis) {
-          for (int g = 0; g < G; g++) {
-            nw += (normalize ? phi[g] * f : phi[g]) * nsf[g];
-            od += old[g] * nsf[g];
-          }
-          if (res_type == MOC_TOTAL_SOURCE) {
-            nw /= k;
-            od /= k;
-          }
-        }
-        if (res_type == MOC_TOTAL_SOURCE) {
-          for (int d = 0; d < G; d++)
-            for (int g = 0; g < G; g++) {
-              nw += ss[d * G + g] * (normalize ? phi[g] * f : phi[g]);
-              od += ss[d * G + g] * old[g];
-            }
-        }
-        if ((res_type == MOC_TOTAL_SOURCE || fis) && od > 0.0) {
-          const double d = (nw - od) / od;
-          res = d * d;
-        }
+      } else if (gd == 0 && G < kFsrResidualGroups) {
+        res = source_residual_term(a, r, res_type, normalize, f, k);
       }
     }
     if (compute_sources) {
       const int it = a.state->iteration + source_iteration_offset;
       double q;
+      const float* sct = a.scat ? a.scat + (size_t)r * (G + 1) : nullptr;
       if (source_kind == kSrcFission) {
         // chi(gd) nu-sigma-f(g) phi(g), without 1/k or fixed sources
         double fsrc = 0.0;
-        if (fis)
-          for (int g = 0; g < G; g++) fsrc += a.chi[(size_t)m * G + gd] * nsf[g] * phi[g];
+        if (fis) {
+          if (sct)
+            fsrc = a.chi[(size_t)m * G + gd] * (double)sct[G];
+          else
+            for (int g = 0; g < G; g++) fsrc += a.chi[(size_t)m * G + gd] * nsf[g] * phi[g];
+        }
         q = fsrc * kOneOverFourPi;
       } else if (source_kind == kSrcScatter) {
         double sc = 0.0;
-        for (int g = 0; g < G; g++) sc += ss[(size_t)gd * G + g] * phi[g];
+        if (sct)
+          sc = (double)sct[gd];
+        else
+          for (int g = 0; g < G; g++) sc += ss[(size_t)gd * G + g] * phi[g];
         q = sc * kOneOverFourPi;
       } else {
-        double new_fis = 0.0;
-        if (fis)
-          for (int g = 0; g < G; g++) new_fis += nsf[g] * (normalize ? phi[g] * f : phi[g]);
-        double sc = 0.0;
-        const double* row = ss + (size_t)gd * G;
-        for (int g = 0; g < G; g++) sc += row[g] * (normalize ? phi[g] * f : phi[g]);
+        double new_fis = 0.0, sc = 0.0;
+        if (sct) {
+          // the contraction is linear: normalising after it is the same
+          if (fis) new_fis = (double)sct[G] * f;
+          sc = (double)sct[gd] * f;
+        } else {
+          if (fis)
+            for (int g = 0; g < G; g++) new_fis += nsf[g] * (normalize ? phi[g] * f : phi[g]);
+          const double* row = ss + (size_t)gd * G;
+          for (int g = 0; g < G; g++) sc += row[g] * (normalize ? phi[g] * f : phi[g]);
+        }
         q = (fis ? a.chi[(size_t)m * G + gd] * new_fis : 0.0) / k + sc;
         if (a.fixed) q += a.fixed[(size_t)r * G + gd];
         q *= kOneOverFourPi;
@@ -1072,13 +1155,19 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
         double src[2];
         for (int c = 0; c < 2; c++) {
           double fis_c = 0.0, sc_c = 0.0;
-          if (fis) {
-            const double cg = a.chi[(size_t)m * G + gd];
-            for (int g = 0; g < G; g++) fis_c += cg * nsf[g] * (mom[2 * g + c] * fm);
-            fis_c /= k;
+          if (sct) {
+            const float* sm = a.scat + ((size_t)(c + 1) * a.N + r) * (G + 1);
+            if (fis) fis_c = a.chi[(size_t)m * G + gd] * ((double)sm[G] * fm) / k;
+            sc_c = (double)sm[gd] * fm;
+          } else {
+            if (fis) {
+              const double cg = a.chi[(size_t)m * G + gd];
+              for (int g = 0; g < G; g++) fis_c += cg * nsf[g] * (mom[2 * g + c] * fm);
+              fis_c /= k;
+            }
+            const double* row = ss + (size_t)gd * G;
+            for (int g = 0; g < G; g++) sc_c += row[g] * (mom[2 * g + c] * fm);
           }
-          const double* row = ss + (size_t)gd * G;
-          for (int g = 0; g < G; g++) sc_c += row[g] * (mom[2 * g + c] * fm);
           src[c] = sc_c + fis_c;
         }
         a.phi_xy_n[2 * i] = mom[2 * gd] * fm;
@@ -1110,6 +1199,189 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
     }
   }
   if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
+}
+
+// ---------------------------------------------------------------------------
+// Scattering-source contraction on the matrix cores
+// ---------------------------------------------------------------------------
+// For every FSR r of material m and component c (the scalar flux, and for the
+// linear source the two flux moments):
+//   y[c][r][gd] = sum_g sigma_s[m](g->gd) x_c[r][g]    gd < G
+//   y[c][r][G]  = sum_g nu-sigma-f[m](g)  x_c[r][g]    (fission rate)
+// (computeFSRSources, src/CPUSolver.cpp:1939-2023; the moment sources of
+// src/CPULSSolver.cpp:386-524).  FSRs are grouped by material into tiles of
+// 16 rows; one wave multiplies a tile's 16 x K block of x by the material's
+// K x (G+1) matrix with v_mfma_f32_16x16x16f16.  Operands are split into
+// fp16 high and low parts (x = hi + lo, each row of x scaled by a power of
+// two so its largest entry lies in [0.5, 1); each matrix column likewise),
+// and hi*hi + hi*lo + lo*hi is accumulated in FP32: about 22 significant
+// bits, against 11 for a plain fp16 product.
+constexpr int kScatRows = 16;
+constexpr int kMfmaMinGroups = 32;   // automatic choice: matrix cores from 32 groups on
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct ScatArgs {
+  int64_t N;
+  int G;
+  int ks_n;                    // K steps of 16 (ceil(G / 16))
+  int nt_n;                    // column tiles of 16 (ceil((G + 1) / 16))
+  int64_t ntiles;
+  const int32_t* rows;         // [ntiles * 16] FSR ids, -1 pads a tile
+  const int32_t* tile_mat;     // [ntiles] material of each tile
+  const uint4* bfrag;          // [M][ks_n][nt_n][64] split fp16 operand fragments
+  const float* colmul;         // [M][nt_n * 16] 2^e undoing each column's scaling
+  const double* sigma_s;       // [M][G][G] dest-major (FP32 path)
+  const double* nsf;           // [M][G] (FP32 path)
+  const int32_t* fsr_mat;      // (FP32 path)
+  const double* x[3];          // components; element g of row r at x[c][(r*G + g)*xs[c]]
+  int xs[3];
+  int C;
+  float* out;                  // [C][N][G + 1]
+  const IterState* state;      // nothing to do once converged (null: always run)
+};
+
+__device__ __forceinline__ void split_f16(double v, _Float16& hi, _Float16& lo) {
+  const float f = (float)v;
+  hi = (_Float16)f;
+  lo = (_Float16)(f - (float)hi);
+}
+
+// 2^-e for the largest magnitude mx = f 2^e, f in [0.5, 1) (1 for mx = 0),
+// with e kept where both 2^e and 2^-e are normal floats.
+__device__ __forceinline__ int scale_exponent(double mx) {
+  if (!(mx > 0.0)) return 0;
+  int e;
+  (void)frexp(mx, &e);
+  return e < -120 ? -120 : (e > 120 ? 120 : e);
+}
+
+// Operand fragments of the matrix side, one thread per (m, ks, nt, lane):
+// lane l holds rows k = 16 ks + 4 (l / 16) + j (j = 0..3) of column
+// n = 16 nt + l % 16, as v_mfma_f32_16x16x16f16 reads its B operand.
+__global__ void scatter_frag_kernel(const double* sigma_s, const double* nsf, int M, int G,
+                                    int ks_n, int nt_n, uint4* bfrag, float* colmul) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)M * ks_n * nt_n * 64;
+  if (t >= total) return;
+  const int lane = (int)(t & 63);
+  int64_t u = t >> 6;
+  const int nt = (int)(u % nt_n);
+  u /= nt_n;
+  const int ks = (int)(u % ks_n);
+  const int m = (int)(u / ks_n);
+  const int n = nt * 16 + (lane & 15);
+  auto coef = [&](int k) -> double {
+    if (k >= G || n > G) return 0.0;
+    return n < G ? sigma_s[((size_t)m * G + n) * G + k] : nsf[(size_t)m * G + k];
+  };
+  double mx = 0.0;
+  for (int k = 0; k < G; k++) mx = fmax(mx, fabs(coef(k)));
+  const int e = scale_exponent(mx);
+  const double s = ldexp(1.0, -e);
+  if (ks == 0 && lane < 16) colmul[(size_t)m * nt_n * 16 + n] = ldexpf(1.f, e);
+  _Float16 hi[4], lo[4];
+  for (int j = 0; j < 4; j++) split_f16(coef(ks * 16 + 4 * (lane >> 4) + j) * s, hi[j], lo[j]);
+  auto pack = [](_Float16 a, _Float16 b) {
+    return (unsigned)__builtin_bit_cast(unsigned short, a) |
+           ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+  };
+  bfrag[t] = make_uint4(pack(hi[0], hi[1]), pack(hi[2], hi[3]), pack(lo[0], lo[1]),
+                        pack(lo[2], lo[3]));
+}
+
+// One wave per tile of 16 FSRs (4 waves per block).  Lane l loads row
+// i = l % 16 of the tile at k = 16 ks + 4 (l / 16) + j, the A operand layout;
+// the accumulator holds rows 4 (l / 16) + j of column 16 nt + l % 16.
+__global__ __launch_bounds__(256) void scatter_mfma_kernel(ScatArgs a) {
+  if (a.state && a.state->converged) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= a.ntiles) return;  // whole waves
+  const int G = a.G;
+  const int i = lane & 15, q = lane >> 4;
+  const int m = a.tile_mat[tile];
+  const int r = a.rows[tile * kScatRows + i];
+  const uint4* bf = a.bfrag + (size_t)m * a.ks_n * a.nt_n * 64 + lane;
+  const float* cm = a.colmul + (size_t)m * a.nt_n * 16;
+  constexpr int kChunk = 5;  // column tiles per pass (80 columns)
+  for (int c = 0; c < a.C; c++) {
+    const double* x = a.x[c];
+    const int xs = a.xs[c];
+    auto xval = [&](int k) -> double {
+      return (r >= 0 && k < G) ? x[((size_t)r * G + k) * xs] : 0.0;
+    };
+    // row scale: largest |x| of the row over the 4 lanes holding it
+    double mx = 0.0;
+    for (int ks = 0; ks < a.ks_n; ks++)
+      for (int j = 0; j < 4; j++) mx = fmax(mx, fabs(xval(ks * 16 + 4 * q + j)));
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    const int e = scale_exponent(mx);
+    const double s = ldexp(1.0, -e);
+    float rmul[4];
+    int rrow[4];
+    for (int j = 0; j < 4; j++) {
+      rmul[j] = ldexpf(1.f, __shfl(e, 4 * q + j, 64));
+      rrow[j] = __shfl(r, 4 * q + j, 64);
+    }
+    float* out = a.out + (size_t)c * a.N * (G + 1);
+    for (int nt0 = 0; nt0 < a.nt_n; nt0 += kChunk) {
+      const int ntc = min(kChunk, a.nt_n - nt0);
+      float4_t acc[kChunk];
+      for (int t = 0; t < kChunk; t++) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < a.ks_n; ks++) {
+        half4_t ah, al;
+        for (int j = 0; j < 4; j++) {
+          _Float16 h, l;
+          split_f16(xval(ks * 16 + 4 * q + j) * s, h, l);
+          ah[j] = h;
+          al[j] = l;
+        }
+#pragma unroll
+        for (int t = 0; t < kChunk; t++) {
+          if (t < ntc) {
+            const uint4 b = bf[((size_t)ks * a.nt_n + nt0 + t) * 64];
+            const half4_t bh = __builtin_bit_cast(half4_t, make_uint2(b.x, b.y));
+            const half4_t bl = __builtin_bit_cast(half4_t, make_uint2(b.z, b.w));
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[t], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kChunk; t++) {
+        const int col = (nt0 + t) * 16 + i;
+        if (t < ntc && col <= G) {
+          const float cmul = cm[col];
+          for (int j = 0; j < 4; j++)
+            if (rrow[j] >= 0) out[(size_t)rrow[j] * (G + 1) + col] = acc[t][j] * rmul[j] * cmul;
+        }
+      }
+    }
+  }
+}
+
+// FP32 fallback of the same contraction on the vector ALUs: one thread per
+// (component, FSR, column), FP32 products and sums.
+__global__ __launch_bounds__(256) void scatter_fp32_kernel(ScatArgs a) {
+  if (a.state && a.state->converged) return;
+  const int G = a.G;
+  const int64_t per = a.N * (G + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= per * a.C) return;
+  const int c = (int)(t / per);
+  const int64_t u = t - (int64_t)c * per;
+  const int64_t r = u / (G + 1);
+  const int col = (int)(u - r * (G + 1));
+  const int m = a.fsr_mat[r];
+  const double* coef = col < G ? a.sigma_s + ((size_t)m * G + col) * G : a.nsf + (size_t)m * G;
+  const double* x = a.x[c] + (size_t)r * G * a.xs[c];
+  float acc = 0.f;
+  for (int g = 0; g < G; g++) acc = fmaf((float)coef[g], (float)x[(size_t)g * a.xs[c]], acc);
+  a.out[t] = acc;
 }
 
 // Fixed-point scale of the next sweep's LDS tally.  Over one sweep,
@@ -1309,6 +1581,12 @@ HipSolver::~HipSolver() {
   dev_free(d_Qxy_);
   dev_free(d_qls_);
   dev_free(d_tally_xy_);
+  dev_free(d_scat_);
+  dev_free(d_scat_rows_);
+  dev_free(d_scat_tile_mat_);
+  dev_free(d_scat_b_);
+  dev_free(d_scat_colmul_);
+  dev_free(d_ss_colsum_);
   dev_free(d_stab_ratio_);
   dev_free(d_Q_);
   dev_free(d_fixed_);
@@ -1656,7 +1934,14 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
       }
     dev_alloc(&d_sigma_a_, (size_t)M * G);
     HIP_CHECK(hipMemcpy(d_sigma_a_, sa.data(), sa.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> cs((size_t)M * G, 0.0);
+    for (int m = 0; m < M; m++)
+      for (int d = 0; d < G; d++)
+        for (int g = 0; g < G; g++) cs[(size_t)m * G + g] += sigma_s[((size_t)m * G + d) * G + g];
+    dev_alloc(&d_ss_colsum_, (size_t)M * G);
+    HIP_CHECK(hipMemcpy(d_ss_colsum_, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  scat_frag_ok_ = false;
   sigma_t_max_ = 0.0;
   for (size_t i = 0; i < (size_t)M * G; i++) sigma_t_max_ = std::max(sigma_t_max_, sigma_t[i]);
   if (G != G_) {
@@ -1717,6 +2002,7 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
   }
   h_fsr_mat_.assign(material, material + num_fsrs);
   h_fsr_vol_.assign(volume, volume + num_fsrs);
+  scat_tiles_ok_ = false;
   layout_tpw_ = 0;
   if (num_fsrs != N_) {
     fixed_on_ = false;
@@ -1753,6 +2039,7 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_tally_xy_, NG * 2 * shape_.tally_copies);
     HIP_CHECK(hipMemset(d_tally_xy_, 0, NG * 2 * shape_.tally_copies * sizeof(float)));
     HIP_CHECK(hipMemset(d_phi_xy_n_, 0, NG * 2 * sizeof(double)));
+    HIP_CHECK(hipMemset(d_phi_xy_, 0, NG * 2 * sizeof(double)));
     ls_ready_ = false;
   }
   dev_alloc(&d_Q_, NG);
@@ -1791,6 +2078,102 @@ void HipSolver::allocateIterationArrays() {
     HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
   }
   psi_cur_ = 0;
+  prepareScatter();
+}
+
+int HipSolver::scatterPrecisionInUse() const {
+  if (scatter_mode_ != MOC_SCATTER_AUTO) return scatter_mode_;
+  return G_ >= kMfmaMinGroups ? MOC_SCATTER_MFMA_F16 : MOC_SCATTER_FP64;
+}
+
+void HipSolver::setScatterPrecision(int mode) {
+  if (mode < MOC_SCATTER_AUTO || mode > MOC_SCATTER_FP32)
+    throw DeviceError(MOC_ERR_INVALID, "unknown scattering precision");
+  if (async_active_) throw DeviceError(MOC_ERR_STATE, "a solve is still running on this solver");
+  HIP_CHECK(hipSetDevice(device_));
+  scatter_mode_ = mode;
+  graph_dirty_ = true;
+  prepareScatter();
+}
+
+// FSRs grouped by material in tiles of kScatRows (ascending FSR ids within a
+// material; the last tile of a material is padded with -1).
+void HipSolver::buildScatterTiles() {
+  std::vector<std::vector<int32_t>> by(M_);
+  for (int64_t r = 0; r < N_; r++) by[h_fsr_mat_[r]].push_back((int32_t)r);
+  std::vector<int32_t> rows, tmat;
+  for (int m = 0; m < M_; m++)
+    for (size_t s = 0; s < by[m].size(); s += kScatRows) {
+      tmat.push_back(m);
+      for (size_t j = 0; j < (size_t)kScatRows; j++)
+        rows.push_back(s + j < by[m].size() ? by[m][s + j] : -1);
+    }
+  scat_ntiles_ = (int64_t)tmat.size();
+  dev_alloc(&d_scat_rows_, rows.size());
+  dev_alloc(&d_scat_tile_mat_, tmat.size());
+  HIP_CHECK(hipMemcpy(d_scat_rows_, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_scat_tile_mat_, tmat.data(), tmat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  scat_tiles_ok_ = true;
+}
+
+// Buffers of the contraction path in use; the matrix-core operand fragments
+// are rebuilt whenever the cross sections change.
+void HipSolver::prepareScatter() {
+  if (!(materials_set_ && fsrs_set_ && tracks_set_)) return;
+  const int path = scatterPrecisionInUse();
+  if (path == MOC_SCATTER_FP64) return;
+  const int C = linear_ ? 3 : 1;
+  dev_alloc(&d_scat_, (size_t)C * N_ * (G_ + 1));
+  if (path != MOC_SCATTER_MFMA_F16) return;
+  if (!scat_tiles_ok_) buildScatterTiles();
+  if (!scat_frag_ok_) {
+    const int ks_n = (G_ + 15) / 16, nt_n = (G_ + 1 + 15) / 16;
+    const int64_t nfrag = (int64_t)M_ * ks_n * nt_n * 64;
+    dev_alloc(&d_scat_b_, (size_t)nfrag * 4);
+    dev_alloc(&d_scat_colmul_, (size_t)M_ * nt_n * 16);
+    hipLaunchKernelGGL(scatter_frag_kernel, dim3(blocks_for(nfrag, 256)), dim3(256), 0, stream_,
+                       d_sigma_s_, d_nsf_, M_, G_, ks_n, nt_n,
+                       reinterpret_cast<uint4*>(d_scat_b_), d_scat_colmul_);
+    HIP_CHECK(hipGetLastError());
+    scat_frag_ok_ = true;
+  }
+}
+
+// The contraction of the current flux (and moments) the next update_kernel
+// reads; nothing on the FP64 path, whose update_kernel loops over groups.
+void HipSolver::launchScatter(hipStream_t s) {
+  const int path = scatterPrecisionInUse();
+  if (path == MOC_SCATTER_FP64) return;
+  if (!d_scat_ || (path == MOC_SCATTER_MFMA_F16 && !(scat_tiles_ok_ && scat_frag_ok_)))
+    throw DeviceError(MOC_ERR_STATE, "scattering contraction buffers are not prepared");
+  ScatArgs a;
+  a.N = N_;
+  a.G = G_;
+  a.ks_n = (G_ + 15) / 16;
+  a.nt_n = (G_ + 1 + 15) / 16;
+  a.ntiles = scat_ntiles_;
+  a.rows = d_scat_rows_;
+  a.tile_mat = d_scat_tile_mat_;
+  a.bfrag = reinterpret_cast<const uint4*>(d_scat_b_);
+  a.colmul = d_scat_colmul_;
+  a.sigma_s = d_sigma_s_;
+  a.nsf = d_nsf_;
+  a.fsr_mat = d_fsr_mat_;
+  a.C = linear_ ? 3 : 1;
+  a.x[0] = d_phi_;
+  a.xs[0] = 1;
+  a.x[1] = linear_ ? d_phi_xy_ : nullptr;
+  a.xs[1] = 2;
+  a.x[2] = linear_ ? d_phi_xy_ + 1 : nullptr;
+  a.xs[2] = 2;
+  a.out = d_scat_;
+  a.state = d_state_;
+  if (path == MOC_SCATTER_MFMA_F16)
+    hipLaunchKernelGGL(scatter_mfma_kernel, dim3(blocks_for(scat_ntiles_, 4)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(scatter_fp32_kernel, dim3(blocks_for((int64_t)a.C * N_ * (G_ + 1), 256)),
+                       dim3(256), 0, s, a);
+  HIP_CHECK(hipGetLastError());
 }
 
 void HipSolver::requireReady() const {
@@ -1961,6 +2344,8 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
 }
 
 FsrArgs HipSolver::fsrArgs() {
+  if (linear_ && !ls_ready_)
+    throw DeviceError(MOC_ERR_STATE, "linear-source data not prepared");
   FsrArgs a;
   a.N = N_;
   a.G = G_;
@@ -1996,6 +2381,8 @@ FsrArgs HipSolver::fsrArgs() {
   a.qls = d_qls_;
   a.ilem = d_ilem_;
   a.src_c = d_src_c_;
+  a.scat = scatterPrecisionInUse() == MOC_SCATTER_FP64 ? nullptr : d_scat_;
+  a.ss_colsum = d_ss_colsum_;
   if (shape_.lds_groups > 0) {
     a.qs = nullptr;
     a.qsub = d_qsub_;
@@ -2030,8 +2417,12 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
                           : (double)N_;
   const int rt = mode == MODE_FLUX ? MOC_SCALAR_FLUX : res_type;
   const unsigned nb_upd = blocks_for((int64_t)N_ * G_, kReduceBlock);
+  if (recompute_sources) launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(nb_upd), dim3(kReduceBlock), 0, stream_, fa,
                      mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1, kSrcTotal);
+  if (rt != MOC_SCALAR_FLUX && G_ >= kFsrResidualGroups)
+    hipLaunchKernelGGL(fsr_residual_kernel, dim3(blocks_for(N_, kReduceBlock)), dim3(kReduceBlock), 0,
+                       stream_, fa, mode == MODE_EIGEN ? 1 : 0, rt);
   hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
                      (int)nb_upd, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
                      tol, mode == MODE_EIGEN ? 0 : 1, sigma_t_max_, recompute_sources ? 1 : 0);
@@ -2041,6 +2432,9 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
 // Fresh state for a solve: zero angular fluxes and tally, initial guess,
 // its normalisation (eigen) and the sources of iteration 0.
 void HipSolver::launchInit(Mode mode, double k0) {
+  // the linear source's per-FSR data (expansion matrices, source constants,
+  // position stream) must exist before its kernels read them
+  if (linear_ && !ls_ready_) prepareLinearSource();
   const size_t NG = (size_t)N_ * G_;
   const size_t psi_n = psiCount();
   IterState st;
@@ -2076,6 +2470,7 @@ void HipSolver::launchInit(Mode mode, double k0) {
                        num_partials_, pstride_, d_state_, (double)N_, 0, 0);
   }
   // normalise (eigen), store old flux, sources of iteration 0
+  launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, mode == MODE_EIGEN ? 1 : 0, 0, 0, 1, 0, kSrcTotal);
   // fixed-point scale of the first sweep (start fluxes are zero); the old
@@ -2166,6 +2561,7 @@ void HipSolver::krylovSweep(int source_kind) {
   FsrArgs fa = fsrArgs();
   fa.leak = nullptr;
   fa.stab_type = -1;
+  launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, 0, 0, 0, 1, 0, source_kind);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
@@ -2176,6 +2572,45 @@ void HipSolver::krylovSweep(int source_kind) {
   HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   state_valid_ = false;
+}
+
+// CPUSolver::computeFSRSources (src/CPUSolver.cpp:1939-2023) on its own:
+// sources of the current flux with the current keff, the negative-source
+// clamp for iterations below 30.
+void HipSolver::computeFSRSources(int iteration, double* out, int64_t n) {
+  requireReady();
+  const size_t NG = (size_t)N_ * G_;
+  if (!out || n != (int64_t)NG) throw DeviceError(MOC_ERR_INVALID, "source buffer must hold N*G values");
+  if (iteration < 0) throw DeviceError(MOC_ERR_INVALID, "iteration must be >= 0");
+  HIP_CHECK(hipSetDevice(device_));
+  if (linear_ && !ls_ready_) prepareLinearSource();
+  if (phi_is_old_) {
+    HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[old_sel_], NG * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+    phi_is_old_ = false;
+  }
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  IterState st;
+  std::memset(&st, 0, sizeof(st));
+  st.k = keff_;
+  st.k_prev = keff_;
+  st.norm_factor = 1.0;
+  st.psi_scale = 1.0;
+  st.scale_cap = scale_cap_;
+  st.iteration = iteration;
+  *h_state_ = st;
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  FsrArgs fa = fsrArgs();
+  fa.leak = nullptr;
+  fa.stab_type = -1;
+  launchScatter(stream_);
+  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(out, d_Q_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  state_valid_ = false;
+  krylov_ready_ = false;
 }
 
 void HipSolver::runSolve(Mode mode, int max_iters, int res_type, double k_fixed) {
@@ -2510,6 +2945,9 @@ size_t HipSolver::solveBytes() const {
              (size_t)M_ * G_ * (G_ + 4) * sizeof(double);
   if (shape_.lds_groups > 0)
     b += nsub * shape_.lds_blocks_per_sub * N_ * gb * sizeof(uint64_t) + nsub * N_ * gb * sizeof(float);
+  if (scatterPrecisionInUse() != MOC_SCATTER_FP64)
+    b += (size_t)N_ * (G_ + 1) * sizeof(float) +
+         (size_t)M_ * ((G_ + 15) / 16) * ((G_ + 16) / 16) * 64 * 16;
   return b;
 }
 

@@ -25,13 +25,14 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL, SCALAR_FLUX,
-                   TOTAL_SOURCE, YAMAMOTO, MOCError, check, lib)
+                   SCATTER_AUTO, SCATTER_FP32, SCATTER_FP64, SCATTER_MFMA_F16, TOTAL_SOURCE,
+                   YAMAMOTO, MOCError, check, lib)
 from .workloads import Problem
 
 __all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "HipLSSolver", "SCALAR_FLUX",
            "FISSION_SOURCE",
            "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "DIAGONAL", "YAMAMOTO", "GLOBAL",
-           "MOCError"]
+           "SCATTER_AUTO", "SCATTER_FP64", "SCATTER_MFMA_F16", "SCATTER_FP32", "MOCError"]
 
 
 def _track_data(tg: "TrackGenerator"):
@@ -250,6 +251,23 @@ class HipSolver:
 
     def allowNegativeFluxes(self, allow: bool) -> None:
         check(lib().moc_solver_allow_negative_fluxes(self._h, int(bool(allow))))
+
+    def setScatterPrecision(self, mode: int = SCATTER_AUTO) -> int:
+        """Arithmetic of the scattering-source contraction: SCATTER_AUTO
+        (matrix cores from 32 groups on, FP64 below), SCATTER_FP64,
+        SCATTER_MFMA_F16 (fp16 MFMA on hi/lo split operands, FP32
+        accumulation) or SCATTER_FP32.  Returns the path in use."""
+        used = C.c_int()
+        check(lib().moc_solver_set_scatter_precision(self._h, int(mode), C.byref(used)))
+        return used.value
+
+    def computeFSRSources(self, iteration: int = 0) -> np.ndarray:
+        """CPUSolver::computeFSRSources (src/CPUSolver.cpp:1939-2023) from the
+        current flux and keff; returns the sources per steradian [N, G]."""
+        n = self._num_fsrs * self._num_groups
+        out = np.empty(n)
+        check(lib().moc_solver_compute_fsr_sources(self._h, int(iteration), _ptr(out), n))
+        return out.reshape(self._num_fsrs, self._num_groups)
 
     def stabilizeTransport(self, stabilization_factor: float,
                            stabilization_type: int = DIAGONAL) -> None:

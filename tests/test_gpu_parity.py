@@ -299,6 +299,24 @@ def test_linear_source_switches_back_to_flat(tracks, golden):
         s.getFluxMoments()
 
 
+def test_linear_source_benchmark_entry_points_without_a_solve(tracks):
+    # runIterations / timeSweeps / computeFSRSources on a fresh linear-source
+    # solver prepare its per-FSR data themselves (regression: they used to
+    # launch the source kernels before the expansion matrices existed)
+    tg = tracks("pin_cell_7g")
+    s = om.HipLSSolver(tg)
+    step, sweep = s.runIterations(3)
+    assert step > 0.0 and sweep > 0.0 and np.isfinite(s.getKeff())
+    assert om.HipLSSolver(tg).timeSweeps(2) > 0.0
+    s2 = om.HipLSSolver(tg)
+    phi = np.random.default_rng(5).uniform(0.5, 1.5, (tg.getNumFSRs(), tg.getNumEnergyGroups()))
+    s2.setFluxes(phi)
+    q = s2.computeFSRSources(30)
+    ref = om.HipSolver(tg)
+    ref.setFluxes(phi)
+    np.testing.assert_allclose(q, ref.computeFSRSources(30), rtol=1e-13)
+
+
 # --- 70 groups ---
 
 def test_70_groups_match_reference(tracks, golden):
