@@ -270,7 +270,7 @@ class HipSolver {
   double* d_phi_xy_n_ = nullptr;       // [N*G*2]
   double* d_Qxy_ = nullptr;            // [N*G*2]
   float4* d_qls_ = nullptr;            // [N*G]
-  float* d_tally_xy_ = nullptr;        // [copies][N*G*2]
+  float* d_tally_xy_ = nullptr;        // [copies][2][N*G] x and y planes
   // scattering contraction on the matrix cores / FP32 fallback
   int scatter_mode_ = MOC_SCATTER_AUTO;
   float* d_scat_ = nullptr;            // [C][N][G+1]

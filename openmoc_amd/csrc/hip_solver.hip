@@ -250,6 +250,15 @@ struct LdsFixedTally {
 // g_begin + gl + k*lpt (k < GPL, valid while < g_begin + g_count), taking
 // sources from `src`, tallying through `tally`, and writes the outgoing
 // angular flux to the linked track's start slot.
+// Running maximum of non-negative floats by their bit patterns (which order
+// like the values).  The current maximum is read first and the atomic is
+// issued only to raise it, so after the first waves most lanes issue none:
+// same-address atomics from every lane of a launch serialise in L2.
+__device__ __forceinline__ void raise_max_bits(unsigned* p, float v) {
+  const unsigned b = __float_as_uint(v);
+  if (b > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, b);
+}
+
 template <int GPL, int EXPM, int NP, class Src, class Tally>
 __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int slot, int dt, int gl,
                                             int lpt, int g_begin, int g_count, const Src& src,
@@ -431,7 +440,7 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   }
   // running max of |start psi| for the next sweep's fixed-point scale
   // (non-negative floats order like their bit patterns)
-  if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
+  if (amax > 0.f) raise_max_bits(&a.psi_max_bits[0], amax);
 }
 
 // Global-atomic sweep: one wave per tile of tpw directed tracks,
@@ -476,7 +485,7 @@ struct LsArgs {
   const float2* seg_pos;    // tile-interleaved [row][tpw]: the segment's entry point
                             // in traversal order, relative to its FSR centroid
   const float4* qls;        // [N*G] (Q, Q_x, Q_y, sigma_t)
-  float* tally_xy;          // [tally_copies][N*G*2]
+  float* tally_xy;          // [tally_copies][2][N*G] (x plane, y plane)
   const float* azim_cos;    // [num_azim/2]
   const float* azim_sin;
 };
@@ -509,7 +518,8 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
   const float dy = dir ? -L.azim_sin[azim] : L.azim_sin[azim];
   const size_t copy = (size_t)(blockIdx.x % a.tally_copies);
   float* tf = a.tally + copy * a.tally_stride;
-  float* txy = L.tally_xy + copy * a.tally_stride * 2;
+  float* tx = L.tally_xy + copy * a.tally_stride * 2;  // x and y planes of this copy
+  float* ty = tx + a.tally_stride;
 
   int g[GPL];
   bool ok[GPL];
@@ -577,8 +587,8 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
       if (ok[k] && len > 0.f) {
         const size_t i = (size_t)fsr * G + g[k];
         unsafeAtomicAdd(&tf[i], acc);
-        unsafeAtomicAdd(&txy[2 * i], ax);
-        unsafeAtomicAdd(&txy[2 * i + 1], ay);
+        unsafeAtomicAdd(&tx[i], ax);
+        unsafeAtomicAdd(&ty[i], ay);
       }
     }
     len_a = len_b;
@@ -604,7 +614,7 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
         }
     }
   }
-  if (amax > 0.f) atomicMax(&a.psi_max_bits[0], __float_as_uint(amax));
+  if (amax > 0.f) raise_max_bits(&a.psi_max_bits[0], amax);
 }
 
 // LDS-privatised sweep.  Block b owns group subset b % nsub (GB groups, one
@@ -752,7 +762,7 @@ struct FsrArgs {
   const double* stab_ratio;   // [G] YAMAMOTO: factor * max |sigma_s(g->g) / sigma_t(g)|
   // linear source (null / 0 for the flat source)
   int linear;
-  const float* tally_xy;      // [tally_copies][N*G*2] sweep moment tallies
+  float* tally_xy;            // [tally_copies][2][N*G] sweep moment tallies (x, y planes)
   double* phi_xy;             // [N*G*2] flux moments of the current sweep
   double* phi_xy_n;           // [N*G*2] normalised flux moments
   double* Qxy;                // [N*G*2] source moments
@@ -874,11 +884,11 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
         double t = 0.0, tx = 0.0, ty = 0.0;
         for (int c = 0; c < a.tally_copies; c++) {
           t += (double)a.tally[c * NG + k];
-          tx += (double)a.tally_xy[(c * NG + k) * 2];
-          ty += (double)a.tally_xy[(c * NG + k) * 2 + 1];
+          tx += (double)a.tally_xy[2 * c * NG + k];
+          ty += (double)a.tally_xy[(2 * c + 1) * NG + k];
           a.tally[c * NG + k] = 0.f;
-          const_cast<float*>(a.tally_xy)[(c * NG + k) * 2] = 0.f;
-          const_cast<float*>(a.tally_xy)[(c * NG + k) * 2 + 1] = 0.f;
+          a.tally_xy[2 * c * NG + k] = 0.f;
+          a.tally_xy[(2 * c + 1) * NG + k] = 0.f;
         }
         f = t / vol;
         f += kFourPi * a.Q[k];
@@ -1195,7 +1205,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
     if (qm > 0.0) {
       // round up so the float bound is not below the double value
       const float qf = __uint_as_float(__float_as_uint((float)qm) + 1u);
-      atomicMax(&a.state->qmax_bits, __float_as_uint(qf));
+      raise_max_bits(&a.state->qmax_bits, qf);
     }
   }
   if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
