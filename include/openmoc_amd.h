@@ -79,6 +79,8 @@ typedef struct {
   double tolerance;        /* from the problem's solver line */
   int32_t max_iters;
   int32_t res_type;
+  int32_t is_3d;           /* 1: explicit 3D tracks (TrackGenerator3D, EXPLICIT_3D) */
+  int32_t num_layers;      /* axial FSR layers (3D) */
 } moc_tracks_info_t;
 
 /* Parses the problem, lays down and traces the tracks (threads<=0: all
@@ -109,6 +111,15 @@ int moc_tracks_export_fsrs(const moc_tracks_t* tracks, int32_t* material /*N*/,
 int moc_tracks_export_quadrature(const moc_tracks_t* tracks, double* phi, double* spacing,
                                  double* azim_weight, double* sin_theta,
                                  double* polar_weight, double* total_weight);
+/* 3D tracks (TrackGenerator3D::get3DTracks, src/TrackGenerator3D.cpp:
+ * 1404-1460; uid order = stacks by azimuthal, xy and polar index, then z):
+ * polar index [T] and z of the start and end points [T]; corrected polar
+ * angles and polar spacings [num_azim/2 * num_polar]
+ * (src/TrackGenerator3D.cpp:750-808); FSR centroid z [N].  Errors on 2D
+ * track sets. */
+int moc_tracks_export_tracks_3d(const moc_tracks_t* tracks, int32_t* polar, double* start_z,
+                                double* end_z, double* theta, double* polar_spacing,
+                                double* fsr_centroid_z);
 /* Materials: sigma_t/nu_sigma_f/chi [M*G], sigma_s [M*G*G] dest-major. */
 int moc_tracks_export_materials(const moc_tracks_t* tracks, double* sigma_t,
                                 double* sigma_s, double* nu_sigma_f, double* chi);
@@ -144,6 +155,13 @@ typedef struct {
   const double* azim_weight;        /* [num_azim/2] azimuthal spacing * azimuthal weight */
   const double* polar_weight;       /* [num_polar] Quadrature::getPolarWeight */
   const double* seg_start;          /* [2S] start point (x, y) of each segment */
+  /* 3D tracks (Track3D, explicit segments): polar index of each track in
+   * [0, num_polar), or NULL for 2D tracks.  With it the sweep is the 3D one
+   * (CPUSolver::tallyScalarFlux 3D branch, src/CPUSolver.cpp:2411-2453): G
+   * fluxes per track end, segment lengths are 3D lengths, the exponential
+   * takes tau itself, and total_weight[azim * num_polar + polar] weights the
+   * whole track (src/TrackTraversingAlgorithms.cpp:902-910). */
+  const int32_t* polar;             /* [T] */
 } moc_track_data_t;
 
 /* device < 0: current device. */

@@ -32,7 +32,7 @@ class TracksInfo(C.Structure):
                 ("num_tracks", C.c_int64), ("num_segments", C.c_int64),
                 ("num_fsrs", C.c_int64), ("max_optical_length", C.c_double),
                 ("tolerance", C.c_double), ("max_iters", C.c_int32),
-                ("res_type", C.c_int32)]
+                ("res_type", C.c_int32), ("is_3d", C.c_int32), ("num_layers", C.c_int32)]
 
 
 class TrackData(C.Structure):
@@ -45,7 +45,8 @@ class TrackData(C.Structure):
                 ("bc_fwd", C.c_void_p), ("bc_bwd", C.c_void_p),
                 ("total_weight", C.c_void_p), ("sin_theta", C.c_void_p),
                 ("azim_phi", C.c_void_p), ("azim_weight", C.c_void_p),
-                ("polar_weight", C.c_void_p), ("seg_start", C.c_void_p)]
+                ("polar_weight", C.c_void_p), ("seg_start", C.c_void_p),
+                ("polar", C.c_void_p)]
 
 
 class RequestResult(C.Structure):
@@ -66,6 +67,7 @@ _SIGS = [
     ("moc_tracks_export_segment_starts", C.c_int, [_P, _P]),
     ("moc_tracks_export_fsrs", C.c_int, [_P, _P, _P, _P, _P]),
     ("moc_tracks_export_quadrature", C.c_int, [_P] * 7),
+    ("moc_tracks_export_tracks_3d", C.c_int, [_P] * 7),
     ("moc_tracks_export_materials", C.c_int, [_P] * 5),
     ("moc_solver_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("moc_solver_destroy", None, [_P]),

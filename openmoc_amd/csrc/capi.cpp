@@ -103,6 +103,22 @@ int moc_tracks_info(const moc_tracks_t* t, moc_tracks_info_t* info) {
   info->tolerance = t->problem.tolerance;
   info->max_iters = t->problem.max_iters;
   info->res_type = residual_code(t->problem.res_type);
+  info->is_3d = t->ts.is_3d ? 1 : 0;
+  info->num_layers = t->ts.num_layers;
+  return MOC_OK;
+}
+
+int moc_tracks_export_tracks_3d(const moc_tracks_t* t, int32_t* polar, double* start_z,
+                                double* end_z, double* theta, double* polar_spacing,
+                                double* fsr_centroid_z) {
+  if (!t) return fail(MOC_ERR_INVALID, "null argument");
+  if (!t->ts.is_3d) return fail(MOC_ERR_INVALID, "not a 3D track set");
+  copy_out(polar, t->ts.polar);
+  copy_out(start_z, t->ts.start_z);
+  copy_out(end_z, t->ts.end_z);
+  copy_out(theta, t->ts.quad.theta);
+  copy_out(polar_spacing, t->ts.quad.polar_spacing);
+  copy_out(fsr_centroid_z, t->ts.fsr_centroid_z);
   return MOC_OK;
 }
 

@@ -201,6 +201,7 @@ class HipSolver {
 
   // sizes
   int G_ = 0, M_ = 0, P_ = 0, num_azim_ = 0, num_polar_ = 0;
+  bool is3d_ = false;                  // 3D tracks: P_ = 1, weight class per track
   int64_t T_ = 0, S_ = 0, N_ = 0;
   int64_t num_fissionable_ = 0;
   bool tracks_set_ = false, materials_set_ = false, fsrs_set_ = false;

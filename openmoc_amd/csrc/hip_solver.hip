@@ -1702,7 +1702,8 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
     throw DeviceError(MOC_ERR_INVALID, "incomplete track data");
   if (td.num_azim < 4 || td.num_azim % 4 || td.num_polar < 2 || td.num_polar % 2)
     throw DeviceError(MOC_ERR_INVALID, "invalid quadrature sizes");
-  if (td.num_polar / 2 > kMaxPolarHalf)
+  const bool is3d = td.polar != nullptr;
+  if (!is3d && td.num_polar / 2 > kMaxPolarHalf)
     throw DeviceError(MOC_ERR_INVALID, "2D sweep supports at most 6 polar angles");
   if (td.num_segments >= (int64_t)1 << 31 || td.num_tracks >= (int64_t)1 << 29)
     throw DeviceError(MOC_ERR_INVALID, "track set too large for 32-bit indexing");
@@ -1710,7 +1711,9 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   if (td.seg_offset[0] != 0 || td.seg_offset[T] != S)
     throw DeviceError(MOC_ERR_INVALID, "seg_offset must start at 0 and end at num_segments");
   const int na2 = td.num_azim / 2;
-  const int P = td.num_polar / 2;
+  // 3D: one polar angle per track; the weight class azim * num_polar + polar
+  // takes the place of the azimuthal index
+  const int P = is3d ? 1 : td.num_polar / 2;
 
   std::vector<int32_t> trk(4 * T), azim(T);
   std::vector<int> in_count(2 * T, 0);
@@ -1719,6 +1722,11 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
     if (e < b) throw DeviceError(MOC_ERR_INVALID, "seg_offset must be non-decreasing");
     if (td.azim[t] < 0 || td.azim[t] >= na2) throw DeviceError(MOC_ERR_INVALID, "azimuthal index out of range");
     azim[t] = td.azim[t];
+    if (is3d) {
+      if (td.polar[t] < 0 || td.polar[t] >= td.num_polar)
+        throw DeviceError(MOC_ERR_INVALID, "polar index out of range");
+      azim[t] = td.azim[t] * td.num_polar + td.polar[t];
+    }
     trk[4 * t] = (int32_t)b;
     trk[4 * t + 1] = (int32_t)(e - b);
     for (int d = 0; d < 2; d++) {
@@ -1762,11 +1770,19 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
     h_order_[2 * i] = 2 * order[i];
     h_order_[2 * i + 1] = 2 * order[i] + 1;
   }
-  std::vector<float> w((size_t)na2 * P);
-  for (int a = 0; a < na2; a++)
-    for (int p = 0; p < P; p++) w[(size_t)a * P + p] = (float)td.total_weight[(size_t)a * td.num_polar + p];
-  for (int p = 0; p < kMaxPolarHalf; p++)
-    inv_sin_[p] = p < P ? (float)(1.0 / td.sin_theta[p]) : 0.f;
+  std::vector<float> w;
+  if (is3d) {
+    w.resize((size_t)na2 * td.num_polar);
+    for (size_t c = 0; c < w.size(); c++) w[c] = (float)td.total_weight[c];
+    inv_sin_[0] = 1.f;  // lengths are 3D lengths
+    for (int p = 1; p < kMaxPolarHalf; p++) inv_sin_[p] = 0.f;
+  } else {
+    w.resize((size_t)na2 * P);
+    for (int a = 0; a < na2; a++)
+      for (int p = 0; p < P; p++) w[(size_t)a * P + p] = (float)td.total_weight[(size_t)a * td.num_polar + p];
+    for (int p = 0; p < kMaxPolarHalf; p++)
+      inv_sin_[p] = p < P ? (float)(1.0 / td.sin_theta[p]) : 0.f;
+  }
 
   dev_alloc(&d_trk_, 4 * T);
   dev_alloc(&d_trk_azim_, T);
@@ -1783,7 +1799,7 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   h_azim_phi_.clear();
   h_azim_w_.clear();
   h_polar_w_.clear();
-  if (td.seg_start && td.azim_phi && td.azim_weight && td.polar_weight) {
+  if (!is3d && td.seg_start && td.azim_phi && td.azim_weight && td.polar_weight) {
     h_seg_start_.assign(td.seg_start, td.seg_start + 2 * S);
     h_azim_phi_.assign(td.azim_phi, td.azim_phi + na2);
     h_azim_w_.assign(td.azim_weight, td.azim_weight + na2);
@@ -1794,6 +1810,7 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   T_ = T;
   S_ = S;
   P_ = P;
+  is3d_ = is3d;
   num_azim_ = td.num_azim;
   num_polar_ = td.num_polar;
   tracks_set_ = true;
@@ -2926,6 +2943,7 @@ void HipSolver::shareGeometry(HipSolver& src) {
   T_ = src.T_;
   S_ = src.S_;
   P_ = src.P_;
+  is3d_ = src.is3d_;
   N_ = src.N_;
   G_ = src.G_;
   num_azim_ = src.num_azim_;
@@ -3100,6 +3118,7 @@ void HipSolver::finishSolve(int max_iters) {
 // ---------------------------------------------------------------------------
 void HipSolver::setLinearSource(bool on) {
   if (on == linear_) return;
+  if (on && is3d_) throw DeviceError(MOC_ERR_INVALID, "the linear source is 2D only");
   if (on && geo_hold_) throw DeviceError(MOC_ERR_INVALID, "linear source on a shared geometry");
   linear_ = on;
   graph_dirty_ = true;

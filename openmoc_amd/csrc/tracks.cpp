@@ -114,6 +114,12 @@ Problem parse_problem(const std::string& text) {
       for (int i = 0; i < 4; i++) p.bc[i] = parse_bc(b[i]);
     } else if (key == "tracks") {
       ss >> p.num_azim >> p.azim_spacing >> p.num_polar >> p.quadrature;
+    } else if (key == "axial") {
+      std::string b0, b1;
+      ss >> p.height >> p.num_layers >> p.z_spacing >> b0 >> b1;
+      if (!ss) throw std::runtime_error("malformed axial line");
+      p.bc_z[0] = parse_bc(b0);
+      p.bc_z[1] = parse_bc(b1);
     } else if (key == "solver") {
       ss >> p.tolerance >> p.max_iters >> p.res_type;
     } else if (key == "end") {
@@ -131,6 +137,14 @@ Problem parse_problem(const std::string& text) {
   if (p.num_polar < 2 || p.num_polar % 2 != 0)
     throw std::runtime_error("num_polar must be a positive even number");
   if (p.azim_spacing <= 0) throw std::runtime_error("azim_spacing must be > 0");
+  if (p.height < 0) throw std::runtime_error("height must be >= 0");
+  if (p.is_3d()) {
+    if (p.num_layers < 1 || p.z_spacing <= 0)
+      throw std::runtime_error("3D problems need >= 1 axial layer and z_spacing > 0");
+    if (p.quadrature != "GL") throw std::runtime_error("3D problems use the GL polar quadrature");
+    if (p.bc_z[0] == BC_PERIODIC || p.bc_z[1] == BC_PERIODIC)
+      throw std::runtime_error("periodic top/bottom boundaries are not supported");
+  }
   // rows given top row first (src/Universe.cpp:1604-1619); stored bottom first
   p.cells.resize(p.nx * p.ny);
   for (int j = 0; j < p.ny; j++)
@@ -373,8 +387,10 @@ struct Tracer {
 
 }  // namespace
 
-TrackSet generate_tracks(const Problem& p, int threads) {
-  TrackSet ts;
+// Azimuthal quadrature, 2D laydown, boundary conditions and links
+// (src/Quadrature.cpp:646-718, src/TrackGenerator.cpp:970-1220), shared by
+// the 2D and the 3D generators.
+static void laydown_2d(const Problem& p, TrackSet& ts) {
   ts.num_groups = p.num_groups;
   const int na = p.num_azim, na2 = na / 2, na4 = na / 4;
   const int np = p.num_polar;
@@ -412,28 +428,6 @@ TrackSet generate_tracks(const Problem& p, int threads) {
     double x2 = (a >= 1) ? 0.5 * (q.phi[a] - q.phi[a - 1]) : q.phi[a];
     q.azim_weight[a] = q.azim_weight[na2 - a - 1] = (x1 + x2) / M_PI;
   }
-  // polar set + total weights w(a,p) = 2pi * w_a * delta_a * w_p * 2 sin(theta_p)
-  std::vector<double> s_half, w_half;
-  polar_set(p.quadrature, np, s_half, w_half);
-  q.sin_theta.assign(np, 0);
-  q.polar_weight.assign(np, 0);
-  q.inv_sin_theta.assign(np, 0);
-  for (int k = 0; k < np / 2; k++) {
-    double theta = std::asin(s_half[k]);
-    double st = std::sin(theta);
-    q.sin_theta[k] = q.sin_theta[np - k - 1] = st;
-    q.polar_weight[k] = q.polar_weight[np - k - 1] = w_half[k];
-    q.inv_sin_theta[k] = q.inv_sin_theta[np - k - 1] = 1.0 / st;
-  }
-  q.total_weight.assign((size_t)na2 * np, 0);
-  for (int a = 0; a < na4; a++)
-    for (int k = 0; k < np / 2; k++) {
-      double w = 2.0 * M_PI * q.azim_weight[a] * q.azim_spacing[a] * q.polar_weight[k] *
-                 2.0 * q.sin_theta[k];
-      for (int aa : {a, na2 - a - 1})
-        for (int kk : {k, np - k - 1}) q.total_weight[(size_t)aa * np + kk] = w;
-    }
-
   // ---- tracks: start/end points (src/TrackGenerator.cpp:1016-1062)
   long T = 0;
   std::vector<long> first(na2 + 1, 0);
@@ -518,6 +512,38 @@ TrackSet generate_tracks(const Problem& p, int threads) {
       }
     }
   }
+
+}
+
+TrackSet generate_tracks(const Problem& p, int threads) {
+  if (p.is_3d()) return generate_tracks_3d(p, threads);
+  TrackSet ts;
+  laydown_2d(p, ts);
+  const int na = p.num_azim, na2 = na / 2, na4 = na / 4;
+  const int np = p.num_polar;
+  Quadrature& q = ts.quad;
+  const long T = ts.num_tracks;
+  // polar set + total weights w(a,p) = 2pi * w_a * delta_a * w_p * 2 sin(theta_p)
+  std::vector<double> s_half, w_half;
+  polar_set(p.quadrature, np, s_half, w_half);
+  q.sin_theta.assign(np, 0);
+  q.polar_weight.assign(np, 0);
+  q.inv_sin_theta.assign(np, 0);
+  for (int k = 0; k < np / 2; k++) {
+    double theta = std::asin(s_half[k]);
+    double st = std::sin(theta);
+    q.sin_theta[k] = q.sin_theta[np - k - 1] = st;
+    q.polar_weight[k] = q.polar_weight[np - k - 1] = w_half[k];
+    q.inv_sin_theta[k] = q.inv_sin_theta[np - k - 1] = 1.0 / st;
+  }
+  q.total_weight.assign((size_t)na2 * np, 0);
+  for (int a = 0; a < na4; a++)
+    for (int k = 0; k < np / 2; k++) {
+      double w = 2.0 * M_PI * q.azim_weight[a] * q.azim_spacing[a] * q.polar_weight[k] *
+                 2.0 * q.sin_theta[k];
+      for (int aa : {a, na2 - a - 1})
+        for (int kk : {k, np - k - 1}) q.total_weight[(size_t)aa * np + kk] = w;
+    }
 
   // ---- ray tracing (parallel over tracks), then first-encounter FSR ids
   Tracer tracer(p);
@@ -632,6 +658,564 @@ void split_segments(TrackSet& ts, const Problem& p, double max_tau) {
   ts.seg_fsr.swap(fsr);
   ts.seg_start.swap(start);
   ts.max_optical_length = std::min(ts.max_optical_length, max_tau);
+}
+
+
+
+// ===========================================================================
+// Explicit 3D tracks
+// ===========================================================================
+namespace {
+
+// Positive roots of the Legendre polynomial P_n, ascending, by Newton's
+// method from the usual cosine initial guesses; Gauss-Legendre weights
+// w = 2 (1 - x^2) / (n P_{n-1}(x))^2 (the reference's getGLWeights form).
+void gauss_legendre_half(int n, std::vector<double>& roots, std::vector<double>& weights) {
+  auto legendre2 = [n](double x, double& pn, double& pn1) {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; k++) {
+      const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    pn = p1;
+    pn1 = p0;
+  };
+  roots.clear();
+  weights.clear();
+  for (int i = 0; i < n / 2; i++) {
+    double x = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double pn, pn1;
+      legendre2(x, pn, pn1);
+      const double dp = n * (x * pn - pn1) / (x * x - 1.0);
+      const double dx = pn / dp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    roots.push_back(x);
+  }
+  std::sort(roots.begin(), roots.end());
+  for (double x : roots) {
+    double pn, pn1;
+    legendre2(x, pn, pn1);
+    weights.push_back(2.0 * (1.0 - x * x) / ((double)n * n * pn1 * pn1));
+  }
+}
+
+struct Track3DRaw {
+  int azim, xy, polar;
+  double x1, y1, z1, x2, y2, z2, theta, phi;
+};
+
+// Position of a boundary point of a z-stack's (s, z) rectangle along its
+// perimeter (s: distance along the 2D track from its start, length L; z in
+// [zmin, zmax]): left side, top, right side, bottom, counter-clockwise.
+inline double perimeter_coord(double s, double z, double L, double zmin, double zmax, double tol) {
+  const double H = zmax - zmin;
+  if (std::fabs(s) <= tol) return z - zmin;
+  if (std::fabs(z - zmax) <= tol) return H + s;
+  if (std::fabs(s - L) <= tol) return H + L + (zmax - z);
+  return 2.0 * H + L + (L - s);  // bottom
+}
+
+}  // namespace
+
+TrackSet generate_tracks_3d(const Problem& p, int threads) {
+  TrackSet ts;
+  laydown_2d(p, ts);
+  ts.is_3d = true;
+  ts.num_layers = p.num_layers;
+  Quadrature& q = ts.quad;
+  const int na = p.num_azim, na2 = na / 2, na4 = na / 4;
+  const int np = p.num_polar, nph = np / 2;
+  const double W = p.width_x(), H = p.width_y(), Z = p.height;
+  const double xmin = -0.5 * W, xmax = 0.5 * W, ymin = -0.5 * H, ymax = 0.5 * H;
+  const double zmin = -0.5 * Z, zmax = 0.5 * Z;
+  const long T2 = ts.num_tracks;
+
+  // ---- Gauss-Legendre polar set, corrected per azimuthal angle
+  // (src/TrackGenerator3D.cpp:750-808, src/Quadrature.cpp:1014-1080)
+  std::vector<double> roots, glw;
+  gauss_legendre_half(np, roots, glw);
+  q.theta.assign((size_t)na2 * np, 0);
+  q.polar_spacing.assign((size_t)na2 * np, 0);
+  q.sin_theta.assign(np, 0);
+  q.inv_sin_theta.assign(np, 0);
+  q.polar_weight.assign(np, 0);
+  for (int k = 0; k < nph; k++) q.polar_weight[k] = q.polar_weight[np - k - 1] = glw[k] / 2.0;
+  std::vector<int> num_l((size_t)na2 * np), num_z((size_t)na2 * np);
+  std::vector<double> dl_eff((size_t)na2 * np), dz_eff((size_t)na2 * np);
+  auto at = [np](int a, int k) { return (size_t)a * np + k; };
+  for (int a = 0; a < na4; a++) {
+    const double phi = q.phi[a];
+    for (int k = 0; k < nph; k++) {
+      const double theta0 = std::acos(roots[k]);
+      const double length = H / std::sin(phi);
+      int nl = (int)std::ceil(length * std::tan(M_PI_2 - theta0) / p.z_spacing);
+      int nz = (int)std::ceil(Z * nl * std::tan(theta0) / length);
+      const double dl = H / (std::sin(phi) * nl);
+      const double dz = Z / nz;
+      const double theta = std::atan(dl / dz);
+      const int as = na2 - a - 1, kc = np - k - 1;
+      for (int aa : {a, as}) {
+        for (int kk : {k, kc}) {
+          num_l[at(aa, kk)] = nl;
+          num_z[at(aa, kk)] = nz;
+          dl_eff[at(aa, kk)] = dl;
+          dz_eff[at(aa, kk)] = dz;
+          q.polar_spacing[at(aa, kk)] = dz * std::sin(theta);
+        }
+        q.theta[at(aa, k)] = theta;
+        q.theta[at(aa, kc)] = M_PI - theta;
+      }
+    }
+  }
+  // w(a,p) = 2 pi * w_a * delta_a * w_p * delta_p (src/Quadrature.cpp:720-741)
+  q.total_weight.assign((size_t)na2 * np, 0);
+  for (int a = 0; a < na2; a++)
+    for (int k = 0; k < np; k++)
+      q.total_weight[at(a, k)] = 2.0 * M_PI * q.azim_weight[a] * q.azim_spacing[a] *
+                                 q.polar_weight[k] * q.polar_spacing[at(a, k)];
+  for (int k = 0; k < np; k++) {
+    q.sin_theta[k] = std::sin(q.theta[at(0, k)]);
+    q.inv_sin_theta[k] = 1.0 / q.sin_theta[k];
+  }
+
+  // ---- 2D track chains along the periodic links (src/TrackGenerator3D.cpp:886-925)
+  std::vector<long> first(na2 + 1, 0);
+  for (int a = 0; a < na2; a++) first[a + 1] = first[a] + ts.num_x[a] + ts.num_y[a];
+  auto id2d = [&](int a, int i) -> long { return first[a] + i; };
+  std::vector<double> len2d(T2);
+  for (long t = 0; t < T2; t++)
+    len2d[t] = std::hypot(ts.end_x[t] - ts.start_x[t], ts.end_y[t] - ts.start_y[t]);
+  std::vector<std::vector<std::vector<long>>> chains(na2);
+  for (int a = 0; a < na2; a++) {
+    chains[a].resize(ts.num_x[a]);
+    for (int x = 0; x < ts.num_x[a]; x++) {
+      int i = x;
+      chains[a][x].push_back(id2d(a, i));
+      while (i < ts.num_y[a]) {
+        i += ts.num_x[a];
+        chains[a][x].push_back(id2d(a, i));
+      }
+    }
+  }
+
+  // ---- 3D tracks along the chains, into z-stacks (azim, xy, polar) in
+  // creation order (src/TrackGenerator3D.cpp:934-1250)
+  std::vector<std::vector<Track3DRaw>> stacks((size_t)T2 * np);
+  auto stack_of = [&](long t2, int k) -> std::vector<Track3DRaw>& { return stacks[(size_t)t2 * np + k]; };
+  const double flt_eps = 1.1920928955078125e-07;  // FLT_EPSILON
+  for (int a = 0; a < na2; a++)
+    for (int x = 0; x < ts.num_x[a]; x++)
+      for (int k = 0; k < np; k++) {
+        const std::vector<long>& ch = chains[a][x];
+        const int nl = num_l[at(a, k)], nz = num_z[at(a, k)];
+        const double dz = dz_eff[at(a, k)], dl = dl_eff[at(a, k)];
+        const double theta = q.theta[at(a, k)];
+        for (int lz = 0; lz < nl + nz; lz++) {
+          // first link and start point (getFirst2DTrackLinkIndex)
+          const long t0 = ch[0];
+          const double phi = ts.phi[t0];
+          const double cph = std::cos(phi), sph = std::sin(phi);
+          double l_start = 0.0;
+          if (k < nph && lz < nl) l_start = H / sph - (lz + 0.5) * dl;
+          else if (k >= nph && lz >= nz) l_start = dl * (lz - nz + 0.5);
+          double x_ext = ts.start_x[t0] - xmin + l_start * cph;
+          double y_ext = ts.start_y[t0] - ymin + l_start * sph;
+          bool nudged = false;
+          if (std::fabs(l_start) > flt_eps) {
+            if (std::fabs(std::round(x_ext / W) * W - x_ext) < kTinyMove ||
+                std::fabs(std::round(y_ext / H) * H - y_ext) < kTinyMove) {
+              l_start += 10 * kTinyMove;
+              x_ext = ts.start_x[t0] - xmin + l_start * cph;
+              y_ext = ts.start_y[t0] - ymin + l_start * sph;
+              nudged = true;
+            }
+          }
+          int link = std::abs((int)std::floor(x_ext / W));
+          double xs = (x_ext < 0.0) ? std::fmod(x_ext, W) + xmax : std::fmod(x_ext, W) + xmin;
+          double ys = std::fmod(y_ext, H) + ymin;
+          double zs, ze;
+          if (k < nph) {
+            zs = zmin + std::max(0., (lz - nl + 0.5)) * dz;
+            ze = zmax + std::min(0., (lz - nz + 0.5)) * dz;
+          } else {
+            zs = zmax + std::min(0., (lz - nz + 0.5)) * dz;
+            ze = zmin + std::max(0., (lz - nl + 0.5)) * dz;
+          }
+          if (nudged) {
+            xs -= 10 * kTinyMove * cph;
+            ys -= 10 * kTinyMove * sph;
+          }
+          // split at the chain's links (set3DTrackData)
+          const int first_link = link;
+          double x2 = xs, y2 = ys, z2 = zs;
+          bool end_of_chain = false;
+          while (!end_of_chain) {
+            if (link >= (int)ch.size()) break;
+            const long t2 = ch[link];
+            const double ph = ts.phi[t2];
+            const double x1 = x2, y1 = y2, z1 = z2;
+            double dl_xy;
+            if (link == first_link)
+              dl_xy = std::hypot(ts.end_x[t2] - x1, ts.end_y[t2] - y1);
+            else
+              dl_xy = len2d[t2];
+            double dl_z;
+            if (k < nph) dl_z = (ze - z1) * std::tan(theta);
+            else dl_z = (z1 - ze) / std::tan(theta - M_PI_2);
+            const double d = std::min(dl_z, dl_xy);
+            x2 = x1 + d * std::cos(ph);
+            y2 = y1 + d * std::sin(ph);
+            if (k < nph) z2 = z1 + d / std::tan(theta);
+            else z2 = z1 - d * std::tan(theta - M_PI_2);
+            if (std::fabs(x2 - x1) < kTinyMove || std::fabs(y2 - y1) < kTinyMove ||
+                std::fabs(z2 - z1) < kTinyMove)
+              break;
+            if (dl_z < dl_xy || ts.xy[t2] >= ts.num_y[a]) end_of_chain = true;
+            Track3DRaw r;
+            r.azim = ts.azim[t2];
+            r.xy = ts.xy[t2];
+            r.polar = k;
+            r.x1 = std::max(xmin, std::min(xmax, x1));
+            r.y1 = std::max(ymin, std::min(ymax, y1));
+            r.z1 = std::max(zmin, std::min(zmax, z1));
+            r.x2 = std::max(xmin, std::min(xmax, x2));
+            r.y2 = std::max(ymin, std::min(ymax, y2));
+            r.z2 = std::max(zmin, std::min(zmax, z2));
+            r.theta = theta;
+            r.phi = ph;
+            stack_of(t2, k).push_back(r);
+            link++;
+            if (!end_of_chain) x2 = (a < na4) ? xmin : xmax;
+          }
+        }
+      }
+
+  // ---- uid order: stacks by (azim, xy, polar), tracks by z index
+  std::vector<Track3DRaw> trk;
+  std::vector<int64_t> stack_first((size_t)T2 * np + 1, 0);
+  for (long t2 = 0; t2 < T2; t2++)
+    for (int k = 0; k < np; k++) {
+      stack_first[(size_t)t2 * np + k] = (int64_t)trk.size();
+      for (const Track3DRaw& r : stack_of(t2, k)) trk.push_back(r);
+    }
+  stack_first[(size_t)T2 * np] = (int64_t)trk.size();
+  const long T = (long)trk.size();
+
+  // ---- links: the track leaving a face continues as the track that starts
+  // at the exit point in the reflected direction (or, traversed backward,
+  // ends there).  Points are matched within each candidate stack through a
+  // sorted perimeter coordinate of the stack's (s, z) rectangle.
+  const double tol = 1e-6;
+  struct Key { double c; long t; };
+  std::vector<std::vector<Key>> starts((size_t)T2 * np), ends((size_t)T2 * np);
+  // 2D track tables of the laydown (ts is refilled with the 3D tracks below)
+  const std::vector<double> sx2 = ts.start_x, sy2 = ts.start_y, ex2 = ts.end_x, ey2 = ts.end_y;
+  const std::vector<double> phi2 = ts.phi;
+  auto s_along = [&](long t2, double x, double y) {
+    return (x - sx2[t2]) * std::cos(phi2[t2]) + (y - sy2[t2]) * std::sin(phi2[t2]);
+  };
+  for (long t = 0; t < T; t++) {
+    const Track3DRaw& r = trk[t];
+    const long t2 = id2d(r.azim, r.xy);
+    const size_t st = (size_t)t2 * np + r.polar;
+    starts[st].push_back({perimeter_coord(s_along(t2, r.x1, r.y1), r.z1, len2d[t2], zmin, zmax, tol), t});
+    ends[st].push_back({perimeter_coord(s_along(t2, r.x2, r.y2), r.z2, len2d[t2], zmin, zmax, tol), t});
+  }
+  for (auto* v : {&starts, &ends})
+    for (auto& s : *v) std::sort(s.begin(), s.end(), [](const Key& a, const Key& b) { return a.c < b.c; });
+  // the track of stack (t2, k) whose start (use_end = false) or end point is (x, y, z)
+  auto find = [&](long t2, int k, bool use_end, double x, double y, double z) -> long {
+    const std::vector<Key>& keys = (use_end ? ends : starts)[(size_t)t2 * np + k];
+    const double L = len2d[t2];
+    const double c = perimeter_coord(s_along(t2, x, y), z, L, zmin, zmax, tol);
+    const double per = 2.0 * (L + (zmax - zmin));
+    for (double cc : {c, c - per, c + per}) {
+      auto it = std::lower_bound(keys.begin(), keys.end(), cc - 1e-5,
+                                 [](const Key& a, double v) { return a.c < v; });
+      for (; it != keys.end() && it->c <= cc + 1e-5; ++it) {
+        const Track3DRaw& o = trk[it->t];
+        const double ox = use_end ? o.x2 : o.x1, oy = use_end ? o.y2 : o.y1, oz = use_end ? o.z2 : o.z1;
+        if (std::fabs(ox - x) < 1e-5 && std::fabs(oy - y) < 1e-5 && std::fabs(oz - z) < 1e-5)
+          return it->t;
+      }
+    }
+    return -1;
+  };
+  ts.num_tracks = T;
+  ts.azim.assign(T, 0);
+  ts.xy.assign(T, 0);
+  ts.polar.assign(T, 0);
+  ts.phi.assign(T, 0);
+  ts.start_x.assign(T, 0);
+  ts.start_y.assign(T, 0);
+  ts.end_x.assign(T, 0);
+  ts.end_y.assign(T, 0);
+  ts.start_z.assign(T, 0);
+  ts.end_z.assign(T, 0);
+  std::vector<int64_t> nf(T, -1), nb(T, -1);
+  std::vector<uint8_t> nff(T, 1), nbf(T, 0), bcf(T, BC_VACUUM), bcb(T, BC_VACUUM);
+  // 2D link tables of the laydown (kept before ts is overwritten below)
+  const std::vector<int64_t> nf2 = ts.next_fwd, nb2 = ts.next_bwd;
+  const std::vector<uint8_t> nff2 = ts.next_fwd_is_fwd, nbf2 = ts.next_bwd_is_fwd;
+  const std::vector<uint8_t> bcf2 = ts.bc_fwd, bcb2 = ts.bc_bwd;
+  long unmatched = 0;
+  for (long t = 0; t < T; t++) {
+    const Track3DRaw& r = trk[t];
+    const long t2 = id2d(r.azim, r.xy);
+    const int k = r.polar, kc = np - 1 - k;
+    const bool up = k < nph;
+    ts.azim[t] = r.azim;
+    ts.xy[t] = r.xy;
+    ts.polar[t] = k;
+    ts.phi[t] = r.phi;
+    ts.start_x[t] = r.x1;
+    ts.start_y[t] = r.y1;
+    ts.start_z[t] = r.z1;
+    ts.end_x[t] = r.x2;
+    ts.end_y[t] = r.y2;
+    ts.end_z[t] = r.z2;
+    // Links are set whatever the boundary condition (as the reference does;
+    // a vacuum end transfers nothing): across a top/bottom face into the
+    // complementary polar angle of the same z-stack, across a side face
+    // along the 2D link, and at an edge (both at once) by a double
+    // reflection into the 2D-linked stack with z reversed.
+    // forward end
+    {
+      const bool on_z = up ? std::fabs(r.z2 - zmax) < tol : std::fabs(r.z2 - zmin) < tol;
+      const long n2 = nf2[t2];
+      const bool f = nff2[t2] != 0;
+      const bool periodic_xy = bcf2[t2] == BC_PERIODIC;
+      if (on_z) {
+        bcf[t] = p.bc_z[up ? 1 : 0];
+        nf[t] = find(t2, kc, false, r.x2, r.y2, r.z2);
+        nff[t] = 1;
+        if (nf[t] < 0 && !periodic_xy) {
+          nf[t] = f ? find(n2, kc, false, r.x2, r.y2, r.z2) : find(n2, k, true, r.x2, r.y2, r.z2);
+          nff[t] = f ? 1 : 0;
+          if (bcf2[t2] == BC_VACUUM) bcf[t] = BC_VACUUM;
+        }
+      } else {
+        bcf[t] = bcf2[t2];
+        // continuing forward keeps the polar index; traversing the next
+        // track backward needs the complementary one (same z direction)
+        const double ex = periodic_xy ? (f ? sx2[n2] : ex2[n2]) : r.x2;
+        const double ey = periodic_xy ? (f ? sy2[n2] : ey2[n2]) : r.y2;
+        nf[t] = f ? find(n2, k, false, ex, ey, r.z2) : find(n2, kc, true, ex, ey, r.z2);
+        nff[t] = f ? 1 : 0;
+      }
+      if (nf[t] < 0 && bcf[t] != BC_VACUUM) unmatched++;
+    }
+    // backward end (start point, travelling against the track)
+    {
+      const bool on_z = up ? std::fabs(r.z1 - zmin) < tol : std::fabs(r.z1 - zmax) < tol;
+      const long n2 = nb2[t2];
+      const bool f = nbf2[t2] != 0;
+      const bool periodic_xy = bcb2[t2] == BC_PERIODIC;
+      if (on_z) {
+        bcb[t] = p.bc_z[up ? 0 : 1];
+        nb[t] = find(t2, kc, true, r.x1, r.y1, r.z1);
+        nbf[t] = 0;
+        if (nb[t] < 0 && !periodic_xy) {
+          nb[t] = f ? find(n2, k, false, r.x1, r.y1, r.z1) : find(n2, kc, true, r.x1, r.y1, r.z1);
+          nbf[t] = f ? 1 : 0;
+          if (bcb2[t2] == BC_VACUUM) bcb[t] = BC_VACUUM;
+        }
+      } else {
+        bcb[t] = bcb2[t2];
+        const double sx = periodic_xy ? (f ? sx2[n2] : ex2[n2]) : r.x1;
+        const double sy = periodic_xy ? (f ? sy2[n2] : ey2[n2]) : r.y1;
+        nb[t] = f ? find(n2, kc, false, sx, sy, r.z1) : find(n2, k, true, sx, sy, r.z1);
+        nbf[t] = f ? 1 : 0;
+      }
+      if (nb[t] < 0 && bcb[t] != BC_VACUUM) unmatched++;
+    }
+  }
+  if (unmatched) throw std::runtime_error("3D laydown: " + std::to_string(unmatched) + " unlinked track ends");
+  ts.next_fwd = nf;
+  ts.next_bwd = nb;
+  ts.next_fwd_is_fwd = nff;
+  ts.next_bwd_is_fwd = nbf;
+  ts.bc_fwd = bcf;
+  ts.bc_bwd = bcb;
+
+  // ---- 3D ray tracing (Geometry::segmentize3D, src/Geometry.cpp:2232-2350):
+  // crossings of the radial surfaces (from the 2D crossings of the track's
+  // projection, t3 = t2 / sin(theta)) and of the axial layer planes, marched
+  // with the reference's TINY_MOVE nudges; the FSR of a segment is the one
+  // containing its nudged start point: (2D region, axial layer).
+  Tracer tracer(p);
+  const double dzl = Z / p.num_layers;
+  const int64_t key2 = (int64_t)p.nx * p.ny * KEY_STRIDE;
+  int nthreads = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  auto parallel = [&](long n, auto&& fn) {
+    const int nt = (int)std::min<long>(nthreads, std::max<long>(1, n));
+    if (nt == 1) {
+      fn(0, 1);
+      return;
+    }
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nt; i++) pool.emplace_back([&, i] { fn(i, nt); });
+    for (auto& th : pool) th.join();
+  };
+  // Every 3D track lies over one 2D track: the radial crossings along each
+  // 2D track (distance from its start) and the region of each interval
+  // between them are found once.
+  struct Line2 { std::vector<double> b; std::vector<int64_t> key; std::vector<int> mat; };
+  std::vector<Line2> lines(T2);
+  parallel(T2, [&](int tid, int nt) {
+    std::vector<double> bp;
+    for (long t2 = tid; t2 < T2; t2 += nt) {
+      const double c = std::cos(phi2[t2]), sn = std::sin(phi2[t2]);
+      tracer.breakpoints(sx2[t2], sy2[t2], c, sn, len2d[t2], bp);
+      Line2& ln = lines[t2];
+      ln.b.push_back(0.0);
+      for (double v : bp) ln.b.push_back(v);
+      ln.b.push_back(len2d[t2]);
+      for (size_t j = 0; j + 1 < ln.b.size(); j++) {
+        const double sm = 0.5 * (ln.b[j] + ln.b[j + 1]);
+        int64_t k2;
+        int m;
+        tracer.classify(sx2[t2] + sm * c, sy2[t2] + sm * sn, k2, m);
+        ln.key.push_back(k2);
+        ln.mat.push_back(m);
+      }
+    }
+  });
+  // The march of one 3D track; emit(length, packed key << 8 | material,
+  // midpoint distance) per segment.  Run twice: to count, then to fill.
+  auto march = [&](long t, auto&& emit) {
+    const Track3DRaw& r = trk[t];
+    const long t2 = id2d(r.azim, r.xy);
+    const Line2& ln = lines[t2];
+    const double st = std::sin(r.theta), uz = std::cos(r.theta);
+    const double L3 = std::sqrt((r.x2 - r.x1) * (r.x2 - r.x1) + (r.y2 - r.y1) * (r.y2 - r.y1) +
+                                (r.z2 - r.z1) * (r.z2 - r.z1));
+    const double sa = s_along(t2, r.x1, r.y1);
+    const double sb = sa + L3 * st;
+    thread_local std::vector<double> bp, bz;
+    bp.clear();
+    const size_t j0 = std::upper_bound(ln.b.begin(), ln.b.end(), sa) - ln.b.begin();
+    for (size_t j = j0; j + 1 < ln.b.size() && ln.b[j] < sb; j++) bp.push_back((ln.b[j] - sa) / st);
+    bz.clear();
+    if (std::fabs(uz) > 0)
+      for (int l = 1; l < p.num_layers; l++) {
+        const double tz = (zmin + l * dzl - r.z1) / uz;
+        if (tz > 0 && tz < L3) bz.push_back(tz);
+      }
+    if (uz < 0) std::reverse(bz.begin(), bz.end());
+    const size_t nrad = bp.size();
+    bp.insert(bp.end(), bz.begin(), bz.end());
+    std::inplace_merge(bp.begin(), bp.begin() + nrad, bp.end());
+    double u = kTinyMove;
+    size_t kk = 0;
+    size_t iv = j0 > 0 ? j0 - 1 : 0;  // 2D interval of the current point
+    while (u < L3) {
+      while (kk < bp.size() && bp[kk] <= u) kk++;
+      const double next = (kk < bp.size()) ? bp[kk] + kTinyMove : L3 + kTinyMove;
+      const double sp = sa + u * st;
+      while (iv + 1 < ln.key.size() && ln.b[iv + 1] <= sp) iv++;
+      int layer = (int)std::floor((r.z1 + u * uz - zmin) / dzl);
+      layer = std::min(std::max(layer, 0), p.num_layers - 1);
+      const int64_t key = (int64_t)layer * key2 + ln.key[iv];
+      const double length = (u == kTinyMove) ? next - kTinyMove : next - u;
+      emit(length, (key << 8) | (int64_t)ln.mat[iv], 0.5 * (u + std::min(next, L3)));
+      u = next;
+    }
+  };
+  std::vector<int64_t> count(T, 0);
+  parallel(T, [&](int tid, int nt) {
+    for (long t = tid; t < T; t += nt) {
+      int64_t c = 0;
+      march(t, [&](double, int64_t, double) { c++; });
+      count[t] = c;
+    }
+  });
+  ts.seg_offset.assign(T + 1, 0);
+  for (long t = 0; t < T; t++) ts.seg_offset[t + 1] = ts.seg_offset[t] + count[t];
+  const long S = (long)ts.seg_offset[T];
+  ts.seg_length.resize(S);
+  ts.seg_fsr.resize(S);
+  ts.seg_start.clear();
+  std::vector<int64_t> packed(S);
+  parallel(T, [&](int tid, int nt) {
+    for (long t = tid; t < T; t += nt) {
+      int64_t k = ts.seg_offset[t];
+      march(t, [&](double length, int64_t pk, double) {
+        ts.seg_length[k] = length;
+        packed[k] = pk;
+        k++;
+      });
+    }
+  });
+
+  // FSR ids by first encounter in uid order
+  std::vector<int32_t> key_to_fsr((size_t)key2 * p.num_layers, -1);
+  ts.num_fsrs = 0;
+  ts.fsr_material.clear();
+  ts.fsr_key.clear();
+  for (long k = 0; k < S; k++) {
+    const int64_t key = packed[k] >> 8;
+    int32_t& id = key_to_fsr[key];
+    if (id < 0) {
+      id = (int32_t)ts.num_fsrs++;
+      ts.fsr_material.push_back((int)(packed[k] & 0xff));
+      ts.fsr_key.push_back(key);
+    }
+    ts.seg_fsr[k] = id;
+  }
+  packed.clear();
+  packed.shrink_to_fit();
+  // volumes and centroids with the 3D volume weight delta_a w_a delta_p w_p
+  // (src/MOCKernel.cpp:86-103), per-thread sums added in thread order
+  const long N = ts.num_fsrs;
+  const int ntv = (int)std::min<long>(nthreads, std::max<long>(1, T));
+  std::vector<std::vector<double>> part(ntv, std::vector<double>(4 * N, 0.0));
+  parallel(T, [&](int tid, int nt) {
+    std::vector<double>& acc = part[tid];
+    for (long t = tid; t < T; t += nt) {
+      const Track3DRaw& r = trk[t];
+      const int a = r.azim, k = r.polar;
+      const double w = q.azim_spacing[a] * q.azim_weight[a] * q.polar_spacing[at(a, k)] * q.polar_weight[k];
+      const double st = std::sin(r.theta), uz = std::cos(r.theta);
+      const double ux = std::cos(r.phi) * st, uy = std::sin(r.phi) * st;
+      int64_t j = ts.seg_offset[t];
+      march(t, [&](double length, int64_t, double um) {
+        const int32_t f = ts.seg_fsr[j++];
+        acc[4 * f] += w * length;
+        acc[4 * f + 1] += w * length * (r.x1 + um * ux);
+        acc[4 * f + 2] += w * length * (r.y1 + um * uy);
+        acc[4 * f + 3] += w * length * (r.z1 + um * uz);
+      });
+    }
+  });
+  ts.fsr_volume.assign(N, 0.0);
+  ts.fsr_centroid.assign(2 * N, 0.0);
+  ts.fsr_centroid_z.assign(N, 0.0);
+  for (int i = 0; i < ntv; i++)
+    for (long f = 0; f < N; f++) {
+      ts.fsr_volume[f] += part[i][4 * f];
+      ts.fsr_centroid[2 * f] += part[i][4 * f + 1];
+      ts.fsr_centroid[2 * f + 1] += part[i][4 * f + 2];
+      ts.fsr_centroid_z[f] += part[i][4 * f + 3];
+    }
+  for (long f = 0; f < N; f++)
+    if (ts.fsr_volume[f] > 0) {
+      ts.fsr_centroid[2 * f] /= ts.fsr_volume[f];
+      ts.fsr_centroid[2 * f + 1] /= ts.fsr_volume[f];
+      ts.fsr_centroid_z[f] /= ts.fsr_volume[f];
+    }
+  double max_tau = 0;
+  std::vector<double> max_sig(p.materials.size(), 0);
+  for (size_t m = 0; m < p.materials.size(); m++)
+    for (double v : p.materials[m].sigma_t) max_sig[m] = std::max(max_sig[m], v);
+  for (long j = 0; j < S; j++)
+    max_tau = std::max(max_tau, ts.seg_length[j] * max_sig[ts.fsr_material[ts.seg_fsr[j]]]);
+  ts.max_optical_length = max_tau;
+  return ts;
 }
 
 }  // namespace moc

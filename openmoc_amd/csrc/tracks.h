@@ -55,12 +55,19 @@ struct Problem {
   double azim_spacing = 0.1;
   int num_polar = 6;
   std::string quadrature = "TY";
+  // 3D (axially extruded lattice; height > 0 selects explicit 3D tracks):
+  // `axial <height> <layers> <z_spacing> <bc_zmin> <bc_zmax>`
+  double height = 0;
+  int num_layers = 1;
+  double z_spacing = 0;
+  BoundaryType bc_z[2] = {BC_REFLECTIVE, BC_REFLECTIVE};  // zmin zmax
   double tolerance = 1e-5;
   int max_iters = 1000;
   std::string res_type = "FISSION_SOURCE";
 
   double width_x() const { return nx * pitch_x; }
   double width_y() const { return ny * pitch_y; }
+  bool is_3d() const { return height > 0; }
 };
 
 // Parses "moc-problem 1" text. Throws std::runtime_error on malformed input.
@@ -76,6 +83,9 @@ struct Quadrature {
   std::vector<double> polar_weight;   // [num_polar]
   std::vector<double> total_weight;   // [num_azim/2][num_polar] w(a,p)
   std::vector<double> inv_sin_theta;  // [num_polar]
+  // 3D only: corrected polar angle and polar spacing per (azim, polar)
+  std::vector<double> theta;          // [num_azim/2][num_polar]
+  std::vector<double> polar_spacing;  // [num_azim/2][num_polar]
 };
 
 // Tracks in uid order plus their segments (SoA) and the FSR table.
@@ -97,14 +107,33 @@ struct TrackSet {
   std::vector<double> fsr_volume;           // [N]
   std::vector<double> fsr_centroid;         // [2N]
   std::vector<int64_t> fsr_key;             // [N] cell*KEY_STRIDE + local region
+                                            //     (3D: + layer * nx*ny*KEY_STRIDE)
   double max_optical_length = 0;            // max sigma_t * length over segments
+  // 3D tracks (explicit Track3D): polar index and z of the end points
+  bool is_3d = false;
+  std::vector<int32_t> polar;               // [T]
+  std::vector<double> start_z, end_z;       // [T]
+  std::vector<double> fsr_centroid_z;       // [N]
+  int num_layers = 1;
 };
 
 constexpr int64_t KEY_STRIDE = 4096;
 
 // Builds the quadrature and the track laydown, traces all tracks and fills
 // FSR volumes / centroids. `threads` <= 0 uses the hardware concurrency.
+// A problem with a height builds explicit 3D tracks (generate_tracks_3d).
 TrackSet generate_tracks(const Problem& p, int threads = 0);
+
+// Explicit 3D tracks of an axially extruded lattice
+// (TrackGenerator3D with EXPLICIT_3D segmentation):
+//   Gauss-Legendre polar set              src/Quadrature.cpp:1014-1290
+//   polar correction, z-stacks, chains    src/TrackGenerator3D.cpp:683-1250
+//   3D ray tracing                        src/Geometry.cpp:2232-2350
+//   3D weights and FSR volumes            src/Quadrature.cpp:674-742,
+//                                         src/MOCKernel.cpp:86-103
+// Links are found geometrically: the track that starts (or, traversed
+// backward, ends) where a track leaves a face, in the reflected direction.
+TrackSet generate_tracks_3d(const Problem& p, int threads = 0);
 
 // Splits every segment whose optical length exceeds `max_tau` into equal
 // pieces (reference TrackGenerator::splitSegments semantics,

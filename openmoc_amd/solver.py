@@ -51,6 +51,10 @@ def _track_data(tg: "TrackGenerator"):
      td.next_fwd_is_fwd, td.next_bwd_is_fwd, td.bc_fwd, td.bc_bwd, td.total_weight,
      td.sin_theta, td.azim_phi, td.azim_weight, td.polar_weight,
      td.seg_start) = [a.ctypes.data for a in keep]
+    if getattr(tg, "polar", None) is not None:
+        keep.append(tg.polar)
+        td.polar = tg.polar.ctypes.data
+        td.seg_start = None
     return td, keep
 
 
@@ -110,8 +114,11 @@ class TrackGenerator:
         self.seg_length = np.zeros(S)
         self.seg_fsr = np.zeros(S, np.int32)
         check(lib().moc_tracks_export_segments(self._h, _ptr(self.seg_length), _ptr(self.seg_fsr)))
-        self.seg_start = np.zeros((S, 2))
-        check(lib().moc_tracks_export_segment_starts(self._h, _ptr(self.seg_start)))
+        if i.is_3d:
+            self.seg_start = np.zeros((0, 2))
+        else:
+            self.seg_start = np.zeros((S, 2))
+            check(lib().moc_tracks_export_segment_starts(self._h, _ptr(self.seg_start)))
         self.fsr_material = np.zeros(N, np.int32)
         self.fsr_volume = np.zeros(N)
         self.fsr_centroid = np.zeros((N, 2))
@@ -133,6 +140,18 @@ class TrackGenerator:
         self.chi = np.zeros((M, G))
         check(lib().moc_tracks_export_materials(self._h, _ptr(self.sigma_t), _ptr(self.sigma_s),
                                                 _ptr(self.nu_sigma_f), _ptr(self.chi)))
+        self.polar = None
+        if i.is_3d:
+            # explicit 3D tracks (TrackGenerator3D, EXPLICIT_3D)
+            self.polar = np.zeros(T, np.int32)
+            self.start_z = np.zeros(T)
+            self.end_z = np.zeros(T)
+            self.theta = np.zeros((na2, npol))
+            self.polar_spacing = np.zeros((na2, npol))
+            self.fsr_centroid_z = np.zeros(N)
+            check(lib().moc_tracks_export_tracks_3d(
+                self._h, _ptr(self.polar), _ptr(self.start_z), _ptr(self.end_z),
+                _ptr(self.theta), _ptr(self.polar_spacing), _ptr(self.fsr_centroid_z)))
 
     # reference TrackGenerator getters
     def getNumTracks(self) -> int:
@@ -371,7 +390,9 @@ class HipSolver:
         p = np.array(psi, np.float32, copy=True).ravel()
         if self._tg is None and not hasattr(self, "_psi_size"):
             raise MOCError(-2, "tracks have not been set")
-        n_psi = (2 * self._tg.getNumTracks() * (self._tg.info.num_polar // 2) * self._num_groups
+        pol = 1 if (self._tg is not None and self._tg.info.is_3d) else (
+            self._tg.info.num_polar // 2 if self._tg is not None else 0)
+        n_psi = (2 * self._tg.getNumTracks() * pol * self._num_groups
                  if self._tg is not None else self._psi_size)
         if q.size != self._num_fsrs * self._num_groups or p.size != n_psi:
             raise MOCError(-1, f"transportSweep expects Q of {self._num_fsrs * self._num_groups} "

@@ -91,6 +91,16 @@ class Problem:
     tolerance: float = 1e-5
     max_iters: int = 1000
     res_type: str = "FISSION_SOURCE"
+    # axially extruded 3D problem (height > 0): explicit 3D tracks with the
+    # Gauss-Legendre polar set, `layers` equal axial FSR layers
+    height: float = 0.0
+    layers: int = 1
+    z_spacing: float = 0.0
+    bc_z: Sequence[str] = ("R", "R")             # zmin zmax
+
+    @property
+    def is_3d(self) -> bool:
+        return self.height > 0
 
     @property
     def num_groups(self) -> int:
@@ -126,6 +136,9 @@ class Problem:
         out.append("cells " + " ".join(str(int(v)) for v in np.asarray(self.cells).ravel()))
         out.append("bc " + " ".join(self.bc))
         out.append(f"tracks {self.num_azim} {self.azim_spacing!r} {self.num_polar} {self.quadrature}")
+        if self.is_3d:
+            out.append(f"axial {self.height!r} {self.layers} {self.z_spacing!r} "
+                       f"{self.bc_z[0]} {self.bc_z[1]}")
         out.append(f"solver {self.tolerance!r} {self.max_iters} {self.res_type}")
         out.append("end")
         return "\n".join(out) + "\n"
@@ -168,6 +181,9 @@ class Problem:
             elif key == "tracks":
                 kw.update(num_azim=int(tok[1]), azim_spacing=float(tok[2]),
                           num_polar=int(tok[3]), quadrature=tok[4])
+            elif key == "axial":
+                kw.update(height=float(tok[1]), layers=int(tok[2]), z_spacing=float(tok[3]),
+                          bc_z=(tok[4], tok[5]))
             elif key == "solver":
                 kw.update(tolerance=float(tok[1]), max_iters=int(tok[2]), res_type=tok[3])
             elif key == "end":
@@ -406,7 +422,51 @@ def pin_cell_7g_tc(seed: int = 5) -> Problem:
                    tolerance=1e-5, max_iters=1000)
 
 
+def pin_cell_3d_7g(seed: int = 1, top: str = "R") -> Problem:
+    """pin_cell_7g extruded to 2 cm in 4 axial FSR layers: explicit 3D tracks,
+    8 azimuthal angles at 0.1 cm, 6 Gauss-Legendre polar angles, 0.5 cm
+    axial spacing, reflective bottom, `top` boundary (R or V)."""
+    p = pin_cell_7g(seed)
+    p.name = "pin_cell_3d_7g" if top == "R" else "pin_cell_3d_vac_7g"
+    p.num_azim, p.azim_spacing, p.num_polar, p.quadrature = 8, 0.1, 6, "GL"
+    p.height, p.layers, p.z_spacing, p.bc_z = 2.0, 4, 0.5, ("R", top)
+    return p
+
+
+def pin_cell_3d_vac_7g(seed: int = 1) -> Problem:
+    """pin_cell_3d_7g with a vacuum top (axial leakage)."""
+    return pin_cell_3d_7g(seed, top="V")
+
+
+def lattice_3x3_3d_7g(seed: int = 4) -> Problem:
+    """lattice_3x3_vac_7g (vacuum on the four sides) extruded to 3 cm in 3
+    axial layers, vacuum bottom and reflective top: explicit 3D tracks, 8
+    azimuthal angles at 0.15 cm, 6 Gauss-Legendre polar angles, 0.6 cm axial
+    spacing."""
+    p = lattice_3x3_vac_7g(seed)
+    p.name = "lattice_3x3_3d_7g"
+    p.num_azim, p.azim_spacing, p.num_polar, p.quadrature = 8, 0.15, 6, "GL"
+    p.height, p.layers, p.z_spacing, p.bc_z = 3.0, 3, 0.6, ("V", "R")
+    return p
+
+
+def lattice_17x17_3d_7g(seed: int = 2, num_azim: int = 16, spacing: float = 0.1,
+                        z_spacing: float = 0.5) -> Problem:
+    """lattice_17x17_7g extruded to 20 cm in 10 axial FSR layers, reflective
+    top and bottom: explicit 3D tracks, 16 azimuthal angles at 0.1 cm, 6
+    Gauss-Legendre polar angles, 0.5 cm axial spacing."""
+    p = lattice_17x17_7g(seed)
+    p.name = "lattice_17x17_3d_7g"
+    p.num_azim, p.azim_spacing, p.num_polar, p.quadrature = num_azim, spacing, 6, "GL"
+    p.height, p.layers, p.z_spacing, p.bc_z = 20.0, 10, z_spacing, ("R", "R")
+    return p
+
+
 WORKLOADS = {
+    "pin_cell_3d_7g": pin_cell_3d_7g,
+    "pin_cell_3d_vac_7g": pin_cell_3d_vac_7g,
+    "lattice_3x3_3d_7g": lattice_3x3_3d_7g,
+    "lattice_17x17_3d_7g": lattice_17x17_3d_7g,
     "pin_cell_70g": pin_cell_70g,
     "lattice_3x3_vac_7g": lattice_3x3_vac_7g,
     "pin_cell_7g_tc": pin_cell_7g_tc,

@@ -109,7 +109,7 @@ View make_view(const OracleProblem* p) {
   View v;
   v.p = p;
   v.G = p->num_groups;
-  v.P = p->num_polar_half;
+  v.P = p->trk_polar ? 1 : p->num_polar_half;
   v.fpt = v.G * v.P;
   v.T = p->num_tracks;
   v.N = p->num_fsrs;
@@ -148,10 +148,71 @@ void fsr_sources(const View& v, const double* phi, double k, int it, bool allow_
   }
 }
 
+// 3D tracks (TransportSweep::onTrack with a Track3D, src/TrackTraversingAlgorithms.cpp:
+// 890-1052; CPUSolver::tallyScalarFlux 3D branch, src/CPUSolver.cpp:2411-2453):
+// G fluxes per track end, delta = (tau psi - l Q) F1(tau) with the segment's 3D
+// length, accumulated unweighted per FSR run and weighted by the track's
+// w(a, p) when flushed (accumulateScalarFluxContribution, :2507-2527).
+template <typename PsiT>
+void sweep3d(const View& v, const double* Q, PsiT* start, double* tally, int threads) {
+  const OracleProblem* p = v.p;
+  const int G = v.G;
+  const int64_t T = v.T;
+  std::memset(tally, 0, sizeof(double) * v.N * G);
+  std::vector<PsiT> boundary(start, start + 2 * T * G);
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+  {
+    std::vector<double> buf(G, 0.0);
+#pragma omp for schedule(dynamic)
+    for (int64_t t = 0; t < T; t++) {
+      const double w = p->total_weight[(int64_t)p->trk_azim[t] * p->num_polar_3d + p->trk_polar[t]];
+      const int64_t s0 = p->seg_offset[t], s1 = p->seg_offset[t + 1];
+      const int64_t ns = s1 - s0;
+      for (int d = 0; d < 2; d++) {
+        PsiT* psi = &boundary[(t * 2 + d) * G];
+        for (int64_t kk = 0; kk < ns; kk++) {
+          const int64_t k = d == 0 ? s0 + kk : s1 - 1 - kk;
+          const int32_t r = p->seg_fsr[k];
+          const double len = p->seg_length[k];
+          const double* st = v.sig_t(r);
+          for (int g = 0; g < G; g++) {
+            const double tau = st[g] * len;
+            const double dl = (tau * (double)psi[g] - len * Q[r * G + g]) * exp_f1(tau);
+            psi[g] = (PsiT)((double)psi[g] - dl);
+            buf[g] += dl;
+          }
+          const bool last = kk == ns - 1;
+          const int64_t knext = d == 0 ? k + 1 : k - 1;
+          const bool flush = (d == 0) ? (!last && p->seg_fsr[knext] != r) : (last || p->seg_fsr[knext] != r);
+          if (flush) {
+            for (int g = 0; g < G; g++) {
+#pragma omp atomic
+              tally[r * G + g] += w * buf[g];
+              buf[g] = 0.0;
+            }
+          }
+        }
+        const uint8_t bc = d == 0 ? p->bc_fwd[t] : p->bc_bwd[t];
+        if (bc == 1 || bc == 2) {
+          const int64_t nt = d == 0 ? p->next_fwd[t] : p->next_bwd[t];
+          const int nd = d == 0 ? !p->next_fwd_is_fwd[t] : !p->next_bwd_is_fwd[t];
+          PsiT* out = start + (nt * 2 + nd) * G;
+          for (int g = 0; g < G; g++) out[g] = psi[g];
+        }
+      }
+    }
+  }
+}
+
 template <typename PsiT>
 void sweep(const View& v, const double* Q, PsiT* start, double* tally, int threads,
            float* leakage = nullptr /* [T] per track, or null */) {
   const OracleProblem* p = v.p;
+  if (p->trk_polar) {
+    sweep3d(v, Q, start, tally, threads);
+    return;
+  }
   const int G = v.G, P = v.P, fpt = v.fpt;
   const int64_t T = v.T;
   std::memset(tally, 0, sizeof(double) * v.N * G);

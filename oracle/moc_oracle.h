@@ -54,6 +54,10 @@ typedef struct {
   const double* azim_weight;       /* [num_azim_half] azimuthal spacing * weight */
   const double* polar_weight;      /* [P] polar weight of the quadrature */
   const double* seg_start;         /* [2S] segment start points (x, y) */
+  /* 3D tracks (NULL for 2D): polar index per track; total_weight is then
+   * [num_azim_half][num_polar_3d] and one polar angle travels per track */
+  const int32_t* trk_polar;        /* [T] */
+  int num_polar_3d;
 } OracleProblem;
 
 enum { ORACLE_SCALAR_FLUX = 0, ORACLE_FISSION_SOURCE = 1, ORACLE_TOTAL_SOURCE = 2 };

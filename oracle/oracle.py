@@ -30,7 +30,7 @@ class OracleProblemC(C.Structure):
             "next_fwd_is_fwd", "next_bwd_is_fwd", "bc_fwd", "bc_bwd", "total_weight",
             "inv_sin_theta", "sigma_t", "sigma_s", "nu_sigma_f", "chi", "fsr_material",
             "fsr_volume", "fixed_source", "azim_phi", "azim_weight", "polar_weight",
-            "seg_start")]
+            "seg_start", "trk_polar")] + [("num_polar_3d", C.c_int)]
 
 
 class OracleOptionsC(C.Structure):
@@ -100,11 +100,11 @@ class OracleProblem:
                  next_fwd, next_bwd, next_fwd_is_fwd, next_bwd_is_fwd, bc_fwd, bc_bwd,
                  total_weight, sin_theta, sigma_t, sigma_s, nu_sigma_f, chi, fsr_material,
                  fsr_volume, fixed_source=None, azim_phi=None, azim_weight=None,
-                 polar_weight=None, seg_start=None):
+                 polar_weight=None, seg_start=None, polar=None):
         c = np.ascontiguousarray
-        P = num_polar // 2
+        P = num_polar // 2 if polar is None else 1
         tw = np.asarray(total_weight, np.float64).reshape(num_azim // 2, -1)
-        if tw.shape[1] != P:
+        if polar is None and tw.shape[1] != P:
             tw = tw[:, :P]
         self.a = dict(
             seg_offset=c(seg_offset, np.int64), seg_length=c(seg_length, np.float64),
@@ -114,12 +114,15 @@ class OracleProblem:
             next_bwd_is_fwd=c(next_bwd_is_fwd, np.uint8),
             bc_fwd=c(bc_fwd, np.uint8), bc_bwd=c(bc_bwd, np.uint8),
             total_weight=c(tw, np.float64),
-            inv_sin_theta=c(1.0 / np.asarray(sin_theta, np.float64)[:P]),
+            inv_sin_theta=c(1.0 / np.asarray(sin_theta, np.float64)[:max(P, 1)]),
             sigma_t=c(sigma_t, np.float64), sigma_s=c(sigma_s, np.float64),
             nu_sigma_f=c(nu_sigma_f, np.float64), chi=c(chi, np.float64),
             fsr_material=c(fsr_material, np.int32), fsr_volume=c(fsr_volume, np.float64))
         if fixed_source is not None:
             self.a["fixed_source"] = c(fixed_source, np.float64).ravel()
+        if polar is not None:
+            # 3D tracks: one polar angle per track, w(a, p) over all polar angles
+            self.a["trk_polar"] = c(polar, np.int32)
         if seg_start is not None:
             # linear-source data: track angle and azimuthal weight (spacing *
             # weight) per azimuthal index, polar weights, segment start points
@@ -135,6 +138,7 @@ class OracleProblem:
         s.num_groups, s.num_polar_half, s.num_azim_half = self.G, P, num_azim // 2
         s.num_tracks, s.num_fsrs = self.T, self.N
         s.num_materials = self.a["sigma_t"].size // self.G
+        s.num_polar_3d = num_polar if polar is not None else 0
         for k, v in self.a.items():
             setattr(s, k, v.ctypes.data)
         self.c = s
@@ -151,7 +155,8 @@ class OracleProblem:
                    nu_sigma_f=tg.nu_sigma_f, chi=tg.chi, fsr_material=tg.fsr_material,
                    fsr_volume=tg.fsr_volume, fixed_source=fixed_source, azim_phi=tg.phi,
                    azim_weight=tg.azim_spacing * tg.azim_weight, polar_weight=tg.polar_weight,
-                   seg_start=tg.seg_start)
+                   seg_start=None if getattr(tg, "polar", None) is not None else tg.seg_start,
+                   polar=getattr(tg, "polar", None))
 
     def _opts(self, max_iters, tol, res_type, psi_fp64, threads, balance=False,
               stabilization=None):
@@ -262,11 +267,16 @@ def from_ref_dump(d: dict, problem, polar_weight=None) -> OracleProblem:
     off[1:] = np.cumsum(d["trk_num_segments"])
     na = int(d["num_azim"])
     npol = int(d["num_polar"])
-    sin_theta = d["quad_sin_theta"].reshape(na // 2, npol)[0]
+    polar = d.get("trk_polar")
+    if polar is not None:
+        # 3D dump: sin(theta) is not used by the 3D sweep
+        sin_theta = np.ones(npol)
+    else:
+        sin_theta = d["quad_sin_theta"].reshape(na // 2, npol)[0]
     # segment start points from the track start and the segment lengths
     # (linear-source data; absent when the dump has no track starts)
     seg_start = None
-    if "trk_start" in d and "quad_phi" in d:
+    if polar is None and "trk_start" in d and "quad_phi" in d:
         phi = np.asarray(d["quad_phi"])[np.asarray(d["trk_azim"])]
         st0 = d["trk_start"].reshape(-1, 2)
         lens = d["seg_length"]
@@ -288,7 +298,7 @@ def from_ref_dump(d: dict, problem, polar_weight=None) -> OracleProblem:
         azim_phi=d.get("quad_phi"),
         azim_weight=(d["quad_azim_spacing"] * d["quad_azim_weight"]
                      if "quad_azim_spacing" in d else None),
-        polar_weight=polar_weight, seg_start=seg_start)
+        polar_weight=polar_weight, seg_start=seg_start, polar=polar)
 
 
 def run_reference(problem_text_path: str, out_path: str, threads: int = 1, fp: str = "f64",

@@ -29,6 +29,7 @@
 #include "Geometry.h"
 #include "Quadrature.h"
 #include "TrackGenerator.h"
+#include "TrackGenerator3D.h"
 #include "log.h"
 
 #include <chrono>
@@ -63,6 +64,10 @@ struct Spec {
   double tol = 1e-5;
   int max_iters = 1000;
   std::string res_type = "FISSION_SOURCE";
+  double height = 0;        // > 0: axially extruded 3D problem
+  int layers = 1;
+  double z_spacing = 0;
+  std::string bcz[2] = {"R", "R"};
 };
 
 std::vector<double> read_doubles(std::istringstream& ss) {
@@ -103,6 +108,7 @@ Spec parse(const char* path) {
     else if (key == "bc") ss >> s.bc[0] >> s.bc[1] >> s.bc[2] >> s.bc[3];
     else if (key == "tracks") ss >> s.num_azim >> s.spacing >> s.num_polar >> s.quad;
     else if (key == "solver") ss >> s.tol >> s.max_iters >> s.res_type;
+    else if (key == "axial") ss >> s.height >> s.layers >> s.z_spacing >> s.bcz[0] >> s.bcz[1];
     else if (key == "end") break;
     else { fprintf(stderr, "unknown key %s\n", key.c_str()); exit(2); }
   }
@@ -235,6 +241,184 @@ int main(int argc, char** argv) {
     if (p.outer_sectors > 1) outer->setNumSectors(p.outer_sectors);
     u->addCell(outer);
     pin_univ.push_back(u);
+  }
+
+  if (s.height > 0) {
+    /* Axially extruded 3D model: the lattice repeated over `layers` equal z
+     * levels inside a root cell bounded by X/Y/Z planes; TrackGenerator3D
+     * with the Gauss-Legendre polar set and EXPLICIT_3D segmentation
+     * (the construction of profile/models/pin-cell/pin-cell-3d.cpp). */
+    const int L = s.layers;
+    Lattice* lat3 = new Lattice();
+    lat3->setWidth(s.px, s.py, s.height / L);
+    std::vector<Universe*> cu((size_t)L * s.nx * s.ny);
+    for (int l = 0; l < L; l++)
+      for (int k = 0; k < s.nx * s.ny; k++) cu[(size_t)l * s.nx * s.ny + k] = pin_univ[s.cells[k]];
+    lat3->setUniverses(L, s.ny, s.nx, cu.data());
+    double hx = 0.5 * s.nx * s.px, hy = 0.5 * s.ny * s.py, hz = 0.5 * s.height;
+    XPlane* xmin = new XPlane(-hx);
+    XPlane* xmax = new XPlane(+hx);
+    YPlane* ymin = new YPlane(-hy);
+    YPlane* ymax = new YPlane(+hy);
+    ZPlane* zmin = new ZPlane(-hz);
+    ZPlane* zmax = new ZPlane(+hz);
+    xmin->setBoundaryType(bc_of(s.bc[0]));
+    xmax->setBoundaryType(bc_of(s.bc[1]));
+    ymin->setBoundaryType(bc_of(s.bc[2]));
+    ymax->setBoundaryType(bc_of(s.bc[3]));
+    zmin->setBoundaryType(bc_of(s.bcz[0]));
+    zmax->setBoundaryType(bc_of(s.bcz[1]));
+    Cell* root_cell = new Cell();
+    root_cell->addSurface(+1, xmin);
+    root_cell->addSurface(-1, xmax);
+    root_cell->addSurface(+1, ymin);
+    root_cell->addSurface(-1, ymax);
+    root_cell->addSurface(+1, zmin);
+    root_cell->addSurface(-1, zmax);
+    root_cell->setFill(lat3);
+    Universe* root = new Universe();
+    root->addCell(root_cell);
+    Geometry* geometry = new Geometry();
+    geometry->setRootUniverse(root);
+    geometry->initializeFlatSourceRegions();
+    TrackGenerator3D* tg = new TrackGenerator3D(geometry, s.num_azim, s.num_polar, s.spacing,
+                                                s.z_spacing);
+    Quadrature* quad = new GLPolarQuad();
+    quad->setNumAzimAngles(s.num_azim);
+    quad->setNumPolarAngles(s.num_polar);
+    tg->setQuadrature(quad);
+    tg->setSegmentFormation(EXPLICIT_3D);
+    tg->setNumThreads(1);
+    tg->generateTracks();
+
+    Writer w(argv[2]);
+    w.l1("num_groups", G);
+    w.l1("num_azim", s.num_azim);
+    w.l1("num_polar", s.num_polar);
+    const int na2 = s.num_azim / 2, np = s.num_polar;
+    {
+      std::vector<double> phi(na2), spc(na2), aw(na2), th(na2 * np), tw(na2 * np), ps(na2 * np),
+          pw(na2 * np);
+      for (int a = 0; a < na2; a++) {
+        phi[a] = quad->getPhi(a);
+        spc[a] = quad->getAzimSpacing(a);
+        aw[a] = quad->getAzimWeight(a);
+        for (int p = 0; p < np; p++) {
+          th[a * np + p] = quad->getTheta(a, p);
+          tw[a * np + p] = quad->getWeight(a, p);
+          ps[a * np + p] = quad->getPolarSpacing(a, p);
+          pw[a * np + p] = quad->getPolarWeight(a, p);
+        }
+      }
+      w.d("quad_phi", phi);
+      w.d("quad_azim_spacing", spc);
+      w.d("quad_azim_weight", aw);
+      w.d("quad_theta", th);
+      w.d("quad_total_weight", tw);
+      w.d("quad_polar_spacing", ps);
+      w.d("quad_polar_weight", pw);
+    }
+    /* 3D tracks in uid order (stacks by azim, xy, polar; z in the stack) */
+    Track3D**** t3 = tg->get3DTracks();
+    int*** tps = tg->getTracksPerStack();
+    std::vector<int32_t> t_azim, t_xy, t_pol, t_nseg, t_nff, t_nbf, t_bcf, t_bcb;
+    std::vector<int64_t> t_uid, t_nf, t_nb, s_fsr;
+    std::vector<double> t_start, t_end, s_len;
+    for (int a = 0; a < na2; a++)
+      for (int i = 0; i < tg->getNumX(a) + tg->getNumY(a); i++)
+        for (int p = 0; p < np; p++)
+          for (int z = 0; z < tps[a][i][p]; z++) {
+            Track3D* tr = &t3[a][i][p][z];
+            t_uid.push_back(tr->getUid());
+            t_azim.push_back(tr->getAzimIndex());
+            t_xy.push_back(tr->getXYIndex());
+            t_pol.push_back(tr->getPolarIndex());
+            t_start.push_back(tr->getStart()->getX());
+            t_start.push_back(tr->getStart()->getY());
+            t_start.push_back(tr->getStart()->getZ());
+            t_end.push_back(tr->getEnd()->getX());
+            t_end.push_back(tr->getEnd()->getY());
+            t_end.push_back(tr->getEnd()->getZ());
+            t_nf.push_back(tr->getTrackNextFwd());
+            t_nb.push_back(tr->getTrackNextBwd());
+            t_nff.push_back(tr->getNextFwdFwd());
+            t_nbf.push_back(tr->getNextBwdFwd());
+            t_bcf.push_back(tr->getBCFwd());
+            t_bcb.push_back(tr->getBCBwd());
+            t_nseg.push_back(tr->getNumSegments());
+            segment* seg = tr->getSegments();
+            for (int k = 0; k < tr->getNumSegments(); k++) {
+              s_len.push_back(seg[k]._length);
+              s_fsr.push_back(seg[k]._region_id);
+            }
+          }
+    w.l("trk_uid", t_uid);
+    w.i("trk_azim", t_azim);
+    w.i("trk_xy", t_xy);
+    w.i("trk_polar", t_pol);
+    w.d("trk_start", t_start);
+    w.d("trk_end", t_end);
+    w.l("trk_next_fwd", t_nf);
+    w.l("trk_next_bwd", t_nb);
+    w.i("trk_next_fwd_is_fwd", t_nff);
+    w.i("trk_next_bwd_is_fwd", t_nbf);
+    w.i("trk_bc_fwd", t_bcf);
+    w.i("trk_bc_bwd", t_bcb);
+    w.i("trk_num_segments", t_nseg);
+    w.d("seg_length", s_len);
+    w.l("seg_fsr", s_fsr);
+    const long N = geometry->getNumFSRs();
+    w.l1("num_fsrs", N);
+    TimedCPUSolver solver(tg);
+    solver.setNumThreads(threads);
+    solver.setConvergenceThreshold(s.tol);
+    if (solve) {
+      residualType rt = FISSION_SOURCE;
+      if (s.res_type == "SCALAR_FLUX") rt = SCALAR_FLUX;
+      else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
+      auto t0 = std::chrono::steady_clock::now();
+      solver.computeEigenvalue(s.max_iters, rt);
+      auto t1 = std::chrono::steady_clock::now();
+      std::vector<double> flux(N * G);
+      for (long r = 0; r < N; r++)
+        for (int g = 0; g < G; g++) flux[r * G + g] = solver.getFlux(r, g + 1);
+      w.d1("keff", solver.getKeff());
+      w.l1("num_iterations", solver.getNumIterations());
+      w.d("flux", flux);
+      w.d1("time_sweep", solver.split("Transport Sweep"));
+      w.d1("time_total", std::chrono::duration<double>(t1 - t0).count());
+      long S3 = (long)s_len.size();
+      w.l1("num_segments", S3);
+      w.l1("num_integrations", 2L * G * S3 * solver.getNumIterations());
+      fprintf(stdout, "keff %.10f iterations %d sweep_s %.6f total_s %.6f tracks %zu segments %ld\n",
+              solver.getKeff(), solver.getNumIterations(), solver.split("Transport Sweep"),
+              std::chrono::duration<double>(t1 - t0).count(), t_uid.size(), S3);
+      FP_PRECISION* vol = tg->getFSRVolumes();
+      std::vector<double> v(N), pt(3 * N);
+      std::vector<int32_t> fm(N);
+      for (long r = 0; r < N; r++) {
+        v[r] = vol[r];
+        Point* q = geometry->getFSRPoint(r);
+        pt[3 * r] = q->getX();
+        pt[3 * r + 1] = q->getY();
+        pt[3 * r + 2] = q->getZ();
+        Material* m = geometry->findFSRMaterial(r);
+        fm[r] = -1;
+        for (size_t k = 0; k < mats.size(); k++) if (mats[k] == m) fm[r] = (int)k;
+      }
+      w.d("fsr_volume", v);
+      w.d("fsr_point", pt);
+      w.i("fsr_material", fm);
+      if (timed_sweeps > 0) {
+        double before = solver.split("Transport Sweep");
+        for (int k = 0; k < timed_sweeps; k++) solver.sweep_once();
+        double dt = solver.split("Transport Sweep") - before;
+        w.d1("timed_sweeps_s", dt);
+        w.l1("timed_sweeps", timed_sweeps);
+        fprintf(stdout, "timed_sweeps %d seconds %.6f\n", timed_sweeps, dt);
+      }
+    }
+    return 0;
   }
 
   Lattice* lat = new Lattice();

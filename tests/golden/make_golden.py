@@ -19,6 +19,12 @@ Outputs (data only: inputs and the reference's outputs):
                             flux moments [N, G, 2] and FSR centroids for
                             pin_cell_7g, lattice_3x3_vac_7g and pin_cell_70g
   pin_cell_70g_ref.npz      70-group keff/flux/iterations
+  pin_cell_3d_7g_ref.npz, pin_cell_3d_vac_7g_ref.npz, lattice_3x3_3d_7g_ref.npz
+                            explicit 3D tracks (TrackGenerator3D, EXPLICIT_3D):
+                            per-track polar index, end points, links, boundary
+                            conditions and segment count, segment lengths and
+                            FSR ids, corrected polar angles and weights, FSR
+                            volumes, keff/flux/iterations
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 """
 import sys
@@ -97,6 +103,19 @@ def main():
         out[f"{tag}_num_iterations"] = np.int64(ds["num_iterations"])
         out[f"{tag}_factor"] = np.float64(factor)
     np.savez_compressed(OUT / "pin_cell_7g_tc_ref.npz", **out)
+
+    for name in ("pin_cell_3d_7g", "pin_cell_3d_vac_7g", "lattice_3x3_3d_7g"):
+        _, d = run(name, threads=1)
+        np.savez_compressed(
+            OUT / f"{name}_ref.npz", flux=d["flux"], seg_length=d["seg_length"],
+            seg_fsr=d["seg_fsr"].astype(np.int32), trk_num_segments=d["trk_num_segments"],
+            trk_azim=d["trk_azim"], trk_xy=d["trk_xy"], trk_polar=d["trk_polar"],
+            trk_start=d["trk_start"], trk_end=d["trk_end"], trk_next_fwd=d["trk_next_fwd"],
+            trk_next_bwd=d["trk_next_bwd"], trk_next_fwd_is_fwd=d["trk_next_fwd_is_fwd"],
+            trk_next_bwd_is_fwd=d["trk_next_bwd_is_fwd"], trk_bc_fwd=d["trk_bc_fwd"],
+            trk_bc_bwd=d["trk_bc_bwd"], quad_theta=d["quad_theta"],
+            quad_total_weight=d["quad_total_weight"], fsr_volume=d["fsr_volume"],
+            fsr_material=d["fsr_material"], **base(d))
 
     _, d = run("pin_cell_70g", threads=1)
     np.savez_compressed(OUT / "pin_cell_70g_ref.npz", flux=d["flux"], **base(d))

@@ -334,3 +334,27 @@ def test_70_groups_linear_source_match_reference(tracks, golden):
     s.computeEigenvalue(1000)
     assert abs(s.getKeff() - float(g["keff"])) < K_TOL
     assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+
+
+# --- explicit 3D tracks ---
+
+@pytest.mark.parametrize("name", ["pin_cell_3d_7g", "pin_cell_3d_vac_7g", "lattice_3x3_3d_7g"])
+def test_3d_eigenvalue_matches_reference(tracks, golden, name):
+    g = golden(name)
+    s = solver_for(tracks(name))
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - float(g["keff"])) < K_TOL
+    assert abs(s.getNumIterations() - int(g["num_iterations"])) <= 2
+    assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
+
+
+def test_3d_single_sweep_matches_oracle(tracks):
+    tg = tracks("lattice_3x3_3d_7g")
+    N, G, T = tg.getNumFSRs(), tg.getNumEnergyGroups(), tg.getNumTracks()
+    rng = np.random.default_rng(11)
+    Q = rng.uniform(0.01, 1.0, N * G)
+    psi = rng.uniform(0.0, 2.0, T * 2 * G).astype(np.float32)   # one polar angle per track
+    tally, psi_out = solver_for(tg).transportSweep(Q, psi)
+    t_ref, p_ref = orc.OracleProblem.from_track_generator(tg).transport_sweep(Q, psi)
+    assert rel_l2(tally, t_ref) < 2e-5
+    np.testing.assert_allclose(psi_out, p_ref, rtol=5e-4, atol=1e-6)

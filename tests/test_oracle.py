@@ -294,3 +294,29 @@ def test_oracle_70_groups_matches_reference(tracks, golden):
     assert abs(sol.keff - float(g["keff"])) < 1e-8
     assert sol.num_iterations == int(g["num_iterations"])
     assert np.linalg.norm(sol.flux.ravel() - g["flux"]) / np.linalg.norm(g["flux"]) < 1e-7
+
+
+@pytest.mark.parametrize("name", ["pin_cell_3d_7g", "pin_cell_3d_vac_7g", "lattice_3x3_3d_7g"])
+def test_oracle_3d_matches_reference(tracks, golden, name):
+    # the 3D sweep (3D segment lengths, F1(tau), w(a, p) per track at the
+    # FSR flush) on this package's tracks, which equal the reference's
+    g = golden(name)
+    sol = orc.OracleProblem.from_track_generator(tracks(name)).eigenvalue(max_iters=1000, tol=1e-5)
+    assert abs(sol.keff - float(g["keff"])) < 1e-12
+    assert sol.num_iterations == int(g["num_iterations"])
+    f, fr = sol.flux.ravel(), g["flux"]
+    assert np.linalg.norm(f - fr) / np.linalg.norm(fr) < 1e-12
+
+
+def test_oracle_3d_axially_uniform_problem_has_uniform_layers(tracks):
+    # reflective top and bottom with axially uniform materials: every axial
+    # layer of a 2D region carries the same flux (up to the axial laydown)
+    tg = tracks("pin_cell_3d_7g")
+    sol = orc.OracleProblem.from_track_generator(tg).eigenvalue(max_iters=1000, tol=1e-7)
+    key2 = 4096
+    by2 = {}
+    for f, k in enumerate(tg.fsr_key % key2):
+        by2.setdefault(int(k), []).append(sol.flux[f])
+    for v in by2.values():
+        v = np.array(v)
+        assert np.abs(v / v.mean(0) - 1).max() < 2e-3

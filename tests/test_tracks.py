@@ -86,3 +86,66 @@ def test_minicore_counts(tracks):
     # 4 fuel assemblies x 289 pins x 32 FSRs + 5 reflector assemblies x 289 blocks
     assert tg.getNumFSRs() == 4 * 289 * 32 + 5 * 289
     assert (tg.bc_fwd == 0).any() and (tg.bc_fwd == 1).any()
+
+
+# --- explicit 3D tracks (TrackGenerator3D, EXPLICIT_3D) ---
+
+@pytest.mark.parametrize("name", ["pin_cell_3d_7g", "pin_cell_3d_vac_7g", "lattice_3x3_3d_7g"])
+def test_3d_tracks_identical_to_reference(tracks, golden, name):
+    g = golden(name)
+    tg = tracks(name)
+    assert tg.info.is_3d == 1
+    assert tg.getNumTracks() == int(g["num_tracks"])
+    assert tg.getNumSegments() == int(g["num_segments"])
+    assert tg.getNumFSRs() == int(g["num_fsrs"])
+    np.testing.assert_array_equal(tg.azim, g["trk_azim"])
+    np.testing.assert_array_equal(tg.xy, g["trk_xy"])
+    np.testing.assert_array_equal(tg.polar, g["trk_polar"])
+    np.testing.assert_allclose(np.c_[tg.start, tg.start_z].ravel(), g["trk_start"], atol=1e-13)
+    np.testing.assert_allclose(np.c_[tg.end, tg.end_z].ravel(), g["trk_end"], atol=1e-13)
+    # links, including the double reflections at edges, and vacuum ends
+    for k in ("next_fwd", "next_bwd", "next_fwd_is_fwd", "next_bwd_is_fwd", "bc_fwd", "bc_bwd"):
+        np.testing.assert_array_equal(getattr(tg, k), g["trk_" + k], err_msg=k)
+    np.testing.assert_array_equal(np.diff(tg.seg_offset), g["trk_num_segments"])
+    np.testing.assert_array_equal(tg.seg_fsr, g["seg_fsr"])          # 3D FSR numbering
+    np.testing.assert_allclose(tg.seg_length, g["seg_length"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tg.theta.ravel(), g["quad_theta"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(tg.total_weight.ravel(), g["quad_total_weight"], rtol=1e-13)
+    np.testing.assert_allclose(tg.fsr_volume, g["fsr_volume"], rtol=1e-12)
+    np.testing.assert_array_equal(tg.fsr_material, g["fsr_material"])
+
+
+@pytest.mark.parametrize("name", ["pin_cell_3d_7g", "lattice_3x3_3d_7g"])
+def test_3d_volumes_and_layers(tracks, name):
+    from openmoc_amd import workloads
+    p = workloads.get(name)
+    tg = tracks(name)
+    assert tg.fsr_volume.sum() == pytest.approx(p.width_x * p.width_y * p.height, rel=1e-12)
+    # every 2D region has all its axial layers; each FSR's track-estimated
+    # centroid lies inside its layer and the layers of a region add up to
+    # the region's 2D area times the height (tracks sample each z-stack
+    # uniformly over the height)
+    key2 = p.nx * p.ny * 4096
+    layer = tg.fsr_key // key2
+    assert sorted(set(layer.tolist())) == list(range(p.layers))
+    dz = p.height / p.layers
+    lo = -p.height / 2 + layer * dz
+    assert np.all(tg.fsr_centroid_z > lo) and np.all(tg.fsr_centroid_z < lo + dz)
+    by2 = {}
+    for f, k in enumerate(tg.fsr_key % key2):
+        by2.setdefault(int(k), []).append(tg.fsr_volume[f])
+    for v in by2.values():
+        assert len(v) == p.layers
+        np.testing.assert_allclose(v, np.mean(v), rtol=0.1)
+
+
+def test_3d_links_are_reciprocal_and_polar_weights_integrate(tracks):
+    tg = tracks("pin_cell_3d_7g")
+    for t in range(tg.getNumTracks()):
+        n, f = tg.next_fwd[t], tg.next_fwd_is_fwd[t]
+        back = tg.next_bwd[n] if f else tg.next_fwd[n]
+        back_f = tg.next_bwd_is_fwd[n] if f else tg.next_fwd_is_fwd[n]
+        assert back == t and back_f == 0
+    # Gauss-Legendre: the polar weights of one hemisphere sum to 1/2 each way
+    w = tg.polar_weight
+    assert w.sum() == pytest.approx(1.0, rel=1e-14)
