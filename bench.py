@@ -217,7 +217,8 @@ def main():
     tg.generateTracks()
     log(f"tracks: {tg.getNumTracks()} tracks, {tg.getNumSegments()} segments, "
         f"{tg.getNumFSRs()} FSRs ({time.perf_counter() - t0:.2f}s)")
-    G, P = tg.getNumEnergyGroups(), tg.info.num_polar // 2
+    # fluxes per track end: G x num_polar/2 in 2D, G in 3D (src/Solver.cpp:433-441)
+    G, P = tg.getNumEnergyGroups(), (1 if tg.info.is_3d else tg.info.num_polar // 2)
     S, T, N = tg.getNumSegments(), tg.getNumTracks(), tg.getNumFSRs()
     w_sweep = 2 * G * P * S
 
