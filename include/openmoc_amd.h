@@ -162,6 +162,10 @@ typedef struct {
    * takes tau itself, and total_weight[azim * num_polar + polar] weights the
    * whole track (src/TrackTraversingAlgorithms.cpp:902-910). */
   const int32_t* polar;             /* [T] */
+  /* 3D only, optional: Track::getXYIndex of each track (the 2D track it lies
+   * over).  With it the tracks over one 2D track (its z-stacks) are swept by
+   * one block with their FSRs' tally in LDS. */
+  const int32_t* xy;                /* [T] */
 } moc_track_data_t;
 
 /* device < 0: current device. */

@@ -46,7 +46,7 @@ class TrackData(C.Structure):
                 ("total_weight", C.c_void_p), ("sin_theta", C.c_void_p),
                 ("azim_phi", C.c_void_p), ("azim_weight", C.c_void_p),
                 ("polar_weight", C.c_void_p), ("seg_start", C.c_void_p),
-                ("polar", C.c_void_p)]
+                ("polar", C.c_void_p), ("xy", C.c_void_p)]
 
 
 class RequestResult(C.Structure):

@@ -202,6 +202,16 @@ class HipSolver {
   // sizes
   int G_ = 0, M_ = 0, P_ = 0, num_azim_ = 0, num_polar_ = 0;
   bool is3d_ = false;                  // 3D tracks: P_ = 1, weight class per track
+  // 3D z-stack groups (tracks over one 2D track), for the LDS stack sweep
+  int32_t ngrp_ = 0;                   // 0: no groups (2D, or no xy given)
+  int32_t max_grp_slots_ = 0;          // most FSRs touched by one group
+  std::vector<int32_t> h_trk_grp_;     // [T] group of each track
+  std::vector<int32_t> h_slot_;        // [S] segment -> slot among its group's FSRs
+  std::vector<int32_t> h_grp_slot0_, h_grp_nslots_, h_slot_fsr_;
+  int32_t* d_grp_ = nullptr;           // [ngrp][4] first tile, tiles, first slot, slots
+  int32_t* d_slot_fsr_ = nullptr;      // [total slots] global FSR of each slot
+  unsigned long long* d_tally_i64_ = nullptr;  // [N*G] fixed-point tally (stack sweep)
+  bool stackMode() const { return is3d_ && ngrp_ > 0 && shape_.lds_groups > 0; }
   int64_t T_ = 0, S_ = 0, N_ = 0;
   int64_t num_fissionable_ = 0;
   bool tracks_set_ = false, materials_set_ = false, fsrs_set_ = false;
@@ -309,6 +319,8 @@ class HipSolver {
   std::vector<float> h_len_;             // host copies for the stream layout
   std::vector<int32_t> h_fsr_, h_trk_, h_order_;
   int layout_tpw_ = 0, layout_ntiles_ = 0;
+  bool layout_stack_ = false;            // streams laid out by z-stack group (3D)
+  std::vector<int32_t> layout_grp_ntiles_;
   int64_t layout_rows_ = 0;
   std::vector<int32_t> mat_fissionable_;  // host copy, per material
   std::vector<int32_t> h_fsr_mat_;        // host copy, per FSR

@@ -701,6 +701,81 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
   for (int i = threadIdx.x; i < nl; i += blockDim.x) out[i] = lds_tally[i];
 }
 
+// 3D z-stack sweep.  A group is the set of 3D tracks over one 2D track (its
+// z-stacks, every polar angle); its segments touch only the FSRs along that
+// 2D line, one per (2D region, axial layer), which get local slots 0..n-1.
+// Block (group, group subset) keeps the fixed-point tally and the
+// {Q/sigma_t, 1/V} table of its slots in LDS, sweeps the group's tiles, and
+// adds its tally to the global int64 tally (exact, order-independent).  The
+// segment keys of the stream are local slots | material << 24.
+struct StackParams {
+  const int4* grp;             // [ngrp] first tile, tiles, first slot, slots
+  const int32_t* slot_fsr;     // global FSR of each slot
+  unsigned long long* tally;   // [N*G] fixed-point sums of contribution / V
+  const float* qsub;           // [nsub][N][GB] Q/sigma_t, subset-major
+  const float* inv_vol;        // [N]
+  const double* sigma_t;       // [M][G]
+  int M;
+  int GB;
+  int nsub;
+  int ngrp;
+  int max_slots;               // LDS layout stride (largest group)
+};
+
+template <int EXPM>
+__global__ __launch_bounds__(1024) void sweep3d_stack_kernel(SweepArgs a, StackParams L) {
+  extern __shared__ unsigned long long lds64[];
+  if (a.state->converged) return;
+  // the nsub subset blocks of one group are dispatched to the same XCD
+  // (blocks are dealt to the 8 XCDs round-robin), so they share its L2
+  const int x = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd;
+  const int grp = x + kNumXcd * (j / L.nsub);
+  const int sub = j % L.nsub;
+  if (grp >= L.ngrp) return;  // uniform per block
+  const int4 gi = L.grp[grp];
+  const int g0 = sub * L.GB;
+  const int gb = min(L.GB, a.G - g0);
+  const int nl = L.max_slots * L.GB;
+  const int ns = gi.w * L.GB;
+  const float s32 = (float)(a.state->tally_scale * 0x1p-32);
+  unsigned long long* lds_tally = lds64;
+  float2* lds_qi = reinterpret_cast<float2*>(lds64 + nl);
+  float* lds_sig = reinterpret_cast<float*>(lds_qi + nl);
+  const float* qg = L.qsub + (size_t)sub * a.N * L.GB;
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    lds_tally[i] = 0ull;
+    const int s = i / L.GB, l = i - s * L.GB;
+    const int f = L.slot_fsr[gi.z + s];
+    lds_qi[i] = make_float2(qg[(size_t)f * L.GB + l], L.inv_vol[f] * s32);
+  }
+  for (int i = threadIdx.x; i < L.M * L.GB; i += blockDim.x) {
+    const int m = i / L.GB, l = i - m * L.GB;
+    lds_sig[i] = (float)L.sigma_t[(size_t)m * a.G + g0 + min(l, gb - 1)];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int slot = lane / L.GB;
+  const int gl = lane - slot * L.GB;
+  if (slot < a.tpw) {
+    LdsFixedTally tally{lds_tally, L.GB};
+    LdsSource src{lds_qi, lds_sig, L.GB};
+    for (int tile = __builtin_amdgcn_readfirstlane(gi.x + wib); tile < gi.x + gi.y; tile += wpb) {
+      const int dt = a.work[tile * a.tpw + slot];
+      if (dt < 0) continue;
+      sweep_track<1, EXPM, 1>(a, tile, slot, dt, gl, L.GB, g0, gb, src, tally);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int s = i / L.GB, l = i - s * L.GB;
+    const unsigned long long v = lds_tally[i];
+    if (l < gb && v != 0ull)
+      atomicAdd(&L.tally[(size_t)L.slot_fsr[gi.z + s] * a.G + g0 + l], v);
+  }
+}
+
 // tally64[r*G + g] = sum over the subset's blocks of partials (fixed order).
 __global__ void reduce_partials_kernel(const unsigned long long* partials, double* tally64,
                                        int64_t N, int G, int GB, int nsub, int blocks_per_sub,
@@ -750,6 +825,7 @@ struct FsrArgs {
   float* tally;               // [tally_copies][N*G] (global-atomic sweep)
   int tally_copies;
   const unsigned long long* lds_partials;  // [nsub][bps][N*GB] fixed point (LDS sweep), or null
+  unsigned long long* tally_i64;           // [N*G] fixed point (3D z-stack sweep), or null
   int lds_bps;
   const float* leak;          // [2T*G] vacuum leakage per directed track and group, or null
   int64_t leak_n;
@@ -878,6 +954,10 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
         const unsigned long long t =
             ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         // the fixed-point tally holds sum(contribution) / V
+        f = ((double)(long long)t / a.state->tally_scale) / st + kFourPi * a.Q[k] / st;
+      } else if (a.tally_i64) {
+        const unsigned long long t = a.tally_i64[k];
+        a.tally_i64[k] = 0ull;
         f = ((double)(long long)t / a.state->tally_scale) / st + kFourPi * a.Q[k] / st;
       } else if (a.linear) {
         // CPULSSolver::addSourceToScalarFlux (src/CPULSSolver.cpp:787-882)
@@ -1592,6 +1672,9 @@ HipSolver::~HipSolver() {
   dev_free(d_qls_);
   dev_free(d_tally_xy_);
   dev_free(d_scat_);
+  dev_free(d_grp_);
+  dev_free(d_slot_fsr_);
+  dev_free(d_tally_i64_);
   dev_free(d_scat_rows_);
   dev_free(d_scat_tile_mat_);
   dev_free(d_scat_b_);
@@ -1653,6 +1736,14 @@ void HipSolver::pickShape() {
       // about two blocks per CU: the strided tile assignment then balances
       // the longest-first tiles across blocks (measured on the 17x17 lattice)
       s.lds_blocks_per_sub = std::max(1, (2 * num_cus_ + nsub - 1) / nsub);
+      // no more blocks than the subset has waves of tiles: a small problem
+      // (a pin cell) then launches a few blocks instead of hundreds of idle
+      // ones, each of which would clear and write a partial tally slice
+      if (T_ > 0) {
+        const int64_t tiles = (2 * T_ + 64 / gb - 1) / (64 / gb);
+        const int64_t waves = std::max<int64_t>(1, (tiles + s.lds_block / 64 - 1) / (s.lds_block / 64));
+        s.lds_blocks_per_sub = (int)std::min<int64_t>(s.lds_blocks_per_sub, waves);
+      }
       // a multiple of the XCD count keeps a slot's subset blocks on one XCD
       s.lds_blocks_per_sub = (s.lds_blocks_per_sub + kNumXcd - 1) / kNumXcd * kNumXcd;
     }
@@ -1662,6 +1753,9 @@ void HipSolver::pickShape() {
 
 // LDS of the LDS sweep: int64 tally [N*gb] | 1/V [N] | sigma_t [M*gb] | Q/sigma_t [N*gb]
 size_t HipSolver::ldsBytes(int gb, int with_q) const {
+  if (is3d_ && ngrp_ > 0)  // z-stack sweep: tally [slots*gb] i64 | {Q/sigma_t, iv} [slots*gb] | sigma_t [M*gb]
+    return (size_t)max_grp_slots_ * gb * (sizeof(uint64_t) + 2 * sizeof(float)) +
+           (size_t)std::max(M_, 1) * gb * sizeof(float);
   return (size_t)N_ * gb * sizeof(uint64_t) +
          (with_q ? (size_t)N_ * gb * 2 * sizeof(float) : (size_t)N_ * sizeof(float)) +
          (size_t)std::max(M_, 1) * gb * sizeof(float);
@@ -1807,6 +1901,52 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   }
   ls_ready_ = false;
 
+  // 3D: group the tracks by the 2D track they lie over (azimuthal and xy
+  // index); each group's FSRs get slots 0..n-1 in order of first appearance
+  ngrp_ = 0;
+  max_grp_slots_ = 0;
+  h_trk_grp_.clear();
+  h_slot_.clear();
+  h_grp_slot0_.clear();
+  h_grp_nslots_.clear();
+  h_slot_fsr_.clear();
+  if (is3d && td.xy) {
+    std::map<std::pair<int32_t, int32_t>, int32_t> gid;
+    h_trk_grp_.resize(T);
+    for (int64_t t = 0; t < T; t++) {
+      auto it = gid.find({td.azim[t], td.xy[t]});
+      if (it == gid.end()) it = gid.emplace(std::make_pair(td.azim[t], td.xy[t]), (int32_t)gid.size()).first;
+      h_trk_grp_[t] = it->second;
+    }
+    ngrp_ = (int32_t)gid.size();
+    std::vector<std::vector<int64_t>> members(ngrp_);
+    for (int64_t t = 0; t < T; t++) members[h_trk_grp_[t]].push_back(t);
+    h_slot_.assign(S, 0);
+    int32_t max_f = 0;
+    for (int64_t k = 0; k < S; k++) max_f = std::max(max_f, td.seg_fsr[k]);
+    std::vector<int32_t> slot_of((size_t)max_f + 1, -1);  // reset per group
+    for (int32_t g = 0; g < ngrp_; g++) {
+      const int32_t s0 = (int32_t)h_slot_fsr_.size();
+      h_grp_slot0_.push_back(s0);
+      for (int64_t t : members[g])
+        for (int64_t k = td.seg_offset[t]; k < td.seg_offset[t + 1]; k++) {
+          int32_t& sl = slot_of[td.seg_fsr[k]];
+          if (sl < 0) {
+            sl = (int32_t)h_slot_fsr_.size() - s0;
+            h_slot_fsr_.push_back(td.seg_fsr[k]);
+          }
+          h_slot_[k] = sl;
+        }
+      const int32_t n = (int32_t)h_slot_fsr_.size() - s0;
+      for (int32_t i = 0; i < n; i++) slot_of[h_slot_fsr_[s0 + i]] = -1;
+      h_grp_nslots_.push_back(n);
+      max_grp_slots_ = std::max(max_grp_slots_, n);
+    }
+    dev_alloc(&d_slot_fsr_, h_slot_fsr_.size());
+    HIP_CHECK(hipMemcpy(d_slot_fsr_, h_slot_fsr_.data(), h_slot_fsr_.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  }
+
   T_ = T;
   S_ = S;
   P_ = P;
@@ -1827,14 +1967,31 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
 // traversal order, of each of its tracks; shorter tracks are padded with
 // length-0 segments that repeat their last FSR.
 void HipSolver::buildTileLayout(int tpw) {
-  if (tpw == layout_tpw_) return;
+  const bool stack = stackMode();
+  if (tpw == layout_tpw_ && stack == layout_stack_) return;
   if (geo_hold_)
     throw DeviceError(MOC_ERR_INVALID,
                       "shared geometry is laid out for another launch shape (same group count needed)");
   if (!(tracks_set_ && fsrs_set_)) return;
   if (N_ >= ((int64_t)1 << kFsrBits) || M_ > 256)
     throw DeviceError(MOC_ERR_INVALID, "at most 2^24 FSRs and 256 materials");
-  const int64_t W = 2 * T_;
+  // work order: directed tracks longest first; for the z-stack sweep, group
+  // by group (each group's tracks longest first, its last tile padded)
+  std::vector<int32_t> order_grp;
+  std::vector<int32_t> grp_tile0, grp_ntiles;
+  const std::vector<int32_t>* order = &h_order_;
+  if (stack) {
+    std::vector<std::vector<int32_t>> mem(ngrp_);
+    for (int32_t dt : h_order_) mem[h_trk_grp_[dt >> 1]].push_back(dt);
+    for (int32_t g = 0; g < ngrp_; g++) {
+      grp_tile0.push_back((int32_t)(order_grp.size() / tpw));
+      order_grp.insert(order_grp.end(), mem[g].begin(), mem[g].end());
+      while (order_grp.size() % tpw) order_grp.push_back(-1);
+      grp_ntiles.push_back((int32_t)(order_grp.size() / tpw) - grp_tile0.back());
+    }
+    order = &order_grp;
+  }
+  const int64_t W = (int64_t)order->size();
   const int64_t ntiles = (W + tpw - 1) / tpw;
   if (ntiles >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "too many tiles");
   std::vector<int32_t> work(ntiles * tpw, -1);
@@ -1845,7 +2002,8 @@ void HipSolver::buildTileLayout(int tpw) {
     for (int sl = 0; sl < tpw; sl++) {
       const int64_t w = tl * tpw + sl;
       if (w >= W) break;
-      const int32_t dt = h_order_[w];
+      const int32_t dt = (*order)[w];
+      if (dt < 0) continue;
       work[tl * tpw + sl] = dt;
       nmax = std::max(nmax, h_trk_[4 * (dt >> 1) + 1]);
     }
@@ -1864,6 +2022,11 @@ void HipSolver::buildTileLayout(int tpw) {
   auto key = [&](int32_t f) {
     return (int32_t)((uint32_t)f | ((uint32_t)h_fsr_mat_[f] << kFsrBits));
   };
+  // key of segment k: its FSR, or (z-stack sweep) its slot in the group
+  auto seg_key = [&](int64_t k) {
+    if (!stack) return key(h_fsr_[k]);
+    return (int32_t)((uint32_t)h_slot_[k] | ((uint32_t)h_fsr_mat_[h_fsr_[k]] << kFsrBits));
+  };
   for (int64_t tl = 0; tl < ntiles; tl++) {
     const int64_t r0 = tile[2 * tl];
     const int32_t nr = tile[2 * tl + 1];
@@ -1872,16 +2035,19 @@ void HipSolver::buildTileLayout(int tpw) {
       if (dt < 0) continue;
       const int32_t t = dt >> 1, dir = dt & 1;
       const int32_t b = h_trk_[4 * t], n = h_trk_[4 * t + 1];
-      int32_t lastf = n > 0 ? h_fsr_[dir ? b : b + n - 1] : 0;
+      // padding repeats the last segment's key (the last segment in
+      // traversal order, which is segment b for a backward track)
+      const int32_t lastk = n > 0 ? (dir ? b : b + n - 1) : -1;
+      const int32_t lastkey = lastk >= 0 ? seg_key(lastk) : 0;
       for (int32_t i = 0; i < nr; i++) {
         const size_t o = (size_t)(r0 + i) * tpw + sl;
         if (i < n) {
           const int32_t sidx = dir ? b + n - 1 - i : b + i;
           ilen[o] = h_len_[sidx];
-          ifsr[o] = key(h_fsr_[sidx]);
+          ifsr[o] = seg_key(sidx);
         } else {
           ilen[o] = 0.f;
-          ifsr[o] = key(lastf);
+          ifsr[o] = lastkey;
         }
       }
     }
@@ -1895,8 +2061,22 @@ void HipSolver::buildTileLayout(int tpw) {
   HIP_CHECK(hipMemcpy(d_work_, work.data(), work.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_tile_, tile.data(), tile.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   layout_tpw_ = tpw;
+  layout_stack_ = stack;
   layout_ntiles_ = (int)ntiles;
   layout_rows_ = rows;
+  layout_grp_ntiles_.clear();
+  if (stack) {
+    std::vector<int32_t> g4((size_t)ngrp_ * 4);
+    for (int32_t g = 0; g < ngrp_; g++) {
+      g4[4 * g] = grp_tile0[g];
+      g4[4 * g + 1] = grp_ntiles[g];
+      g4[4 * g + 2] = h_grp_slot0_[g];
+      g4[4 * g + 3] = h_grp_nslots_[g];
+    }
+    dev_alloc(&d_grp_, g4.size());
+    HIP_CHECK(hipMemcpy(d_grp_, g4.data(), g4.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    layout_grp_ntiles_ = grp_ntiles;
+  }
   // start-flux slots live at work positions: position of each directed
   // track, and the position its outgoing flux is written to
   h_wpos_.assign(2 * T_, -1);
@@ -2071,7 +2251,13 @@ void HipSolver::allocateIterationArrays() {
   }
   dev_alloc(&d_Q_, NG);
   if (!shape_forced_) pickShape();
-  if (shape_.lds_groups > 0) {
+  if (stackMode()) {
+    const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
+    dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
+    HIP_CHECK(hipMemset(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float)));
+    dev_alloc(&d_tally_i64_, NG);
+    HIP_CHECK(hipMemset(d_tally_i64_, 0, NG * sizeof(unsigned long long)));
+  } else if (shape_.lds_groups > 0) {
     const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
     dev_alloc(&d_lds_partials_, (size_t)nsub * shape_.lds_blocks_per_sub * N_ * shape_.lds_groups);
     dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
@@ -2300,6 +2486,37 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
     return;
   }
+  if (stackMode()) {
+    StackParams L;
+    L.grp = reinterpret_cast<const int4*>(d_grp_);
+    L.slot_fsr = d_slot_fsr_;
+    L.tally = d_tally_i64_;
+    L.qsub = d_qsub_;
+    L.inv_vol = d_inv_vol_;
+    L.sigma_t = d_sigma_t_;
+    L.M = M_;
+    L.GB = shape_.lds_groups;
+    L.nsub = (G_ + L.GB - 1) / L.GB;
+    L.ngrp = ngrp_;
+    L.max_slots = max_grp_slots_;
+    const size_t lds_bytes = ldsBytes(L.GB, 1);
+    if (lds_bytes > lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
+    if (layout_grp_ntiles_.size() != (size_t)ngrp_)
+      throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out by z-stack group");
+    // groups padded to a multiple of the XCD count; block b -> group, subset
+    const unsigned grid = (unsigned)((ngrp_ + kNumXcd - 1) / kNumXcd * kNumXcd * L.nsub);
+    auto launch = [&](auto kernel) {
+      ensure_lds_attribute((const void*)kernel, device_, lds_max_);
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(shape_.lds_block), lds_bytes, s, a, L);
+    };
+    if (exp_mode_ == MOC_EXP_HW)
+      launch(sweep3d_stack_kernel<MOC_EXP_HW>);
+    else
+      launch(sweep3d_stack_kernel<MOC_EXP_RATIONAL>);
+    HIP_CHECK(hipGetLastError());
+    if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
+    return;
+  }
   if (shape_.lds_groups > 0) {
     LdsParams L;
     L.GB = shape_.lds_groups;
@@ -2410,7 +2627,15 @@ FsrArgs HipSolver::fsrArgs() {
   a.src_c = d_src_c_;
   a.scat = scatterPrecisionInUse() == MOC_SCATTER_FP64 ? nullptr : d_scat_;
   a.ss_colsum = d_ss_colsum_;
-  if (shape_.lds_groups > 0) {
+  a.tally_i64 = nullptr;
+  if (stackMode()) {
+    a.qs = nullptr;
+    a.qsub = d_qsub_;
+    a.qsub_groups = shape_.lds_groups;
+    a.lds_partials = nullptr;
+    a.lds_bps = 0;
+    a.tally_i64 = d_tally_i64_;
+  } else if (shape_.lds_groups > 0) {
     a.qs = nullptr;
     a.qsub = d_qsub_;
     a.qsub_groups = shape_.lds_groups;
@@ -2476,6 +2701,9 @@ void HipSolver::launchInit(Mode mode, double k0) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[1], (int64_t)psi_n);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
+  if (stackMode())
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
+                       reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
   psi_cur_ = 0;
 
   refreshStabilization();
@@ -2756,9 +2984,17 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   // CPUSolver::_start_flux
   HIP_CHECK(hipMemcpyAsync(d_psi_[1], slots.data(), psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
+  if (stackMode())
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
+                       reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
   launchSweep(d_psi_[0], d_psi_[1], stream_);
   std::vector<float> t32;
-  if (shape_.lds_groups > 0) {
+  std::vector<unsigned long long> t64;
+  if (stackMode()) {
+    t64.resize(NG);
+    HIP_CHECK(hipMemcpyAsync(t64.data(), d_tally_i64_, NG * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, stream_));
+  } else if (shape_.lds_groups > 0) {
     const int gb = shape_.lds_groups, nsub = (G_ + gb - 1) / gb;
     const int64_t nred = (int64_t)nsub * N_ * gb;
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(nred, 256)), dim3(256), 0, stream_,
@@ -2774,7 +3010,15 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   }
   HIP_CHECK(hipMemcpyAsync(slots.data(), d_psi_[1], psi_n * sizeof(float), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
-  if (shape_.lds_groups == 0) {
+  if (stackMode()) {
+    // the fixed-point tally holds sum / V
+    for (size_t i = 0; i < NG; i++) {
+      const double v = h_fsr_vol_[i / G_];
+      tally[i] = (double)(long long)t64[i] / st.tally_scale * (v < kFltEpsilon ? 0.0 : v);
+    }
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
+                       reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
+  } else if (shape_.lds_groups == 0) {
     for (size_t i = 0; i < NG; i++) {
       double t = 0.0;
       for (int c = 0; c < shape_.tally_copies; c++) t += t32[c * NG + i];
@@ -2917,6 +3161,7 @@ void HipSolver::shareGeometry(HipSolver& src) {
   if (!(src.tracks_set_ && src.fsrs_set_ && src.layout_tpw_ > 0))
     throw DeviceError(MOC_ERR_STATE, "the source solver has no complete geometry (tracks, FSRs, materials)");
   if (src.device_ != device_) throw DeviceError(MOC_ERR_INVALID, "solvers on different devices");
+  if (src.is3d_) throw DeviceError(MOC_ERR_INVALID, "3D track sets are not shared between solvers");
   if (tracks_set_ || fsrs_set_ || geo_hold_)
     throw DeviceError(MOC_ERR_STATE, "geometry can only be shared into a fresh solver");
   if (!src.geo_hold_) {
@@ -2998,7 +3243,8 @@ void HipSolver::ensureGraph(int mode, int res_type) {
         (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 1, 2>, (const void*)sweep2d_lds_kernel<MOC_EXP_RATIONAL, 1, 3>,
         (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 1>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 2>,
         (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 0, 3>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 1>,
-        (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 2>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 3>};
+        (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 2>, (const void*)sweep2d_lds_kernel<MOC_EXP_HW, 1, 3>,
+        (const void*)sweep3d_stack_kernel<MOC_EXP_RATIONAL>, (const void*)sweep3d_stack_kernel<MOC_EXP_HW>};
     for (const void* k : ks) ensure_lds_attribute(k, device_, lds_max_);
   }
   hipGraph_t g = nullptr;

@@ -52,8 +52,9 @@ def _track_data(tg: "TrackGenerator"):
      td.sin_theta, td.azim_phi, td.azim_weight, td.polar_weight,
      td.seg_start) = [a.ctypes.data for a in keep]
     if getattr(tg, "polar", None) is not None:
-        keep.append(tg.polar)
+        keep += [tg.polar, tg.xy]
         td.polar = tg.polar.ctypes.data
+        td.xy = tg.xy.ctypes.data
         td.seg_start = None
     return td, keep
 

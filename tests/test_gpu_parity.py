@@ -358,3 +358,21 @@ def test_3d_single_sweep_matches_oracle(tracks):
     t_ref, p_ref = orc.OracleProblem.from_track_generator(tg).transport_sweep(Q, psi)
     assert rel_l2(tally, t_ref) < 2e-5
     np.testing.assert_allclose(psi_out, p_ref, rtol=5e-4, atol=1e-6)
+
+
+def test_3d_zstack_sweep_equals_global_tally_sweep(tracks, golden):
+    # the default 3D shape keeps each z-stack group's FSR tally in LDS; the
+    # global-atomic shape is the other path: same solve within FP32 rounding
+    tg = tracks("lattice_3x3_3d_7g")
+    g = golden("lattice_3x3_3d_7g")
+    s = solver_for(tg)
+    assert s.getLaunchShape()["lds_groups"] > 0
+    s.computeEigenvalue(1000)
+    t = solver_for(tg)
+    t.setLaunchShape(lds_groups=0, groups_per_lane=1, lanes_per_track=7, tracks_per_wave=9,
+                     block=64, tally_copies=8)
+    t.computeEigenvalue(1000)
+    for x in (s, t):
+        assert abs(x.getKeff() - float(g["keff"])) < K_TOL
+        assert rel_l2(x.getFluxes(), g["flux"]) < FLUX_RL2
+    assert abs(s.getKeff() - t.getKeff()) < 1e-6
