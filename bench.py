@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch-shape", default="", metavar="K=V,...",
+                    help="fixed sweep launch shape (keys of getLaunchShape), no autotuning")
     ap.add_argument("--serving", type=int, default=48, metavar="N",
                     help="requests in the serving-stream measurement (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -230,7 +232,11 @@ def main():
     scatter_name = {om.SCATTER_FP64: "fp64", om.SCATTER_MFMA_F16: "mfma_f16_split",
                     om.SCATTER_FP32: "fp32"}[scatter]
     shape = solver.getLaunchShape()
-    if not args.no_autotune:
+    if args.launch_shape:
+        kv = dict(item.split("=") for item in args.launch_shape.split(","))
+        solver.setLaunchShape(**{k: int(v) for k, v in kv.items()})
+        shape = solver.getLaunchShape()
+    elif not args.no_autotune:
         from openmoc_amd.autotune import autotune
         solver.computeEigenvalue(3)
         shape, t_best = autotune(solver, repeats=5)
