@@ -279,6 +279,8 @@ def main():
     achieved_tflops = FLOPS_PER_INTEGRATION * w_sweep / sweep_s / 1e12
     if args.linear_source:
         kernel = "sweep2d_ls_kernel"
+    elif tg.info.is_3d and shape.get("lds_groups", 0) > 0:
+        kernel = "sweep3d_stack_kernel"
     else:
         kernel = "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel"
     traffic, traffic_src = measured_traffic(kernel, args.workload)
