@@ -37,6 +37,7 @@ extern "C" {
 #define MOC_ERR_HIP (-3)       /* HIP runtime failure */
 #define MOC_ERR_MEMORY (-4)    /* allocation / admission refused */
 #define MOC_ERR_NO_DEVICE (-5) /* no GPU visible */
+#define MOC_ERR_IO (-6)        /* file missing, short or malformed */
 
 /* residualType (src/Solver.h / constants) */
 #define MOC_SCALAR_FLUX 0
@@ -265,6 +266,53 @@ int moc_solver_get_fluxes(moc_solver_t* solver, double* out, int64_t num_fluxes)
 int moc_solver_set_fluxes(moc_solver_t* solver, const double* in, int64_t num_fluxes);
 /* CPUSolver::computeFSRFissionRates(rates, N) (src/CPUSolver.cpp:2850-2890). */
 int moc_solver_compute_fsr_fission_rates(moc_solver_t* solver, double* rates, int64_t num_fsrs);
+
+/* ---- FSR flux files (checkpoint / restart) ----
+ * Layout of Solver::dumpFSRFluxes (src/Solver.cpp:2066-2112), native byte
+ * order: f64 k_eff | i32 num_groups | i64 num_fsrs | per FSR f64 x, y, z
+ * (centroid) and f64 flux[num_groups].  FSRs of a file are matched to the
+ * solver's by nearest centroid among the 3x3x3 cells of width `tolerance` cm
+ * around each centroid (src/Solver.cpp:2198-2306). */
+
+/* FSR centroids [3N] naming the solver's FSRs in flux files: the reference's
+ * Geometry::getFSRCentroid values, from TrackGenerator::generateFSRCentroids
+ * (src/TrackGenerator.cpp:1681-1735; z = 0 for 2D tracks). */
+int moc_solver_set_fsr_centroids(moc_solver_t* solver, const double* xyz, int64_t num_values);
+/* Solver::dumpFSRFluxes(fname) (src/Solver.cpp:2066-2112): k_eff and the
+ * current scalar fluxes. */
+int moc_solver_dump_fsr_fluxes(moc_solver_t* solver, const char* path);
+/* Solver::loadFSRFluxes(fname, assign_k_eff, tolerance) (src/Solver.cpp:
+ * 2135-2318): the matched fluxes become the current fluxes (as set_fluxes);
+ * assign_k_eff sets k_eff to the file's.  *max_centroid_error (nullable)
+ * receives the largest centroid mismatch (cm).  MOC_ERR_IO for an unreadable
+ * file, MOC_ERR_INVALID for another FSR or group count or an unmatched FSR. */
+int moc_solver_load_fsr_fluxes(moc_solver_t* solver, const char* path, int32_t assign_k_eff,
+                               double tolerance, double* max_centroid_error);
+/* Solver::loadInitialFSRFluxes(fname) (src/Solver.cpp:2120-2123; used in
+ * computeInitialFluxGuess, src/Solver.cpp:1733-1751): eigenvalue and source
+ * solves start from the file's fluxes and k_eff, normalised to the FSR count,
+ * then run 30 sweeps on their sources before the source iteration.  The file
+ * is read when each solve starts (tolerance 0.01 cm).  NULL or "" switches it
+ * off. */
+int moc_solver_load_initial_fsr_fluxes(moc_solver_t* solver, const char* path);
+/* Solver::setResidualByReference(fname) (src/Solver.cpp:2056-2059; used in
+ * CPUSolver::computeResidual, src/CPUSolver.cpp:2113-2252): residuals compare
+ * the normalised flux with the file's fluxes instead of the previous
+ * iteration's.  Read when each solve starts; NULL or "" switches it off. */
+int moc_solver_set_residual_by_reference(moc_solver_t* solver, const char* path);
+
+/* Host-only flux-file helpers (no GPU needed), the steps the solver calls
+ * above use.  moc_flux_file_read with null xyz / flux returns the header only;
+ * otherwise xyz_len must be 3*num_fsrs and flux_len num_fsrs*num_groups. */
+int moc_flux_file_write(const char* path, double keff, int32_t num_groups, int64_t num_fsrs,
+                        const double* xyz, const double* flux);
+int moc_flux_file_read(const char* path, double* keff, int32_t* num_groups, int64_t* num_fsrs,
+                       double* xyz, int64_t xyz_len, double* flux, int64_t flux_len);
+/* index[i] = the file centroid (of num_file, [3*num_file]) matched to own
+ * centroid i (of num_own) by the rule above; *max_error (nullable) the largest
+ * matched distance. */
+int moc_fsr_match_centroids(const double* file_xyz, int64_t num_file, const double* own_xyz,
+                            int64_t num_own, double tolerance, int64_t* index, double* max_error);
 
 /* Timing of the last solve: device time of all sweeps (s), total wall time
  * (s), number of integrations (2 * fluxes_per_track * segments * iters,

@@ -10,5 +10,6 @@ from .workloads import Problem, get as get_workload  # noqa: F401
 from .solver import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL,  # noqa: F401
                      SCALAR_FLUX, SCATTER_AUTO, SCATTER_FP32, SCATTER_FP64, SCATTER_MFMA_F16,
                      TOTAL_SOURCE, YAMAMOTO, GPUSolver, HipLSSolver, HipSolver, MOCError,
-                     TrackGenerator)
+                     TrackGenerator, fsr_centroids, match_centroids, read_flux_file,
+                     write_flux_file)
 from . import process  # noqa: F401

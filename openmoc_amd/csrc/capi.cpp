@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/openmoc_amd.h"
+#include "fluxfile.h"
 #include "hip_solver.h"
 #include "serve.h"
 #include "tracks.h"
@@ -433,6 +434,65 @@ int moc_solver_set_fluxes(moc_solver_t* s, const double* in, int64_t num_fluxes)
 
 int moc_solver_compute_fsr_fission_rates(moc_solver_t* s, double* rates, int64_t num_fsrs) {
   MOC_SOLVER_CALL(s, s->impl->computeFSRFissionRates(rates, num_fsrs));
+}
+
+int moc_solver_set_fsr_centroids(moc_solver_t* s, const double* xyz, int64_t num_values) {
+  MOC_SOLVER_CALL(s, s->impl->setFSRCentroids(xyz, num_values));
+}
+
+int moc_solver_dump_fsr_fluxes(moc_solver_t* s, const char* path) {
+  if (!path || !*path) return fail(MOC_ERR_INVALID, "empty file name");
+  MOC_SOLVER_CALL(s, s->impl->dumpFSRFluxes(path));
+}
+
+int moc_solver_load_fsr_fluxes(moc_solver_t* s, const char* path, int32_t assign_k_eff,
+                               double tolerance, double* max_centroid_error) {
+  if (!path || !*path) return fail(MOC_ERR_INVALID, "empty file name");
+  MOC_SOLVER_CALL(s, {
+    const double e = s->impl->loadFSRFluxes(path, assign_k_eff != 0, tolerance);
+    if (max_centroid_error) *max_centroid_error = e;
+  });
+}
+
+int moc_solver_load_initial_fsr_fluxes(moc_solver_t* s, const char* path) {
+  MOC_SOLVER_CALL(s, s->impl->loadInitialFSRFluxes(path ? path : ""));
+}
+
+int moc_solver_set_residual_by_reference(moc_solver_t* s, const char* path) {
+  MOC_SOLVER_CALL(s, s->impl->setResidualByReference(path ? path : ""));
+}
+
+int moc_flux_file_write(const char* path, double keff, int32_t num_groups, int64_t num_fsrs,
+                        const double* xyz, const double* flux) {
+  if (!path || !*path) return fail(MOC_ERR_INVALID, "empty file name");
+  return guarded([&] { moc::write_flux_file(path, keff, num_groups, num_fsrs, xyz, flux); });
+}
+
+int moc_flux_file_read(const char* path, double* keff, int32_t* num_groups, int64_t* num_fsrs,
+                       double* xyz, int64_t xyz_len, double* flux, int64_t flux_len) {
+  if (!path || !*path) return fail(MOC_ERR_INVALID, "empty file name");
+  return guarded([&] {
+    const moc::FluxFile f = moc::read_flux_file(path);
+    if (xyz && xyz_len != (int64_t)f.xyz.size())
+      throw moc::DeviceError(MOC_ERR_INVALID, "xyz_len must be 3*num_fsrs");
+    if (flux && flux_len != (int64_t)f.flux.size())
+      throw moc::DeviceError(MOC_ERR_INVALID, "flux_len must be num_fsrs*num_groups");
+    if (keff) *keff = f.keff;
+    if (num_groups) *num_groups = f.num_groups;
+    if (num_fsrs) *num_fsrs = f.num_fsrs;
+    if (xyz) copy_out(xyz, f.xyz);
+    if (flux) copy_out(flux, f.flux);
+  });
+}
+
+int moc_fsr_match_centroids(const double* file_xyz, int64_t num_file, const double* own_xyz,
+                            int64_t num_own, double tolerance, int64_t* index, double* max_error) {
+  if (!index && num_own > 0) return fail(MOC_ERR_INVALID, "null index array");
+  return guarded([&] {
+    const std::vector<int64_t> idx =
+        moc::match_centroids(file_xyz, num_file, own_xyz, num_own, tolerance, max_error);
+    copy_out(index, idx);
+  });
 }
 
 int moc_solver_get_timing(const moc_solver_t* s, double* sweep_seconds, double* total_seconds,

@@ -168,6 +168,24 @@ class HipSolver {
   const LaunchShape& shape() const { return shape_; }
   void setShape(const LaunchShape& s);
 
+  // FSR centroids [3N] (x, y, z) that identify FSRs in flux files.
+  void setFSRCentroids(const double* xyz, int64_t n);
+  // Solver::dumpFSRFluxes (src/Solver.cpp:2066-2112): k_eff, centroids and
+  // the current scalar fluxes.
+  void dumpFSRFluxes(const std::string& path);
+  // Solver::loadFSRFluxes (src/Solver.cpp:2135-2318): scalar fluxes of the
+  // file FSRs nearest to this solver's centroids become the current fluxes;
+  // returns the largest centroid mismatch.
+  double loadFSRFluxes(const std::string& path, bool assign_k_eff, double tolerance);
+  // Solver::loadInitialFSRFluxes (src/Solver.cpp:2120-2123): eigenvalue and
+  // source solves start from the file's fluxes and k_eff, normalised, after 30
+  // sweeps on their sources (src/Solver.cpp:1733-1751).  "" switches it off.
+  void loadInitialFSRFluxes(const std::string& path);
+  // Solver::setResidualByReference (src/Solver.cpp:2056-2059): residuals
+  // compare the flux with the file's fluxes instead of the previous
+  // iteration's.  "" switches it off.
+  void setResidualByReference(const std::string& path);
+
  private:
   enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
   void requireReady() const;
@@ -178,6 +196,11 @@ class HipSolver {
   void launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   void launchInit(Mode mode, double k0);
+  void readFluxFiles(Mode mode);
+  void launchStartupSweeps();
+  void zeroTallies(hipStream_t s);
+  std::vector<double> matchedFlux(const std::string& path, double tolerance, double* keff,
+                                  double* max_error);
   void ensureGraph(int mode, int res_type);
   void launchIterations(int mode, int res_type, int n);
   void launchAsyncBatch();
@@ -267,6 +290,13 @@ class HipSolver {
   std::vector<double> h_seg_start_;    // [2S]
   std::vector<double> h_azim_phi_, h_azim_w_, h_polar_w_;
   std::vector<double> h_centroid_;     // [2N]
+  std::vector<double> h_fsr_xyz_;      // [3N] centroids naming FSRs in flux files
+  std::string init_flux_file_;         // loadInitialFSRFluxes
+  std::string ref_flux_file_;          // setResidualByReference
+  std::vector<double> h_init_flux_;    // [N*G] loaded at solve start
+  double init_keff_ = 1.0;
+  double* d_ref_flux_ = nullptr;       // [N*G] reference flux of the residual
+  size_t ref_flux_n_ = 0;              // values read into d_ref_flux_
   std::vector<int32_t> h_work_, h_tile_;  // host copies of the stream layout
   std::vector<int32_t> h_wpos_;        // [2T] work position of each directed track
   int32_t* d_out_wp_ = nullptr;        // [2T] work position of the linked start slot

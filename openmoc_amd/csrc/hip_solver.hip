@@ -34,6 +34,7 @@
 #include <type_traits>
 
 #include "hip_solver.h"
+#include "fluxfile.h"
 
 namespace moc {
 
@@ -851,6 +852,9 @@ struct FsrArgs {
   // sum_g nu-sigma-f(g) x(g), of x = phi (c = 0) and the moments (c = 1, 2)
   const float* scat;
   const double* ss_colsum;    // [M*G] sum over destinations of sigma_s(g->d)
+  // residual against a fixed reference flux [N*G] instead of the old flux
+  // (setResidualByReference), or null
+  const double* ref_flux;
 };
 
 namespace {
@@ -1104,7 +1108,7 @@ __device__ double source_residual_term(const FsrArgs& a, int64_t r, int res_type
   const bool fis = a.fissionable[m] != 0;
   const double* nsf = a.nsf + (size_t)m * G;
   const double* phi = a.phi + (size_t)r * G;
-  const double* old = a.phi_old[a.state->old_sel] + (size_t)r * G;
+  const double* old = (a.ref_flux ? a.ref_flux : a.phi_old[a.state->old_sel]) + (size_t)r * G;
   double nw = 0.0, od = 0.0;
   if (fis) {
     for (int g = 0; g < G; g++) {
@@ -1185,8 +1189,9 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
     if (do_residual) {
       // computeResidual (src/CPUSolver.cpp:2113-2252)
       if (res_type == MOC_SCALAR_FLUX) {
-        if (old[gd] > 0.0) {
-          const double d = (phin - old[gd]) / old[gd];
+        const double ref = a.ref_flux ? a.ref_flux[i] : old[gd];
+        if (ref > 0.0) {
+          const double d = (phin - ref) / ref;
           res = d * d;
         }
       } else if (gd == 0 && G < kFsrResidualGroups) {
@@ -1681,6 +1686,7 @@ HipSolver::~HipSolver() {
   dev_free(d_scat_colmul_);
   dev_free(d_ss_colsum_);
   dev_free(d_stab_ratio_);
+  dev_free(d_ref_flux_);
   dev_free(d_Q_);
   dev_free(d_fixed_);
   dev_free(d_sigma_t_);
@@ -2627,6 +2633,7 @@ FsrArgs HipSolver::fsrArgs() {
   a.src_c = d_src_c_;
   a.scat = scatterPrecisionInUse() == MOC_SCATTER_FP64 ? nullptr : d_scat_;
   a.ss_colsum = d_ss_colsum_;
+  a.ref_flux = (!ref_flux_file_.empty() && ref_flux_n_ == (size_t)N_ * G_) ? d_ref_flux_ : nullptr;
   a.tally_i64 = nullptr;
   if (stackMode()) {
     a.qs = nullptr;
@@ -2689,6 +2696,10 @@ void HipSolver::launchInit(Mode mode, double k0) {
   if (linear_ && !ls_ready_) prepareLinearSource();
   const size_t NG = (size_t)N_ * G_;
   const size_t psi_n = psiCount();
+  // a flux file's fluxes and k_eff replace the flat guess of eigenvalue and
+  // source solves (loadFSRFluxes(file, true), src/Solver.cpp:1733-1736)
+  const bool restart = mode != MODE_FLUX && h_init_flux_.size() == NG;
+  if (restart) k0 = init_keff_;
   IterState st;
   std::memset(&st, 0, sizeof(st));
   st.k = k0;
@@ -2700,42 +2711,94 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // fresh angular fluxes and tally (initializeFluxArrays, src/CPUSolver.cpp:281-380)
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[1], (int64_t)psi_n);
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
-  if (stackMode())
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
-                       reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
+  zeroTallies(stream_);
   psi_cur_ = 0;
 
   refreshStabilization();
   const FsrArgs fa = fsrArgs();
   // initial guess (computeInitialFluxGuess, src/Solver.cpp:1710-1731; computeFlux
   // starts from phi = 0, src/Solver.cpp:1383-1390)
-  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_phi_,
-                     (int64_t)NG, mode == MODE_FLUX ? 0.0 : 1.0);
-  if (linear_) {
+  if (restart)
+    HIP_CHECK(hipMemcpyAsync(d_phi_, h_init_flux_.data(), NG * sizeof(double),
+                             hipMemcpyHostToDevice, stream_));
+  else
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_phi_,
+                       (int64_t)NG, mode == MODE_FLUX ? 0.0 : 1.0);
+  if (linear_)
     // flattenFSRFluxes zeroes the moments (src/CPULSSolver.cpp:343-356)
     hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
                        d_phi_xy_, (int64_t)(2 * NG), 0.0);
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG * shape_.tally_copies, 256)),
-                       dim3(256), 0, stream_, d_tally_xy_, (int64_t)(2 * NG * shape_.tally_copies));
-  }
-  if (mode == MODE_EIGEN) {
+  // the loaded fluxes are normalised in source solves too (src/Solver.cpp:1736)
+  const bool normalize = mode == MODE_EIGEN || restart;
+  if (normalize) {
     hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 0, 0);
     hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
                        num_partials_, pstride_, d_state_, (double)N_, 0, 0);
   }
-  // normalise (eigen), store old flux, sources of iteration 0
+  // normalise, store old flux, sources of iteration 0
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, mode == MODE_EIGEN ? 1 : 0, 0, 0, 1, 0, kSrcTotal);
+                     stream_, fa, normalize ? 1 : 0, 0, 0, 1, 0, kSrcTotal);
   // fixed-point scale of the first sweep (start fluxes are zero); the old
   // flux just written becomes the current one
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
   HIP_CHECK(hipGetLastError());
+  if (restart) {
+    launchStartupSweeps();
+    // the copy above is complete once the startup sweeps synchronised
+    h_init_flux_.clear();
+  }
 
   state_valid_ = true;
   phi_is_old_ = true;
   krylov_ready_ = false;
+}
+
+// Zero tallies that sweeps accumulate into (the LDS sweep's per-block
+// partials are overwritten instead).
+void HipSolver::zeroTallies(hipStream_t s) {
+  const size_t NG = (size_t)N_ * G_;
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, s,
+                     d_tally_, (int64_t)(NG * shape_.tally_copies));
+  if (stackMode())
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, s,
+                       reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
+  if (linear_)
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG * shape_.tally_copies, 256)),
+                       dim3(256), 0, s, d_tally_xy_, (int64_t)(2 * NG * shape_.tally_copies));
+}
+
+// After a flux file's fluxes were normalised, stored and turned into the
+// sources of iteration 0: 30 sweeps on those sources converge the angular
+// fluxes, then addSourceToScalarFlux (src/Solver.cpp:1744-1751).  The source
+// iteration then starts from computeFSRSources(0) of that flux, the old flux
+// staying the loaded one.
+void HipSolver::launchStartupSweeps() {
+  constexpr int kStartupSweeps = 30;
+  const size_t NG = (size_t)N_ * G_;
+  // the start fluxes are zero, so the normalisation has nothing to scale in
+  // them and the 30 sweeps run unscaled
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  h_state_->psi_scale = 1.0;
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  const FsrArgs fa = fsrArgs();
+  for (int i = 0; i < kStartupSweeps; i++) {
+    // transportSweep starts from a zero scalar-flux tally (src/CPUSolver.cpp:2338-2389)
+    zeroTallies(stream_);
+    // fixed-point scale from the previous sweep's outgoing angular fluxes
+    if (i > 0)
+      hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 0, 0);
+    launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
+    psi_cur_ ^= 1;
+  }
+  hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1, 0);
+  // computeFSRSources(0) of this flux, unnormalised; the old flux is kept
+  launchScatter(stream_);
+  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
+  HIP_CHECK(hipGetLastError());
 }
 
 // YAMAMOTO: per group, the factor times the largest |sigma_s(g->g) / sigma_t(g)|
@@ -2918,6 +2981,85 @@ void HipSolver::setFluxes(const double* in, int64_t n) {
   HIP_CHECK(hipMemcpyAsync(d_phi_, in, n * sizeof(double), hipMemcpyHostToDevice, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   phi_is_old_ = false;
+}
+
+void HipSolver::setFSRCentroids(const double* xyz, int64_t n) {
+  if (!fsrs_set_) throw DeviceError(MOC_ERR_STATE, "FSRs have not been set");
+  if (!xyz || n != 3 * N_) throw DeviceError(MOC_ERR_INVALID, "centroid array must hold 3*N values");
+  h_fsr_xyz_.assign(xyz, xyz + n);
+}
+
+void HipSolver::dumpFSRFluxes(const std::string& path) {
+  requireReady();
+  if (h_fsr_xyz_.size() != (size_t)(3 * N_))
+    throw DeviceError(MOC_ERR_STATE, "FSR centroids have not been set");
+  std::vector<double> flux((size_t)N_ * G_);
+  getFluxes(flux.data(), (int64_t)flux.size());
+  write_flux_file(path, keff_, G_, N_, h_fsr_xyz_.data(), flux.data());
+}
+
+// The file's fluxes reordered to this solver's FSRs by nearest centroid
+// (src/Solver.cpp:2135-2318).
+std::vector<double> HipSolver::matchedFlux(const std::string& path, double tolerance, double* keff,
+                                           double* max_error) {
+  if (h_fsr_xyz_.size() != (size_t)(3 * N_))
+    throw DeviceError(MOC_ERR_STATE, "FSR centroids have not been set");
+  const FluxFile f = read_flux_file(path);
+  if (f.num_fsrs != N_)
+    throw DeviceError(MOC_ERR_INVALID, "the flux file holds " + std::to_string(f.num_fsrs) +
+                                           " FSRs, the geometry " + std::to_string(N_));
+  if (f.num_groups != G_)
+    throw DeviceError(MOC_ERR_INVALID, "the flux file holds " + std::to_string(f.num_groups) +
+                                           " groups, the materials " + std::to_string(G_));
+  const std::vector<int64_t> idx =
+      match_centroids(f.xyz.data(), f.num_fsrs, h_fsr_xyz_.data(), N_, tolerance, max_error);
+  std::vector<double> flux((size_t)N_ * G_);
+  for (int64_t r = 0; r < N_; r++)
+    std::memcpy(flux.data() + r * G_, f.flux.data() + idx[(size_t)r] * G_, G_ * sizeof(double));
+  if (keff) *keff = f.keff;
+  return flux;
+}
+
+double HipSolver::loadFSRFluxes(const std::string& path, bool assign_k_eff, double tolerance) {
+  requireReady();
+  double k = 0.0, err = 0.0;
+  const std::vector<double> flux = matchedFlux(path, tolerance, &k, &err);
+  setFluxes(flux.data(), (int64_t)flux.size());
+  if (assign_k_eff) keff_ = k;
+  return err;
+}
+
+void HipSolver::loadInitialFSRFluxes(const std::string& path) {
+  init_flux_file_ = path;
+  h_init_flux_.clear();
+}
+
+void HipSolver::setResidualByReference(const std::string& path) {
+  ref_flux_file_ = path;
+  graph_dirty_ = true;
+}
+
+// Files named by loadInitialFSRFluxes / setResidualByReference are read when
+// a solve starts, as the reference reads them in computeInitialFluxGuess
+// (src/Solver.cpp:1710-1736); centroid tolerance 0.01 cm (src/Solver.h:470).
+void HipSolver::readFluxFiles(Mode mode) {
+  constexpr double kTolerance = 0.01;
+  h_init_flux_.clear();
+  if (!init_flux_file_.empty() && mode != MODE_FLUX) {
+    if (num_fissionable_ == 0)
+      throw DeviceError(MOC_ERR_INVALID, "initial fluxes from a file are normalised by the fission "
+                                         "rate, which needs fissionable material");
+    h_init_flux_ = matchedFlux(init_flux_file_, kTolerance, &init_keff_, nullptr);
+  }
+  if (!ref_flux_file_.empty()) {
+    const std::vector<double> ref = matchedFlux(ref_flux_file_, kTolerance, nullptr, nullptr);
+    dev_alloc(&d_ref_flux_, ref.size());
+    ref_flux_n_ = ref.size();
+    HIP_CHECK(hipMemcpyAsync(d_ref_flux_, ref.data(), ref.size() * sizeof(double),
+                             hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    graph_dirty_ = true;
+  }
 }
 
 void HipSolver::computeFSRFissionRates(double* out, int64_t n) {
@@ -3311,6 +3453,7 @@ void HipSolver::startSolve(int mode, int max_iters, int res_type, double k_fixed
   if (!async_ev0_) HIP_CHECK(hipEventCreate(&async_ev0_));
   async_ms_ = 0.0;
   async_t0_ = std::chrono::steady_clock::now();
+  readFluxFiles((Mode)mode);
   launchInit((Mode)mode, mode == MODE_SOURCE ? k_fixed : 1.0);
   if (use_graphs_ && max_iters >= kGraphIters) ensureGraph(mode, res_type);
   async_mode_ = mode;

@@ -19,6 +19,7 @@ Usage, as in the reference's sample inputs
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Optional, Union
 
 import numpy as np
@@ -57,6 +58,55 @@ def _track_data(tg: "TrackGenerator"):
         td.xy = tg.xy.ctypes.data
         td.seg_start = None
     return td, keep
+
+
+def _path(fname) -> bytes:
+    return os.fsencode(fname)
+
+
+def fsr_centroids(tg: "TrackGenerator") -> np.ndarray:
+    """FSR centroids [N, 3] of a TrackGenerator's tracks, as the reference's
+    Geometry::getFSRCentroid holds them (TrackGenerator::generateFSRCentroids,
+    src/TrackGenerator.cpp:1681-1735); z is 0 for 2D tracks."""
+    n = tg.getNumFSRs()
+    xyz = np.zeros((n, 3))
+    xyz[:, :2] = tg.fsr_centroid.reshape(n, 2)
+    if getattr(tg, "fsr_centroid_z", None) is not None:
+        xyz[:, 2] = tg.fsr_centroid_z
+    return xyz
+
+
+def write_flux_file(fname, keff: float, xyz: np.ndarray, flux: np.ndarray) -> None:
+    """Writes a flux file in the layout of Solver::dumpFSRFluxes
+    (src/Solver.cpp:2066-2112) from centroids [N, 3] and fluxes [N, G]."""
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(flux, np.float64).reshape(len(x), -1)
+    check(lib().moc_flux_file_write(_path(fname), float(keff), f.shape[1], len(x), _ptr(x),
+                                    _ptr(f)))
+
+
+def read_flux_file(fname):
+    """(k_eff, centroids [N, 3], fluxes [N, G]) of a flux file."""
+    k, g, n = C.c_double(), C.c_int32(), C.c_int64()
+    check(lib().moc_flux_file_read(_path(fname), C.byref(k), C.byref(g), C.byref(n), None, 0,
+                                   None, 0))
+    xyz = np.zeros((n.value, 3))
+    flux = np.zeros((n.value, g.value))
+    check(lib().moc_flux_file_read(_path(fname), None, None, None, _ptr(xyz), xyz.size,
+                                   _ptr(flux), flux.size))
+    return k.value, xyz, flux
+
+
+def match_centroids(file_xyz: np.ndarray, own_xyz: np.ndarray, tolerance: float = 0.01):
+    """(index, max_error): for every own centroid the matched file centroid
+    (the rule of Solver::loadFSRFluxes, src/Solver.cpp:2198-2306)."""
+    a = np.ascontiguousarray(file_xyz, np.float64).reshape(-1, 3)
+    b = np.ascontiguousarray(own_xyz, np.float64).reshape(-1, 3)
+    idx = np.zeros(len(b), np.int64)
+    err = C.c_double()
+    check(lib().moc_fsr_match_centroids(_ptr(a), len(a), _ptr(b), len(b), float(tolerance),
+                                        _ptr(idx), C.byref(err)))
+    return idx, err.value
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -221,6 +271,7 @@ class HipSolver:
         self._num_fsrs = int(i.num_fsrs)
         self._num_groups = int(i.num_groups)
         self.setConvergenceThreshold(i.tolerance)
+        self.setFSRCentroids(fsr_centroids(tg))
 
     def setArrays(self, *, num_azim, num_polar, seg_offset, seg_length, seg_fsr, azim,
                   next_fwd, next_bwd, next_fwd_is_fwd, next_bwd_is_fwd, bc_fwd, bc_bwd,
@@ -251,6 +302,39 @@ class HipSolver:
         self._num_fsrs, self._num_groups = len(fm), G
         self._tg = None
         self._psi_size = 2 * len(arrs["azim"]) * (int(num_polar) // 2) * G
+
+    def setFSRCentroids(self, xyz: np.ndarray) -> None:
+        """FSR centroids [N, 3] that name the FSRs in flux files (set from the
+        TrackGenerator by setTrackGenerator)."""
+        a = np.ascontiguousarray(xyz, np.float64)
+        check(lib().moc_solver_set_fsr_centroids(self._h, _ptr(a), a.size))
+
+    # --- flux files (Solver.cpp:2056-2318) --------------------------------
+    def dumpFSRFluxes(self, fname: str) -> None:
+        """Solver::dumpFSRFluxes (src/Solver.cpp:2066-2112): k_eff, FSR
+        centroids and the current scalar fluxes, binary."""
+        check(lib().moc_solver_dump_fsr_fluxes(self._h, _path(fname)))
+
+    def loadFSRFluxes(self, fname: str, assign_k_eff: bool = False,
+                      tolerance: float = 0.01) -> float:
+        """Solver::loadFSRFluxes (src/Solver.cpp:2135-2318): the fluxes of
+        the file's FSRs nearest to this geometry's centroids become the current
+        fluxes.  Returns the largest centroid mismatch (cm)."""
+        err = C.c_double()
+        check(lib().moc_solver_load_fsr_fluxes(self._h, _path(fname), int(bool(assign_k_eff)),
+                                               float(tolerance), C.byref(err)))
+        return err.value
+
+    def loadInitialFSRFluxes(self, fname: Optional[str]) -> None:
+        """Solver::loadInitialFSRFluxes (src/Solver.cpp:2120-2123): eigenvalue
+        and source solves restart from the file's fluxes and k_eff after 30
+        sweeps on their sources.  None switches it off."""
+        check(lib().moc_solver_load_initial_fsr_fluxes(self._h, _path(fname or "")))
+
+    def setResidualByReference(self, fname: Optional[str]) -> None:
+        """Solver::setResidualByReference (src/Solver.cpp:2056-2059):
+        residuals against the file's fluxes.  None switches it off."""
+        check(lib().moc_solver_set_residual_by_reference(self._h, _path(fname or "")))
 
     def setMaterials(self, sigma_t, sigma_s, nu_sigma_f, chi) -> None:
         """Replace the cross sections (sigma_t/nu_sigma_f/chi [M, G], sigma_s

@@ -179,7 +179,9 @@ int main(int argc, char** argv) {
   if (argc < 3) {
     fprintf(stderr, "usage: %s problem.txt out.bin [--threads N] [--no-solve] "
                     "[--max-iters N] [--tol X] [--sweeps N] [--balance] "
-                    "[--stabilize FACTOR TYPE] [--krylov] [--linear-source]\n", argv[0]);
+                    "[--stabilize FACTOR TYPE] [--krylov] [--linear-source] "
+                    "[--dump-fluxes F] [--restart-from F] [--residual-reference F] "
+                    "[--load-fluxes F]\n", argv[0]);
     return 2;
   }
   int threads = 1;
@@ -190,6 +192,12 @@ int main(int argc, char** argv) {
   bool balance = false, krylov = false, linear_source = false;
   double stab_factor = 0.0;
   int stab_type = -1;
+  /* flux files (Solver::dumpFSRFluxes / loadInitialFSRFluxes /
+   * setResidualByReference / loadFSRFluxes, src/Solver.cpp:2056-2318) */
+  const char* dump_fluxes = nullptr;
+  const char* restart_from = nullptr;
+  const char* residual_reference = nullptr;
+  const char* load_fluxes = nullptr;
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -203,6 +211,10 @@ int main(int argc, char** argv) {
     }
     else if (!strcmp(argv[a], "--krylov")) krylov = true;
     else if (!strcmp(argv[a], "--linear-source")) linear_source = true;
+    else if (!strcmp(argv[a], "--dump-fluxes")) dump_fluxes = argv[++a];
+    else if (!strcmp(argv[a], "--restart-from")) restart_from = argv[++a];
+    else if (!strcmp(argv[a], "--residual-reference")) residual_reference = argv[++a];
+    else if (!strcmp(argv[a], "--load-fluxes")) load_fluxes = argv[++a];
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -376,9 +388,12 @@ int main(int argc, char** argv) {
       residualType rt = FISSION_SOURCE;
       if (s.res_type == "SCALAR_FLUX") rt = SCALAR_FLUX;
       else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
+      if (restart_from) solver.loadInitialFSRFluxes(restart_from);
+      if (residual_reference) solver.setResidualByReference(residual_reference);
       auto t0 = std::chrono::steady_clock::now();
       solver.computeEigenvalue(s.max_iters, rt);
       auto t1 = std::chrono::steady_clock::now();
+      if (dump_fluxes) solver.dumpFSRFluxes(dump_fluxes);
       std::vector<double> flux(N * G);
       for (long r = 0; r < N; r++)
         for (int g = 0; g < G; g++) flux[r * G + g] = solver.getFlux(r, g + 1);
@@ -549,9 +564,12 @@ int main(int argc, char** argv) {
     residualType rt = FISSION_SOURCE;
     if (s.res_type == "SCALAR_FLUX") rt = SCALAR_FLUX;
     else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
+    if (restart_from) solver.loadInitialFSRFluxes(restart_from);
+    if (residual_reference) solver.setResidualByReference(residual_reference);
     auto t0 = std::chrono::steady_clock::now();
     solver.computeEigenvalue(s.max_iters, rt);
     auto t1 = std::chrono::steady_clock::now();
+    if (dump_fluxes) solver.dumpFSRFluxes(dump_fluxes);
     std::vector<double> flux(N * G);
     for (long r = 0; r < N; r++)
       for (int g = 0; g < G; g++) flux[r * G + g] = solver.getFlux(r, g + 1);
@@ -591,6 +609,16 @@ int main(int argc, char** argv) {
     w.d("fsr_centroid", cen);
     w.d("fsr_point", pt);
     w.i("fsr_material", fm);
+  }
+
+  /* Optional: the fluxes Solver::loadFSRFluxes reads from a flux file onto
+   * this geometry's FSRs (k_eff not assigned, tolerance 0.01 cm). */
+  if (solve && load_fluxes) {
+    solver.loadFSRFluxes(load_fluxes, false, 0.01);
+    std::vector<double> lf(N * G);
+    for (long r = 0; r < N; r++)
+      for (int g = 0; g < G; g++) lf[r * G + g] = solver.getFlux(r, g + 1);
+    w.d("loaded_flux", lf);
   }
 
   /* Optional: Krylov operator sweeps on the converged flux (the pattern of

@@ -25,7 +25,21 @@ Outputs (data only: inputs and the reference's outputs):
                             conditions and segment count, segment lengths and
                             FSR ids, corrected polar angles and weights, FSR
                             volumes, keff/flux/iterations
+  lattice_3x3_vac_7g_ref.fluxes, lattice_3x3_vac_7g_it10.fluxes
+                            flux files written by the reference's
+                            Solver::dumpFSRFluxes after the converged solve and
+                            after 10 iterations
+  lattice_3x3_vac_7g_fluxfile_ref.npz
+                            restarts from the 10-iteration file
+                            (loadInitialFSRFluxes: 5 iterations, and to
+                            convergence), the solve with residuals against the
+                            converged file (setResidualByReference), and the
+                            fluxes the reference's loadFSRFluxes reads from a
+                            file this library wrote with its FSRs shuffled
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
+
+`python tests/golden/make_golden.py flux_files` regenerates only the flux-file
+fixtures.
 """
 import sys
 import tempfile
@@ -58,7 +72,49 @@ def base(d):
                 num_tracks=np.int64(len(d["trk_azim"])))
 
 
+def flux_files():
+    """Flux-file fixtures (Solver::dumpFSRFluxes / loadInitialFSRFluxes /
+    setResidualByReference / loadFSRFluxes, src/Solver.cpp:2056-2318)."""
+    import openmoc_amd as om
+
+    name = "lattice_3x3_vac_7g"
+    conv, part = OUT / f"{name}_ref.fluxes", OUT / f"{name}_it10.fluxes"
+    _, d = run(name, threads=1, extra=("--dump-fluxes", str(conv)))
+    _, d10 = run(name, threads=1, extra=("--max-iters", "10", "--dump-fluxes", str(part)))
+    _, r5 = run(name, threads=1, extra=("--restart-from", str(part), "--max-iters", "5"))
+    _, rc = run(name, threads=1, extra=("--restart-from", str(part)))
+    _, rr = run(name, threads=1, extra=("--residual-reference", str(conv)))
+    # a file written by this library: synthetic fluxes on its own centroids,
+    # FSR order shuffled, read back by the reference onto its FSRs
+    tg = om.TrackGenerator(workloads.get(name))
+    tg.generateTracks()
+    xyz = om.fsr_centroids(tg)
+    N, G = len(xyz), tg.getNumEnergyGroups()
+    rng = np.random.default_rng(11)
+    flux = rng.uniform(0.5, 2.0, (N, G))
+    perm = rng.permutation(N)
+    with tempfile.TemporaryDirectory() as td:
+        ours = Path(td) / "ours.fluxes"
+        om.write_flux_file(ours, 1.2345, xyz[perm], flux[perm])
+        _, dl = run(name, threads=1, extra=("--load-fluxes", str(ours)))
+    np.savez_compressed(
+        OUT / f"{name}_fluxfile_ref.npz",
+        keff=np.float64(d["keff"]), flux=d["flux"], num_iterations=np.int64(d["num_iterations"]),
+        it10_keff=np.float64(d10["keff"]), it10_flux=d10["flux"],
+        restart5_keff=np.float64(r5["keff"]), restart5_flux=r5["flux"],
+        restart5_num_iterations=np.int64(r5["num_iterations"]),
+        restart_keff=np.float64(rc["keff"]), restart_flux=rc["flux"],
+        restart_num_iterations=np.int64(rc["num_iterations"]),
+        residual_ref_keff=np.float64(rr["keff"]), residual_ref_flux=rr["flux"],
+        residual_ref_num_iterations=np.int64(rr["num_iterations"]),
+        written_flux=flux, written_perm=perm, loaded_flux=dl["loaded_flux"].reshape(N, G),
+        fsr_centroid=d["fsr_centroid"].reshape(N, 2))
+
+
 def main():
+    if sys.argv[1:] == ["flux_files"]:
+        flux_files()
+        return
     for name in ("c5g7_pin_cell", "hom_inf_medium"):
         _, d = run(name, threads=1)
         np.savez_compressed(OUT / f"{name}_ref.npz", flux=d["flux"], **base(d))
@@ -126,6 +182,7 @@ def main():
         mom = d["flux_xyz"].reshape(N, 3, G)[:, :2, :].transpose(0, 2, 1)
         np.savez_compressed(OUT / f"{name}_ls_ref.npz", flux=d["flux"], flux_moments=mom,
                             fsr_centroid=d["fsr_centroid_ls"].reshape(N, 2), **base(d))
+    flux_files()
     print("golden fixtures written to", OUT)
 
 
