@@ -225,7 +225,8 @@ class HipSolver {
   // sizes
   int G_ = 0, M_ = 0, P_ = 0, num_azim_ = 0, num_polar_ = 0;
   bool is3d_ = false;                  // 3D tracks: P_ = 1, weight class per track
-  // 3D z-stack groups (tracks over one 2D track), for the LDS stack sweep
+  // 3D z-stack groups (tracks over a pair of neighbouring 2D tracks), for the
+  // LDS stack sweep
   int32_t ngrp_ = 0;                   // 0: no groups (2D, or no xy given)
   int32_t max_grp_slots_ = 0;          // most FSRs touched by one group
   std::vector<int32_t> h_trk_grp_;     // [T] group of each track

@@ -702,9 +702,10 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
   for (int i = threadIdx.x; i < nl; i += blockDim.x) out[i] = lds_tally[i];
 }
 
-// 3D z-stack sweep.  A group is the set of 3D tracks over one 2D track (its
-// z-stacks, every polar angle); its segments touch only the FSRs along that
-// 2D line, one per (2D region, axial layer), which get local slots 0..n-1.
+// 3D z-stack sweep.  A group is the set of 3D tracks over a pair of
+// neighbouring parallel 2D tracks (their z-stacks, every polar angle); its
+// segments touch only the FSRs along those two 2D lines, one per (2D region,
+// axial layer), which get local slots 0..n-1.
 // Block (group, group subset) keeps the fixed-point tally and the
 // {Q/sigma_t, 1/V} table of its slots in LDS, sweeps the group's tiles, and
 // adds its tally to the global int64 tally (exact, order-independent).  The
@@ -1907,8 +1908,13 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   }
   ls_ready_ = false;
 
-  // 3D: group the tracks by the 2D track they lie over (azimuthal and xy
-  // index); each group's FSRs get slots 0..n-1 in order of first appearance
+  // 3D: group the tracks by the 2D track they lie over (azimuthal index and
+  // xy index / kStackPairing: neighbouring parallel 2D tracks, which cross
+  // nearly the same FSRs, share a group, so a group's tally flush covers the
+  // work of both); each group's FSRs get slots 0..n-1 in order of first
+  // appearance.  On the 3D 17x17 lattice pairs sweep 12% faster than single
+  // 2D tracks; groups of 4 and 8 gain less or lose (larger LDS slices).
+  constexpr int32_t kStackPairing = 2;
   ngrp_ = 0;
   max_grp_slots_ = 0;
   h_trk_grp_.clear();
@@ -1920,8 +1926,9 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
     std::map<std::pair<int32_t, int32_t>, int32_t> gid;
     h_trk_grp_.resize(T);
     for (int64_t t = 0; t < T; t++) {
-      auto it = gid.find({td.azim[t], td.xy[t]});
-      if (it == gid.end()) it = gid.emplace(std::make_pair(td.azim[t], td.xy[t]), (int32_t)gid.size()).first;
+      const std::pair<int32_t, int32_t> key(td.azim[t], td.xy[t] / kStackPairing);
+      auto it = gid.find(key);
+      if (it == gid.end()) it = gid.emplace(key, (int32_t)gid.size()).first;
       h_trk_grp_[t] = it->second;
     }
     ngrp_ = (int32_t)gid.size();
