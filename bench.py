@@ -34,6 +34,9 @@ sys.path.insert(0, str(ROOT))
 METRIC = "segment-group integrations/s + keff-solves/s, p50/p99 latency at ≤1 pcm"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 (vector, packed)
+# the same VALUs without packed FP32 instructions, which this build does not
+# emit (one FP32 FMA per lane per clock)
+FP32_UNPACKED_PEAK_TFLOPS = FP32_PEAK_TFLOPS / 2
 FLOPS_PER_INTEGRATION = 32     # rational F1 + attenuation + tally (FMA = 2)
 
 
@@ -295,6 +298,8 @@ def main():
                 "launch_seconds": sweep_s}
     roofline_valu = {"achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+                     "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
+                     "frac_unpacked": achieved_tflops / FP32_UNPACKED_PEAK_TFLOPS,
                      "flops_per_integration": FLOPS_PER_INTEGRATION}
 
     serving = None
