@@ -88,6 +88,18 @@ typedef struct {
  * cores).  Replaces Geometry::initializeFlatSourceRegions +
  * TrackGenerator::generateTracks. */
 int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** out);
+/* TrackGenerator::dumpSegmentsToFile (src/TrackGenerator.cpp:1388-1501;
+ * records of DumpSegments::onTrack, src/TrackTraversingAlgorithms.cpp:
+ * 1101-1144): the problem text, the laydown, every track's segment records
+ * (f64 length, i32 material, i64 region, i32 track index, f64 start x, y, z)
+ * and the FSR table.  MOC_ERR_IO if the file cannot be written. */
+int moc_tracks_dump_segments(const moc_tracks_t* tracks, const char* path);
+/* TrackGenerator::readSegmentsFromFile (src/TrackGenerator.cpp:1518-1638):
+ * tracks and segments from a file written by moc_tracks_dump_segments for the
+ * same problem text, without laydown or ray tracing.  *out is NULL (and the
+ * call returns MOC_OK) if the file was written for another problem, as the
+ * reference returns false; MOC_ERR_IO for a missing or malformed file. */
+int moc_tracks_read_segments(const char* problem_text, const char* path, moc_tracks_t** out);
 void moc_tracks_destroy(moc_tracks_t* tracks);
 int moc_tracks_info(const moc_tracks_t* tracks, moc_tracks_info_t* info);
 

@@ -61,6 +61,8 @@ _SIGS = [
     ("moc_version", C.c_char_p, []),
     ("moc_tracks_generate", C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
     ("moc_tracks_destroy", None, [_P]),
+    ("moc_tracks_dump_segments", C.c_int, [_P, C.c_char_p]),
+    ("moc_tracks_read_segments", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(_P)]),
     ("moc_tracks_info", C.c_int, [_P, C.POINTER(TracksInfo)]),
     ("moc_tracks_export_tracks", C.c_int, [_P] + [_P] * 11),
     ("moc_tracks_export_segments", C.c_int, [_P, _P, _P]),

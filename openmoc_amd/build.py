@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libopenmoc_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tracks.cpp", "fluxfile.cpp", "capi.cpp", "serve.cpp", "hip_solver.hip"]
+SOURCES = ["tracks.cpp", "fluxfile.cpp", "segfile.cpp", "capi.cpp", "serve.cpp", "hip_solver.hip"]
 
 # gfx950 only; packed FP32 instructions disabled (-packed-fp32-ops); hardware
 # float atomics for the sweep tally (-munsafe-fp-atomics).

@@ -7,6 +7,7 @@
 
 #include "../../include/openmoc_amd.h"
 #include "fluxfile.h"
+#include "segfile.h"
 #include "hip_solver.h"
 #include "serve.h"
 #include "tracks.h"
@@ -14,6 +15,7 @@
 struct moc_tracks {
   moc::Problem problem;
   moc::TrackSet ts;
+  std::string text;  // the problem text the tracks were made from
 };
 
 struct moc_solver {
@@ -76,6 +78,7 @@ int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** ou
   return guarded([&] {
     auto* t = new moc_tracks;
     try {
+      t->text = problem_text;
       t->problem = moc::parse_problem(problem_text);
       t->ts = moc::generate_tracks(t->problem, threads);
       // the exponential evaluators support optical lengths up to 100
@@ -90,6 +93,39 @@ int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** ou
 }
 
 void moc_tracks_destroy(moc_tracks_t* tracks) { delete tracks; }
+
+int moc_tracks_dump_segments(const moc_tracks_t* t, const char* path) {
+  if (!t || !path || !*path) return fail(MOC_ERR_INVALID, "null argument");
+  return guarded([&] { moc::dump_segment_file(path, t->text, t->ts); });
+}
+
+int moc_tracks_read_segments(const char* problem_text, const char* path, moc_tracks_t** out) {
+  if (!problem_text || !path || !*path || !out) return fail(MOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto* t = new moc_tracks;
+    try {
+      t->text = problem_text;
+      t->problem = moc::parse_problem(problem_text);
+      if (!moc::read_segment_file(path, t->text, t->ts)) {
+        delete t;
+        return;  // written for another problem
+      }
+      if (t->ts.num_groups != t->problem.num_groups ||
+          t->ts.fsr_material.size() != t->ts.fsr_volume.size())
+        throw moc::DeviceError(MOC_ERR_IO, std::string("segment file ") + path +
+                                               " does not match its problem text");
+      for (int32_t m : t->ts.fsr_material)
+        if (m < 0 || m >= (int32_t)t->problem.materials.size())
+          throw moc::DeviceError(MOC_ERR_IO, std::string("segment file ") + path +
+                                                 ": material id out of range");
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
 
 int moc_tracks_info(const moc_tracks_t* t, moc_tracks_info_t* info) {
   if (!t || !info) return fail(MOC_ERR_INVALID, "null argument");

@@ -219,6 +219,55 @@ class TrackGenerator:
     def getNumEnergyGroups(self) -> int:
         return int(self.info.num_groups)
 
+    # --- segment files (TrackGenerator.cpp:1388-1638) ----------------------
+    def setDumpSegments(self, dump_segments: bool) -> None:
+        """TrackGenerator::setDumpSegments: whether dumpSegmentsToFile writes."""
+        self._dump_segments = bool(dump_segments)
+
+    def getTestFilename(self, directory: str = "tracks") -> str:
+        """Segment-file name for these tracks in `directory`, in the pattern
+        of TrackGenerator(3D)::getTestFilename (TrackGenerator.cpp:1353-1371,
+        TrackGenerator3D.cpp:1480-1532)."""
+        p = self.problem
+        if p is None:
+            raise MOCError(-2, "the problem text could not be parsed for a file name")
+        if p.is_3d:
+            return (f"{directory}/3D_{p.num_azim}_azim_{p.num_polar}_polar_"
+                    f"{p.azim_spacing:g}x{p.z_spacing:g}_cm_spacing_quad_GAUSS_LEGENDRE_"
+                    f"track_EXPLICIT_3D.data")
+        return f"{directory}/2D_{p.num_azim}_angles_{p.azim_spacing:g}_cm_spacing.data"
+
+    def dumpSegmentsToFile(self, fname: Optional[str] = None) -> None:
+        """TrackGenerator::dumpSegmentsToFile (TrackGenerator.cpp:1388-1501):
+        nothing if setDumpSegments(False); otherwise the tracks and segments go
+        to `fname` (default getTestFilename(), its directory created)."""
+        if not getattr(self, "_dump_segments", True):
+            return
+        if not self.containsSegments():
+            raise MOCError(-2, "Unable to dump Segments to a file since no Segments have "
+                               "been generated")
+        path = fname or self.getTestFilename()
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        check(lib().moc_tracks_dump_segments(self._h, _path(path)))
+
+    def readSegmentsFromFile(self, fname: Optional[str] = None) -> bool:
+        """TrackGenerator::readSegmentsFromFile (TrackGenerator.cpp:1518-1638):
+        takes the tracks and segments from `fname` (default getTestFilename())
+        instead of generating them.  False, with nothing changed, if the file
+        was written for another problem."""
+        h = C.c_void_p()
+        check(lib().moc_tracks_read_segments(self._text.encode(),
+                                             _path(fname or self.getTestFilename()), C.byref(h)))
+        if not h.value:
+            return False
+        self._free()
+        self._h = h
+        info = _lib.TracksInfo()
+        check(lib().moc_tracks_info(h, C.byref(info)))
+        self.info = info
+        self._export()
+        return True
+
     def getFSRVolumes(self) -> np.ndarray:
         return self.fsr_volume
 
