@@ -66,6 +66,7 @@ int ServeEngine::addGeometry(const moc_track_data_t& tracks, int64_t num_fsrs,
   geo->G = G;
   geo->N = num_fsrs;
   geo->bytes = geo->tmpl->solveBytes();
+  geo->work = 2.0 * (double)tracks.num_segments * G * (tracks.polar ? 1 : tracks.num_polar / 2);
   std::lock_guard<std::mutex> lk(geo_mu_);
   geos_.push_back(std::move(geo));
   return (int)geos_.size() - 1;
@@ -192,7 +193,10 @@ void ServeEngine::release(const Active& a) {
 void ServeEngine::loop() {
   (void)hipSetDevice(device_);
   while (true) {
-    // admission: free slots and memory, in arrival order
+    // admission: free slots and memory.  Longest first: the pending request
+    // with the most work per sweep starts next, so long solves overlap the
+    // short ones instead of trailing them; a request that has been passed
+    // over max_active_ times is admitted before anything younger.
     std::vector<std::shared_ptr<Request>> admit_now;
     {
       std::unique_lock<std::mutex> lk(mu_);
@@ -203,17 +207,28 @@ void ServeEngine::loop() {
       size_t mem = mem_used_;
       size_t slots = active_.size();
       while (!pending_.empty() && slots < (size_t)max_active_) {
-        const auto& r = pending_.front();
+        size_t pick = 0;
         size_t need;
         {
           std::lock_guard<std::mutex> gl(geo_mu_);
-          need = geos_[r->geo]->bytes;
+          if (pending_.front()->passed_over < max_active_) {
+            double best = -1.0;
+            for (size_t i = 0; i < pending_.size(); i++) {
+              const double w = geos_[pending_[i]->geo]->work;
+              if (w > best) {
+                best = w;
+                pick = i;
+              }
+            }
+          }
+          need = geos_[pending_[pick]->geo]->bytes;
         }
         if (slots > 0 && mem + need > budget_) break;  // wait for memory
         mem += need;
         slots++;
-        admit_now.push_back(r);
-        pending_.pop_front();
+        for (size_t i = 0; i < pick; i++) pending_[i]->passed_over++;
+        admit_now.push_back(pending_[pick]);
+        pending_.erase(pending_.begin() + (long)pick);
       }
       if (stop_ && !admit_now.empty()) {
         // shutting down: fail what was not started

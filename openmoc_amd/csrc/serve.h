@@ -68,6 +68,7 @@ class ServeEngine {
     int iters = 0;
     std::vector<double> flux;
     double t_submit = 0, t_start = 0, t_done = 0;
+    int passed_over = 0;  // admissions that went to a later request
   };
   struct Geometry {
     std::unique_ptr<HipSolver> tmpl;                 // owns the shared buffers
@@ -77,6 +78,7 @@ class ServeEngine {
     int M = 0, G = 0;
     int64_t N = 0;
     size_t bytes = 0;                                // device bytes of one solve
+    double work = 0.0;                               // integrations per sweep
   };
   struct Active {
     std::shared_ptr<Request> req;
