@@ -29,6 +29,10 @@ Outputs (data only: inputs and the reference's outputs):
                             flux files written by the reference's
                             Solver::dumpFSRFluxes after the converged solve and
                             after 10 iterations
+  pin_cell_3d_7g_ref.fluxes, pin_cell_70g_ref.fluxes and *_fluxfile_ref.npz
+                            the same restart and residual-by-reference runs
+                            from the converged file on the 3D z-stack sweep and
+                            at 70 groups
   lattice_3x3_vac_7g_fluxfile_ref.npz
                             restarts from the 10-iteration file
                             (loadInitialFSRFluxes: 5 iterations, and to
@@ -111,9 +115,31 @@ def flux_files():
         fsr_centroid=d["fsr_centroid"].reshape(N, 2))
 
 
+def flux_files_other_paths():
+    """Restart (loadInitialFSRFluxes) and residual-by-reference runs from the
+    reference's own converged file, on the 3D pin cell (z-stack sweep) and the
+    70-group pin cell (per-FSR source residual terms)."""
+    for name in ("pin_cell_3d_7g", "pin_cell_70g"):
+        conv = OUT / f"{name}_ref.fluxes"
+        _, d = run(name, threads=1, extra=("--dump-fluxes", str(conv)))
+        _, rc = run(name, threads=1, extra=("--restart-from", str(conv)))
+        _, rr = run(name, threads=1, extra=("--residual-reference", str(conv)))
+        np.savez_compressed(
+            OUT / f"{name}_fluxfile_ref.npz",
+            keff=np.float64(d["keff"]), num_iterations=np.int64(d["num_iterations"]),
+            restart_keff=np.float64(rc["keff"]), restart_flux=rc["flux"],
+            restart_num_iterations=np.int64(rc["num_iterations"]),
+            residual_ref_keff=np.float64(rr["keff"]), residual_ref_flux=rr["flux"],
+            residual_ref_num_iterations=np.int64(rr["num_iterations"]))
+
+
 def main():
     if sys.argv[1:] == ["flux_files"]:
         flux_files()
+        flux_files_other_paths()
+        return
+    if sys.argv[1:] == ["flux_files_other_paths"]:
+        flux_files_other_paths()
         return
     for name in ("c5g7_pin_cell", "hom_inf_medium"):
         _, d = run(name, threads=1)
@@ -183,6 +209,7 @@ def main():
         np.savez_compressed(OUT / f"{name}_ls_ref.npz", flux=d["flux"], flux_moments=mom,
                             fsr_centroid=d["fsr_centroid_ls"].reshape(N, 2), **base(d))
     flux_files()
+    flux_files_other_paths()
     print("golden fixtures written to", OUT)
 
 
