@@ -948,16 +948,22 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
       if (a.lds_partials) {
         const unsigned long long* p = a.lds_partials + (size_t)sub * a.lds_bps * nl + j;
         // exact (two's complement) sums, so any order gives the same bits;
-        // eight independent accumulators keep eight loads in flight
-        unsigned long long acc[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-        int b = 0;
-        for (; b + 8 <= a.lds_bps; b += 8) {
+        // sixteen independent accumulators keep sixteen loads in flight
+        constexpr int kAcc = 16;
+        unsigned long long acc[kAcc];
 #pragma unroll
-          for (int u = 0; u < 8; u++) acc[u] += p[(size_t)(b + u) * nl];
+        for (int u = 0; u < kAcc; u++) acc[u] = 0ull;
+        int b = 0;
+        for (; b + kAcc <= a.lds_bps; b += kAcc) {
+#pragma unroll
+          for (int u = 0; u < kAcc; u++) acc[u] += p[(size_t)(b + u) * nl];
         }
         for (; b < a.lds_bps; b++) acc[0] += p[(size_t)b * nl];
-        const unsigned long long t =
-            ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+        for (int w = kAcc / 2; w > 0; w /= 2)
+#pragma unroll
+          for (int u = 0; u < w; u++) acc[u] += acc[u + w];
+        const unsigned long long t = acc[0];
         // the fixed-point tally holds sum(contribution) / V
         f = ((double)(long long)t / a.state->tally_scale) / st + kFourPi * a.Q[k] / st;
       } else if (a.tally_i64) {
