@@ -312,6 +312,13 @@ int moc_solver_load_initial_fsr_fluxes(moc_solver_t* solver, const char* path);
  * the normalised flux with the file's fluxes instead of the previous
  * iteration's.  Read when each solve starts; NULL or "" switches it off. */
 int moc_solver_set_residual_by_reference(moc_solver_t* solver, const char* path);
+/* Solver::setRestartStatus(is_restart) (src/Solver.cpp:610-615): while on,
+ * compute_eigenvalue keeps k_eff, and compute_eigenvalue / compute_source keep
+ * the angular fluxes and the scalar flux of the solver's previous solve
+ * instead of a flat guess (src/Solver.cpp:1554-1572, 1721-1731); the iteration
+ * count starts at 1.  Flat source only; MOC_ERR_STATE without an earlier
+ * solve on this solver. */
+int moc_solver_set_restart_status(moc_solver_t* solver, int32_t is_restart);
 
 /* Host-only flux-file helpers (no GPU needed), the steps the solver calls
  * above use.  moc_flux_file_read with null xyz / flux returns the header only;

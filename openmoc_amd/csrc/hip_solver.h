@@ -185,6 +185,11 @@ class HipSolver {
   // compare the flux with the file's fluxes instead of the previous
   // iteration's.  "" switches it off.
   void setResidualByReference(const std::string& path);
+  // Solver::setRestartStatus (src/Solver.cpp:610-615): the next eigenvalue
+  // and source solves keep k_eff (eigenvalue), the angular fluxes and the
+  // scalar flux of the previous solve instead of starting from a flat guess
+  // (src/Solver.cpp:1554-1572, 1721-1731).
+  void setRestartStatus(bool on) { restart_status_ = on; }
 
  private:
   enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
@@ -198,6 +203,7 @@ class HipSolver {
   void launchInit(Mode mode, double k0);
   void readFluxFiles(Mode mode);
   void launchStartupSweeps();
+  void launchRestart(Mode mode, double k0);
   void zeroTallies(hipStream_t s);
   std::vector<double> matchedFlux(const std::string& path, double tolerance, double* keff,
                                   double* max_error);
@@ -298,6 +304,11 @@ class HipSolver {
   double init_keff_ = 1.0;
   double* d_ref_flux_ = nullptr;       // [N*G] reference flux of the residual
   size_t ref_flux_n_ = 0;              // values read into d_ref_flux_
+  bool restart_status_ = false;        // setRestartStatus
+  bool restart_active_ = false;        // the running solve continues the previous one
+  bool solved_ = false;                // a solve finished since the arrays were laid out
+  int psi_cur0_ = 0;                   // start-flux buffer at the running solve's start
+  int graph_psi0_ = -1;                // start-flux buffer the iteration graph was captured at
   std::vector<int32_t> h_work_, h_tile_;  // host copies of the stream layout
   std::vector<int32_t> h_wpos_;        // [2T] work position of each directed track
   int32_t* d_out_wp_ = nullptr;        // [2T] work position of the linked start slot

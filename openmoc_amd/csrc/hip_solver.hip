@@ -2244,6 +2244,7 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
 void HipSolver::allocateIterationArrays() {
   graph_dirty_ = true;
   state_valid_ = false;
+  solved_ = false;
   if (!(tracks_set_ && materials_set_ && fsrs_set_)) return;
   const size_t NG = (size_t)N_ * G_;
   if (!shape_forced_) pickShape();
@@ -2707,6 +2708,11 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // the linear source's per-FSR data (expansion matrices, source constants,
   // position stream) must exist before its kernels read them
   if (linear_ && !ls_ready_) prepareLinearSource();
+  restart_active_ = false;
+  if (restart_status_ && mode != MODE_FLUX) {
+    launchRestart(mode, k0);
+    return;
+  }
   const size_t NG = (size_t)N_ * G_;
   const size_t psi_n = psiCount();
   // a flux file's fluxes and k_eff replace the flat guess of eigenvalue and
@@ -2762,6 +2768,56 @@ void HipSolver::launchInit(Mode mode, double k0) {
     h_init_flux_.clear();
   }
 
+  state_valid_ = true;
+  phi_is_old_ = true;
+  krylov_ready_ = false;
+}
+
+// A solve that continues the previous one (setRestartStatus): k_eff (kept
+// for eigenvalue solves, src/Solver.cpp:1563-1564; set to k0 by
+// computeSource, :1474), the start fluxes with their pending normalisation and
+// the current scalar flux carry over; the flux is stored as the old flux and
+// gives the sources of iteration 0, without the flat guess or its
+// normalisation (src/Solver.cpp:1721-1731).
+void HipSolver::launchRestart(Mode mode, double k0) {
+  if (!solved_ || !state_valid_)
+    throw DeviceError(MOC_ERR_STATE, "a restart needs an earlier solve on this solver");
+  if (linear_) throw DeviceError(MOC_ERR_INVALID, "restarts are available for the flat source");
+  if (h_init_flux_.size() == (size_t)N_ * G_)
+    throw DeviceError(MOC_ERR_INVALID, "a restart and initial fluxes from a file are exclusive");
+  const size_t NG = (size_t)N_ * G_;
+  HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  IterState st = *h_state_;
+  if (mode == MODE_SOURCE) st.k = k0;
+  st.k_prev = st.k;
+  st.residual = 0.0;
+  st.iteration = 0;
+  st.converged = 0;
+  st.dk = 0;
+  st.reported_iters = 0;
+  st.negative_fluxes = 0;
+  st.negative_sources = 0;
+  // the next sweep's start fluxes are the last sweep's outgoing ones, bounded
+  // by its running maximum (psi_scale still applies to them)
+  st.psi_max_bits = __builtin_bit_cast(uint32_t, st.psi_max);
+  st.qmax_bits = 0u;
+  *h_state_ = st;
+  HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
+  if (phi_is_old_)
+    HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[st.old_sel & 1], NG * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+  zeroTallies(stream_);
+  refreshStabilization();
+  const FsrArgs fa = fsrArgs();
+  // storeFSRFluxes and the sources of iteration 0, no normalisation
+  launchScatter(stream_);
+  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+  hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  restart_active_ = true;
   state_valid_ = true;
   phi_is_old_ = true;
   krylov_ready_ = false;
@@ -3382,7 +3438,11 @@ size_t HipSolver::solveBytes() const {
 
 void HipSolver::ensureGraph(int mode, int res_type) {
   const bool recompute = mode != MODE_FLUX;
-  if (graph_exec_ && graph_mode_ == mode && graph_res_type_ == res_type && !graph_dirty_) return;
+  // the graph's sweeps alternate the two start-flux buffers from the parity
+  // they were captured at
+  if (graph_exec_ && graph_mode_ == mode && graph_res_type_ == res_type && !graph_dirty_ &&
+      graph_psi0_ == psi_cur_)
+    return;
   if (graph_exec_) {
     (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
@@ -3421,6 +3481,7 @@ void HipSolver::ensureGraph(int mode, int res_type) {
     throw DeviceError(MOC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   psi_cur_ = psi0;  // an even number of iterations was recorded
+  graph_psi0_ = psi0;
   graph_mode_ = mode;
   graph_res_type_ = res_type;
   graph_dirty_ = false;
@@ -3468,6 +3529,7 @@ void HipSolver::startSolve(int mode, int max_iters, int res_type, double k_fixed
   async_t0_ = std::chrono::steady_clock::now();
   readFluxFiles((Mode)mode);
   launchInit((Mode)mode, mode == MODE_SOURCE ? k_fixed : 1.0);
+  psi_cur0_ = psi_cur_;
   if (use_graphs_ && max_iters >= kGraphIters) ensureGraph(mode, res_type);
   async_mode_ = mode;
   async_res_type_ = res_type;
@@ -3508,7 +3570,13 @@ void HipSolver::finishSolve(int max_iters) {
   keff_ = h_state_->k;
   old_sel_ = h_state_->old_sel;
   phi_is_old_ = true;
-  num_iterations_ = h_state_->converged ? h_state_->reported_iters : max_iters;
+  // a restart counts from 1 (src/Solver.cpp:1554)
+  num_iterations_ = (h_state_->converged ? h_state_->reported_iters : max_iters) +
+                    (restart_active_ ? 1 : 0);
+  // iterations launched after convergence return at entry, so the start
+  // fluxes of a following sweep are where the last live iteration put them
+  psi_cur_ = psi_cur0_ ^ (h_state_->iteration & 1);
+  solved_ = true;
   total_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - async_t0_).count();
   // device time of the iteration batches (sweeps plus the per-FSR steps)
   sweep_seconds_ = async_ms_ * 1e-3;

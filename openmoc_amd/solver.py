@@ -385,6 +385,11 @@ class HipSolver:
         residuals against the file's fluxes.  None switches it off."""
         check(lib().moc_solver_set_residual_by_reference(self._h, _path(fname or "")))
 
+    def setRestartStatus(self, is_restart: bool) -> None:
+        """Solver::setRestartStatus (src/Solver.cpp:610-615): following solves
+        continue from this solver's k_eff, angular and scalar fluxes."""
+        check(lib().moc_solver_set_restart_status(self._h, int(bool(is_restart))))
+
     def setMaterials(self, sigma_t, sigma_s, nu_sigma_f, chi) -> None:
         """Replace the cross sections (sigma_t/nu_sigma_f/chi [M, G], sigma_s
         [M, G*G] dest-major), e.g. a perturbed copy of the geometry's materials

@@ -198,6 +198,7 @@ int main(int argc, char** argv) {
   const char* restart_from = nullptr;
   const char* residual_reference = nullptr;
   const char* load_fluxes = nullptr;
+  int restart_after = 0;  /* --restart-after N: N iterations, then setRestartStatus(true) */
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -215,6 +216,7 @@ int main(int argc, char** argv) {
     else if (!strcmp(argv[a], "--restart-from")) restart_from = argv[++a];
     else if (!strcmp(argv[a], "--residual-reference")) residual_reference = argv[++a];
     else if (!strcmp(argv[a], "--load-fluxes")) load_fluxes = argv[++a];
+    else if (!strcmp(argv[a], "--restart-after")) restart_after = atoi(argv[++a]);
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -566,6 +568,10 @@ int main(int argc, char** argv) {
     else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
     if (restart_from) solver.loadInitialFSRFluxes(restart_from);
     if (residual_reference) solver.setResidualByReference(residual_reference);
+    if (restart_after > 0) {
+      solver.computeEigenvalue(restart_after, rt);
+      solver.setRestartStatus(true);
+    }
     auto t0 = std::chrono::steady_clock::now();
     solver.computeEigenvalue(s.max_iters, rt);
     auto t1 = std::chrono::steady_clock::now();

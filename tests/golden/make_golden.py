@@ -37,9 +37,11 @@ Outputs (data only: inputs and the reference's outputs):
                             restarts from the 10-iteration file
                             (loadInitialFSRFluxes: 5 iterations, and to
                             convergence), the solve with residuals against the
-                            converged file (setResidualByReference), and the
-                            fluxes the reference's loadFSRFluxes reads from a
-                            file this library wrote with its FSRs shuffled
+                            converged file (setResidualByReference), solves
+                            continued with setRestartStatus after 10 iterations
+                            and after convergence, and the fluxes the
+                            reference's loadFSRFluxes reads from a file this
+                            library wrote with its FSRs shuffled
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 
 `python tests/golden/make_golden.py flux_files` regenerates only the flux-file
@@ -88,6 +90,9 @@ def flux_files():
     _, r5 = run(name, threads=1, extra=("--restart-from", str(part), "--max-iters", "5"))
     _, rc = run(name, threads=1, extra=("--restart-from", str(part)))
     _, rr = run(name, threads=1, extra=("--residual-reference", str(conv)))
+    # setRestartStatus after 10 iterations, and after a converged solve
+    _, rs10 = run(name, threads=1, extra=("--restart-after", "10"))
+    _, rs40 = run(name, threads=1, extra=("--restart-after", "40"))
     # a file written by this library: synthetic fluxes on its own centroids,
     # FSR order shuffled, read back by the reference onto its FSRs
     tg = om.TrackGenerator(workloads.get(name))
@@ -111,6 +116,10 @@ def flux_files():
         restart_num_iterations=np.int64(rc["num_iterations"]),
         residual_ref_keff=np.float64(rr["keff"]), residual_ref_flux=rr["flux"],
         residual_ref_num_iterations=np.int64(rr["num_iterations"]),
+        restart10_keff=np.float64(rs10["keff"]), restart10_flux=rs10["flux"],
+        restart10_num_iterations=np.int64(rs10["num_iterations"]),
+        restart40_keff=np.float64(rs40["keff"]), restart40_flux=rs40["flux"],
+        restart40_num_iterations=np.int64(rs40["num_iterations"]),
         written_flux=flux, written_perm=perm, loaded_flux=dl["loaded_flux"].reshape(N, G),
         fsr_centroid=d["fsr_centroid"].reshape(N, 2))
 

@@ -151,3 +151,44 @@ def test_restart_and_reference_residual_on_other_sweep_paths(tracks, name):
     assert abs(q.getNumIterations() - int(ref["residual_ref_num_iterations"])) <= 1
     assert abs(q.getKeff() - ref["residual_ref_keff"]) < 1e-6
     assert rel_l2(q.getFluxes(), ref["residual_ref_flux"]) < 1e-4
+
+
+def test_restart_status_continues_the_previous_solve(tracks, ref):
+    """setRestartStatus (src/Solver.cpp:610-615): 10 iterations, then a solve
+    that keeps k_eff, the angular and the scalar fluxes.  The reference stops
+    after 1 + 8 iterations with the eigenvalue of the uninterrupted 18; after
+    a converged solve it runs 1 + 1 more."""
+    tg = tracks(NAME)
+    full = om.HipSolver(tg)
+    full.computeEigenvalue(1000)
+    s = om.HipSolver(tg)
+    s.computeEigenvalue(10)
+    s.setRestartStatus(True)
+    s.computeEigenvalue(1000)
+    assert s.getNumIterations() == int(ref["restart10_num_iterations"]) == 9
+    assert abs(s.getKeff() - ref["restart10_keff"]) < 1e-7
+    assert rel_l2(s.getFluxes(), ref["restart10_flux"]) < 1e-5
+    # the continued iteration is the uninterrupted one
+    assert abs(s.getKeff() - full.getKeff()) < 1e-12
+    np.testing.assert_allclose(s.getFluxes(), full.getFluxes(), rtol=1e-12, atol=0)
+    # continuing a converged solve
+    full.setRestartStatus(True)
+    full.computeEigenvalue(1000)
+    assert full.getNumIterations() == int(ref["restart40_num_iterations"])
+    assert abs(full.getKeff() - ref["restart40_keff"]) < 1e-7
+    assert rel_l2(full.getFluxes(), ref["restart40_flux"]) < 1e-5
+    # off again: a fresh solve
+    full.setRestartStatus(False)
+    full.computeEigenvalue(1000)
+    assert abs(full.getNumIterations() - int(ref["num_iterations"])) <= 1
+
+
+def test_restart_status_needs_an_earlier_solve(tracks):
+    s = om.HipSolver(tracks(NAME))
+    s.setRestartStatus(True)
+    with pytest.raises(om.MOCError) as e:
+        s.computeEigenvalue(10)
+    assert e.value.code == -2
+    s.setRestartStatus(False)
+    s.computeEigenvalue(10)
+    assert s.getNumIterations() == 10
