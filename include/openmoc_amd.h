@@ -259,7 +259,9 @@ int moc_solver_scatter_transport_sweep(moc_solver_t* solver);
 
 /* Solver::computeEigenvalue(max_iters, res_type) (src/Solver.cpp:1542-1689). */
 int moc_solver_compute_eigenvalue(moc_solver_t* solver, int32_t max_iters, int32_t res_type);
-/* Solver::computeFlux(max_iters, only_fixed_source) (src/Solver.cpp:1352-1420). */
+/* Solver::computeFlux(max_iters, only_fixed_source) (src/Solver.cpp:1352-1420):
+ * k_eff = 1; only_fixed_source = 0 after an earlier solve starts from that
+ * solve's scalar and angular fluxes, otherwise from zero (:1379-1386). */
 int moc_solver_compute_flux(moc_solver_t* solver, int32_t max_iters, int32_t only_fixed_source);
 /* Solver::computeSource(max_iters, k_eff, res_type) (src/Solver.cpp:1459-1516). */
 int moc_solver_compute_source(moc_solver_t* solver, int32_t max_iters, double k_eff,

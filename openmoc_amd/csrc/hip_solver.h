@@ -306,6 +306,7 @@ class HipSolver {
   size_t ref_flux_n_ = 0;              // values read into d_ref_flux_
   bool restart_status_ = false;        // setRestartStatus
   bool restart_active_ = false;        // the running solve continues the previous one
+  bool continue_flux_ = false;         // computeFlux(only_fixed_source = false) after a solve
   bool solved_ = false;                // a solve finished since the arrays were laid out
   int psi_cur0_ = 0;                   // start-flux buffer at the running solve's start
   int graph_psi0_ = -1;                // start-flux buffer the iteration graph was captured at

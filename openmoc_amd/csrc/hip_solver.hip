@@ -2709,7 +2709,9 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // position stream) must exist before its kernels read them
   if (linear_ && !ls_ready_) prepareLinearSource();
   restart_active_ = false;
-  if (restart_status_ && mode != MODE_FLUX) {
+  const bool cont = mode == MODE_FLUX ? continue_flux_ : restart_status_;
+  continue_flux_ = false;
+  if (cont) {
     launchRestart(mode, k0);
     return;
   }
@@ -2773,9 +2775,10 @@ void HipSolver::launchInit(Mode mode, double k0) {
   krylov_ready_ = false;
 }
 
-// A solve that continues the previous one (setRestartStatus): k_eff (kept
-// for eigenvalue solves, src/Solver.cpp:1563-1564; set to k0 by
-// computeSource, :1474), the start fluxes with their pending normalisation and
+// A solve that continues the previous one (setRestartStatus, or computeFlux
+// without only_fixed_source): k_eff (kept for eigenvalue solves,
+// src/Solver.cpp:1563-1564; set to k0 by computeSource and computeFlux,
+// :1474, :1366), the start fluxes with their pending normalisation and
 // the current scalar flux carry over; the flux is stored as the old flux and
 // gives the sources of iteration 0, without the flat guess or its
 // normalisation (src/Solver.cpp:1721-1731).
@@ -2789,7 +2792,9 @@ void HipSolver::launchRestart(Mode mode, double k0) {
   HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
   IterState st = *h_state_;
-  if (mode == MODE_SOURCE) st.k = k0;
+  // computeSource sets k_eff (src/Solver.cpp:1474), computeFlux sets 1
+  // (src/Solver.cpp:1366-1367)
+  if (mode != MODE_EIGEN) st.k = k0;
   st.k_prev = st.k;
   st.residual = 0.0;
   st.iteration = 0;
@@ -2817,7 +2822,8 @@ void HipSolver::launchRestart(Mode mode, double k0) {
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(stream_));
-  restart_active_ = true;
+  // only computeEigenvalue counts a restart as an iteration (src/Solver.cpp:1554)
+  restart_active_ = mode == MODE_EIGEN;
   state_valid_ = true;
   phi_is_old_ = true;
   krylov_ready_ = false;
@@ -3013,8 +3019,10 @@ void HipSolver::computeEigenvalue(int max_iters, int res_type) {
 }
 
 void HipSolver::computeFlux(int max_iters, bool only_fixed_source) {
-  (void)only_fixed_source;  // the flux arrays are always re-initialised
   if (!fixed_on_) throw DeviceError(MOC_ERR_STATE, "computeFlux requires fixed sources");
+  // only_fixed_source = false after an earlier solve: its scalar and angular
+  // fluxes are the starting point instead of zero (src/Solver.cpp:1379-1386)
+  continue_flux_ = !only_fixed_source && solved_ && state_valid_;
   runSolve(MODE_FLUX, max_iters, MOC_SCALAR_FLUX, 1.0);
 }
 
