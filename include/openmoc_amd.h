@@ -318,8 +318,8 @@ int moc_solver_set_residual_by_reference(moc_solver_t* solver, const char* path)
  * compute_eigenvalue keeps k_eff, and compute_eigenvalue / compute_source keep
  * the angular fluxes and the scalar flux of the solver's previous solve
  * instead of a flat guess (src/Solver.cpp:1554-1572, 1721-1731); the iteration
- * count starts at 1.  Flat source only; MOC_ERR_STATE without an earlier
- * solve on this solver. */
+ * count starts at 1 (flux moments carry over with the linear source).
+ * MOC_ERR_STATE without an earlier solve on this solver. */
 int moc_solver_set_restart_status(moc_solver_t* solver, int32_t is_restart);
 
 /* Host-only flux-file helpers (no GPU needed), the steps the solver calls

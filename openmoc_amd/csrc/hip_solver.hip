@@ -2785,7 +2785,6 @@ void HipSolver::launchInit(Mode mode, double k0) {
 void HipSolver::launchRestart(Mode mode, double k0) {
   if (!solved_ || !state_valid_)
     throw DeviceError(MOC_ERR_STATE, "a restart needs an earlier solve on this solver");
-  if (linear_) throw DeviceError(MOC_ERR_INVALID, "restarts are available for the flat source");
   if (h_init_flux_.size() == (size_t)N_ * G_)
     throw DeviceError(MOC_ERR_INVALID, "a restart and initial fluxes from a file are exclusive");
   const size_t NG = (size_t)N_ * G_;
@@ -2811,6 +2810,10 @@ void HipSolver::launchRestart(Mode mode, double k0) {
   HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
   if (phi_is_old_)
     HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[st.old_sel & 1], NG * sizeof(double),
+                             hipMemcpyDeviceToDevice, stream_));
+  // linear source: the normalised flux moments of the last iteration too
+  if (linear_)
+    HIP_CHECK(hipMemcpyAsync(d_phi_xy_, d_phi_xy_n_, 2 * NG * sizeof(double),
                              hipMemcpyDeviceToDevice, stream_));
   zeroTallies(stream_);
   refreshStabilization();

@@ -42,6 +42,8 @@ Outputs (data only: inputs and the reference's outputs):
                             and after convergence, and the fluxes the
                             reference's loadFSRFluxes reads from a file this
                             library wrote with its FSRs shuffled
+  lattice_3x3_vac_7g_ls_restart_ref.npz
+                            linear source: setRestartStatus after 10 iterations
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 
 `python tests/golden/make_golden.py flux_files` regenerates only the flux-file
@@ -142,6 +144,16 @@ def flux_files_other_paths():
             residual_ref_num_iterations=np.int64(rr["num_iterations"]))
 
 
+def ls_restart():
+    """Linear source: setRestartStatus after 10 iterations."""
+    _, d = run("lattice_3x3_vac_7g", threads=1, extra=("--linear-source", "--restart-after", "10"))
+    N, G = int(d["num_fsrs"]), int(d["num_groups"])
+    mom = d["flux_xyz"].reshape(N, 3, G)[:, :2, :].transpose(0, 2, 1)
+    np.savez_compressed(OUT / "lattice_3x3_vac_7g_ls_restart_ref.npz", flux=d["flux"],
+                        flux_moments=mom, keff=np.float64(d["keff"]),
+                        num_iterations=np.int64(d["num_iterations"]))
+
+
 def main():
     if sys.argv[1:] == ["flux_files"]:
         flux_files()
@@ -149,6 +161,9 @@ def main():
         return
     if sys.argv[1:] == ["flux_files_other_paths"]:
         flux_files_other_paths()
+        return
+    if sys.argv[1:] == ["ls_restart"]:
+        ls_restart()
         return
     for name in ("c5g7_pin_cell", "hom_inf_medium"):
         _, d = run(name, threads=1)
@@ -219,6 +234,7 @@ def main():
                             fsr_centroid=d["fsr_centroid_ls"].reshape(N, 2), **base(d))
     flux_files()
     flux_files_other_paths()
+    ls_restart()
     print("golden fixtures written to", OUT)
 
 

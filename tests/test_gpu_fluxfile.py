@@ -192,3 +192,22 @@ def test_restart_status_needs_an_earlier_solve(tracks):
     s.setRestartStatus(False)
     s.computeEigenvalue(10)
     assert s.getNumIterations() == 10
+
+
+def test_restart_status_with_the_linear_source(tracks):
+    """setRestartStatus on CPULSSolver semantics: the flux moments carry over
+    too; 1 + 8 iterations after 10, the eigenvalue of the uninterrupted solve."""
+    ref = dict(np.load(GOLDEN / f"{NAME}_ls_restart_ref.npz"))
+    tg = tracks(NAME)
+    full = om.HipLSSolver(tg)
+    full.computeEigenvalue(1000)
+    s = om.HipLSSolver(tg)
+    s.computeEigenvalue(10)
+    s.setRestartStatus(True)
+    s.computeEigenvalue(1000)
+    assert s.getNumIterations() == int(ref["num_iterations"]) == 9
+    assert abs(s.getKeff() - ref["keff"]) < 1e-6
+    assert rel_l2(s.getFluxes(), ref["flux"]) < 1e-4
+    assert abs(s.getKeff() - full.getKeff()) < 1e-9
+    m, mr = s.getFluxMoments(), ref["flux_moments"]
+    np.testing.assert_allclose(m.reshape(mr.shape), mr, atol=1e-4 * np.abs(mr).max())
