@@ -44,6 +44,10 @@ extern "C" {
 #define MOC_FISSION_SOURCE 1
 #define MOC_TOTAL_SOURCE 2
 
+/* solverMode (src/constants.h) */
+#define MOC_FORWARD 0
+#define MOC_ADJOINT 1
+
 /* exponential evaluation in the sweep */
 #define MOC_EXP_RATIONAL 0   /* the reference's 5/6 rational fit (default) */
 #define MOC_EXP_HW 1         /* hardware exp2: 1 - 2^(-x log2 e) */
@@ -321,6 +325,13 @@ int moc_solver_set_residual_by_reference(moc_solver_t* solver, const char* path)
  * count starts at 1 (flux moments carry over with the linear source).
  * MOC_ERR_STATE without an earlier solve on this solver. */
 int moc_solver_set_restart_status(moc_solver_t* solver, int32_t is_restart);
+/* Solver::setSolverMode(solver_mode) (src/Solver.cpp:597-602; applied in
+ * initializeMaterials, :792-812): MOC_ADJOINT solves with the scattering and
+ * fission matrices transposed (Material::transposeProductionMatrices,
+ * src/Material.cpp:1091-1103); keff, normalisation and the fission-source
+ * residual keep nu-sigma-f, as in the reference.  Stays in effect for every
+ * following solve. */
+int moc_solver_set_solver_mode(moc_solver_t* solver, int32_t mode);
 
 /* Host-only flux-file helpers (no GPU needed), the steps the solver calls
  * above use.  moc_flux_file_read with null xyz / flux returns the header only;

@@ -15,6 +15,8 @@ EXP_RATIONAL, EXP_HW = 0, 1
 SCATTER_AUTO, SCATTER_FP64, SCATTER_MFMA_F16, SCATTER_FP32 = 0, 1, 2, 3
 # stabilizationType (src/Solver.h:103-113)
 DIAGONAL, YAMAMOTO, GLOBAL = 0, 1, 2
+# solverMode (src/constants.h)
+FORWARD, ADJOINT = 0, 1
 
 
 class MOCError(RuntimeError):
@@ -117,6 +119,7 @@ _SIGS = [
     ("moc_solver_load_initial_fsr_fluxes", C.c_int, [_P, C.c_char_p]),
     ("moc_solver_set_residual_by_reference", C.c_int, [_P, C.c_char_p]),
     ("moc_solver_set_restart_status", C.c_int, [_P, C.c_int32]),
+    ("moc_solver_set_solver_mode", C.c_int, [_P, C.c_int32]),
     ("moc_flux_file_write", C.c_int, [C.c_char_p, C.c_double, C.c_int32, C.c_int64, _P, _P]),
     ("moc_flux_file_read", C.c_int, [C.c_char_p, _P, _P, _P, _P, C.c_int64, _P, C.c_int64]),
     ("moc_fsr_match_centroids", C.c_int, [_P, C.c_int64, _P, C.c_int64, C.c_double, _P, _P]),

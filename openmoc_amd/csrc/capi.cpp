@@ -502,6 +502,10 @@ int moc_solver_set_restart_status(moc_solver_t* s, int32_t is_restart) {
   MOC_SOLVER_CALL(s, s->impl->setRestartStatus(is_restart != 0));
 }
 
+int moc_solver_set_solver_mode(moc_solver_t* s, int32_t mode) {
+  MOC_SOLVER_CALL(s, s->impl->setSolverMode(mode));
+}
+
 int moc_flux_file_write(const char* path, double keff, int32_t num_groups, int64_t num_fsrs,
                         const double* xyz, const double* flux) {
   if (!path || !*path) return fail(MOC_ERR_INVALID, "empty file name");

@@ -190,6 +190,9 @@ class HipSolver {
   // scalar flux of the previous solve instead of starting from a flat guess
   // (src/Solver.cpp:1554-1572, 1721-1731).
   void setRestartStatus(bool on) { restart_status_ = on; }
+  // Solver::setSolverMode (src/Solver.cpp:597-602): MOC_FORWARD or
+  // MOC_ADJOINT (transposed scattering and fission matrices).
+  void setSolverMode(int mode);
 
  private:
   enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
@@ -204,6 +207,7 @@ class HipSolver {
   void readFluxFiles(Mode mode);
   void launchStartupSweeps();
   void launchRestart(Mode mode, double k0);
+  void uploadProductionMatrices(int M, int G);
   void zeroTallies(hipStream_t s);
   std::vector<double> matchedFlux(const std::string& path, double tolerance, double* keff,
                                   double* max_error);
@@ -270,7 +274,8 @@ class HipSolver {
   double* d_sigma_t_ = nullptr;        // [M*G]
   double* d_sigma_s_ = nullptr;        // [M*G*G]
   double* d_nsf_ = nullptr;            // [M*G]
-  double* d_chi_ = nullptr;            // [M*G]
+  double* d_chi_ = nullptr;            // [M*G] fission-source spectrum (nu-sigma-f when adjoint)
+  double* d_fis_nsf_ = nullptr;        // [M*G] fission-source production (chi when adjoint)
   int32_t* d_mat_fissionable_ = nullptr;  // [M]
   int32_t* d_fsr_mat_ = nullptr;       // [N]
   double* d_fsr_vol_ = nullptr;        // [N]
@@ -283,7 +288,9 @@ class HipSolver {
   double* d_sigma_a_ = nullptr;        // [M*G] absorption
   float* d_leak_ = nullptr;            // [2T*G] vacuum leakage (neutron-balance keff)
   double* d_stab_ratio_ = nullptr;     // [G] YAMAMOTO ratios
-  std::vector<double> h_sigma_t_, h_sigma_s_;  // host copies (dest-major scattering)
+  std::vector<double> h_sigma_t_, h_sigma_s_;  // host copies (dest-major scattering in use)
+  std::vector<double> h_sigma_s_fwd_, h_nsf_fwd_, h_chi_fwd_;  // the materials as given
+  int solver_mode_ = MOC_FORWARD;
   int stab_type_ = -1;
   double stab_factor_ = 0.0;
   bool balance_ = false;

@@ -813,8 +813,9 @@ struct FsrArgs {
   const double* fsr_vol;
   const double* sigma_t;
   const double* sigma_s;      // dest-major
-  const double* nsf;
-  const double* chi;
+  const double* nsf;          // nu-sigma-f: keff, normalisation, residual
+  const double* chi;          // fission-source spectrum (nu-sigma-f when adjoint)
+  const double* fis_nsf;      // fission-source production (chi when adjoint)
   const double* sigma_a;      // [M*G] absorption (neutron-balance keff)
   const int32_t* fissionable;
   const double* fixed;        // nullable
@@ -1185,7 +1186,8 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
     const double k = a.state->k;
     const double st = a.sigma_t[(size_t)m * G + gd];
     const double* ss = a.sigma_s + (size_t)m * G * G;
-    const double* nsf = a.nsf + (size_t)m * G;
+    // fission source: chi(gd) sum_g fnsf(g) x(g) (transposed roles when adjoint)
+    const double* fnsf = a.fis_nsf + (size_t)m * G;
     const bool fis = a.fissionable[m] != 0;
     const double* phi = a.phi + (size_t)r * G;
     const int sel = a.state->old_sel;
@@ -1216,7 +1218,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
           if (sct)
             fsrc = a.chi[(size_t)m * G + gd] * (double)sct[G];
           else
-            for (int g = 0; g < G; g++) fsrc += a.chi[(size_t)m * G + gd] * nsf[g] * phi[g];
+            for (int g = 0; g < G; g++) fsrc += a.chi[(size_t)m * G + gd] * fnsf[g] * phi[g];
         }
         q = fsrc * kOneOverFourPi;
       } else if (source_kind == kSrcScatter) {
@@ -1234,7 +1236,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
           sc = (double)sct[gd] * f;
         } else {
           if (fis)
-            for (int g = 0; g < G; g++) new_fis += nsf[g] * (normalize ? phi[g] * f : phi[g]);
+            for (int g = 0; g < G; g++) new_fis += fnsf[g] * (normalize ? phi[g] * f : phi[g]);
           const double* row = ss + (size_t)gd * G;
           for (int g = 0; g < G; g++) sc += row[g] * (normalize ? phi[g] * f : phi[g]);
         }
@@ -1264,7 +1266,7 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
           } else {
             if (fis) {
               const double cg = a.chi[(size_t)m * G + gd];
-              for (int g = 0; g < G; g++) fis_c += cg * nsf[g] * (mom[2 * g + c] * fm);
+              for (int g = 0; g < G; g++) fis_c += cg * fnsf[g] * (mom[2 * g + c] * fm);
               fis_c /= k;
             }
             const double* row = ss + (size_t)gd * G;
@@ -1700,6 +1702,7 @@ HipSolver::~HipSolver() {
   dev_free(d_sigma_s_);
   dev_free(d_nsf_);
   dev_free(d_chi_);
+  dev_free(d_fis_nsf_);
   dev_free(d_mat_fissionable_);
   dev_free(d_fsr_mat_);
   dev_free(d_fsr_vol_);
@@ -2136,38 +2139,17 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
     }
   }
   dev_alloc(&d_sigma_t_, (size_t)M * G);
-  dev_alloc(&d_sigma_s_, (size_t)M * G * G);
   dev_alloc(&d_nsf_, (size_t)M * G);
-  dev_alloc(&d_chi_, (size_t)M * G);
   dev_alloc(&d_mat_fissionable_, M);
   HIP_CHECK(hipMemcpy(d_sigma_t_, sigma_t, (size_t)M * G * sizeof(double), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(d_sigma_s_, sigma_s, (size_t)M * G * G * sizeof(double), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_nsf_, nu_sigma_f, (size_t)M * G * sizeof(double), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(d_chi_, chi, (size_t)M * G * sizeof(double), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(d_mat_fissionable_, fis.data(), M * sizeof(int32_t), hipMemcpyHostToDevice));
   mat_fissionable_ = fis;
   h_sigma_t_.assign(sigma_t, sigma_t + (size_t)M * G);
-  h_sigma_s_.assign(sigma_s, sigma_s + (size_t)M * G * G);
-  {
-    // Material::getSigmaA: sigma_t minus the out-scattering of the group
-    // (src/Material.cpp:228-257)
-    std::vector<double> sa((size_t)M * G);
-    for (int m = 0; m < M; m++)
-      for (int g = 0; g < G; g++) {
-        double v = sigma_t[(size_t)m * G + g];
-        for (int d = 0; d < G; d++) v -= sigma_s[((size_t)m * G + d) * G + g];
-        sa[(size_t)m * G + g] = v;
-      }
-    dev_alloc(&d_sigma_a_, (size_t)M * G);
-    HIP_CHECK(hipMemcpy(d_sigma_a_, sa.data(), sa.size() * sizeof(double), hipMemcpyHostToDevice));
-    std::vector<double> cs((size_t)M * G, 0.0);
-    for (int m = 0; m < M; m++)
-      for (int d = 0; d < G; d++)
-        for (int g = 0; g < G; g++) cs[(size_t)m * G + g] += sigma_s[((size_t)m * G + d) * G + g];
-    dev_alloc(&d_ss_colsum_, (size_t)M * G);
-    HIP_CHECK(hipMemcpy(d_ss_colsum_, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
-  scat_frag_ok_ = false;
+  h_sigma_s_fwd_.assign(sigma_s, sigma_s + (size_t)M * G * G);
+  h_nsf_fwd_.assign(nu_sigma_f, nu_sigma_f + (size_t)M * G);
+  h_chi_fwd_.assign(chi, chi + (size_t)M * G);
+  uploadProductionMatrices(M, G);
   sigma_t_max_ = 0.0;
   for (size_t i = 0; i < (size_t)M * G; i++) sigma_t_max_ = std::max(sigma_t_max_, sigma_t[i]);
   if (G != G_) {
@@ -2189,6 +2171,66 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
     countFissionable();
   }
   allocateIterationArrays();
+}
+
+// The production matrices the sources use (Solver::initializeMaterials,
+// src/Solver.cpp:792-812): forward, sigma_s and the fission matrix
+// chi (x) nu-sigma-f; adjoint, both transposed (Material::
+// transposeProductionMatrices, src/Material.cpp:1091-1103), i.e. sigma_s^T and
+// nu-sigma-f (x) chi.  keff, the normalisation and the fission-source residual
+// keep nu-sigma-f (src/CPUSolver.cpp:1860-1931, 2113-2328); the absorption and
+// the total-source residual follow the matrix in use, as the reference's read
+// its transposed sigma_s.
+void HipSolver::uploadProductionMatrices(int M, int G) {
+  const bool adj = solver_mode_ == MOC_ADJOINT;
+  const size_t MG = (size_t)M * G;
+  h_sigma_s_.resize(MG * G);
+  for (int m = 0; m < M; m++)
+    for (int d = 0; d < G; d++)
+      for (int g = 0; g < G; g++)
+        h_sigma_s_[((size_t)m * G + d) * G + g] =
+            adj ? h_sigma_s_fwd_[((size_t)m * G + g) * G + d] : h_sigma_s_fwd_[((size_t)m * G + d) * G + g];
+  const std::vector<double>& src_chi = adj ? h_nsf_fwd_ : h_chi_fwd_;
+  const std::vector<double>& src_nsf = adj ? h_chi_fwd_ : h_nsf_fwd_;
+  dev_alloc(&d_sigma_s_, MG * G);
+  dev_alloc(&d_chi_, MG);
+  dev_alloc(&d_fis_nsf_, MG);
+  HIP_CHECK(hipMemcpy(d_sigma_s_, h_sigma_s_.data(), MG * G * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_chi_, src_chi.data(), MG * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_fis_nsf_, src_nsf.data(), MG * sizeof(double), hipMemcpyHostToDevice));
+  // Material::getSigmaA: sigma_t minus the out-scattering of the group
+  // (src/Material.cpp:228-257); column sums for the total-source residual
+  std::vector<double> sa(MG), cs(MG, 0.0);
+  for (int m = 0; m < M; m++)
+    for (int g = 0; g < G; g++) {
+      double v = h_sigma_t_[(size_t)m * G + g];
+      for (int d = 0; d < G; d++) {
+        v -= h_sigma_s_[((size_t)m * G + d) * G + g];
+        cs[(size_t)m * G + g] += h_sigma_s_[((size_t)m * G + d) * G + g];
+      }
+      sa[(size_t)m * G + g] = v;
+    }
+  dev_alloc(&d_sigma_a_, MG);
+  HIP_CHECK(hipMemcpy(d_sigma_a_, sa.data(), sa.size() * sizeof(double), hipMemcpyHostToDevice));
+  dev_alloc(&d_ss_colsum_, MG);
+  HIP_CHECK(hipMemcpy(d_ss_colsum_, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice));
+  scat_frag_ok_ = false;
+  graph_dirty_ = true;
+}
+
+void HipSolver::setSolverMode(int mode) {
+  if (mode != MOC_FORWARD && mode != MOC_ADJOINT) throw DeviceError(MOC_ERR_INVALID, "unknown solver mode");
+  if (mode == solver_mode_) return;
+  solver_mode_ = mode;
+  if (materials_set_) {
+    HIP_CHECK(hipSetDevice(device_));
+    uploadProductionMatrices(M_, G_);
+    // the matrix-core operands follow the matrices in use
+    if (tracks_set_ && fsrs_set_) prepareScatter();
+  }
+  state_valid_ = false;
+  solved_ = false;
+  krylov_ready_ = false;
 }
 
 void HipSolver::countFissionable() {
@@ -2365,7 +2407,7 @@ void HipSolver::prepareScatter() {
     dev_alloc(&d_scat_b_, (size_t)nfrag * 4);
     dev_alloc(&d_scat_colmul_, (size_t)M_ * nt_n * 16);
     hipLaunchKernelGGL(scatter_frag_kernel, dim3(blocks_for(nfrag, 256)), dim3(256), 0, stream_,
-                       d_sigma_s_, d_nsf_, M_, G_, ks_n, nt_n,
+                       d_sigma_s_, d_fis_nsf_, M_, G_, ks_n, nt_n,
                        reinterpret_cast<uint4*>(d_scat_b_), d_scat_colmul_);
     HIP_CHECK(hipGetLastError());
     scat_frag_ok_ = true;
@@ -2390,7 +2432,7 @@ void HipSolver::launchScatter(hipStream_t s) {
   a.bfrag = reinterpret_cast<const uint4*>(d_scat_b_);
   a.colmul = d_scat_colmul_;
   a.sigma_s = d_sigma_s_;
-  a.nsf = d_nsf_;
+  a.nsf = d_fis_nsf_;
   a.fsr_mat = d_fsr_mat_;
   a.C = linear_ ? 3 : 1;
   a.x[0] = d_phi_;
@@ -2619,6 +2661,7 @@ FsrArgs HipSolver::fsrArgs() {
   a.sigma_s = d_sigma_s_;
   a.nsf = d_nsf_;
   a.chi = d_chi_;
+  a.fis_nsf = d_fis_nsf_;
   a.sigma_a = d_sigma_a_;
   a.fissionable = d_mat_fissionable_;
   a.fixed = fixed_on_ ? d_fixed_ : nullptr;

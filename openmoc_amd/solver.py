@@ -25,7 +25,8 @@ from typing import Optional, Union
 import numpy as np
 
 from . import _lib
-from ._lib import (DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, GLOBAL, SCALAR_FLUX,
+from ._lib import (ADJOINT, DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, FORWARD, GLOBAL,
+                   SCALAR_FLUX,
                    SCATTER_AUTO, SCATTER_FP32, SCATTER_FP64, SCATTER_MFMA_F16, TOTAL_SOURCE,
                    YAMAMOTO, MOCError, check, lib)
 from .workloads import Problem
@@ -384,6 +385,11 @@ class HipSolver:
         """Solver::setResidualByReference (src/Solver.cpp:2056-2059):
         residuals against the file's fluxes.  None switches it off."""
         check(lib().moc_solver_set_residual_by_reference(self._h, _path(fname or "")))
+
+    def setSolverMode(self, solver_mode: int) -> None:
+        """Solver::setSolverMode (src/Solver.cpp:597-602): FORWARD or ADJOINT
+        (transposed scattering and fission matrices for the sources)."""
+        check(lib().moc_solver_set_solver_mode(self._h, int(solver_mode)))
 
     def setRestartStatus(self, is_restart: bool) -> None:
         """Solver::setRestartStatus (src/Solver.cpp:610-615): following solves

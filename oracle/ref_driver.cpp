@@ -199,6 +199,7 @@ int main(int argc, char** argv) {
   const char* residual_reference = nullptr;
   const char* load_fluxes = nullptr;
   int restart_after = 0;  /* --restart-after N: N iterations, then setRestartStatus(true) */
+  bool adjoint = false;   /* --adjoint: setSolverMode(ADJOINT) */
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -217,6 +218,7 @@ int main(int argc, char** argv) {
     else if (!strcmp(argv[a], "--residual-reference")) residual_reference = argv[++a];
     else if (!strcmp(argv[a], "--load-fluxes")) load_fluxes = argv[++a];
     else if (!strcmp(argv[a], "--restart-after")) restart_after = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--adjoint")) adjoint = true;
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -568,6 +570,7 @@ int main(int argc, char** argv) {
     else if (s.res_type == "TOTAL_SOURCE") rt = TOTAL_SOURCE;
     if (restart_from) solver.loadInitialFSRFluxes(restart_from);
     if (residual_reference) solver.setResidualByReference(residual_reference);
+    if (adjoint) solver.setSolverMode(ADJOINT);
     if (restart_after > 0) {
       solver.computeEigenvalue(restart_after, rt);
       solver.setRestartStatus(true);

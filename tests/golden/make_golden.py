@@ -44,6 +44,8 @@ Outputs (data only: inputs and the reference's outputs):
                             library wrote with its FSRs shuffled
   lattice_3x3_vac_7g_ls_restart_ref.npz
                             linear source: setRestartStatus after 10 iterations
+  *_adjoint_ref.npz         adjoint solves (setSolverMode(ADJOINT)) of
+                            pin_cell_7g, lattice_3x3_vac_7g and pin_cell_70g
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 
 `python tests/golden/make_golden.py flux_files` regenerates only the flux-file
@@ -154,6 +156,16 @@ def ls_restart():
                         num_iterations=np.int64(d["num_iterations"]))
 
 
+def adjoint():
+    """Adjoint solves (setSolverMode(ADJOINT)): the 2D LDS path at 7 groups
+    and the matrix-core contraction at 70 groups."""
+    for name in ("pin_cell_7g", "lattice_3x3_vac_7g", "pin_cell_70g"):
+        _, d = run(name, threads=1, extra=("--adjoint",))
+        np.savez_compressed(OUT / f"{name}_adjoint_ref.npz", flux=d["flux"],
+                            keff=np.float64(d["keff"]),
+                            num_iterations=np.int64(d["num_iterations"]))
+
+
 def main():
     if sys.argv[1:] == ["flux_files"]:
         flux_files()
@@ -164,6 +176,9 @@ def main():
         return
     if sys.argv[1:] == ["ls_restart"]:
         ls_restart()
+        return
+    if sys.argv[1:] == ["adjoint"]:
+        adjoint()
         return
     for name in ("c5g7_pin_cell", "hom_inf_medium"):
         _, d = run(name, threads=1)
@@ -235,6 +250,7 @@ def main():
     flux_files()
     flux_files_other_paths()
     ls_restart()
+    adjoint()
     print("golden fixtures written to", OUT)
 
 
