@@ -409,6 +409,56 @@ class HipSolver:
 
     def setConvergenceThreshold(self, threshold: float) -> None:
         check(lib().moc_solver_set_convergence_threshold(self._h, float(threshold)))
+        self._threshold = float(threshold)
+
+    # --- reference getters (src/Solver.h:443-462) ---------------------------
+    def getConvergenceThreshold(self) -> float:
+        return getattr(self, "_threshold", 1e-5)
+
+    def getTrackGenerator(self):
+        return self._tg
+
+    def _need_tracks(self):
+        if self._tg is None:
+            raise MOCError(-2, "the solver has no TrackGenerator")
+        return self._tg
+
+    def getNumPolarAngles(self) -> int:
+        return int(self._need_tracks().info.num_polar)
+
+    def getMaxOpticalLength(self) -> float:
+        return self._need_tracks().getMaxOpticalLength()
+
+    def is3D(self) -> bool:
+        return bool(self._need_tracks().info.is_3d)
+
+    def getFSRVolume(self, fsr_id: int) -> float:
+        """Solver::getFSRVolume, with its range check (src/Solver.cpp:192-203)."""
+        tg = self._need_tracks()
+        if not 0 <= int(fsr_id) < tg.getNumFSRs():
+            raise MOCError(-1, f"Unable to get the volume for FSR {fsr_id} since the max "
+                               f"FSR ID = {tg.getNumFSRs() - 1}")
+        return float(tg.fsr_volume[int(fsr_id)])
+
+    def getTotalTime(self) -> float:
+        return self.getTiming()["total_seconds"]
+
+    def isUsingDoublePrecision(self) -> bool:
+        """Angular fluxes and the sweep are FP32 (per-FSR arithmetic FP64)."""
+        return False
+
+    def isUsingExponentialInterpolation(self) -> bool:
+        return False
+
+    def useExponentialIntrinsic(self) -> None:
+        """Solver::useExponentialIntrinsic: the hardware exponential."""
+        self.setExponential(EXP_HW)
+
+    def setNumThreads(self, num_threads: int) -> None:
+        """Accepted for API compatibility; the sweep runs on the GPU."""
+        if int(num_threads) <= 0:
+            raise MOCError(-1, f"Unable to set the number of threads to {num_threads} "
+                               "since it is less than or equal to 0")
 
     def setExponential(self, mode: int) -> None:
         check(lib().moc_solver_set_exponential(self._h, int(mode)))

@@ -52,3 +52,24 @@ def test_unknown_solver_mode(tracks):
     s = om.HipSolver(tracks("pin_cell_7g"))
     with pytest.raises(om.MOCError):
         s.setSolverMode(2)
+
+
+def test_reference_getters(tracks):
+    tg = tracks("pin_cell_7g")
+    s = om.HipSolver(tg)
+    s.setConvergenceThreshold(3e-6)
+    assert s.getConvergenceThreshold() == 3e-6
+    assert s.getTrackGenerator() is tg
+    assert s.getNumPolarAngles() == tg.info.num_polar
+    assert not s.is3D() and om.HipSolver(tracks("pin_cell_3d_7g")).is3D()
+    assert s.getFSRVolume(0) == tg.fsr_volume[0]
+    with pytest.raises(om.MOCError):
+        s.getFSRVolume(tg.getNumFSRs())
+    assert s.getMaxOpticalLength() == tg.getMaxOpticalLength()
+    assert not s.isUsingDoublePrecision() and not s.isUsingExponentialInterpolation()
+    s.setNumThreads(8)
+    s.useExponentialIntrinsic()
+    s.computeEigenvalue(1000)
+    assert s.getTotalTime() > 0
+    ref = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "pin_cell_7g_ref.npz")
+    assert abs(s.getKeff() - float(ref["keff"])) < 1e-5      # hardware exp within 1 pcm
