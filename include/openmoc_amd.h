@@ -220,6 +220,15 @@ int moc_solver_compute_fsr_sources(moc_solver_t* solver, int32_t iteration, doub
 /* Solver::setFixedSourceByFSR (1-based group). */
 int moc_solver_set_fixed_source_by_fsr(moc_solver_t* solver, int64_t fsr, int32_t group,
                                        double source);
+/* Solver::setFixedSourceByMaterial(material, group, source) (src/Solver.cpp:
+ * 538-543), material = index into the materials of moc_solver_set_materials;
+ * applied to every FSR of the material at the call (the reference applies it
+ * when the next solve starts, src/Solver.cpp:1119-1137). */
+int moc_solver_set_fixed_source_by_material(moc_solver_t* solver, int32_t material, int32_t group,
+                                            double source);
+/* CPUSolver::resetFixedSources (src/CPUSolver.cpp:228-250): all fixed
+ * sources zero. */
+int moc_solver_reset_fixed_sources(moc_solver_t* solver);
 
 /* stabilizationType (src/Solver.h:103-113) */
 #define MOC_STAB_NONE (-1)

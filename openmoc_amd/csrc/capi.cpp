@@ -502,6 +502,15 @@ int moc_solver_set_restart_status(moc_solver_t* s, int32_t is_restart) {
   MOC_SOLVER_CALL(s, s->impl->setRestartStatus(is_restart != 0));
 }
 
+int moc_solver_set_fixed_source_by_material(moc_solver_t* s, int32_t material, int32_t group,
+                                            double source) {
+  MOC_SOLVER_CALL(s, s->impl->setFixedSourceByMaterial(material, group, source));
+}
+
+int moc_solver_reset_fixed_sources(moc_solver_t* s) {
+  MOC_SOLVER_CALL(s, s->impl->resetFixedSources());
+}
+
 int moc_solver_set_solver_mode(moc_solver_t* s, int32_t mode) {
   MOC_SOLVER_CALL(s, s->impl->setSolverMode(mode));
 }

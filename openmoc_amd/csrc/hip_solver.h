@@ -136,6 +136,8 @@ class HipSolver {
   void useGraphs(bool on) { use_graphs_ = on; }
   int device() const { return device_; }
   void setFixedSourceByFSR(int64_t fsr, int group, double source);
+  void setFixedSourceByMaterial(int material, int group, double source);
+  void resetFixedSources();
   // Arithmetic of the scattering-source contraction (MOC_SCATTER_*);
   // scatterPrecisionInUse resolves MOC_SCATTER_AUTO for the current groups.
   void setScatterPrecision(int mode);

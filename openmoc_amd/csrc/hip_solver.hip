@@ -2485,6 +2485,37 @@ void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
   HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
 }
 
+// Solver::setFixedSourceByMaterial (src/Solver.cpp:538-543): every FSR of the
+// material, applied now (the reference applies it when a solve starts,
+// src/Solver.cpp:1119-1137).
+void HipSolver::setFixedSourceByMaterial(int material, int group, double source) {
+  graph_dirty_ = true;
+  requireReady();
+  if (material < 0 || material >= M_) throw DeviceError(MOC_ERR_INVALID, "material id out of range");
+  if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
+  const size_t NG = (size_t)N_ * G_;
+  if (h_fixed_.size() != NG) h_fixed_.assign(NG, 0.0);
+  for (int64_t r = 0; r < N_; r++)
+    if (h_fsr_mat_[r] == material) h_fixed_[(size_t)r * G_ + group - 1] = source;
+  fixed_on_ = true;
+  HIP_CHECK(hipSetDevice(device_));
+  if (!d_fixed_) dev_alloc(&d_fixed_, NG);
+  HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// CPUSolver::resetFixedSources (src/CPUSolver.cpp:228-250): every fixed source
+// becomes zero (the solver still counts as having fixed sources).
+void HipSolver::resetFixedSources() {
+  graph_dirty_ = true;
+  requireReady();
+  if (h_fixed_.empty()) return;
+  std::fill(h_fixed_.begin(), h_fixed_.end(), 0.0);
+  HIP_CHECK(hipSetDevice(device_));
+  if (d_fixed_)
+    HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), h_fixed_.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+}
+
 void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
                             hipEvent_t ev_after_sweep, float* leak) {
   SweepArgs a;

@@ -520,6 +520,22 @@ class HipSolver:
         check(lib().moc_solver_set_fixed_source_by_fsr(self._h, int(fsr_id), int(group),
                                                        float(source)))
 
+    def setFixedSourceByMaterial(self, material, group: int, source: float) -> None:
+        """Solver::setFixedSourceByMaterial: `material` is the index of the
+        material (or its name in the problem description)."""
+        if isinstance(material, str):
+            names = [m.name for m in self._tg.problem.materials] if self._tg is not None and \
+                self._tg.problem is not None else []
+            if material not in names:
+                raise MOCError(-1, f"unknown material {material!r}")
+            material = names.index(material)
+        check(lib().moc_solver_set_fixed_source_by_material(self._h, int(material), int(group),
+                                                            float(source)))
+
+    def resetFixedSources(self) -> None:
+        """CPUSolver::resetFixedSources: every fixed source zero."""
+        check(lib().moc_solver_reset_fixed_sources(self._h))
+
     # --- solves ----------------------------------------------------------
     def computeEigenvalue(self, max_iters: int = 1000, res_type: int = FISSION_SOURCE) -> None:
         check(lib().moc_solver_compute_eigenvalue(self._h, int(max_iters), int(res_type)))

@@ -406,3 +406,25 @@ def test_3d_zstack_sweep_equals_global_tally_sweep(tracks, golden):
         assert abs(x.getKeff() - float(g["keff"])) < K_TOL
         assert rel_l2(x.getFluxes(), g["flux"]) < FLUX_RL2
     assert abs(s.getKeff() - t.getKeff()) < 1e-6
+
+
+def test_fixed_source_by_material_and_reset(tracks):
+    """setFixedSourceByMaterial equals setting every FSR of the material;
+    resetFixedSources zeroes them (src/Solver.cpp:538-543,
+    src/CPUSolver.cpp:228-250)."""
+    tg = tracks("c5g7_pin_cell")
+    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
+    m0 = int(tg.fsr_material[0])
+    a = solver_for(tg)
+    a.setFixedSourceByMaterial(m0, 1, 2.5)
+    a.computeFlux(1000)
+    b = solver_for(tg)
+    for r in np.flatnonzero(tg.fsr_material == m0):
+        b.setFixedSourceByFSR(int(r), 1, 2.5)
+    b.computeFlux(1000)
+    np.testing.assert_array_equal(a.getFluxes(), b.getFluxes())
+    a.resetFixedSources()
+    a.computeFlux(1000)
+    assert np.all(a.getFluxes() == 0.0)
+    with pytest.raises(om.MOCError):
+        a.setFixedSourceByMaterial(len(tg.sigma_t), 1, 1.0)
