@@ -341,6 +341,12 @@ int moc_solver_set_restart_status(moc_solver_t* solver, int32_t is_restart);
  * residual keep nu-sigma-f, as in the reference.  Stays in effect for every
  * following solve. */
 int moc_solver_set_solver_mode(moc_solver_t* solver, int32_t mode);
+/* Solver::setChiSpectrumMaterial(material) (src/Solver.cpp:708-710; used by
+ * computeInitialFluxGuess, :1722-1725, and flattenFSRFluxesChiSpectrum,
+ * src/CPUSolver.cpp:1829-1840): eigenvalue and source solves start from the
+ * material's chi in every FSR.  material = index into the materials; -1 for
+ * the flat guess. */
+int moc_solver_set_chi_spectrum_material(moc_solver_t* solver, int32_t material);
 
 /* Host-only flux-file helpers (no GPU needed), the steps the solver calls
  * above use.  moc_flux_file_read with null xyz / flux returns the header only;

@@ -511,6 +511,10 @@ int moc_solver_reset_fixed_sources(moc_solver_t* s) {
   MOC_SOLVER_CALL(s, s->impl->resetFixedSources());
 }
 
+int moc_solver_set_chi_spectrum_material(moc_solver_t* s, int32_t material) {
+  MOC_SOLVER_CALL(s, s->impl->setChiSpectrumMaterial(material));
+}
+
 int moc_solver_set_solver_mode(moc_solver_t* s, int32_t mode) {
   MOC_SOLVER_CALL(s, s->impl->setSolverMode(mode));
 }

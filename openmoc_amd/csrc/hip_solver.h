@@ -195,6 +195,10 @@ class HipSolver {
   // Solver::setSolverMode (src/Solver.cpp:597-602): MOC_FORWARD or
   // MOC_ADJOINT (transposed scattering and fission matrices).
   void setSolverMode(int mode);
+  // Solver::setChiSpectrumMaterial (src/Solver.cpp:708-710): the initial flux
+  // guess of eigenvalue and source solves is that material's chi in every FSR
+  // (src/Solver.cpp:1722-1725); -1 restores the flat guess.
+  void setChiSpectrumMaterial(int material);
 
  private:
   enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
@@ -293,6 +297,8 @@ class HipSolver {
   std::vector<double> h_sigma_t_, h_sigma_s_;  // host copies (dest-major scattering in use)
   std::vector<double> h_sigma_s_fwd_, h_nsf_fwd_, h_chi_fwd_;  // the materials as given
   int solver_mode_ = MOC_FORWARD;
+  int chi_material_ = -1;              // setChiSpectrumMaterial
+  std::vector<double> h_chi_guess_;    // [N*G] staging of that guess
   int stab_type_ = -1;
   double stab_factor_ = 0.0;
   bool balance_ = false;

@@ -2218,6 +2218,12 @@ void HipSolver::uploadProductionMatrices(int M, int G) {
   graph_dirty_ = true;
 }
 
+void HipSolver::setChiSpectrumMaterial(int material) {
+  if (material >= 0 && (!materials_set_ || material >= M_))
+    throw DeviceError(MOC_ERR_INVALID, "material id out of range");
+  chi_material_ = material < 0 ? -1 : material;
+}
+
 void HipSolver::setSolverMode(int mode) {
   if (mode != MOC_FORWARD && mode != MOC_ADJOINT) throw DeviceError(MOC_ERR_INVALID, "unknown solver mode");
   if (mode == solver_mode_) return;
@@ -2813,12 +2819,22 @@ void HipSolver::launchInit(Mode mode, double k0) {
   const FsrArgs fa = fsrArgs();
   // initial guess (computeInitialFluxGuess, src/Solver.cpp:1710-1731; computeFlux
   // starts from phi = 0, src/Solver.cpp:1383-1390)
-  if (restart)
+  if (restart) {
     HIP_CHECK(hipMemcpyAsync(d_phi_, h_init_flux_.data(), NG * sizeof(double),
                              hipMemcpyHostToDevice, stream_));
-  else
+  } else if (mode != MODE_FLUX && chi_material_ >= 0) {
+    // flattenFSRFluxesChiSpectrum (src/CPUSolver.cpp:1829-1840): every FSR
+    // starts from the chi of the chosen material
+    h_chi_guess_.resize(NG);
+    for (int64_t r = 0; r < N_; r++)
+      for (int g = 0; g < G_; g++)
+        h_chi_guess_[(size_t)r * G_ + g] = h_chi_fwd_[(size_t)chi_material_ * G_ + g];
+    HIP_CHECK(hipMemcpyAsync(d_phi_, h_chi_guess_.data(), NG * sizeof(double),
+                             hipMemcpyHostToDevice, stream_));
+  } else {
     hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, d_phi_,
                        (int64_t)NG, mode == MODE_FLUX ? 0.0 : 1.0);
+  }
   if (linear_)
     // flattenFSRFluxes zeroes the moments (src/CPULSSolver.cpp:343-356)
     hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,

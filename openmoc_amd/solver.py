@@ -532,6 +532,19 @@ class HipSolver:
         check(lib().moc_solver_set_fixed_source_by_material(self._h, int(material), int(group),
                                                             float(source)))
 
+    def setChiSpectrumMaterial(self, material) -> None:
+        """Solver::setChiSpectrumMaterial: solves start from that material's
+        chi (index or name; None for the flat guess)."""
+        if material is None:
+            material = -1
+        elif isinstance(material, str):
+            names = [m.name for m in self._tg.problem.materials] if self._tg is not None and \
+                self._tg.problem is not None else []
+            if material not in names:
+                raise MOCError(-1, f"unknown material {material!r}")
+            material = names.index(material)
+        check(lib().moc_solver_set_chi_spectrum_material(self._h, int(material)))
+
     def resetFixedSources(self) -> None:
         """CPUSolver::resetFixedSources: every fixed source zero."""
         check(lib().moc_solver_reset_fixed_sources(self._h))

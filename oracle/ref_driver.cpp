@@ -200,6 +200,7 @@ int main(int argc, char** argv) {
   const char* load_fluxes = nullptr;
   int restart_after = 0;  /* --restart-after N: N iterations, then setRestartStatus(true) */
   bool adjoint = false;   /* --adjoint: setSolverMode(ADJOINT) */
+  int chi_material = -1;  /* --chi-spectrum M: setChiSpectrumMaterial(material M) */
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -219,6 +220,7 @@ int main(int argc, char** argv) {
     else if (!strcmp(argv[a], "--load-fluxes")) load_fluxes = argv[++a];
     else if (!strcmp(argv[a], "--restart-after")) restart_after = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--adjoint")) adjoint = true;
+    else if (!strcmp(argv[a], "--chi-spectrum")) chi_material = atoi(argv[++a]);
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -571,6 +573,7 @@ int main(int argc, char** argv) {
     if (restart_from) solver.loadInitialFSRFluxes(restart_from);
     if (residual_reference) solver.setResidualByReference(residual_reference);
     if (adjoint) solver.setSolverMode(ADJOINT);
+    if (chi_material >= 0) solver.setChiSpectrumMaterial(mats.at(chi_material));
     if (restart_after > 0) {
       solver.computeEigenvalue(restart_after, rt);
       solver.setRestartStatus(true);

@@ -44,6 +44,9 @@ Outputs (data only: inputs and the reference's outputs):
                             library wrote with its FSRs shuffled
   lattice_3x3_vac_7g_ls_restart_ref.npz
                             linear source: setRestartStatus after 10 iterations
+  lattice_3x3_vac_7g_chi_spectrum_ref.npz
+                            solve from the chi spectrum of the first fissile
+                            material (setChiSpectrumMaterial)
   *_adjoint_ref.npz         adjoint solves (setSolverMode(ADJOINT)) of
                             pin_cell_7g, lattice_3x3_vac_7g and pin_cell_70g
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
@@ -166,6 +169,17 @@ def adjoint():
                             num_iterations=np.int64(d["num_iterations"]))
 
 
+def chi_spectrum():
+    """Initial guess from a material's chi (setChiSpectrumMaterial)."""
+    name = "lattice_3x3_vac_7g"
+    p = workloads.get(name)
+    m = next(i for i, mat in enumerate(p.materials) if any(c > 0 for c in mat.chi))
+    _, d = run(name, threads=1, extra=("--chi-spectrum", str(m)))
+    np.savez_compressed(OUT / f"{name}_chi_spectrum_ref.npz", flux=d["flux"],
+                        keff=np.float64(d["keff"]), num_iterations=np.int64(d["num_iterations"]),
+                        material=np.int64(m))
+
+
 def main():
     if sys.argv[1:] == ["flux_files"]:
         flux_files()
@@ -179,6 +193,9 @@ def main():
         return
     if sys.argv[1:] == ["adjoint"]:
         adjoint()
+        return
+    if sys.argv[1:] == ["chi_spectrum"]:
+        chi_spectrum()
         return
     for name in ("c5g7_pin_cell", "hom_inf_medium"):
         _, d = run(name, threads=1)
@@ -251,6 +268,7 @@ def main():
     flux_files_other_paths()
     ls_restart()
     adjoint()
+    chi_spectrum()
     print("golden fixtures written to", OUT)
 
 

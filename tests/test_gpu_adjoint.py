@@ -73,3 +73,22 @@ def test_reference_getters(tracks):
     assert s.getTotalTime() > 0
     ref = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "pin_cell_7g_ref.npz")
     assert abs(s.getKeff() - float(ref["keff"])) < 1e-5      # hardware exp within 1 pcm
+
+
+def test_chi_spectrum_initial_guess_matches_reference(tracks, golden):
+    """setChiSpectrumMaterial (src/Solver.cpp:708-710, 1722-1725): the solve
+    starts from the material's chi in every FSR."""
+    ref = golden("lattice_3x3_vac_7g_chi_spectrum")
+    tg = tracks("lattice_3x3_vac_7g")
+    s = om.HipSolver(tg)
+    s.setChiSpectrumMaterial(int(ref["material"]))
+    s.computeEigenvalue(1000)
+    assert abs(s.getNumIterations() - int(ref["num_iterations"])) <= 1
+    assert abs(s.getKeff() - ref["keff"]) < 1e-6
+    assert rel_l2(s.getFluxes(), ref["flux"]) < 1e-4
+    s.setChiSpectrumMaterial("fuel")       # by name
+    s.setChiSpectrumMaterial(None)         # flat guess again
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - golden("lattice_3x3_vac_7g")["keff"]) < 1e-6
+    with pytest.raises(om.MOCError):
+        s.setChiSpectrumMaterial(99)
