@@ -291,7 +291,10 @@ int moc_solver_get_fluxes(moc_solver_t* solver, double* out, int64_t num_fluxes)
  * A compute_* call starts from its own flat guess, as the reference does
  * without a restart (Solver::computeInitialFluxGuess, src/Solver.cpp:1710-1731). */
 int moc_solver_set_fluxes(moc_solver_t* solver, const double* in, int64_t num_fluxes);
-/* CPUSolver::computeFSRFissionRates(rates, N) (src/CPUSolver.cpp:2850-2890). */
+/* CPUSolver::computeFSRFissionRates(rates, N, nu = true) (src/CPUSolver.cpp:
+ * 2825-2886): volume-weighted nu-sigma-f rates.  The problem description
+ * carries nu-sigma-f only, so the reference's default nu = false (sigma-f
+ * rates) has no counterpart. */
 int moc_solver_compute_fsr_fission_rates(moc_solver_t* solver, double* rates, int64_t num_fsrs);
 
 /* ---- FSR flux files (checkpoint / restart) ----

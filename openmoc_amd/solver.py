@@ -587,7 +587,12 @@ class HipSolver:
         a = np.ascontiguousarray(fluxes, np.float64).ravel()
         check(lib().moc_solver_set_fluxes(self._h, _ptr(a), a.size))
 
-    def computeFSRFissionRates(self, num_fsrs: Optional[int] = None) -> np.ndarray:
+    def computeFSRFissionRates(self, num_fsrs: Optional[int] = None, nu: bool = True) -> np.ndarray:
+        """CPUSolver::computeFSRFissionRates (src/CPUSolver.cpp:2825-2886):
+        volume-weighted nu-sigma-f rates.  Problems carry nu-sigma-f only, so
+        nu=False (sigma-f rates) is refused."""
+        if not nu:
+            raise MOCError(-1, "sigma-f is not part of the problem description; use nu=True")
         n = self._num_fsrs if num_fsrs is None else int(num_fsrs)
         out = np.zeros(n)
         check(lib().moc_solver_compute_fsr_fission_rates(self._h, _ptr(out), n))
