@@ -2825,6 +2825,8 @@ void HipSolver::launchInit(Mode mode, double k0) {
   } else if (mode != MODE_FLUX && chi_material_ >= 0) {
     // flattenFSRFluxesChiSpectrum (src/CPUSolver.cpp:1829-1840): every FSR
     // starts from the chi of the chosen material
+    if (chi_material_ >= M_ || h_chi_fwd_.size() != (size_t)M_ * G_)
+      throw DeviceError(MOC_ERR_STATE, "the chi-spectrum material is not among the materials");
     h_chi_guess_.resize(NG);
     for (int64_t r = 0; r < N_; r++)
       for (int g = 0; g < G_; g++)
