@@ -222,10 +222,23 @@ int moc_solver_set_fixed_source_by_fsr(moc_solver_t* solver, int64_t fsr, int32_
                                        double source);
 /* Solver::setFixedSourceByMaterial(material, group, source) (src/Solver.cpp:
  * 538-543), material = index into the materials of moc_solver_set_materials;
- * applied to every FSR of the material at the call (the reference applies it
- * when the next solve starts, src/Solver.cpp:1119-1137). */
+ * applied to every FSR of the material when the next solve starts, after the
+ * per-FSR and cell sources (src/Solver.cpp:1119-1137). */
 int moc_solver_set_fixed_source_by_material(moc_solver_t* solver, int32_t material, int32_t group,
                                             double source);
+/* Geometry::findCellContainingFSR (src/Geometry.cpp) as a table: the id of
+ * the material cell each FSR lies in [num_fsrs], the cell ids that
+ * moc_solver_set_fixed_source_by_cell names (any numbering the caller keeps;
+ * the Python mirror numbers a pin's cells pin * 64 + index, cylinders
+ * innermost first, then the outer cell). */
+int moc_solver_set_fsr_cells(moc_solver_t* solver, const int32_t* cell_of_fsr, int64_t num_fsrs);
+/* Solver::setFixedSourceByCell(cell, group, source) (src/Solver.cpp:510-527):
+ * every FSR of the cell.  Fixed sources are applied when a solve starts, as
+ * Solver::initializeFixedSources does (src/Solver.cpp:1089-1137): cell
+ * sources, then material sources, overwrite the per-FSR ones of their FSRs,
+ * whatever the call order.  MOC_ERR_STATE without the FSR cell table. */
+int moc_solver_set_fixed_source_by_cell(moc_solver_t* solver, int32_t cell, int32_t group,
+                                        double source);
 /* CPUSolver::resetFixedSources (src/CPUSolver.cpp:228-250): all fixed
  * sources zero. */
 int moc_solver_reset_fixed_sources(moc_solver_t* solver);

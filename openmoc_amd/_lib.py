@@ -121,6 +121,8 @@ _SIGS = [
     ("moc_solver_set_restart_status", C.c_int, [_P, C.c_int32]),
     ("moc_solver_set_solver_mode", C.c_int, [_P, C.c_int32]),
     ("moc_solver_set_fixed_source_by_material", C.c_int, [_P, C.c_int32, C.c_int32, C.c_double]),
+    ("moc_solver_set_fsr_cells", C.c_int, [_P, _P, C.c_int64]),
+    ("moc_solver_set_fixed_source_by_cell", C.c_int, [_P, C.c_int32, C.c_int32, C.c_double]),
     ("moc_solver_reset_fixed_sources", C.c_int, [_P]),
     ("moc_solver_set_chi_spectrum_material", C.c_int, [_P, C.c_int32]),
     ("moc_flux_file_write", C.c_int, [C.c_char_p, C.c_double, C.c_int32, C.c_int64, _P, _P]),

@@ -507,6 +507,15 @@ int moc_solver_set_fixed_source_by_material(moc_solver_t* s, int32_t material, i
   MOC_SOLVER_CALL(s, s->impl->setFixedSourceByMaterial(material, group, source));
 }
 
+int moc_solver_set_fsr_cells(moc_solver_t* s, const int32_t* cell_of_fsr, int64_t num_fsrs) {
+  MOC_SOLVER_CALL(s, s->impl->setFSRCells(cell_of_fsr, num_fsrs));
+}
+
+int moc_solver_set_fixed_source_by_cell(moc_solver_t* s, int32_t cell, int32_t group,
+                                        double source) {
+  MOC_SOLVER_CALL(s, s->impl->setFixedSourceByCell(cell, group, source));
+}
+
 int moc_solver_reset_fixed_sources(moc_solver_t* s) {
   MOC_SOLVER_CALL(s, s->impl->resetFixedSources());
 }

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/openmoc_amd.h"
@@ -137,6 +138,8 @@ class HipSolver {
   int device() const { return device_; }
   void setFixedSourceByFSR(int64_t fsr, int group, double source);
   void setFixedSourceByMaterial(int material, int group, double source);
+  void setFSRCells(const int32_t* cell, int64_t n);
+  void setFixedSourceByCell(int cell, int group, double source);
   void resetFixedSources();
   // Arithmetic of the scattering-source contraction (MOC_SCATTER_*);
   // scatterPrecisionInUse resolves MOC_SCATTER_AUTO for the current groups.
@@ -385,6 +388,18 @@ class HipSolver {
   std::vector<double> h_fsr_vol_;         // host copy, per FSR
   std::vector<double> h_fixed_;        // host copy of fixed sources
   bool fixed_on_ = false;
+  // Fixed sources as the reference keeps them (Solver::_fix_src_FSR_map and
+  // _fix_src_material_map, src/Solver.cpp:479-543): folded into the per-FSR
+  // array when a solve starts (Solver::initializeFixedSources,
+  // src/Solver.cpp:1089-1137, CPUSolver::initializeFixedSources,
+  // src/CPUSolver.cpp:425-456), so a material's source wins over an
+  // overlapping per-FSR one whatever the call order.
+  std::map<std::pair<int64_t, int>, double> fix_fsr_map_;
+  std::map<std::pair<int, int>, double> fix_mat_map_;
+  std::map<std::pair<int, int>, double> fix_cell_map_;  // Solver::_fix_src_cell_map
+  std::vector<int32_t> h_fsr_cell_;                      // cell of each FSR (findCellContainingFSR)
+  bool fixed_dirty_ = false;
+  void applyFixedSources();
 
   // results
   double keff_ = 1.0;

@@ -2157,6 +2157,10 @@ void HipSolver::setMaterials(int num_materials, int num_groups, const double* si
     shape_forced_ = false;
     fixed_on_ = false;
     h_fixed_.clear();
+    fix_fsr_map_.clear();
+    fix_mat_map_.clear();
+    fix_cell_map_.clear();
+    fixed_dirty_ = false;
   }
   M_ = M;
   materials_set_ = true;
@@ -2281,6 +2285,10 @@ void HipSolver::setFSRs(int64_t num_fsrs, const int32_t* material, const double*
   if (num_fsrs != N_) {
     fixed_on_ = false;
     h_fixed_.clear();
+    fix_fsr_map_.clear();
+    fix_mat_map_.clear();
+    fix_cell_map_.clear();
+    fixed_dirty_ = false;
   }
   N_ = num_fsrs;
   fsrs_set_ = true;
@@ -2358,6 +2366,7 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_fixed_, NG);
     HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
   }
+  fixed_dirty_ = fixed_on_;
   psi_cur_ = 0;
   prepareScatter();
 }
@@ -2477,43 +2486,86 @@ void HipSolver::setExponential(int mode) {
   exp_mode_ = mode;
 }
 
+// Solver::setFixedSourceByFSR (src/Solver.cpp:479-499): recorded in the
+// per-FSR map, applied when the next solve starts.
 void HipSolver::setFixedSourceByFSR(int64_t fsr, int group, double source) {
   graph_dirty_ = true;
   requireReady();
   if (fsr < 0 || fsr >= N_) throw DeviceError(MOC_ERR_INVALID, "FSR id out of range");
   if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
-  const size_t NG = (size_t)N_ * G_;
-  if (h_fixed_.size() != NG) h_fixed_.assign(NG, 0.0);
-  h_fixed_[(size_t)fsr * G_ + group - 1] = source;
+  fix_fsr_map_[{fsr, group}] = source;
   fixed_on_ = true;
-  HIP_CHECK(hipSetDevice(device_));
-  if (!d_fixed_) dev_alloc(&d_fixed_, NG);
-  HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
+  fixed_dirty_ = true;
 }
 
 // Solver::setFixedSourceByMaterial (src/Solver.cpp:538-543): every FSR of the
-// material, applied now (the reference applies it when a solve starts,
-// src/Solver.cpp:1119-1137).
+// material, applied when the next solve starts after the per-FSR sources.
 void HipSolver::setFixedSourceByMaterial(int material, int group, double source) {
   graph_dirty_ = true;
   requireReady();
   if (material < 0 || material >= M_) throw DeviceError(MOC_ERR_INVALID, "material id out of range");
   if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
-  const size_t NG = (size_t)N_ * G_;
-  if (h_fixed_.size() != NG) h_fixed_.assign(NG, 0.0);
-  for (int64_t r = 0; r < N_; r++)
-    if (h_fsr_mat_[r] == material) h_fixed_[(size_t)r * G_ + group - 1] = source;
+  fix_mat_map_[{material, group}] = source;
   fixed_on_ = true;
-  HIP_CHECK(hipSetDevice(device_));
-  if (!d_fixed_) dev_alloc(&d_fixed_, NG);
-  HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
+  fixed_dirty_ = true;
 }
 
-// CPUSolver::resetFixedSources (src/CPUSolver.cpp:228-250): every fixed source
-// becomes zero (the solver still counts as having fixed sources).
+// Geometry::findCellContainingFSR (src/Geometry.cpp) as a table: the id of
+// the material cell holding each FSR, for setFixedSourceByCell.
+void HipSolver::setFSRCells(const int32_t* cell, int64_t n) {
+  requireReady();
+  if (!cell || n != N_) throw DeviceError(MOC_ERR_INVALID, "FSR cell table must hold N values");
+  h_fsr_cell_.assign(cell, cell + n);
+  fixed_dirty_ = fixed_on_;
+}
+
+// Solver::setFixedSourceByCell (src/Solver.cpp:510-527): every FSR of the
+// cell, applied when the next solve starts, before the material sources.
+void HipSolver::setFixedSourceByCell(int cell, int group, double source) {
+  graph_dirty_ = true;
+  requireReady();
+  if ((int64_t)h_fsr_cell_.size() != N_)
+    throw DeviceError(MOC_ERR_STATE, "the FSR cell table has not been set");
+  if (group < 1 || group > G_) throw DeviceError(MOC_ERR_INVALID, "group out of range (1-based)");
+  if (std::find(h_fsr_cell_.begin(), h_fsr_cell_.end(), cell) == h_fsr_cell_.end())
+    throw DeviceError(MOC_ERR_INVALID, "no FSR lies in this cell");
+  fix_cell_map_[{cell, group}] = source;
+  fixed_on_ = true;
+  fixed_dirty_ = true;
+}
+
+// Solver::initializeFixedSources + CPUSolver::initializeFixedSources
+// (src/Solver.cpp:1089-1137, src/CPUSolver.cpp:425-456): each cell entry, then
+// each material entry, is written into the per-FSR map for every FSR of the
+// cell or material (where it stays, as in the reference), then the per-FSR
+// map fills the array.
+void HipSolver::applyFixedSources() {
+  if (!fixed_on_ || !fixed_dirty_) return;
+  if ((int64_t)h_fsr_cell_.size() == N_)
+    for (const auto& e : fix_cell_map_)
+      for (int64_t r = 0; r < N_; r++)
+        if (h_fsr_cell_[r] == e.first.first) fix_fsr_map_[{r, e.first.second}] = e.second;
+  for (const auto& e : fix_mat_map_)
+    for (int64_t r = 0; r < N_; r++)
+      if (h_fsr_mat_[r] == e.first.first) fix_fsr_map_[{r, e.first.second}] = e.second;
+  const size_t NG = (size_t)N_ * G_;
+  h_fixed_.assign(NG, 0.0);
+  for (const auto& e : fix_fsr_map_) h_fixed_[(size_t)e.first.first * G_ + e.first.second - 1] = e.second;
+  HIP_CHECK(hipSetDevice(device_));
+  dev_alloc(&d_fixed_, NG);
+  HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
+  fixed_dirty_ = false;
+}
+
+// CPUSolver::resetFixedSources (src/CPUSolver.cpp:228-250): every recorded
+// fixed source and the array become zero (the solver still counts as having
+// fixed sources).
 void HipSolver::resetFixedSources() {
   graph_dirty_ = true;
   requireReady();
+  for (auto& e : fix_fsr_map_) e.second = 0.0;
+  for (auto& e : fix_mat_map_) e.second = 0.0;
+  for (auto& e : fix_cell_map_) e.second = 0.0;
   if (h_fixed_.empty()) return;
   std::fill(h_fixed_.begin(), h_fixed_.end(), 0.0);
   HIP_CHECK(hipSetDevice(device_));
@@ -3071,6 +3123,7 @@ void HipSolver::computeFSRSources(int iteration, double* out, int64_t n) {
   if (!out || n != (int64_t)NG) throw DeviceError(MOC_ERR_INVALID, "source buffer must hold N*G values");
   if (iteration < 0) throw DeviceError(MOC_ERR_INVALID, "iteration must be >= 0");
   HIP_CHECK(hipSetDevice(device_));
+  applyFixedSources();
   if (linear_ && !ls_ready_) prepareLinearSource();
   if (phi_is_old_) {
     HIP_CHECK(hipMemcpyAsync(d_phi_, d_phi_old_[old_sel_], NG * sizeof(double),
@@ -3114,10 +3167,11 @@ void HipSolver::computeEigenvalue(int max_iters, int res_type) {
 }
 
 void HipSolver::computeFlux(int max_iters, bool only_fixed_source) {
-  if (!fixed_on_) throw DeviceError(MOC_ERR_STATE, "computeFlux requires fixed sources");
   // only_fixed_source = false after an earlier solve: its scalar and angular
   // fluxes are the starting point instead of zero (src/Solver.cpp:1379-1386)
   continue_flux_ = !only_fixed_source && solved_ && state_valid_;
+  if (!fixed_on_ && !continue_flux_)
+    throw DeviceError(MOC_ERR_STATE, "computeFlux from zero requires fixed sources");
   runSolve(MODE_FLUX, max_iters, MOC_SCALAR_FLUX, 1.0);
 }
 
@@ -3618,7 +3672,9 @@ void HipSolver::startSolve(int mode, int max_iters, int res_type, double k_fixed
                       "cannot compute a FISSION_SOURCE residual without fissionable FSRs");
   if (mode == MODE_EIGEN && num_fissionable_ == 0)
     throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
-  if (mode == MODE_FLUX && !fixed_on_) throw DeviceError(MOC_ERR_STATE, "computeFlux requires fixed sources");
+  if (mode == MODE_FLUX && !fixed_on_ && !continue_flux_)
+    throw DeviceError(MOC_ERR_STATE, "computeFlux from zero requires fixed sources");
+  applyFixedSources();
   if (linear_) {
     if (mode != MODE_EIGEN) throw DeviceError(MOC_ERR_INVALID, "the linear source supports computeEigenvalue");
     if (stab_type_ >= 0 || balance_)

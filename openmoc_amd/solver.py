@@ -31,6 +31,10 @@ from ._lib import (ADJOINT, DIAGONAL, EXP_HW, EXP_RATIONAL, FISSION_SOURCE, FORW
                    YAMAMOTO, MOCError, check, lib)
 from .workloads import Problem
 
+# FSR keys of the tracer: lattice cell * KEY_STRIDE + zone * 64 + sector
+# (3D: + axial layer * lattice cells * KEY_STRIDE)
+KEY_STRIDE = 4096
+
 __all__ = ["TrackGenerator", "HipSolver", "GPUSolver", "HipLSSolver", "SCALAR_FLUX",
            "FISSION_SOURCE",
            "TOTAL_SOURCE", "EXP_RATIONAL", "EXP_HW", "DIAGONAL", "YAMAMOTO", "GLOBAL",
@@ -271,6 +275,27 @@ class TrackGenerator:
 
     def getFSRVolumes(self) -> np.ndarray:
         return self.fsr_volume
+
+    def getFSRCells(self) -> np.ndarray:
+        """Geometry::findCellContainingFSR as a table [N]: the cell of each
+        FSR as pin * 64 + index, with `pin` the pin universe's index in the
+        problem and `index` its cell (cylinders innermost first, the outer
+        cell last).  A cylinder's rings and sectors are all of its cell."""
+        p = self.problem
+        if p is None:
+            raise MOCError(-2, "the problem text could not be parsed for the cells")
+        key = np.asarray(self.fsr_key, np.int64)
+        lat = (key // KEY_STRIDE) % (p.nx * p.ny)          # iy * nx + ix, iy from ymin
+        zone = (key % KEY_STRIDE) // 64
+        iy, ix = lat // p.nx, lat % p.nx
+        pin = np.asarray(p.cells).reshape(p.ny, p.nx)[p.ny - 1 - iy, ix]
+        # zone -> cell index: each cylinder covers its rings' zones
+        out = np.empty(len(key), np.int32)
+        for q, t in enumerate(p.pins):
+            first = np.cumsum([0] + [max(1, c.rings) for c in t.cylinders])
+            sel = pin == q
+            out[sel] = q * 64 + np.searchsorted(first, zone[sel], side="right") - 1
+        return out
 
     def getMaxOpticalLength(self) -> float:
         return float(self.info.max_optical_length)
@@ -531,6 +556,22 @@ class HipSolver:
             material = names.index(material)
         check(lib().moc_solver_set_fixed_source_by_material(self._h, int(material), int(group),
                                                             float(source)))
+
+    def setFixedSourceByCell(self, cell, group: int, source: float) -> None:
+        """Solver::setFixedSourceByCell: `cell` is (pin, index), the pin
+        universe's index in the problem and the cell's index in it
+        (cylinders innermost first, then the outer cell; a cylinder's rings
+        and sectors are all of its cell).  Applied when a solve starts, before
+        the material sources, as Solver::initializeFixedSources does."""
+        if self._tg is None:
+            raise MOCError(-2, "setFixedSourceByCell needs the solver's TrackGenerator")
+        if not getattr(self, "_cells_set", False):
+            cells = np.ascontiguousarray(self._tg.getFSRCells(), np.int32)
+            check(lib().moc_solver_set_fsr_cells(self._h, _ptr(cells), len(cells)))
+            self._cells_set = True
+        pin, index = cell
+        check(lib().moc_solver_set_fixed_source_by_cell(self._h, int(pin) * 64 + int(index),
+                                                        int(group), float(source)))
 
     def setChiSpectrumMaterial(self, material) -> None:
         """Solver::setChiSpectrumMaterial: solves start from that material's

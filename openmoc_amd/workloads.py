@@ -331,6 +331,21 @@ def c5g7_pin_cell() -> Problem:
                    tolerance=1e-5, max_iters=500)
 
 
+def water_box_source() -> Problem:
+    """Fixed-source box of the reference's tests/test_compute_flux and
+    tests/test_compute_source (test_compute_flux.py:13-91): the 2.5 cm
+    HomInfMedInput grid of 10x10 cells (tests/input_set.py:54-92) filled with
+    C5G7 water, vacuum on all four sides, and the cell at lattice row 3,
+    column 3 (from the top left, the universes[0][3][3] of the test) a
+    separate universe that carries the fixed source.  4 azimuthal angles at
+    0.1 cm (the test harness defaults, tests/testing_harness.py:33-37)."""
+    cells = np.zeros((10, 10), np.int32)
+    cells[3, 3] = 1
+    return Problem("water_box_source", [C5G7_WATER], [PinTemplate([], 0, 0), PinTemplate([], 0, 0)],
+                   10, 10, 0.25, 0.25, cells, bc=("V", "V", "V", "V"), num_azim=4,
+                   azim_spacing=0.1, tolerance=1e-5, max_iters=500, res_type="TOTAL_SOURCE")
+
+
 def pin_cell_7g(seed: int = 1) -> Problem:
     """Single 1.26 cm pin cell, r=0.54 (3 rings x 8 sectors), moderator 8
     sectors, reflective; 7 synthetic groups; 32 azimuthal, 0.05 cm, TY 6."""
@@ -472,6 +487,7 @@ WORKLOADS = {
     "pin_cell_7g_tc": pin_cell_7g_tc,
     "hom_inf_medium": hom_inf_medium,
     "c5g7_pin_cell": c5g7_pin_cell,
+    "water_box_source": water_box_source,
     "pin_cell_7g": pin_cell_7g,
     "lattice_17x17_7g": lattice_17x17_7g,
     "minicore_70g": minicore_70g,

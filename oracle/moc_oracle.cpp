@@ -28,8 +28,16 @@
  *       tests/test_forward_hom_inf_medium/results_true.dat) and the compiled
  *       reference on the synthetic workloads.
  *   oracle_compute_flux           src/Solver.cpp:1352-1420
+ *   oracle_compute_source         src/Solver.cpp:1459-1516
+ *   oracle_solve_state            the same loops from a previous solve's
+ *                                 scalar and start fluxes (computeFlux with
+ *                                 only_fixed_source = false, :1379-1386)
  *       pinned: infinite-medium fixed-source closed form phi = 4pi q / sigma_a
- *       (q isotropic source per steradian, Q = fixed/4pi).
+ *       (q isotropic source per steradian, Q = fixed/4pi); the compiled
+ *       reference's computeFlux / computeSource with fixed sources by cell,
+ *       FSR and material, including its own tests/test_compute_flux and
+ *       tests/test_compute_source regression files, and the continuation
+ *       of an eigenvalue and a source solve.
  *   neutron-balance keff          src/CPUSolver.cpp:2258-2328 (rates of
  *                                 fission, absorption and the vacuum leakage
  *                                 tallied by transferBoundaryFlux :2593-2600)
@@ -363,9 +371,15 @@ enum Mode { MODE_EIGEN = 0, MODE_FLUX = 1, MODE_SOURCE = 2 };
 
 // Solver::computeEigenvalue (:1542-1689), computeFlux (:1352-1420) and
 // computeSource (:1459-1516) share this loop.
+// phi0 / psi0 (nullable, MODE_FLUX only): the scalar flux [N*G] and start
+// fluxes [2*T*fpt] a previous solve left; computeFlux(only_fixed_source =
+// false) after a solve keeps them instead of the zero start (:1379-1386), with
+// k = 1 for the sources of computeFSRSources(0) (:1370, :1389).  psi_out
+// (nullable) receives the start fluxes at the end.
 template <typename PsiT>
 int solve(const OracleProblem* p, const OracleOptions* o, OracleResult* out, double* flux_out,
-          Mode mode, double k_fixed) {
+          Mode mode, double k_fixed, const double* phi0 = nullptr, const PsiT* psi0 = nullptr,
+          PsiT* psi_out = nullptr) {
   View v = make_view(p);
   const int G = v.G;
   const int64_t N = v.N, T = v.T;
@@ -373,6 +387,8 @@ int solve(const OracleProblem* p, const OracleOptions* o, OracleResult* out, dou
   std::vector<double> phi(N * G, mode == MODE_FLUX ? 0.0 : 1.0), old(N * G), Q(N * G),
       scratch(N), tally(N * G);
   std::vector<PsiT> start(2 * T * v.fpt, (PsiT)0);
+  if (mode == MODE_FLUX && phi0) std::memcpy(phi.data(), phi0, sizeof(double) * N * G);
+  if (mode == MODE_FLUX && psi0) std::memcpy(start.data(), psi0, sizeof(PsiT) * start.size());
   bool allow_neg = o->allow_negative_fluxes != 0;
   double k = mode == MODE_SOURCE ? k_fixed : 1.0;
   double sweep_s = 0;
@@ -490,6 +506,7 @@ int solve(const OracleProblem* p, const OracleOptions* o, OracleResult* out, dou
   }
   if (!converged && mode != MODE_EIGEN) iters = o->max_iters;
   std::memcpy(flux_out, phi.data(), sizeof(double) * N * G);
+  if (psi_out) std::memcpy(psi_out, start.data(), sizeof(PsiT) * start.size());
   out->keff = k;
   out->num_iterations = iters;
   out->residual = res;
@@ -921,6 +938,17 @@ int oracle_compute_source(const OracleProblem* p, const OracleOptions* o, double
   if (!p || !o || !r || !flux_out || !(k_eff > 0)) return -1;
   return o->psi_fp64 ? solve<double>(p, o, r, flux_out, MODE_SOURCE, k_eff)
                      : solve<float>(p, o, r, flux_out, MODE_SOURCE, k_eff);
+}
+
+int oracle_solve_state(const OracleProblem* p, const OracleOptions* o, int mode, double k_eff,
+                       const double* phi0, const float* psi0, OracleResult* r, double* flux_out,
+                       float* psi_out) {
+  if (!p || !o || !r || !flux_out || mode < MODE_EIGEN || mode > MODE_SOURCE || o->psi_fp64)
+    return -1;
+  if (mode == MODE_SOURCE && !(k_eff > 0)) return -1;
+  if ((phi0 || psi0) && mode != MODE_FLUX) return -1;
+  return solve<float>(p, o, r, flux_out, (Mode)mode, mode == MODE_SOURCE ? k_eff : 1.0, phi0,
+                      psi0, psi_out);
 }
 
 }  // extern "C"

@@ -105,6 +105,13 @@ int oracle_compute_flux(const OracleProblem* p, const OracleOptions* o,
 /* Solver::computeSource(max_iters, k_eff, res_type): src/Solver.cpp:1459-1516. */
 int oracle_compute_source(const OracleProblem* p, const OracleOptions* o, double k_eff,
                           OracleResult* r, double* flux_out);
+/* mode 0 eigenvalue, 1 flux, 2 source (k_eff); phi0 [N*G] / psi0 [2*T*fpt]
+ * (nullable, flux mode only): the state a previous solve left, for
+ * computeFlux(max_iters, only_fixed_source = false); psi_out (nullable)
+ * receives the start fluxes at the end.  FP32 angular fluxes only. */
+int oracle_solve_state(const OracleProblem* p, const OracleOptions* o, int mode, double k_eff,
+                       const double* phi0, const float* psi0, OracleResult* r, double* flux_out,
+                       float* psi_out);
 
 /* One CPUSolver::transportSweep (src/CPUSolver.cpp:2338-2389) on given
  * reduced sources Q [N*G] and start fluxes psi_start [T*2*P*G] (float, the

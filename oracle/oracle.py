@@ -63,6 +63,9 @@ def lib():
             getattr(h, name).restype = C.c_int
         h.oracle_compute_source.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
         h.oracle_compute_source.restype = C.c_int
+        h.oracle_solve_state.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        h.oracle_solve_state.restype = C.c_int
         h.oracle_transport_sweep.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         h.oracle_transport_sweep.restype = C.c_int
         h.oracle_compute_fsr_sources.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int,
@@ -134,6 +137,7 @@ class OracleProblem:
         self.N = len(self.a["fsr_material"])
         self.T = len(self.a["trk_azim"])
         self.P = P
+        self.psi_count = 2 * self.T * self.G * P   # start fluxes [T][2][P*G] (fpt = G*P)
         s = OracleProblemC()
         s.num_groups, s.num_polar_half, s.num_azim_half = self.G, P, num_azim // 2
         s.num_tracks, s.num_fsrs = self.T, self.N
@@ -208,6 +212,29 @@ class OracleProblem:
     def source(self, k_eff: float, **kw) -> OracleSolution:
         kw.setdefault("res_type", TOTAL_SOURCE)
         return self._run(lib().oracle_compute_source, C.c_double(k_eff), **kw)
+
+    def solve_state(self, mode: int, k_eff: float = 1.0, phi0=None, psi0=None, max_iters=1000,
+                    tol=1e-5, res_type=None, threads=0):
+        """One solve (mode 0 eigenvalue, 1 flux, 2 source) that also returns
+        the start fluxes it ends with; in flux mode (phi0, psi0) continue a
+        previous solve, as computeFlux(only_fixed_source=False).  Returns
+        (OracleSolution, start fluxes)."""
+        if res_type is None:
+            res_type = (FISSION_SOURCE, SCALAR_FLUX, TOTAL_SOURCE)[mode]
+        o = self._opts(max_iters, tol, res_type, False, threads)
+        r = OracleResultC()
+        flux = np.zeros(self.N * self.G)
+        psi = np.zeros(self.psi_count, np.float32)
+        f0 = None if phi0 is None else np.ascontiguousarray(phi0, np.float64).ravel()
+        p0 = None if psi0 is None else np.ascontiguousarray(psi0, np.float32).ravel()
+        rc = lib().oracle_solve_state(C.byref(self.c), C.byref(o), int(mode), float(k_eff),
+                                      None if f0 is None else f0.ctypes.data,
+                                      None if p0 is None else p0.ctypes.data, C.byref(r),
+                                      flux.ctypes.data, psi.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oracle failed with code {rc}")
+        return (OracleSolution(r.keff, r.num_iterations, r.residual, r.sweep_seconds,
+                               r.num_integrations, flux.reshape(self.N, self.G)), psi)
 
     def transport_sweep(self, Q: np.ndarray, psi: np.ndarray, threads: int = 0):
         q = np.ascontiguousarray(Q, np.float64).ravel()

@@ -181,7 +181,9 @@ int main(int argc, char** argv) {
                     "[--max-iters N] [--tol X] [--sweeps N] [--balance] "
                     "[--stabilize FACTOR TYPE] [--krylov] [--linear-source] "
                     "[--dump-fluxes F] [--restart-from F] [--residual-reference F] "
-                    "[--load-fluxes F]\n", argv[0]);
+                    "[--load-fluxes F] [--fixed-source-cell PIN CELL G V] "
+                    "[--fixed-source-fsr R G V] [--fixed-source-material M G V] "
+                    "[--sequence STEPS]\n", argv[0]);
     return 2;
   }
   int threads = 1;
@@ -201,6 +203,19 @@ int main(int argc, char** argv) {
   int restart_after = 0;  /* --restart-after N: N iterations, then setRestartStatus(true) */
   bool adjoint = false;   /* --adjoint: setSolverMode(ADJOINT) */
   int chi_material = -1;  /* --chi-spectrum M: setChiSpectrumMaterial(material M) */
+  /* Fixed sources (Solver::setFixedSourceByCell / ByFSR / ByMaterial,
+   * src/Solver.cpp:479-543), applied in the order given.  A cell is named by
+   * its pin universe and its index in that pin (cylinders innermost first,
+   * then the outer cell). */
+  struct FixedSrc { char kind; int a, b, group; double value; };
+  std::vector<FixedSrc> fixed_sources;
+  /* --sequence STEPS: comma-separated solves run in order instead of the one
+   * computeEigenvalue: "eig", "flux" (computeFlux(max_iters, true)),
+   * "flux0" (computeFlux(max_iters, false)), "source:K" or "source:K:RT"
+   * (computeSource(max_iters, K, res_type or RT)).  Step i writes step<i>_keff,
+   * step<i>_num_iterations and step<i>_flux; the last step also writes the
+   * usual keff / num_iterations / flux records. */
+  const char* sequence = nullptr;
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -221,6 +236,22 @@ int main(int argc, char** argv) {
     else if (!strcmp(argv[a], "--restart-after")) restart_after = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--adjoint")) adjoint = true;
     else if (!strcmp(argv[a], "--chi-spectrum")) chi_material = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--fixed-source-cell")) {
+      FixedSrc f{'c', atoi(argv[a + 1]), atoi(argv[a + 2]), atoi(argv[a + 3]), atof(argv[a + 4])};
+      fixed_sources.push_back(f);
+      a += 4;
+    }
+    else if (!strcmp(argv[a], "--fixed-source-fsr")) {
+      FixedSrc f{'r', atoi(argv[a + 1]), 0, atoi(argv[a + 2]), atof(argv[a + 3])};
+      fixed_sources.push_back(f);
+      a += 3;
+    }
+    else if (!strcmp(argv[a], "--fixed-source-material")) {
+      FixedSrc f{'m', atoi(argv[a + 1]), 0, atoi(argv[a + 2]), atof(argv[a + 3])};
+      fixed_sources.push_back(f);
+      a += 3;
+    }
+    else if (!strcmp(argv[a], "--sequence")) sequence = argv[++a];
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -240,8 +271,10 @@ int main(int argc, char** argv) {
   }
 
   std::vector<Universe*> pin_univ;
+  std::vector<std::vector<Cell*>> pin_cells;  /* cylinders innermost first, then the outer cell */
   for (const Pin& p : s.pins) {
     Universe* u = new Universe();
+    pin_cells.emplace_back();
     std::vector<ZCylinder*> zc;
     for (const Cyl& c : p.cyl) zc.push_back(new ZCylinder(0.0, 0.0, c.r));
     for (size_t i = 0; i < p.cyl.size(); i++) {
@@ -252,12 +285,14 @@ int main(int argc, char** argv) {
       if (p.cyl[i].rings > 1) c->setNumRings(p.cyl[i].rings);
       if (p.cyl[i].sectors > 1) c->setNumSectors(p.cyl[i].sectors);
       u->addCell(c);
+      pin_cells.back().push_back(c);
     }
     Cell* outer = new Cell();
     outer->setFill(mats[p.outer_mat]);
     if (!zc.empty()) outer->addSurface(+1, zc.back());
     if (p.outer_sectors > 1) outer->setNumSectors(p.outer_sectors);
     u->addCell(outer);
+    pin_cells.back().push_back(outer);
     pin_univ.push_back(u);
   }
 
@@ -566,6 +601,12 @@ int main(int argc, char** argv) {
   long N = geometry->getNumFSRs();
   w.l1("num_fsrs", N);
 
+  for (const FixedSrc& f : fixed_sources) {
+    if (f.kind == 'c') solver.setFixedSourceByCell(pin_cells.at(f.a).at(f.b), f.group, f.value);
+    else if (f.kind == 'r') solver.setFixedSourceByFSR(f.a, f.group, f.value);
+    else solver.setFixedSourceByMaterial(mats.at(f.a), f.group, f.value);
+  }
+
   if (solve) {
     residualType rt = FISSION_SOURCE;
     if (s.res_type == "SCALAR_FLUX") rt = SCALAR_FLUX;
@@ -579,7 +620,41 @@ int main(int argc, char** argv) {
       solver.setRestartStatus(true);
     }
     auto t0 = std::chrono::steady_clock::now();
-    solver.computeEigenvalue(s.max_iters, rt);
+    if (sequence) {
+      std::string steps(sequence);
+      size_t pos = 0;
+      int i = 0;
+      while (pos <= steps.size()) {
+        size_t end = steps.find(',', pos);
+        if (end == std::string::npos) end = steps.size();
+        const std::string st = steps.substr(pos, end - pos);
+        if (st == "eig") solver.computeEigenvalue(s.max_iters, rt);
+        else if (st == "flux") solver.computeFlux(s.max_iters, true);
+        else if (st == "flux0") solver.computeFlux(s.max_iters, false);
+        else if (st.rfind("source:", 0) == 0) {
+          /* source:K or source:K:RESIDUAL_TYPE */
+          residualType srt = rt;
+          const size_t c2 = st.find(':', 7);
+          if (c2 != std::string::npos) {
+            const std::string r = st.substr(c2 + 1);
+            srt = r == "SCALAR_FLUX" ? SCALAR_FLUX : r == "TOTAL_SOURCE" ? TOTAL_SOURCE : FISSION_SOURCE;
+          }
+          solver.computeSource(s.max_iters, atof(st.substr(7, c2 == std::string::npos ? std::string::npos : c2 - 7).c_str()), srt);
+        }
+        else { fprintf(stderr, "unknown sequence step %s\n", st.c_str()); exit(2); }
+        std::vector<double> fl(N * G);
+        for (long r = 0; r < N; r++)
+          for (int g = 0; g < G; g++) fl[r * G + g] = solver.getFlux(r, g + 1);
+        const std::string pre = "step" + std::to_string(i);
+        w.d1((pre + "_keff").c_str(), solver.getKeff());
+        w.l1((pre + "_num_iterations").c_str(), solver.getNumIterations());
+        w.d((pre + "_flux").c_str(), fl);
+        pos = end + 1;
+        i++;
+      }
+    } else {
+      solver.computeEigenvalue(s.max_iters, rt);
+    }
     auto t1 = std::chrono::steady_clock::now();
     if (dump_fluxes) solver.dumpFSRFluxes(dump_fluxes);
     std::vector<double> flux(N * G);

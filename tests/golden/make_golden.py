@@ -49,10 +49,17 @@ Outputs (data only: inputs and the reference's outputs):
                             material (setChiSpectrumMaterial)
   *_adjoint_ref.npz         adjoint solves (setSolverMode(ADJOINT)) of
                             pin_cell_7g, lattice_3x3_vac_7g and pin_cell_70g
+  water_box_source_ref.npz, c5g7_pin_cell_fixed_source_ref.npz,
+  lattice_3x3_vac_7g_fixed_source_ref.npz, pin_cell_7g_eig_flux0_ref.npz
+                            computeFlux (from zero and continued) and
+                            computeSource with fixed sources by cell, by FSR
+                            and by material (the water box is the reference's
+                            tests/test_compute_flux and test_compute_source
+                            problem)
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 
 `python tests/golden/make_golden.py flux_files` regenerates only the flux-file
-fixtures.
+fixtures, `... fixed_source` only the fixed-source ones.
 """
 import sys
 import tempfile
@@ -180,7 +187,90 @@ def chi_spectrum():
                         material=np.int64(m))
 
 
+WATER_BOX_FLUX_SOURCES = ((1, 1.0), (2, 0.5), (3, 0.25))   # test_compute_flux.py:86-88
+WATER_BOX_SOURCE_SOURCES = ((1, 1.0),)                      # test_compute_source.py:84
+
+
+def fsr_source_args(fixed):
+    """--fixed-source-fsr arguments for a [N, G] array of per-FSR sources."""
+    out = []
+    for r, g in zip(*np.nonzero(fixed)):
+        out += ["--fixed-source-fsr", str(r), str(g + 1), repr(float(fixed[r, g]))]
+    return out
+
+
+def pin_cell_fixed_sources(N, G):
+    """Per-FSR sources used on the C5G7 pin cell: group 1 everywhere and
+    group 3 in the first half of the FSRs."""
+    fixed = np.zeros((N, G))
+    fixed[:, 0] = 1.0
+    fixed[: N // 2, 2] = 0.5
+    return fixed
+
+
+def fixed_source():
+    """computeFlux / computeSource (src/Solver.cpp:1352-1516) of the compiled
+    reference: the water box of the reference's own tests/test_compute_flux
+    and tests/test_compute_source (a fixed source by cell), per-FSR sources on
+    the C5G7 pin cell, a material source overlapping per-FSR sources on the
+    leaking 3x3 lattice, and computeFlux(only_fixed_source=False) continuing
+    an eigenvalue solve without fixed sources."""
+    out = {}
+    cell = lambda srcs: sum((["--fixed-source-cell", "1", "0", str(g), repr(v)] for g, v in srcs), [])
+    _, d = run("water_box_source", threads=1,
+               extra=(*cell(WATER_BOX_FLUX_SOURCES), "--sequence", "flux,flux0"))
+    out.update(flux_flux=d["step0_flux"], flux_num_iterations=np.int64(d["step0_num_iterations"]),
+               flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]),
+               fsr_point=d["fsr_point"], fsr_volume=d["fsr_volume"])
+    _, d = run("water_box_source", threads=1,
+               extra=(*cell(WATER_BOX_SOURCE_SOURCES), "--sequence", "source:1.0"))
+    out.update(source_flux=d["step0_flux"], source_num_iterations=np.int64(d["step0_num_iterations"]))
+    np.savez_compressed(OUT / "water_box_source_ref.npz", **out)
+
+    _, d0 = run("c5g7_pin_cell", threads=1, extra=("--no-solve",))
+    N, G = int(d0["num_fsrs"]), int(d0["num_groups"])
+    fixed = pin_cell_fixed_sources(N, G)
+    _, d = run("c5g7_pin_cell", threads=1,
+               extra=(*fsr_source_args(fixed), "--tol", "1e-7", "--max-iters", "1000",
+                      "--sequence", "flux,flux0,source:3.0:TOTAL_SOURCE"))
+    np.savez_compressed(
+        OUT / "c5g7_pin_cell_fixed_source_ref.npz", fixed=fixed,
+        flux_flux=d["step0_flux"], flux_num_iterations=np.int64(d["step0_num_iterations"]),
+        flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]),
+        source_flux=d["step2_flux"], source_num_iterations=np.int64(d["step2_num_iterations"]))
+
+    # material 1 (moderator) group 1 overlaps a per-FSR source set before
+    # and one set after it: the material's value wins in both FSRs
+    name = "lattice_3x3_vac_7g"
+    _, d0 = run(name, threads=1)
+    mat = np.asarray(d0["fsr_material"])
+    mod = np.nonzero(mat == 1)[0]
+    fuel = np.nonzero(mat == 0)[0]
+    srcs = [("fsr", int(mod[0]), 1, 3.0), ("material", 1, 1, 0.5), ("fsr", int(mod[5]), 1, 4.0),
+            ("fsr", int(fuel[0]), 2, 1.0)]
+    args = []
+    for kind, a, g, v in srcs:
+        args += [f"--fixed-source-{kind}", str(a), str(g), repr(v)]
+    _, d = run(name, threads=1, extra=(*args, "--tol", "1e-7", "--sequence",
+                                       "source:0.8:TOTAL_SOURCE,flux0"))
+    np.savez_compressed(
+        OUT / f"{name}_fixed_source_ref.npz",
+        sources=np.array([(0 if k == "fsr" else 1, a, g, v) for k, a, g, v in srcs]),
+        source_flux=d["step0_flux"], source_num_iterations=np.int64(d["step0_num_iterations"]),
+        flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]))
+
+    _, d = run("pin_cell_7g", threads=1, extra=("--sequence", "eig,flux0"))
+    np.savez_compressed(
+        OUT / "pin_cell_7g_eig_flux0_ref.npz",
+        eig_keff=np.float64(d["step0_keff"]), eig_flux=d["step0_flux"],
+        eig_num_iterations=np.int64(d["step0_num_iterations"]),
+        flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]))
+
+
 def main():
+    if sys.argv[1:] == ["fixed_source"]:
+        fixed_source()
+        return
     if sys.argv[1:] == ["flux_files"]:
         flux_files()
         flux_files_other_paths()
@@ -269,6 +359,7 @@ def main():
     ls_restart()
     adjoint()
     chi_spectrum()
+    fixed_source()
     print("golden fixtures written to", OUT)
 
 

@@ -1,0 +1,150 @@
+"""computeFlux / computeSource (src/Solver.cpp:1352-1516) on the GPU against
+the compiled reference's fixtures (tests/golden/make_golden.py fixed_source)
+and the reference's own regression files, through the C ABI.
+
+Bars (FP32 sweep against the FP64 reference): FSR scalar flux relative L2
+<= 1e-4 and iteration counts within one of the reference's; groups with no
+source anywhere stay exactly zero."""
+import numpy as np
+import pytest
+
+import openmoc_amd as om
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FLUX_RL2 = 1e-4
+WATER_BOX_FLUX_SOURCES = ((1, 1.0), (2, 0.5), (3, 0.25))   # test_compute_flux.py:86-88
+SOURCE_CELL = (1, 0)
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def regression_fluxes(name):
+    """Iterations and fluxes of a reference results_true.dat."""
+    lines = (GOLDEN / name).read_text().splitlines()
+    return int(lines[0].split(":")[1]), np.array([float(v) for v in lines[2:]])
+
+
+def test_water_box_compute_flux_matches_reference(tracks):
+    """tests/test_compute_flux: fixed source by cell in groups 1-3, vacuum
+    box of C5G7 water, computeFlux(500)."""
+    tg = tracks("water_box_source")
+    s = om.HipSolver(tg)
+    for g, v in WATER_BOX_FLUX_SOURCES:
+        s.setFixedSourceByCell(SOURCE_CELL, g, v)
+    s.computeFlux(500)
+    iters, ref = regression_fluxes("test_compute_flux_results_true.dat")
+    flux = s.getFluxes()
+    print(f"water box flux: iterations {s.getNumIterations()} (reference {iters}), "
+          f"rel L2 {rel_l2(flux, ref):.2e}")
+    assert s.getNumIterations() == iters
+    assert rel_l2(flux, ref) < FLUX_RL2
+    # groups 4-7 have no source and no scattering source in this mode
+    assert np.all(flux.reshape(-1, 7)[:, 3:] == 0.0)
+
+
+def test_water_box_compute_source_matches_reference(tracks):
+    """tests/test_compute_source: group-1 source by cell,
+    computeSource(500, k=1, TOTAL_SOURCE)."""
+    tg = tracks("water_box_source")
+    s = om.HipSolver(tg)
+    s.setFixedSourceByCell(SOURCE_CELL, 1, 1.0)
+    s.computeSource(500, 1.0, om.TOTAL_SOURCE)
+    iters, ref = regression_fluxes("test_compute_source_results_true.dat")
+    flux = s.getFluxes()
+    print(f"water box source: iterations {s.getNumIterations()} (reference {iters}), "
+          f"rel L2 {rel_l2(flux, ref):.2e}")
+    assert abs(s.getNumIterations() - iters) <= 1
+    assert rel_l2(flux, ref) < FLUX_RL2
+
+
+def test_water_box_flux_continued_matches_reference(tracks, golden):
+    """computeFlux(500) then computeFlux(500, only_fixed_source=False): the
+    second solve's sources are the scattering of the first's flux plus the
+    fixed source."""
+    tg = tracks("water_box_source")
+    g = golden("water_box_source")
+    s = om.HipSolver(tg)
+    for gg, v in WATER_BOX_FLUX_SOURCES:
+        s.setFixedSourceByCell(SOURCE_CELL, gg, v)
+    s.computeFlux(500)
+    s.computeFlux(500, only_fixed_source=False)
+    assert abs(s.getNumIterations() - g["flux0_num_iterations"]) <= 1
+    assert rel_l2(s.getFluxes(), g["flux0_flux"]) < FLUX_RL2
+
+
+def test_c5g7_fixed_source_solves_match_reference(tracks, golden):
+    """Per-FSR sources on the C5G7 pin cell: computeFlux from zero, continued,
+    and computeSource(k=3, TOTAL_SOURCE), threshold 1e-7."""
+    tg = tracks("c5g7_pin_cell")
+    g = golden("c5g7_pin_cell_fixed_source")
+    s = om.HipSolver(tg)
+    for r, gg in zip(*np.nonzero(g["fixed"])):
+        s.setFixedSourceByFSR(int(r), int(gg) + 1, float(g["fixed"][r, gg]))
+    s.setConvergenceThreshold(1e-7)
+    s.computeFlux(1000)
+    got = [("flux", s.getNumIterations(), s.getFluxes())]
+    s.computeFlux(1000, only_fixed_source=False)
+    got.append(("flux0", s.getNumIterations(), s.getFluxes()))
+    s.computeSource(1000, 3.0, om.TOTAL_SOURCE)
+    got.append(("source", s.getNumIterations(), s.getFluxes()))
+    for tag, it, flux in got:
+        print(f"c5g7 {tag}: iterations {it} (reference {int(g[tag + '_num_iterations'])}), "
+              f"rel L2 {rel_l2(flux, g[tag + '_flux']):.2e}")
+        assert abs(it - g[f"{tag}_num_iterations"]) <= 1, tag
+        assert rel_l2(flux, g[f"{tag}_flux"]) < FLUX_RL2, tag
+
+
+@pytest.mark.parametrize("order", ["reference", "material_last"])
+def test_material_source_wins_over_fsr_sources(tracks, golden, order):
+    """A material source overlapping per-FSR sources on the leaking 3x3
+    lattice: the material's value holds in its FSRs whatever the call order
+    (Solver::initializeFixedSources, src/Solver.cpp:1089-1137);
+    computeSource(k=0.8) then computeFlux(only_fixed_source=False)."""
+    tg = tracks("lattice_3x3_vac_7g")
+    g = golden("lattice_3x3_vac_7g_fixed_source")
+    rows = [tuple(r) for r in g["sources"]]
+    if order == "material_last":
+        rows = [r for r in rows if r[0] == 0] + [r for r in rows if r[0] == 1]
+    s = om.HipSolver(tg)
+    for kind, a, gg, v in rows:
+        if int(kind) == 0:
+            s.setFixedSourceByFSR(int(a), int(gg), float(v))
+        else:
+            s.setFixedSourceByMaterial(int(a), int(gg), float(v))
+    s.setConvergenceThreshold(1e-7)
+    s.computeSource(1000, 0.8, om.TOTAL_SOURCE)
+    assert abs(s.getNumIterations() - g["source_num_iterations"]) <= 1
+    assert rel_l2(s.getFluxes(), g["source_flux"]) < FLUX_RL2
+    s.computeFlux(1000, only_fixed_source=False)
+    assert abs(s.getNumIterations() - g["flux0_num_iterations"]) <= 1
+    assert rel_l2(s.getFluxes(), g["flux0_flux"]) < FLUX_RL2
+
+
+def test_flux_continues_an_eigenvalue_solve(tracks, golden):
+    """computeEigenvalue, then computeFlux(only_fixed_source=False) without
+    any fixed source (the reference allows it: the sources are the k = 1
+    fission and scattering sources of the eigenvalue flux)."""
+    tg = tracks("pin_cell_7g")
+    g = golden("pin_cell_7g_eig_flux0")
+    s = om.HipSolver(tg)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - g["eig_keff"]) < 1e-5
+    s.computeFlux(1000, only_fixed_source=False)
+    print(f"eig -> flux0: iterations {s.getNumIterations()} "
+          f"(reference {int(g['flux0_num_iterations'])}), "
+          f"rel L2 {rel_l2(s.getFluxes(), g['flux0_flux']):.2e}")
+    assert abs(s.getNumIterations() - g["flux0_num_iterations"]) <= 1
+    assert rel_l2(s.getFluxes(), g["flux0_flux"]) < FLUX_RL2
+
+
+def test_compute_flux_from_zero_needs_fixed_sources(tracks):
+    tg = tracks("pin_cell_7g")
+    s = om.HipSolver(tg)
+    with pytest.raises(om.MOCError):
+        s.computeFlux(10)

@@ -119,75 +119,6 @@ def test_hw_exponential_within_tolerance(tracks, golden):
     assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
 
 
-def _fixed_source_problem(tg):
-    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
-    fixed = np.zeros((N, G))
-    fixed[:, 0] = 1.0
-    fixed[: N // 2, 2] = 0.5
-    return fixed
-
-
-def test_compute_source_matches_oracle(tracks):
-    tg = tracks("c5g7_pin_cell")
-    fixed = _fixed_source_problem(tg)
-    s = solver_for(tg)
-    for r in range(fixed.shape[0]):
-        for gg in range(fixed.shape[1]):
-            if fixed[r, gg]:
-                s.setFixedSourceByFSR(r, gg + 1, fixed[r, gg])
-    s.setConvergenceThreshold(1e-7)
-    s.computeSource(1000, 0.9, om.TOTAL_SOURCE)
-    op = orc.OracleProblem.from_track_generator(tg, fixed_source=fixed)
-    sol = op.source(0.9, max_iters=1000, tol=1e-7, res_type=orc.TOTAL_SOURCE)
-    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
-    assert abs(s.getNumIterations() - sol.num_iterations) <= 2
-
-
-def test_compute_flux_matches_oracle(tracks):
-    tg = tracks("c5g7_pin_cell")
-    fixed = _fixed_source_problem(tg)
-    s = solver_for(tg)
-    for r in range(fixed.shape[0]):
-        for gg in range(fixed.shape[1]):
-            if fixed[r, gg]:
-                s.setFixedSourceByFSR(r, gg + 1, fixed[r, gg])
-    s.setConvergenceThreshold(1e-7)
-    s.computeFlux(1000)
-    sol = orc.OracleProblem.from_track_generator(tg, fixed_source=fixed).flux(
-        max_iters=1000, tol=1e-7)
-    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
-
-
-def test_compute_flux_continues_without_only_fixed_source(tracks):
-    """computeFlux(only_fixed_source=False) after a solve (src/Solver.cpp:
-    1379-1396): no zero start; computeFSRSources(0) of the previous flux (k = 1:
-    scattering, fission and the fixed source) is the source of the sweeps.  The
-    result is the fixed-source transport solution of that total source, which
-    the oracle gives when handed it as the fixed source."""
-    tg = tracks("c5g7_pin_cell")
-    fixed = _fixed_source_problem(tg)
-    s = solver_for(tg)
-    for r in range(fixed.shape[0]):
-        for gg in range(fixed.shape[1]):
-            if fixed[r, gg]:
-                s.setFixedSourceByFSR(r, gg + 1, fixed[r, gg])
-    s.setConvergenceThreshold(1e-7)
-    s.computeFlux(1000)
-    N, G = fixed.shape
-    phi1 = s.getFluxes().reshape(N, G)
-    s.computeFlux(1000, only_fixed_source=False)
-    mat = tg.fsr_material
-    S = tg.sigma_s.reshape(-1, G, G)[mat]                  # [N, dest, origin]
-    total = (np.einsum("rdg,rg->rd", S, phi1)
-             + tg.chi[mat] * (tg.nu_sigma_f[mat] * phi1).sum(1)[:, None] + fixed)
-    sol = orc.OracleProblem.from_track_generator(tg, fixed_source=total).flux(
-        max_iters=1000, tol=1e-7)
-    assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
-    # and from zero again with only_fixed_source
-    s.computeFlux(1000)
-    np.testing.assert_allclose(s.getFluxes(), phi1.ravel(), rtol=1e-6, atol=0)
-
-
 def test_repeat_solves_agree(tracks):
     """Same result on repeated solves with one solver (reference
     tests/test_multisim_simple): within tolerance (atomic order may differ)."""

@@ -7,6 +7,8 @@ import pytest
 
 from oracle import oracle as orc
 
+from conftest import GOLDEN
+
 # Reference unit-test golden values of expF1_fractional
 # (tests/unit_tests/test_exponentials.py:11,75-78).
 TAU = [1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1, 10, 100]
@@ -320,3 +322,150 @@ def test_oracle_3d_axially_uniform_problem_has_uniform_layers(tracks):
     for v in by2.values():
         v = np.array(v)
         assert np.abs(v / v.mean(0) - 1).max() < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# computeFlux / computeSource (src/Solver.cpp:1352-1516) against the compiled
+# reference and the reference's own regression files
+# ---------------------------------------------------------------------------
+WATER_BOX_FLUX_SOURCES = ((1, 1.0), (2, 0.5), (3, 0.25))   # test_compute_flux.py:86-88
+WATER_BOX_SOURCE_SOURCE = ((1, 1.0),)                       # test_compute_source.py:84
+SOURCE_CELL = (1, 0)   # the water box's source universe, its only cell
+
+
+def _results_string(num_iterations, flux):
+    """The regression-file text of tests/testing_harness.py:158-207 for a
+    fixed-source solve: iterations, then every flux as {:12.6E}."""
+    return (f"# Iterations: {num_iterations}\nfluxes:\n"
+            + "\n".join("{0:12.6E}".format(f) for f in np.ravel(flux)) + "\n")
+
+
+def _cell_sources(tg, cell, sources):
+    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
+    fixed = np.zeros((N, G))
+    sel = tg.getFSRCells() == cell[0] * 64 + cell[1]
+    for g, v in sources:
+        fixed[sel, g - 1] = v
+    return fixed
+
+
+def _maxrel(a, b):
+    b = np.ravel(b)
+    return float(np.max(np.abs(np.ravel(a) - b)) / np.max(np.abs(b)))
+
+
+def test_water_box_source_cell_is_the_tests_lattice_cell(tracks, golden):
+    """The source universe sits at universes[0][3][3] of the reference test
+    (row 3 from the top, column 3): the one FSR of that cell is where the
+    reference's own FSR point lies."""
+    tg = tracks("water_box_source")
+    cells = tg.getFSRCells()
+    assert (cells == 64).sum() == 1
+    r = int(np.nonzero(cells == 64)[0][0])
+    x, y = golden("water_box_source")["fsr_point"].reshape(tg.getNumFSRs(), -1)[r, :2]
+    assert -0.5 < x < -0.25 and 0.25 < y < 0.5
+
+
+def test_oracle_reproduces_reference_compute_flux_results(tracks):
+    """tests/test_compute_flux/results_true.dat, byte for byte."""
+    tg = tracks("water_box_source")
+    op = orc.OracleProblem.from_track_generator(
+        tg, fixed_source=_cell_sources(tg, SOURCE_CELL, WATER_BOX_FLUX_SOURCES))
+    sol, _ = op.solve_state(1, max_iters=500, tol=1e-5)
+    expect = (GOLDEN / "test_compute_flux_results_true.dat").read_text()
+    assert _results_string(sol.num_iterations, sol.flux) == expect
+
+
+def test_oracle_reproduces_reference_compute_source_results(tracks):
+    """tests/test_compute_source/results_true.dat, byte for byte
+    (computeSource(500, k=1, TOTAL_SOURCE))."""
+    tg = tracks("water_box_source")
+    op = orc.OracleProblem.from_track_generator(
+        tg, fixed_source=_cell_sources(tg, SOURCE_CELL, WATER_BOX_SOURCE_SOURCE))
+    sol, _ = op.solve_state(2, k_eff=1.0, max_iters=500, tol=1e-5, res_type=orc.TOTAL_SOURCE)
+    expect = (GOLDEN / "test_compute_source_results_true.dat").read_text()
+    assert _results_string(sol.num_iterations, sol.flux) == expect
+
+
+def test_oracle_fixed_source_modes_match_reference(tracks, golden):
+    """Flux from zero, flux continued (only_fixed_source=False), and source
+    solves: same iteration counts and fluxes to 1e-12 as the compiled
+    reference, on the water box (cell source), the C5G7 pin cell (per-FSR
+    sources) and the leaking 3x3 lattice (a material source overlapping
+    per-FSR sources)."""
+    tg = tracks("water_box_source")
+    g = golden("water_box_source")
+    op = orc.OracleProblem.from_track_generator(
+        tg, fixed_source=_cell_sources(tg, SOURCE_CELL, WATER_BOX_FLUX_SOURCES))
+    s1, psi = op.solve_state(1, max_iters=500, tol=1e-5)
+    s2, _ = op.solve_state(1, phi0=s1.flux, psi0=psi, max_iters=500, tol=1e-5)
+    assert (s1.num_iterations, s2.num_iterations) == (g["flux_num_iterations"],
+                                                      g["flux0_num_iterations"])
+    assert _maxrel(s1.flux, g["flux_flux"]) < 1e-12
+    assert _maxrel(s2.flux, g["flux0_flux"]) < 1e-12
+
+    tg = tracks("c5g7_pin_cell")
+    g = golden("c5g7_pin_cell_fixed_source")
+    op = orc.OracleProblem.from_track_generator(tg, fixed_source=g["fixed"])
+    s1, psi = op.solve_state(1, max_iters=1000, tol=1e-7)
+    s2, _ = op.solve_state(1, phi0=s1.flux, psi0=psi, max_iters=1000, tol=1e-7)
+    s3, _ = op.solve_state(2, k_eff=3.0, max_iters=1000, tol=1e-7, res_type=orc.TOTAL_SOURCE)
+    for s, tag in ((s1, "flux"), (s2, "flux0"), (s3, "source")):
+        assert s.num_iterations == g[f"{tag}_num_iterations"], tag
+        assert _maxrel(s.flux, g[f"{tag}_flux"]) < 1e-12, tag
+
+    tg = tracks("lattice_3x3_vac_7g")
+    g = golden("lattice_3x3_vac_7g_fixed_source")
+    fixed = fixed_sources_as_the_reference_applies_them(tg, g["sources"])
+    op = orc.OracleProblem.from_track_generator(tg, fixed_source=fixed)
+    s1, psi = op.solve_state(2, k_eff=0.8, max_iters=1000, tol=1e-7, res_type=orc.TOTAL_SOURCE)
+    s2, _ = op.solve_state(1, phi0=s1.flux, psi0=psi, max_iters=1000, tol=1e-7)
+    assert (s1.num_iterations, s2.num_iterations) == (g["source_num_iterations"],
+                                                      g["flux0_num_iterations"])
+    assert _maxrel(s1.flux, g["source_flux"]) < 1e-12
+    assert _maxrel(s2.flux, g["flux0_flux"]) < 1e-12
+
+
+def fixed_sources_as_the_reference_applies_them(tg, sources):
+    """[N, G] fixed sources from (kind, id, group, value) rows, kind 0 per FSR
+    and 1 per material, in the order of Solver::initializeFixedSources
+    (src/Solver.cpp:1089-1137): material entries overwrite the per-FSR ones."""
+    N, G = tg.getNumFSRs(), tg.getNumEnergyGroups()
+    per_fsr = {}
+    for kind, a, g, v in sources:
+        if int(kind) == 0:
+            per_fsr[(int(a), int(g))] = v
+    for kind, a, g, v in sources:
+        if int(kind) == 1:
+            for r in np.nonzero(tg.fsr_material == int(a))[0]:
+                per_fsr[(int(r), int(g))] = v
+    fixed = np.zeros((N, G))
+    for (r, g), v in per_fsr.items():
+        fixed[r, g - 1] = v
+    return fixed
+
+
+def test_material_source_wins_over_overlapping_fsr_sources(golden):
+    """The fixture sets a per-FSR source in a moderator FSR before and one
+    after the moderator's material source: the reference's fluxes are those
+    of the material value in both (checked through the oracle above); here,
+    that the fixture does overlap."""
+    g = golden("lattice_3x3_vac_7g_fixed_source")
+    rows = g["sources"]
+    assert rows[0][0] == 0 and rows[1][0] == 1 and rows[2][0] == 0
+    assert rows[0][2] == rows[1][2] == rows[2][2] == 1
+
+
+def test_oracle_flux_continues_an_eigenvalue_solve(tracks, golden):
+    """computeEigenvalue, then computeFlux(only_fixed_source=False) without
+    fixed sources: k = 1 sources of the eigenvalue flux, started from its
+    scalar and angular fluxes."""
+    tg = tracks("pin_cell_7g")
+    g = golden("pin_cell_7g_eig_flux0")
+    op = orc.OracleProblem.from_track_generator(tg)
+    s1, psi = op.solve_state(0, max_iters=1000, tol=1e-5)
+    s2, _ = op.solve_state(1, phi0=s1.flux, psi0=psi, max_iters=1000, tol=1e-5)
+    assert s1.num_iterations == g["eig_num_iterations"]
+    assert abs(s1.keff - g["eig_keff"]) < 1e-12
+    assert s2.num_iterations == g["flux0_num_iterations"]
+    assert _maxrel(s2.flux, g["flux0_flux"]) < 1e-12
