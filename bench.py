@@ -83,6 +83,31 @@ def cpu_baseline(problem_text: str, G: int, P: int, S: int, sweeps: int, threads
                           f"({threads} OpenMP threads)", "seconds": sol.sweep_seconds}
 
 
+def host_cpu():
+    """(threads to use, description) of the host cores this process may run
+    on: the scheduler affinity, capped by the cgroup CPU quota where one is
+    set (a GPU box grants each GPU's jobs a share of its cores)."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(affinity, quota) if quota else affinity
+    return threads, {"cpu_model": model, "affinity_cores": affinity, "cgroup_quota_cores": quota,
+                     "os_cpu_count": os.cpu_count()}
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
@@ -268,7 +293,9 @@ def main():
             solve["reference_iterations"] = int(g["num_iterations"])
         log(f"solve: {solve}")
 
-    # steps: full source iterations, stopping rule disabled
+    # steps: full source iterations, stopping rule disabled.  The warm-up
+    # call also captures the iteration graphs (both start-flux parities, with
+    # the stopping rule off), which the timed call then replays
     solver.runIterations(max(args.warmup, 1), time_sweep=False)
     # exactly `steps` iterations; runIterations synchronises the device on
     # entry (state read) and before returning
@@ -290,17 +317,29 @@ def main():
     else:
         kernel = "sweep2d_lds_kernel" if shape.get("lds_groups", 0) > 0 else "sweep2d_kernel"
     traffic, traffic_src = measured_traffic(kernel, args.workload)
-    roofline = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                "traffic_source": traffic_src,
-                "kernel": kernel,
-                "algorithmic_bytes_per_launch": bytes_sweep,
-                "launch_seconds": sweep_s}
+    # the binding roof is the lower of the two ceilings on integrations/s:
+    # BW_HBM / B_int (algorithmic bytes) and P_FP32 / F_int
+    b_int = bytes_sweep / w_sweep
+    ceil_hbm = HBM_PEAK_GBS * 1e9 / b_int
+    ceil_valu = FP32_PEAK_TFLOPS * 1e12 / FLOPS_PER_INTEGRATION
+    roofline_hbm = {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                    "traffic_source": traffic_src, "kernel": kernel,
+                    "algorithmic_bytes_per_launch": bytes_sweep,
+                    "bytes_per_integration": b_int, "ceiling_integrations_per_s": ceil_hbm,
+                    "launch_seconds": sweep_s}
     roofline_valu = {"achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                      "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
                      "frac_unpacked": achieved_tflops / FP32_UNPACKED_PEAK_TFLOPS,
-                     "flops_per_integration": FLOPS_PER_INTEGRATION}
+                     "flops_per_integration": FLOPS_PER_INTEGRATION,
+                     "ceiling_integrations_per_s": ceil_valu, "kernel": kernel,
+                     "launch_seconds": sweep_s}
+    if ceil_valu <= ceil_hbm:
+        roofline = dict(roofline_valu, bound="valu", traffic=traffic, traffic_source=traffic_src,
+                        algorithmic_bytes_per_launch=bytes_sweep)
+    else:
+        roofline = dict(roofline_hbm, bound="hbm")
 
     serving = None
     if args.serving > 0:
@@ -310,12 +349,12 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-        threads = max(1, min(threads, os.cpu_count() or 1))
-        log(f"cpu baseline: {args.cpu_sweeps} sweeps on {threads} threads ...")
+        threads, host = host_cpu()
+        log(f"cpu baseline: {args.cpu_sweeps} sweeps on {threads} threads ({host}) ...")
         three_d = bool(tg.info.is_3d)
         sweeps = max(2, args.cpu_sweeps // 4) if three_d else args.cpu_sweeps
         cpu = cpu_baseline(text, G, P, S, sweeps, threads, use_reference=not three_d)
+        cpu.update(host)
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -331,7 +370,7 @@ def main():
                    "launch_shape": shape,
                    "step": "one source iteration (sweep + per-FSR FP64 steps)"},
         "sweep_integrations_per_s": w_sweep / sweep_s,
-        "roofline": roofline, "roofline_valu": roofline_valu,
+        "roofline": roofline, "roofline_hbm": roofline_hbm, "roofline_valu": roofline_valu,
         "cpu_baseline": cpu, "solve": solve, "serving": serving,
     }
     if cpu:

@@ -100,9 +100,14 @@ int moc_tracks_generate(const char* problem_text, int threads, moc_tracks_t** ou
 int moc_tracks_dump_segments(const moc_tracks_t* tracks, const char* path);
 /* TrackGenerator::readSegmentsFromFile (src/TrackGenerator.cpp:1518-1638):
  * tracks and segments from a file written by moc_tracks_dump_segments for the
- * same problem text, without laydown or ray tracing.  *out is NULL (and the
- * call returns MOC_OK) if the file was written for another problem, as the
- * reference returns false; MOC_ERR_IO for a missing or malformed file. */
+ * same problem text, without laydown or ray tracing; or from a 2D file the
+ * reference's own dumpSegmentsToFile wrote: the tracks are then this
+ * problem's laydown (as the reference lays them down before reading), the
+ * segments the file's, and each FSR's material and key come from the point
+ * the file gives for it.  *out is NULL (and the call returns MOC_OK) if the
+ * file was written for another problem (for a reference-written file: its
+ * records do not fit this problem's laydown), as the reference returns false;
+ * MOC_ERR_IO for a missing or malformed file. */
 int moc_tracks_read_segments(const char* problem_text, const char* path, moc_tracks_t** out);
 void moc_tracks_destroy(moc_tracks_t* tracks);
 int moc_tracks_info(const moc_tracks_t* tracks, moc_tracks_info_t* info);

@@ -327,7 +327,7 @@ class HipSolver {
   bool continue_flux_ = false;         // computeFlux(only_fixed_source = false) after a solve
   bool solved_ = false;                // a solve finished since the arrays were laid out
   int psi_cur0_ = 0;                   // start-flux buffer at the running solve's start
-  int graph_psi0_ = -1;                // start-flux buffer the iteration graph was captured at
+  double graph_tol_ = -1.0;            // stopping threshold the iteration graphs were captured with
   std::vector<int32_t> h_work_, h_tile_;  // host copies of the stream layout
   std::vector<int32_t> h_wpos_;        // [2T] work position of each directed track
   int32_t* d_out_wp_ = nullptr;        // [2T] work position of the linked start slot
@@ -356,7 +356,10 @@ class HipSolver {
   struct GeoHold;
   std::shared_ptr<GeoHold> geo_hold_;  // set when the geometry buffers are shared
   // captured iteration batch
-  hipGraphExec_t graph_exec_ = nullptr;
+  // iteration graphs, one per start-flux buffer the batch begins at (the
+  // sweeps alternate the two buffers, so each parity needs its own capture)
+  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+  void dropGraphs();
   int graph_mode_ = -1, graph_res_type_ = -1;
   bool graph_dirty_ = true;
   bool use_graphs_ = true;

@@ -1642,7 +1642,7 @@ HipSolver::HipSolver(int device) {
 HipSolver::~HipSolver() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  dropGraphs();
   if (async_ev_) (void)hipEventDestroy(async_ev_);
   if (async_ev0_) (void)hipEventDestroy(async_ev0_);
   if (geo_hold_) {
@@ -3421,14 +3421,22 @@ double HipSolver::runIterations(int n, double* sweep_seconds) {
   HIP_CHECK(hipMemcpyAsync(d_state_, h_state_, sizeof(IterState), hipMemcpyHostToDevice, stream_));
   const double tol = threshold_;
   threshold_ = 0.0;
-  graph_dirty_ = true;
   hipEvent_t e0, e1;
   HIP_CHECK(hipEventCreate(&e0));
   HIP_CHECK(hipEventCreate(&e1));
   double step_ms = 0.0;
   try {
-    // the iterations as a solve runs them (graph batches)
-    if (use_graphs_ && n >= kGraphIters) ensureGraph(MODE_EIGEN, MOC_FISSION_SOURCE);
+    // the iterations as a solve runs them (graph batches); the graphs of
+    // both start-flux parities are captured here, before the timed launches,
+    // and kept for the next call (the threshold is part of their key)
+    if (use_graphs_) {
+      const int p0 = psi_cur_;
+      for (int q = 0; q < 2; q++) {
+        psi_cur_ = q;
+        ensureGraph(MODE_EIGEN, MOC_FISSION_SOURCE);
+      }
+      psi_cur_ = p0;
+    }
     HIP_CHECK(hipEventRecord(e0, stream_));
     launchIterations(MODE_EIGEN, MOC_FISSION_SOURCE, n);
     HIP_CHECK(hipEventRecord(e1, stream_));
@@ -3461,7 +3469,6 @@ double HipSolver::runIterations(int n, double* sweep_seconds) {
     throw;
   }
   threshold_ = tol;
-  graph_dirty_ = true;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIP_CHECK(hipMemcpyAsync(h_state_, d_state_, sizeof(IterState), hipMemcpyDeviceToHost, stream_));
@@ -3593,17 +3600,28 @@ size_t HipSolver::solveBytes() const {
 }
 
 
+void HipSolver::dropGraphs() {
+  for (auto& g : graph_exec_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// The graph of kGraphIters iterations starting at start-flux buffer psi_cur_
+// (its sweeps alternate the two buffers from there).  Both parities' graphs
+// are kept while mode, residual type and threshold stay the same.
 void HipSolver::ensureGraph(int mode, int res_type) {
   const bool recompute = mode != MODE_FLUX;
-  // the graph's sweeps alternate the two start-flux buffers from the parity
-  // they were captured at
-  if (graph_exec_ && graph_mode_ == mode && graph_res_type_ == res_type && !graph_dirty_ &&
-      graph_psi0_ == psi_cur_)
-    return;
-  if (graph_exec_) {
-    (void)hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
+  if (graph_dirty_ || graph_mode_ != mode || graph_res_type_ != res_type ||
+      graph_tol_ != threshold_) {
+    dropGraphs();
+    graph_dirty_ = false;
+    graph_mode_ = mode;
+    graph_res_type_ = res_type;
+    graph_tol_ = threshold_;
   }
+  if (graph_exec_[psi_cur_]) return;
   if (layout_tpw_ != sweepTracksPerWave())
     throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
   // kernels that need the LDS opt-in get it before the capture
@@ -3631,17 +3649,13 @@ void HipSolver::ensureGraph(int mode, int res_type) {
     throw;
   }
   HIP_CHECK(hipStreamEndCapture(stream_, &g));
-  hipError_t e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&graph_exec_[psi0], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
+  psi_cur_ = psi0;  // an even number of iterations was recorded
   if (e != hipSuccess) {
-    graph_exec_ = nullptr;
+    graph_exec_[psi0] = nullptr;
     throw DeviceError(MOC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
-  psi_cur_ = psi0;  // an even number of iterations was recorded
-  graph_psi0_ = psi0;
-  graph_mode_ = mode;
-  graph_res_type_ = res_type;
-  graph_dirty_ = false;
 }
 
 // Launches `n` iterations: whole graphs while at least kGraphIters remain and
@@ -3651,7 +3665,7 @@ void HipSolver::launchIterations(int mode, int res_type, int n) {
   while (n > 0) {
     if (use_graphs_ && n >= kGraphIters) {
       ensureGraph(mode, res_type);
-      HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+      HIP_CHECK(hipGraphLaunch(graph_exec_[psi_cur_], stream_));
       n -= kGraphIters;
     } else {
       launchIteration((Mode)mode, res_type, recompute, threshold_);

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -95,12 +96,95 @@ class In {
     left_ -= n;
   }
   uint64_t left() const { return left_; }
+  // the next n bytes without consuming them; false if fewer are left
+  bool peek(void* p, size_t n) {
+    if (n > left_) return false;
+    const long at = std::ftell(f_.get());
+    const bool ok = std::fread(p, 1, n, f_.get()) == n;
+    if (at < 0 || std::fseek(f_.get(), at, SEEK_SET) != 0) io_error(path_, "not seekable");
+    return ok;
+  }
 
  private:
   std::string path_;
   FilePtr f_;
   uint64_t left_ = 0;
 };
+
+// A 2D file in the reference's own layout (TrackGenerator::dumpSegmentsToFile,
+// src/TrackGenerator.cpp:1388-1501, EXPLICIT_2D without CMFD): identity string,
+// per track (uid order, DumpSegments::onTrack, src/TrackTraversingAlgorithms.cpp:
+// 1101-1144) i32 count and 48-byte records, then the FSR table.  The identity
+// is the reference's Geometry::toString, which names its CSG objects, so it
+// cannot be compared with the problem text: as the reference lays its tracks
+// down before reading (src/TrackGenerator.cpp:818-836), the tracks here are
+// this problem's laydown, and the file is taken for this problem when its
+// records fit that laydown exactly.  Each FSR's material and key come from the
+// point the file gives for it (the reference writes the point where the FSR was
+// first found); volumes and centroids from the segments.
+bool read_reference_layout(In& in, const std::string& problem_text, TrackSet& ts) {
+  const Problem p = parse_problem(problem_text);
+  if (p.is_3d()) return false;
+  TrackSet out = laydown_tracks_2d(p);
+  const int64_t T = out.num_tracks;
+  constexpr uint64_t kRecordBytes = 8 + 4 + 8 + 4 + 3 * 8;
+  out.seg_offset.assign((size_t)T + 1, 0);
+  std::vector<int32_t> seg_ref_mat;
+  for (int64_t t = 0; t < T; t++) {
+    int32_t n;
+    if (in.left() < 4) return false;
+    n = in.get<int32_t>();
+    if (n < 0 || (uint64_t)n > in.left() / kRecordBytes) return false;
+    for (int32_t k = 0; k < n; k++) {
+      const double l = in.get<double>();
+      const int32_t m = in.get<int32_t>();
+      const int64_t f = in.get<int64_t>();
+      (void)in.get<int32_t>();  // track_idx
+      const double x = in.get<double>(), y = in.get<double>();
+      (void)in.get<double>();   // z
+      if (f < 0 || f >= ((int64_t)1 << 31) || !(l >= 0.0)) return false;
+      out.seg_length.push_back(l);
+      out.seg_fsr.push_back((int32_t)f);
+      out.seg_start.push_back(x);
+      out.seg_start.push_back(y);
+      seg_ref_mat.push_back(m);
+    }
+    out.seg_offset[(size_t)t + 1] = (int64_t)out.seg_length.size();
+  }
+  if (in.left() < 8) return false;
+  const int64_t N = in.get<int64_t>();
+  if (N <= 0 || (uint64_t)N > in.left() / 40) return false;
+  std::vector<double> point((size_t)(2 * N), 0.0);
+  std::vector<int32_t> ref_mat((size_t)N, 0);
+  std::vector<char> seen((size_t)N, 0);
+  for (int64_t i = 0; i < N; i++) {
+    (void)in.str();                      // key (FSR_keys_map order)
+    const int64_t id = in.get<int64_t>();
+    const double x = in.get<double>(), y = in.get<double>();
+    (void)in.get<double>();
+    ref_mat[(size_t)i] = in.get<int32_t>();  // material of FSR i (FSRs_to_material_IDs)
+    (void)in.str();                      // key of FSR i
+    if (id < 0 || id >= N || seen[(size_t)id]) return false;
+    seen[(size_t)id] = 1;
+    point[(size_t)(2 * id)] = x;
+    point[(size_t)(2 * id + 1)] = y;
+  }
+  if (in.left() != 0) return false;
+  if (!complete_fsrs_from_points(p, out, point)) return false;
+  // the file's material ids name the problem's materials one to one
+  std::map<int32_t, int32_t> to_ours, to_ref;
+  auto same = [&](int32_t ref, int32_t ours) {
+    auto a = to_ours.emplace(ref, ours).first;
+    auto b = to_ref.emplace(ours, ref).first;
+    return a->second == ours && b->second == ref;
+  };
+  for (int64_t r = 0; r < N; r++)
+    if (!same(ref_mat[(size_t)r], out.fsr_material[(size_t)r])) return false;
+  for (size_t k = 0; k < seg_ref_mat.size(); k++)
+    if (!same(seg_ref_mat[k], out.fsr_material[(size_t)out.seg_fsr[k]])) return false;
+  ts = std::move(out);
+  return true;
+}
 
 std::string fsr_key_string(const TrackSet& ts, int64_t r) {
   return ts.fsr_key.empty() ? std::to_string(r) : std::to_string(ts.fsr_key[(size_t)r]);
@@ -240,7 +324,11 @@ void dump_segment_file(const std::string& path, const std::string& problem_text,
 
 bool read_segment_file(const std::string& path, const std::string& problem_text, TrackSet& ts) {
   In in(path);
-  if (in.str() != problem_text) return false;
+  const std::string identity = in.str();
+  char magic[4] = {0, 0, 0, 0};
+  if (!in.peek(magic, 4) || std::memcmp(magic, kLaydownMagic, 4) != 0)
+    return read_reference_layout(in, problem_text, ts);
+  if (identity != problem_text) return false;
   TrackSet out;
   get_laydown(in, path, out);
   const int64_t T = out.num_tracks;

@@ -26,9 +26,12 @@ namespace moc {
 void dump_segment_file(const std::string& path, const std::string& problem_text,
                        const TrackSet& ts);
 
-// Fills `ts` from a segment file.  Returns false, leaving `ts` untouched, if
-// the file was written for another problem text (the reference's geometry
-// check); throws DeviceError(MOC_ERR_IO) on a missing, short or malformed file.
+// Fills `ts` from a segment file: this layout, or a 2D file the reference's
+// TrackGenerator::dumpSegmentsToFile wrote (no laydown section; the tracks
+// are then this problem's laydown and the records must fit it exactly).
+// Returns false, leaving `ts` untouched, if the file was written for another
+// problem (the reference's geometry check); throws DeviceError(MOC_ERR_IO) on
+// a missing, short or malformed file of this layout.
 bool read_segment_file(const std::string& path, const std::string& problem_text, TrackSet& ts);
 
 }  // namespace moc

@@ -515,15 +515,12 @@ static void laydown_2d(const Problem& p, TrackSet& ts) {
 
 }
 
-TrackSet generate_tracks(const Problem& p, int threads) {
-  if (p.is_3d()) return generate_tracks_3d(p, threads);
-  TrackSet ts;
-  laydown_2d(p, ts);
+// Polar set and total weights of the 2D quadrature (src/Quadrature.cpp:
+// 674-742, 822-894): w(a,p) = 2pi * w_a * delta_a * w_p * 2 sin(theta_p).
+static void quadrature_2d(const Problem& p, TrackSet& ts) {
   const int na = p.num_azim, na2 = na / 2, na4 = na / 4;
   const int np = p.num_polar;
   Quadrature& q = ts.quad;
-  const long T = ts.num_tracks;
-  // polar set + total weights w(a,p) = 2pi * w_a * delta_a * w_p * 2 sin(theta_p)
   std::vector<double> s_half, w_half;
   polar_set(p.quadrature, np, s_half, w_half);
   q.sin_theta.assign(np, 0);
@@ -544,6 +541,80 @@ TrackSet generate_tracks(const Problem& p, int threads) {
       for (int aa : {a, na2 - a - 1})
         for (int kk : {k, np - k - 1}) q.total_weight[(size_t)aa * np + kk] = w;
     }
+}
+
+TrackSet laydown_tracks_2d(const Problem& p) {
+  if (p.is_3d()) throw std::invalid_argument("the 2D laydown needs a 2D problem");
+  TrackSet ts;
+  laydown_2d(p, ts);
+  quadrature_2d(p, ts);
+  return ts;
+}
+
+bool complete_fsrs_from_points(const Problem& p, TrackSet& ts, const std::vector<double>& point) {
+  const long N = (long)point.size() / 2, T = ts.num_tracks;
+  const long S = (long)ts.seg_length.size();
+  if (N <= 0 || (long)ts.seg_offset.size() != T + 1 || ts.seg_offset[T] != S ||
+      (long)ts.seg_start.size() != 2 * S)
+    return false;
+  for (long k = 0; k < S; k++)
+    if (ts.seg_fsr[k] < 0 || ts.seg_fsr[k] >= N) return false;
+  Tracer tracer(p);
+  ts.num_fsrs = N;
+  ts.fsr_material.assign(N, 0);
+  ts.fsr_key.assign(N, 0);
+  std::vector<int64_t> keys;
+  for (long r = 0; r < N; r++) {
+    int64_t key;
+    int mat;
+    const double x = point[2 * r], y = point[2 * r + 1];
+    if (!(x > tracer.xmin && x < tracer.xmax && y > tracer.ymin && y < tracer.ymax)) return false;
+    tracer.classify(x, y, key, mat);
+    ts.fsr_key[r] = key;
+    ts.fsr_material[r] = mat;
+    keys.push_back(key);
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) return false;  // one FSR per region
+  // volumes and centroids from the segments (src/MOCKernel.cpp:86-160), the
+  // midpoint of a segment from its start point along the track
+  const Quadrature& q = ts.quad;
+  ts.fsr_volume.assign(N, 0.0);
+  ts.fsr_centroid.assign(2 * N, 0.0);
+  for (long t = 0; t < T; t++) {
+    const int a = ts.azim[t];
+    const double w = q.azim_spacing[a] * q.azim_weight[a];
+    const double ux = std::cos(ts.phi[t]), uy = std::sin(ts.phi[t]);
+    for (long k = ts.seg_offset[t]; k < ts.seg_offset[t + 1]; k++) {
+      const int32_t f = ts.seg_fsr[k];
+      const double l = ts.seg_length[k];
+      ts.fsr_volume[f] += w * l;
+      ts.fsr_centroid[2 * f] += w * l * (ts.seg_start[2 * k] + 0.5 * l * ux);
+      ts.fsr_centroid[2 * f + 1] += w * l * (ts.seg_start[2 * k + 1] + 0.5 * l * uy);
+    }
+  }
+  for (long f = 0; f < N; f++) {
+    if (!(ts.fsr_volume[f] > 0)) return false;  // every FSR is crossed
+    ts.fsr_centroid[2 * f] /= ts.fsr_volume[f];
+    ts.fsr_centroid[2 * f + 1] /= ts.fsr_volume[f];
+  }
+  double max_tau = 0;
+  std::vector<double> max_sig(p.materials.size(), 0);
+  for (size_t m = 0; m < p.materials.size(); m++)
+    for (double v : p.materials[m].sigma_t) max_sig[m] = std::max(max_sig[m], v);
+  for (long k = 0; k < S; k++)
+    max_tau = std::max(max_tau, ts.seg_length[k] * max_sig[ts.fsr_material[ts.seg_fsr[k]]]);
+  ts.max_optical_length = max_tau;
+  return true;
+}
+
+TrackSet generate_tracks(const Problem& p, int threads) {
+  if (p.is_3d()) return generate_tracks_3d(p, threads);
+  TrackSet ts;
+  laydown_2d(p, ts);
+  quadrature_2d(p, ts);
+  Quadrature& q = ts.quad;
+  const long T = ts.num_tracks;
 
   // ---- ray tracing (parallel over tracks), then first-encounter FSR ids
   Tracer tracer(p);

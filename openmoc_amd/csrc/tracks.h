@@ -135,6 +135,19 @@ TrackSet generate_tracks(const Problem& p, int threads = 0);
 // backward, ends) where a track leaves a face, in the reflected direction.
 TrackSet generate_tracks_3d(const Problem& p, int threads = 0);
 
+// The 2D quadrature and track laydown with links, without ray tracing (what
+// the reference's TrackGenerator holds before readSegmentsFromFile,
+// src/TrackGenerator.cpp:818-836).
+TrackSet laydown_tracks_2d(const Problem& p);
+
+// FSR data of a laid-down 2D TrackSet whose segments (seg_offset, seg_length,
+// seg_fsr, seg_start) came from elsewhere, given one point inside each FSR
+// [2N]: each FSR's key and material from its point, volumes and centroids from
+// the segments, the maximum optical length.  False if the data do not fit the
+// problem (a point outside the domain, two FSRs in one region, an FSR no
+// segment crosses, a region id out of range).
+bool complete_fsrs_from_points(const Problem& p, TrackSet& ts, const std::vector<double>& point);
+
 // Splits every segment whose optical length exceeds `max_tau` into equal
 // pieces (reference TrackGenerator::splitSegments semantics,
 // src/TrackTraversingAlgorithms.cpp:195-268).

@@ -258,7 +258,9 @@ class TrackGenerator:
     def readSegmentsFromFile(self, fname: Optional[str] = None) -> bool:
         """TrackGenerator::readSegmentsFromFile (TrackGenerator.cpp:1518-1638):
         takes the tracks and segments from `fname` (default getTestFilename())
-        instead of generating them.  False, with nothing changed, if the file
+        instead of generating them: a file dumpSegmentsToFile wrote, or a 2D
+        file the reference's own dumpSegmentsToFile wrote (read on this
+        problem's track laydown).  False, with nothing changed, if the file
         was written for another problem."""
         h = C.c_void_p()
         check(lib().moc_tracks_read_segments(self._text.encode(),

@@ -216,6 +216,10 @@ int main(int argc, char** argv) {
    * step<i>_num_iterations and step<i>_flux; the last step also writes the
    * usual keff / num_iterations / flux records. */
   const char* sequence = nullptr;
+  /* --dump-segments DIR: TrackGenerator::dumpSegmentsToFile (2D) into
+   * DIR/tracks/ under the reference's own file name (set_output_directory +
+   * initializeTrackFileDirectory, src/TrackGenerator.cpp:1326-1346, 1388) */
+  const char* dump_segments_dir = nullptr;
   for (int a = 3; a < argc; a++) {
     if (!strcmp(argv[a], "--threads")) threads = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--no-solve")) solve = false;
@@ -252,6 +256,7 @@ int main(int argc, char** argv) {
       a += 3;
     }
     else if (!strcmp(argv[a], "--sequence")) sequence = argv[++a];
+    else if (!strcmp(argv[a], "--dump-segments")) dump_segments_dir = argv[++a];
   }
   Spec s = parse(argv[1]);
   if (max_iters_override > 0) s.max_iters = max_iters_override;
@@ -512,6 +517,11 @@ int main(int argc, char** argv) {
   tg->setQuadrature(quad);
   tg->setNumThreads(1);
   tg->generateTracks();
+  if (dump_segments_dir) {
+    set_output_directory(const_cast<char*>(dump_segments_dir));
+    tg->initializeTrackFileDirectory();
+    tg->dumpSegmentsToFile();
+  }
 
   Writer w(argv[2]);
   w.l1("num_groups", G);

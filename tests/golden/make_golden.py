@@ -56,6 +56,9 @@ Outputs (data only: inputs and the reference's outputs):
                             and by material (the water box is the reference's
                             tests/test_compute_flux and test_compute_source
                             problem)
+  c5g7_pin_cell_ref_segments.data, pin_cell_7g_ref_segments.data
+                            segment files written by the reference's
+                            TrackGenerator::dumpSegmentsToFile
   *.txt                     the "moc-problem 1" inputs (synthetic XS bytes)
 
 `python tests/golden/make_golden.py flux_files` regenerates only the flux-file
@@ -267,7 +270,27 @@ def fixed_source():
         flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]))
 
 
+def segment_files():
+    """Segment files written by the reference's own
+    TrackGenerator::dumpSegmentsToFile (src/TrackGenerator.cpp:1388-1501) for
+    the C5G7 pin cell and pin_cell_7g, kept byte for byte."""
+    import glob
+    import shutil
+    for name in ("c5g7_pin_cell", "pin_cell_7g"):
+        p = workloads.get(name)
+        txt = OUT / f"{name}.txt"
+        txt.write_text(p.to_text())
+        with tempfile.TemporaryDirectory() as td:
+            oracle.run_reference(str(txt), str(Path(td) / "ref.bin"), threads=1, fp="f64",
+                                 extra=("--no-solve", "--dump-segments", td))
+            (src,) = glob.glob(str(Path(td) / "tracks" / "*.data"))
+            shutil.copyfile(src, OUT / f"{name}_ref_segments.data")
+
+
 def main():
+    if sys.argv[1:] == ["segment_files"]:
+        segment_files()
+        return
     if sys.argv[1:] == ["fixed_source"]:
         fixed_source()
         return
@@ -360,6 +383,7 @@ def main():
     adjoint()
     chi_spectrum()
     fixed_source()
+    segment_files()
     print("golden fixtures written to", OUT)
 
 

@@ -1,6 +1,7 @@
-"""Solves on tracks read back from a segment file (readSegmentsFromFile,
-src/TrackGenerator.cpp:1518-1638) give bitwise the results of the generated
-tracks: the file restores every array the solver is given."""
+"""Solves on tracks read from segment files (readSegmentsFromFile,
+src/TrackGenerator.cpp:1518-1638): files the reference itself wrote give the
+reference's results, and files this library wrote give bitwise the results
+of the generated tracks."""
 import numpy as np
 import pytest
 
@@ -22,3 +23,22 @@ def test_solve_on_tracks_from_a_segment_file(tracks, tmp_path, name):
     assert b.getNumIterations() == a.getNumIterations()
     assert b.getKeff() == a.getKeff()
     np.testing.assert_array_equal(b.getFluxes(), a.getFluxes())
+
+
+@pytest.mark.parametrize("name", ["c5g7_pin_cell", "pin_cell_7g"])
+def test_solve_on_segment_files_the_reference_wrote(golden, name):
+    """Tracks and segments from the reference's own dumpSegmentsToFile output
+    (tests/golden/*_ref_segments.data): keff within 1 pcm and FSR flux
+    relative L2 <= 1e-4 of the reference's converged solve."""
+    from conftest import GOLDEN
+    rd = om.TrackGenerator(om.get_workload(name))
+    assert rd.readSegmentsFromFile(GOLDEN / f"{name}_ref_segments.data")
+    g = golden(name)
+    s = om.HipSolver(rd)
+    s.computeEigenvalue(1000)
+    ref = np.asarray(g["flux"], np.float64).ravel()
+    rl2 = float(np.linalg.norm(s.getFluxes() - ref) / np.linalg.norm(ref))
+    print(f"{name} on the reference's segment file: keff {s.getKeff():.8f} "
+          f"(reference {float(g['keff']):.8f}), flux rel L2 {rl2:.2e}")
+    assert abs(s.getKeff() - float(g["keff"])) < 1e-5
+    assert rl2 < 1e-4

@@ -165,3 +165,50 @@ def test_malformed_files(tracks, tmp_path):
     (tmp_path / "x.tracks").write_bytes(b + b"\0")
     with pytest.raises(om.MOCError):
         fresh().readSegmentsFromFile(tmp_path / "x.tracks")
+
+
+# ---------------------------------------------------------------------------
+# Files the reference itself wrote (tests/golden/*_ref_segments.data, by its
+# TrackGenerator::dumpSegmentsToFile): read with this problem's laydown, they
+# give the tracer's FSR numbering, materials, keys and links, and its segment
+# lengths to rounding.
+# ---------------------------------------------------------------------------
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("name", ["c5g7_pin_cell", "pin_cell_7g"])
+def test_reads_segment_files_the_reference_wrote(tracks, name):
+    tg = tracks(name)
+    rd = om.TrackGenerator(om.get_workload(name))
+    assert rd.readSegmentsFromFile(GOLDEN / f"{name}_ref_segments.data")
+    for a in ("seg_offset", "seg_fsr", "fsr_material", "fsr_key", "azim", "next_fwd", "next_bwd",
+              "next_fwd_is_fwd", "next_bwd_is_fwd", "bc_fwd", "bc_bwd"):
+        np.testing.assert_array_equal(getattr(rd, a), getattr(tg, a), err_msg=a)
+    np.testing.assert_allclose(rd.seg_length, tg.seg_length, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rd.seg_start, tg.seg_start, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rd.fsr_volume, tg.fsr_volume, rtol=1e-12)
+    # the reference's segment start points carry its 1e-8 cm TINY_MOVE nudges
+    np.testing.assert_allclose(rd.fsr_centroid, tg.fsr_centroid, rtol=0, atol=1e-8)
+    assert rd.getNumFSRs() == tg.getNumFSRs() and rd.getNumSegments() == tg.getNumSegments()
+
+
+def test_reference_written_file_for_another_problem_is_not_taken():
+    """readSegmentsFromFile returns False when the file does not fit the
+    problem's laydown (the reference's geometry check)."""
+    for other in ("pin_cell_7g", "lattice_3x3_vac_7g"):
+        rd = om.TrackGenerator(om.get_workload(other))
+        assert rd.readSegmentsFromFile(GOLDEN / "c5g7_pin_cell_ref_segments.data") is False
+
+
+def test_solve_on_reference_written_segments_matches_reference(golden):
+    """The oracle on the tracks read from the reference's file reproduces the
+    reference's own C5G7 pin-cell result (tests/test_forward_pin_cell):
+    1.04666, 261 iterations."""
+    from oracle import oracle as orc
+    rd = om.TrackGenerator(om.get_workload("c5g7_pin_cell"))
+    assert rd.readSegmentsFromFile(GOLDEN / "c5g7_pin_cell_ref_segments.data")
+    g = golden("c5g7_pin_cell")
+    sol = orc.OracleProblem.from_track_generator(rd).eigenvalue(max_iters=500, tol=1e-5, threads=1)
+    assert sol.num_iterations == g["num_iterations"] == 261
+    assert abs(sol.keff - g["keff"]) < 1e-10
+    assert f"{sol.keff:12.5E}" == " 1.04666E+00"
