@@ -398,23 +398,26 @@ int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
 int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
                                double* tally);
 
-/* Sweep launch shape, shape9 = {groups_per_lane, lanes_per_track,
+/* Sweep launch shape, shape10 = {groups_per_lane, lanes_per_track,
  * tracks_per_wave, block, tally_copies, lds_groups, lds_block,
- * lds_blocks_per_sub, lds_q} (chosen by default from the device's LDS size,
- * or by the autotuner, openmoc_amd/autotune.py).
+ * lds_blocks_per_sub, lds_q, lds_flush} (chosen by default from the device's
+ * LDS size, or by the autotuner, openmoc_amd/autotune.py).
  *   lds_groups > 0: LDS-privatised tally; each block holds the [N][lds_groups]
  *     int64 fixed-point tally slice of one group subset, 1/V per FSR and
  *     sigma_t by material in LDS, plus the subset's FP32 Q/sigma_t slice when
  *     lds_q = 1 (8*N*lds_groups + 4*N + 4*M*lds_groups
  *     + 4*N*lds_groups*lds_q bytes must fit the device's LDS); lds_block
  *     threads (multiple of 64, <= 1024), lds_blocks_per_sub blocks per
- *     subset; the integer partial slices are summed exactly.
+ *     subset.  lds_flush = 0: each block writes its slice to a partial buffer
+ *     that the per-FSR step sums; lds_flush = 1: each block adds the
+ *     slice's non-zero entries to one int64 tally with atomics.  Both sums
+ *     are exact, so the results are the same bits.
  *   lds_groups = 0: device-scope float atomics; groups_per_lane in
  *     {1,2,4,9,16}, lanes_per_track = ceil(G/groups_per_lane),
  *     tracks_per_wave * lanes_per_track <= 64, block a multiple of 64
  *     (<= 256), tally_copies private tallies (block b uses copy b % copies). */
-int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape9);
-int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape9);
+int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape10);
+int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape10);
 
 /* Benchmark form of the eigenvalue iteration (no counterpart in the
  * reference): runs num_iterations source iterations (sweep + all per-FSR

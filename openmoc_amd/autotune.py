@@ -7,7 +7,8 @@ keeps the fastest.  Candidates cover both tally paths:
 
   * LDS sweep (lds_groups > 0): groups per block subset (= lanes per track),
     whether the subset's Q/sigma_t slice is staged in LDS (lds_q), threads per
-    block and blocks per subset;
+    block, blocks per subset, and how blocks hand their tally on (lds_flush:
+    per-block partial slices, or exact atomics into one int64 tally);
   * global-atomic sweep (lds_groups = 0): groups per lane, block size and the
     number of private tally copies.
 
@@ -37,12 +38,13 @@ def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
         if gb > num_groups:
             continue
         nsub = -(-num_groups // gb)
-        for q, block, per_cu in itertools.product((1, 0), (512, 1024), (1, 2, 4)):
+        for q, block, per_cu, flush in itertools.product((1, 0), (512, 1024), (1, 2, 4), (0, 1)):
             # blocks per subset in multiples of 8: the subsets of one track
             # slot then run on one XCD and share its L2 for the segment stream
             # (about a fifth of the HBM fetch of other counts, at equal speed)
             bps = max(8, ((num_cus * per_cu) // nsub + 7) // 8 * 8)
-            out.append(dict(lds_groups=gb, lds_q=q, lds_block=block, lds_blocks_per_sub=bps))
+            out.append(dict(lds_groups=gb, lds_q=q, lds_block=block, lds_blocks_per_sub=bps,
+                            lds_flush=flush))
     for gpl in (1, 2, 4, 9):
         lpt = -(-num_groups // gpl)
         if lpt > 64:

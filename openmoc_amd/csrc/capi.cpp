@@ -572,37 +572,37 @@ int moc_solver_transport_sweep(moc_solver_t* s, const double* Q, float* psi, dou
   MOC_SOLVER_CALL(s, s->impl->transportSweepOnce(Q, psi, tally));
 }
 
-int moc_solver_set_launch_shape(moc_solver_t* s, const int32_t* shape9) {
-  if (!shape9) return fail(MOC_ERR_INVALID, "null argument");
-  const int32_t* shape8 = shape9;
+int moc_solver_set_launch_shape(moc_solver_t* s, const int32_t* shape10) {
+  if (!shape10) return fail(MOC_ERR_INVALID, "null argument");
   MOC_SOLVER_CALL(s, {
     moc::LaunchShape sh;
-    sh.groups_per_lane = shape8[0];
-    sh.lanes_per_track = shape8[1];
-    sh.tracks_per_wave = shape8[2];
-    sh.block = shape8[3];
-    sh.tally_copies = shape8[4];
-    sh.lds_groups = shape8[5];
-    sh.lds_block = shape8[6];
-    sh.lds_blocks_per_sub = shape8[7];
-    sh.lds_q = shape8[8];
+    sh.groups_per_lane = shape10[0];
+    sh.lanes_per_track = shape10[1];
+    sh.tracks_per_wave = shape10[2];
+    sh.block = shape10[3];
+    sh.tally_copies = shape10[4];
+    sh.lds_groups = shape10[5];
+    sh.lds_block = shape10[6];
+    sh.lds_blocks_per_sub = shape10[7];
+    sh.lds_q = shape10[8];
+    sh.lds_flush = shape10[9];
     s->impl->setShape(sh);
   });
 }
 
-int moc_solver_get_launch_shape(const moc_solver_t* s, int32_t* shape9) {
-  if (!s || !s->impl || !shape9) return fail(MOC_ERR_INVALID, "null argument");
-  int32_t* shape8 = shape9;
+int moc_solver_get_launch_shape(const moc_solver_t* s, int32_t* shape10) {
+  if (!s || !s->impl || !shape10) return fail(MOC_ERR_INVALID, "null argument");
   const moc::LaunchShape& sh = s->impl->shape();
-  shape8[0] = sh.groups_per_lane;
-  shape8[1] = sh.lanes_per_track;
-  shape8[2] = sh.tracks_per_wave;
-  shape8[3] = sh.block;
-  shape8[4] = sh.tally_copies;
-  shape8[5] = sh.lds_groups;
-  shape8[6] = sh.lds_block;
-  shape8[7] = sh.lds_blocks_per_sub;
-  shape8[8] = sh.lds_q;
+  shape10[0] = sh.groups_per_lane;
+  shape10[1] = sh.lanes_per_track;
+  shape10[2] = sh.tracks_per_wave;
+  shape10[3] = sh.block;
+  shape10[4] = sh.tally_copies;
+  shape10[5] = sh.lds_groups;
+  shape10[6] = sh.lds_block;
+  shape10[7] = sh.lds_blocks_per_sub;
+  shape10[8] = sh.lds_q;
+  shape10[9] = sh.lds_flush;
   return MOC_OK;
 }
 

@@ -64,6 +64,10 @@ struct LaunchShape {
   int lds_block = 512;
   int lds_blocks_per_sub = 0;
   int lds_q = 0;             // 1: the subset's Q/sigma_t slice is staged in LDS too
+  // 1: each block adds the non-zero entries of its LDS tally slice to one
+  // global int64 tally with exact atomics; 0: it writes its whole slice to
+  // a per-block partial buffer that the per-FSR step sums
+  int lds_flush = 0;
 };
 
 class DeviceError : public std::exception {
@@ -255,6 +259,9 @@ class HipSolver {
   int32_t* d_slot_fsr_ = nullptr;      // [total slots] global FSR of each slot
   unsigned long long* d_tally_i64_ = nullptr;  // [N*G] fixed-point tally (stack sweep)
   bool stackMode() const { return is3d_ && ngrp_ > 0 && shape_.lds_groups > 0; }
+  // the sweep tallies into the global int64 fixed-point tally d_tally_i64_
+  // (3D z-stack sweep, or the 2D LDS sweep with the atomic flush)
+  bool int64Tally() const { return stackMode() || (shape_.lds_groups > 0 && shape_.lds_flush); }
   int64_t T_ = 0, S_ = 0, N_ = 0;
   int64_t num_fissionable_ = 0;
   bool tracks_set_ = false, materials_set_ = false, fsrs_set_ = false;

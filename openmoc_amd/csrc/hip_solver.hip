@@ -625,9 +625,12 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
 // subset-major global table).  The block processes the tiles
 // slot*wpb + wave, + blocks_per_sub*wpb, ... (longest first, so the strided
 // assignment balances) and finally writes its tally slice to its own partial
-// buffer; reduce_partials_kernel sums the partials in a fixed order.
+// buffer, which the per-FSR step sums, or adds the slice's non-zero entries to
+// one global int64 tally with atomics.  Both sums are exact integer sums, so
+// the two flushes give the same bits.
 struct LdsParams {
-  unsigned long long* partials;  // [nsub][blocks_per_sub][N*GB] fixed-point sums
+  unsigned long long* partials;  // [nsub][blocks_per_sub][N*GB] fixed-point sums, or null
+  unsigned long long* tally64;   // [N*G] fixed-point sums (atomic flush), or null
   const float* qsub;      // [nsub][N][GB] Q/sigma_t, subset-major
   const float* inv_vol;   // [N]
   const double* sigma_t;  // [M][G]
@@ -698,6 +701,14 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
     }
   }
   __syncthreads();
+  if (L.tally64) {
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+      const unsigned long long v = lds_tally[i];
+      const int r = i / L.GB, l = i - r * L.GB;
+      if (l < gb && v != 0ull) atomicAdd(&L.tally64[(size_t)r * a.G + g0 + l], v);
+    }
+    return;
+  }
   unsigned long long* out = L.partials + ((size_t)sub * L.blocks_per_sub + bslot) * nl;
   for (int i = threadIdx.x; i < nl; i += blockDim.x) out[i] = lds_tally[i];
 }
@@ -1789,6 +1800,7 @@ void HipSolver::setShape(const LaunchShape& s) {
       s.tally_copies > 64)
     throw DeviceError(MOC_ERR_INVALID, "invalid sweep launch shape");
   if (s.lds_groups < 0 || s.lds_groups > 64 || s.lds_groups > G_ || s.lds_q < 0 || s.lds_q > 1 ||
+      s.lds_flush < 0 || s.lds_flush > 1 ||
       (s.lds_groups > 0 &&
        (ldsBytes(s.lds_groups, s.lds_q) > lds_max_ || s.lds_block <= 0 ||
         s.lds_block % 64 != 0 || s.lds_block > 1024 || s.lds_blocks_per_sub <= 0)))
@@ -2335,7 +2347,12 @@ void HipSolver::allocateIterationArrays() {
     HIP_CHECK(hipMemset(d_tally_i64_, 0, NG * sizeof(unsigned long long)));
   } else if (shape_.lds_groups > 0) {
     const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
-    dev_alloc(&d_lds_partials_, (size_t)nsub * shape_.lds_blocks_per_sub * N_ * shape_.lds_groups);
+    if (shape_.lds_flush) {
+      dev_alloc(&d_tally_i64_, NG);
+      HIP_CHECK(hipMemset(d_tally_i64_, 0, NG * sizeof(unsigned long long)));
+    } else {
+      dev_alloc(&d_lds_partials_, (size_t)nsub * shape_.lds_blocks_per_sub * N_ * shape_.lds_groups);
+    }
     dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
     HIP_CHECK(hipMemset(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float)));
     dev_alloc(&d_tally64_, NG);
@@ -2673,7 +2690,8 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     L.GB = shape_.lds_groups;
     L.nsub = (G_ + L.GB - 1) / L.GB;
     L.blocks_per_sub = shape_.lds_blocks_per_sub;
-    L.partials = d_lds_partials_;
+    L.partials = shape_.lds_flush ? nullptr : d_lds_partials_;
+    L.tally64 = shape_.lds_flush ? d_tally_i64_ : nullptr;
     L.qsub = d_qsub_;
     L.inv_vol = d_inv_vol_;
     L.sigma_t = d_sigma_t_;
@@ -2792,8 +2810,9 @@ FsrArgs HipSolver::fsrArgs() {
     a.qs = nullptr;
     a.qsub = d_qsub_;
     a.qsub_groups = shape_.lds_groups;
-    a.lds_partials = d_lds_partials_;
-    a.lds_bps = shape_.lds_blocks_per_sub;
+    a.lds_partials = shape_.lds_flush ? nullptr : d_lds_partials_;
+    a.lds_bps = shape_.lds_flush ? 0 : shape_.lds_blocks_per_sub;
+    a.tally_i64 = shape_.lds_flush ? d_tally_i64_ : nullptr;
   } else {
     a.qs = d_qs_;
     a.qsub = nullptr;
@@ -2982,7 +3001,7 @@ void HipSolver::zeroTallies(hipStream_t s) {
   const size_t NG = (size_t)N_ * G_;
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, s,
                      d_tally_, (int64_t)(NG * shape_.tally_copies));
-  if (stackMode())
+  if (int64Tally())
     hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, s,
                        reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
   if (linear_)
@@ -3352,13 +3371,13 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   // CPUSolver::_start_flux
   HIP_CHECK(hipMemcpyAsync(d_psi_[1], slots.data(), psi_n * sizeof(float), hipMemcpyHostToDevice, stream_));
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(NG * shape_.tally_copies, 256)), dim3(256), 0, stream_, d_tally_, (int64_t)(NG * shape_.tally_copies));
-  if (stackMode())
+  if (int64Tally())
     hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
                        reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
   launchSweep(d_psi_[0], d_psi_[1], stream_);
   std::vector<float> t32;
   std::vector<unsigned long long> t64;
-  if (stackMode()) {
+  if (int64Tally()) {
     t64.resize(NG);
     HIP_CHECK(hipMemcpyAsync(t64.data(), d_tally_i64_, NG * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, stream_));
@@ -3378,7 +3397,7 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
   }
   HIP_CHECK(hipMemcpyAsync(slots.data(), d_psi_[1], psi_n * sizeof(float), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
-  if (stackMode()) {
+  if (int64Tally()) {
     // the fixed-point tally holds sum / V
     for (size_t i = 0; i < NG; i++) {
       const double v = h_fsr_vol_[i / G_];

@@ -674,7 +674,8 @@ class HipSolver:
         return tally, p
 
     SHAPE_KEYS = ("groups_per_lane", "lanes_per_track", "tracks_per_wave", "block",
-                  "tally_copies", "lds_groups", "lds_block", "lds_blocks_per_sub", "lds_q")
+                  "tally_copies", "lds_groups", "lds_block", "lds_blocks_per_sub", "lds_q",
+                  "lds_flush")
 
     def setLaunchShape(self, **shape) -> None:
         """Override the sweep launch shape (keys of getLaunchShape())."""
@@ -687,7 +688,7 @@ class HipSolver:
         check(lib().moc_solver_set_launch_shape(self._h, _ptr(a)))
 
     def getLaunchShape(self) -> dict:
-        a = np.zeros(9, np.int32)
+        a = np.zeros(len(self.SHAPE_KEYS), np.int32)
         check(lib().moc_solver_get_launch_shape(self._h, _ptr(a)))
         return dict(zip(self.SHAPE_KEYS, (int(x) for x in a)))
 

@@ -359,3 +359,21 @@ def test_fixed_source_by_material_and_reset(tracks):
     assert np.all(a.getFluxes() == 0.0)
     with pytest.raises(om.MOCError):
         a.setFixedSourceByMaterial(len(tg.sigma_t), 1, 1.0)
+
+
+@pytest.mark.parametrize("name", ["pin_cell_7g", "lattice_3x3_vac_7g", "lattice_17x17_7g"])
+def test_lds_tally_flush_modes_agree_bitwise(tracks, name):
+    """The LDS sweep's two ways of handing its fixed-point tally on (per-block
+    partial slices, or exact int64 atomics into one tally) are both exact
+    integer sums: the same keff, iterations and fluxes, bit for bit."""
+    tg = tracks(name)
+    res = []
+    for flush in (0, 1):
+        s = solver_for(tg)
+        if s.getLaunchShape()["lds_groups"] == 0:
+            pytest.skip("the default shape is not the LDS sweep")
+        s.setLaunchShape(lds_flush=flush)
+        s.computeEigenvalue(1000)
+        res.append((s.getKeff(), s.getNumIterations(), s.getFluxes()))
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+    np.testing.assert_array_equal(res[0][2], res[1][2])
