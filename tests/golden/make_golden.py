@@ -79,14 +79,31 @@ from oracle import mocdump, oracle  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
-def run(name: str, threads: int = 8, extra=()):
+def run(name: str, threads: int = 8, extra=(), fp: str = "f64"):
     p = workloads.get(name)
     txt = OUT / f"{name}.txt"
     txt.write_text(p.to_text())
     with tempfile.TemporaryDirectory() as td:
         dump = Path(td) / "ref.bin"
-        oracle.run_reference(str(txt), str(dump), threads=threads, fp="f64", extra=extra)
+        oracle.run_reference(str(txt), str(dump), threads=threads, fp=fp, extra=extra)
         return p, mocdump.read(str(dump))
+
+
+F32_THREADS = (1, 2, 4, 8)
+
+
+def f32_iterations(name: str, extra, steps: int):
+    """Iteration counts [steps][len(F32_THREADS)] of the same solve sequence
+    in the reference's FP32 build (its default), on 1, 2, 4 and 8 threads.
+    Near a 1e-7 threshold the FP32 sweep's rounding decides the last
+    iterations, so these counts spread; the spread is the bar for this
+    library's FP32 sweep."""
+    out = np.zeros((steps, len(F32_THREADS)), np.int64)
+    for j, th in enumerate(F32_THREADS):
+        _, d = run(name, threads=th, extra=extra, fp="f32")
+        for i in range(steps):
+            out[i, j] = int(d[f"step{i}_num_iterations"])
+    return out
 
 
 def base(d):
@@ -233,14 +250,17 @@ def fixed_source():
     _, d0 = run("c5g7_pin_cell", threads=1, extra=("--no-solve",))
     N, G = int(d0["num_fsrs"]), int(d0["num_groups"])
     fixed = pin_cell_fixed_sources(N, G)
-    _, d = run("c5g7_pin_cell", threads=1,
-               extra=(*fsr_source_args(fixed), "--tol", "1e-7", "--max-iters", "1000",
-                      "--sequence", "flux,flux0,source:3.0:TOTAL_SOURCE"))
+    extra = (*fsr_source_args(fixed), "--tol", "1e-7", "--max-iters", "1000",
+             "--sequence", "flux,flux0,source:3.0:TOTAL_SOURCE")
+    _, d = run("c5g7_pin_cell", threads=1, extra=extra)
+    f32 = f32_iterations("c5g7_pin_cell", extra, 3)
     np.savez_compressed(
         OUT / "c5g7_pin_cell_fixed_source_ref.npz", fixed=fixed,
         flux_flux=d["step0_flux"], flux_num_iterations=np.int64(d["step0_num_iterations"]),
         flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]),
-        source_flux=d["step2_flux"], source_num_iterations=np.int64(d["step2_num_iterations"]))
+        source_flux=d["step2_flux"], source_num_iterations=np.int64(d["step2_num_iterations"]),
+        flux_num_iterations_f32=f32[0], flux0_num_iterations_f32=f32[1],
+        source_num_iterations_f32=f32[2])
 
     # material 1 (moderator) group 1 overlaps a per-FSR source set before
     # and one set after it: the material's value wins in both FSRs
@@ -254,13 +274,15 @@ def fixed_source():
     args = []
     for kind, a, g, v in srcs:
         args += [f"--fixed-source-{kind}", str(a), str(g), repr(v)]
-    _, d = run(name, threads=1, extra=(*args, "--tol", "1e-7", "--sequence",
-                                       "source:0.8:TOTAL_SOURCE,flux0"))
+    extra = (*args, "--tol", "1e-7", "--sequence", "source:0.8:TOTAL_SOURCE,flux0")
+    _, d = run(name, threads=1, extra=extra)
+    f32 = f32_iterations(name, extra, 2)
     np.savez_compressed(
         OUT / f"{name}_fixed_source_ref.npz",
         sources=np.array([(0 if k == "fsr" else 1, a, g, v) for k, a, g, v in srcs]),
         source_flux=d["step0_flux"], source_num_iterations=np.int64(d["step0_num_iterations"]),
-        flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]))
+        flux0_flux=d["step1_flux"], flux0_num_iterations=np.int64(d["step1_num_iterations"]),
+        source_num_iterations_f32=f32[0], flux0_num_iterations_f32=f32[1])
 
     _, d = run("pin_cell_7g", threads=1, extra=("--sequence", "eig,flux0"))
     np.savez_compressed(

