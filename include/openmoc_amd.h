@@ -398,10 +398,11 @@ int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
 int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
                                double* tally);
 
-/* Sweep launch shape, shape10 = {groups_per_lane, lanes_per_track,
+
+/* Sweep launch shape, shape11 = {groups_per_lane, lanes_per_track,
  * tracks_per_wave, block, tally_copies, lds_groups, lds_block,
- * lds_blocks_per_sub, lds_q, lds_flush} (chosen by default from the device's
- * LDS size, or by the autotuner, openmoc_amd/autotune.py).
+ * lds_blocks_per_sub, lds_q, lds_flush, seg_store} (chosen by default from
+ * the device's LDS size, or by the autotuner, openmoc_amd/autotune.py).
  *   lds_groups > 0: LDS-privatised tally; each block holds the [N][lds_groups]
  *     int64 fixed-point tally slice of one group subset, 1/V per FSR and
  *     sigma_t by material in LDS, plus the subset's FP32 Q/sigma_t slice when
@@ -412,12 +413,18 @@ int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi
  *     that the per-FSR step sums; lds_flush = 1: each block adds the
  *     slice's non-zero entries to one int64 tally with atomics.  Both sums
  *     are exact, so the results are the same bits.
- *   lds_groups = 0: device-scope float atomics; groups_per_lane in
+ *   lds_groups = 0: the global-tally sweep; groups_per_lane in
  *     {1,2,4,9,16}, lanes_per_track = ceil(G/groups_per_lane),
  *     tracks_per_wave * lanes_per_track <= 64, block a multiple of 64
- *     (<= 256), tally_copies private tallies (block b uses copy b % copies). */
-int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape10);
-int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape10);
+ *     (<= 256).  seg_store = 1 (default): each segment's weighted
+ *     delta-psi per group is stored (no atomics) and the per-FSR step sums
+ *     an FSR's segments in a fixed order in FP64, so repeated solves give the
+ *     same bits (device memory: 4 * segments * 2 * ceil(G/32)*32 bytes, three
+ *     times that with the linear source); seg_store = 0: device-scope float
+ *     atomics into tally_copies private tallies (block b uses copy
+ *     b % copies), whose order varies from run to run. */
+int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape11);
+int moc_solver_get_launch_shape(const moc_solver_t* solver, int32_t* shape11);
 
 /* Benchmark form of the eigenvalue iteration (no counterpart in the
  * reference): runs num_iterations source iterations (sweep + all per-FSR
@@ -479,6 +486,12 @@ int moc_engine_wait(moc_engine_t* engine, int64_t request_id, double timeout_s,
                     moc_request_result_t* result, double* flux, int64_t num_fluxes);
 int moc_engine_stats(moc_engine_t* engine, int64_t* completed, int64_t* failed,
                      int64_t* iterations, int32_t* pending);
+/* Device memory of the engine's solves: the budget, the bytes the solvers of
+ * all batches hold now (live or idle; an idle solver is freed when another
+ * geometry needs the memory), and the most they have held at once.  A new
+ * solver is created only while held + its bytes <= budget, except when no
+ * solve is live at all (the queue always moves). */
+int moc_engine_memory(moc_engine_t* engine, int64_t* budget, int64_t* held, int64_t* held_max);
 
 /* Diagnostic (no counterpart in the reference): a plain device-to-device copy
  * of `bytes` bytes (a multiple of 16) run `repeats` times on `device`; the mean

@@ -98,6 +98,8 @@ _SIGS = [
                                   C.c_int64]),
     ("moc_engine_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    ("moc_engine_memory", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]),
     ("moc_solver_stabilize_transport", C.c_int, [_P, C.c_double, C.c_int32]),
     ("moc_solver_set_keff_from_neutron_balance", C.c_int, [_P, C.c_int]),
     ("moc_solver_fission_transport_sweep", C.c_int, [_P]),

@@ -9,8 +9,9 @@ keeps the fastest.  Candidates cover both tally paths:
     whether the subset's Q/sigma_t slice is staged in LDS (lds_q), threads per
     block, blocks per subset, and how blocks hand their tally on (lds_flush:
     per-block partial slices, or exact atomics into one int64 tally);
-  * global-atomic sweep (lds_groups = 0): groups per lane, block size and the
-    number of private tally copies.
+  * global-tally sweep (lds_groups = 0): groups per lane, block size and how
+    segments hand their tally on (seg_store: stored per segment and summed
+    per FSR in a fixed order, or float atomics into private tally copies).
 
 Shapes the device rejects (LDS size) are skipped.  Results are cached per
 problem signature for the life of the process.
@@ -49,9 +50,10 @@ def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
         lpt = -(-num_groups // gpl)
         if lpt > 64:
             continue
-        for block, copies in itertools.product((64, 256), (1, 8)):
+        for block, (store, copies) in itertools.product((64, 256), ((1, 1), (0, 1), (0, 8))):
             out.append(dict(lds_groups=0, groups_per_lane=gpl, lanes_per_track=lpt,
-                            tracks_per_wave=64 // lpt, block=block, tally_copies=copies))
+                            tracks_per_wave=64 // lpt, block=block, tally_copies=copies,
+                            seg_store=store))
     return out
 
 

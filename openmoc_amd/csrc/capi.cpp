@@ -395,6 +395,15 @@ int moc_engine_stats(moc_engine_t* e, int64_t* completed, int64_t* failed, int64
   });
 }
 
+int moc_engine_memory(moc_engine_t* e, int64_t* budget, int64_t* held, int64_t* held_max) {
+  MOC_ENGINE_CALL(e, {
+    const moc::ServeStats s = e->impl->stats();
+    if (budget) *budget = s.budget;
+    if (held) *held = s.held;
+    if (held_max) *held_max = s.held_max;
+  });
+}
+
 int moc_device_copy_calibration(int device, int64_t bytes, int32_t repeats, double* seconds) {
   return guarded([&] {
     const double t = moc::deviceCopyCalibration(device, bytes, repeats);
@@ -572,37 +581,39 @@ int moc_solver_transport_sweep(moc_solver_t* s, const double* Q, float* psi, dou
   MOC_SOLVER_CALL(s, s->impl->transportSweepOnce(Q, psi, tally));
 }
 
-int moc_solver_set_launch_shape(moc_solver_t* s, const int32_t* shape10) {
-  if (!shape10) return fail(MOC_ERR_INVALID, "null argument");
+int moc_solver_set_launch_shape(moc_solver_t* s, const int32_t* shape11) {
+  if (!shape11) return fail(MOC_ERR_INVALID, "null argument");
   MOC_SOLVER_CALL(s, {
     moc::LaunchShape sh;
-    sh.groups_per_lane = shape10[0];
-    sh.lanes_per_track = shape10[1];
-    sh.tracks_per_wave = shape10[2];
-    sh.block = shape10[3];
-    sh.tally_copies = shape10[4];
-    sh.lds_groups = shape10[5];
-    sh.lds_block = shape10[6];
-    sh.lds_blocks_per_sub = shape10[7];
-    sh.lds_q = shape10[8];
-    sh.lds_flush = shape10[9];
+    sh.groups_per_lane = shape11[0];
+    sh.lanes_per_track = shape11[1];
+    sh.tracks_per_wave = shape11[2];
+    sh.block = shape11[3];
+    sh.tally_copies = shape11[4];
+    sh.lds_groups = shape11[5];
+    sh.lds_block = shape11[6];
+    sh.lds_blocks_per_sub = shape11[7];
+    sh.lds_q = shape11[8];
+    sh.lds_flush = shape11[9];
+    sh.seg_store = shape11[10];
     s->impl->setShape(sh);
   });
 }
 
-int moc_solver_get_launch_shape(const moc_solver_t* s, int32_t* shape10) {
-  if (!s || !s->impl || !shape10) return fail(MOC_ERR_INVALID, "null argument");
+int moc_solver_get_launch_shape(const moc_solver_t* s, int32_t* shape11) {
+  if (!s || !s->impl || !shape11) return fail(MOC_ERR_INVALID, "null argument");
   const moc::LaunchShape& sh = s->impl->shape();
-  shape10[0] = sh.groups_per_lane;
-  shape10[1] = sh.lanes_per_track;
-  shape10[2] = sh.tracks_per_wave;
-  shape10[3] = sh.block;
-  shape10[4] = sh.tally_copies;
-  shape10[5] = sh.lds_groups;
-  shape10[6] = sh.lds_block;
-  shape10[7] = sh.lds_blocks_per_sub;
-  shape10[8] = sh.lds_q;
-  shape10[9] = sh.lds_flush;
+  shape11[0] = sh.groups_per_lane;
+  shape11[1] = sh.lanes_per_track;
+  shape11[2] = sh.tracks_per_wave;
+  shape11[3] = sh.block;
+  shape11[4] = sh.tally_copies;
+  shape11[5] = sh.lds_groups;
+  shape11[6] = sh.lds_block;
+  shape11[7] = sh.lds_blocks_per_sub;
+  shape11[8] = sh.lds_q;
+  shape11[9] = sh.lds_flush;
+  shape11[10] = sh.seg_store;
   return MOC_OK;
 }
 

@@ -35,7 +35,7 @@ struct IterState {
   double residual;
   double fission_total;
   int iteration;        // completed iterations of the current solve
-  int converged;        // 1 once the stopping rule is met
+  int converged;        // 1 once the stopping rule is met (2: iteration limit, batched solves)
   int dk;               // int(1e5 * (k - k_prev)) of the last iteration
   int reported_iters;   // Solver::_num_iterations semantics
   int64_t negative_fluxes;
@@ -50,6 +50,9 @@ struct IterState {
   float psi_max;            // max |start psi| of the sweep in use (before psi_scale)
   double scale_cap;         // upper limit of the FP32 scale s32 (1e38 / max 1/V)
   int old_sel;              // which old-flux buffer holds the current normalised flux
+  // iteration limit enforced on the device (batched solves; 0: none): the
+  // solve stops with converged = 2 once `iteration` reaches it
+  int max_iters;
 };
 
 struct LaunchShape {
@@ -68,6 +71,11 @@ struct LaunchShape {
   // global int64 tally with exact atomics; 0: it writes its whole slice to
   // a per-block partial buffer that the per-FSR step sums
   int lds_flush = 0;
+  // global-tally sweep (lds_groups = 0): 1 stores each segment's weighted
+  // delta-psi per group (no atomics) and the per-FSR step sums an FSR's
+  // segments in a fixed order (deterministic); 0 adds them with float atomics
+  // into tally_copies private tallies
+  int seg_store = 0;
 };
 
 class DeviceError : public std::exception {
@@ -85,7 +93,13 @@ class DeviceError : public std::exception {
 // mean kernel time (s).  Calibrates HBM byte counters against a known size.
 double deviceCopyCalibration(int device, int64_t bytes, int repeats);
 
-struct FsrArgs;  // arguments of the per-FSR kernels
+struct FsrArgs;      // arguments of the per-FSR kernels
+struct SweepArgs;    // arguments of the 2D sweep kernels
+struct LdsParams;    // LDS-privatised tally of the 2D sweep
+struct ScatArgs;     // scattering contraction on the matrix cores / FP32
+struct BatchSlot;    // one solve's row in a batched launch
+struct SweepBuffers; // per-sweep device buffers that replace the solver's own
+class SolveBatch;
 
 class HipSolver {
  public:
@@ -213,7 +227,8 @@ class HipSolver {
   void allocateIterationArrays();
   void runSolve(Mode mode, int max_iters, int res_type, double k_fixed);
   void launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
-                   hipEvent_t ev_after_sweep = nullptr, float* leak = nullptr);
+                   hipEvent_t ev_after_sweep = nullptr, float* leak = nullptr,
+                   const SweepBuffers* bufs = nullptr);
   void launchIteration(Mode mode, int res_type, bool recompute_sources, double tol,
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
   void launchInit(Mode mode, double k0);
@@ -234,6 +249,11 @@ class HipSolver {
   void launchScatter(hipStream_t s);
   void buildPositionStream();
   FsrArgs fsrArgs();
+  void sweepArgs(SweepArgs& a, const float* psi_in, float* psi_out, float* leak,
+                 const SweepBuffers* bufs) const;
+  void ldsParams(LdsParams& L, const SweepBuffers* bufs) const;
+  bool scatArgs(ScatArgs& a) const;
+  friend class SolveBatch;
   void refreshStabilization();
   const double* currentFlux() const;
   void pickShape();
@@ -377,6 +397,15 @@ class HipSolver {
   double async_ms_ = 0.0;
   std::chrono::steady_clock::time_point async_t0_;
   double* d_tally64_ = nullptr;        // [N*G] reduced LDS-path tally
+  // per-segment store of the global-tally sweep (shape_.seg_store)
+  bool segStore() const { return shape_.seg_store && shape_.lds_groups == 0 && !is3d_; }
+  float* d_segout_ = nullptr;          // [planes][seg_rows_][seg_stride_]
+  int64_t* d_csr_off_ = nullptr;       // [N+1] each FSR's rows of d_segout_ (geometry)
+  int32_t* d_seg_slot_ = nullptr;      // tile-interleaved: each segment's row (geometry)
+  bool csr_ready_ = false;             // d_csr_off_, d_seg_slot_ match the stream layout
+  int seg_stride_ = 0;                 // floats per row: G rounded up to 32
+  int64_t seg_rows_ = 0;               // segments + 1 (a row for the padding)
+  void buildSegmentIndex();
   int num_cus_ = 0;
   size_t lds_max_ = 0;                 // max dynamic LDS per block (bytes)
   size_t lds_per_cu_ = 0;
@@ -416,6 +445,101 @@ class HipSolver {
   int num_iterations_ = 0;
   double sweep_seconds_ = 0, total_seconds_ = 0;
   int64_t integrations_ = 0;
+
+  // batched solves (SolveBatch)
+  bool stream_borrowed_ = false;       // stream_ belongs to a SolveBatch
+  int device_max_iters_ = 0;           // IterState::max_iters of the next launchInit
+  void adoptStream(hipStream_t s);
+};
+
+// A solve of a SolveBatch that has stopped.
+struct BatchResult {
+  void* tag = nullptr;
+  double keff = 0.0;
+  int num_iterations = 0;
+  std::vector<double> flux;  // normalised FSR scalar flux [N*G] when requested
+};
+
+// Eigenvalue solves of one geometry advanced together (serving).  Every
+// launch of an iteration step carries all live solves, one row each of a
+// device table of per-solve arguments (blockIdx.y = row); the steps of
+// kGraphIters iterations are one captured graph per live-row count.  Each
+// solve stops on the device (stopping rule or iteration limit), after which
+// its rows return at entry, and a finished row is refilled with the next
+// request between graph launches (continuous batching).  Every row keeps
+// its angular and scalar fluxes resident; its arithmetic is that of a plain
+// HipSolver solve of the same inputs.
+class SolveBatch {
+ public:
+  // geometry: a solver with tracks, FSRs and materials set; the rows share
+  // its geometry buffers and launch shape.  res_type: residual of every solve.
+  SolveBatch(HipSolver& geometry, int capacity, int res_type);
+  ~SolveBatch();
+  SolveBatch(const SolveBatch&) = delete;
+  SolveBatch& operator=(const SolveBatch&) = delete;
+
+  int capacity() const { return capacity_; }
+  int live() const { return (int)live_.size(); }
+  bool busy() const { return !live_.empty() || !inflight_.empty(); }
+  bool hasIdleSolver() const { return !idle_.empty(); }
+  size_t bytesPerSolve() const { return bytes_per_solve_; }
+  size_t bytesHeld() const;
+  // Starts an eigenvalue solve with these cross sections in a free row
+  // (its initial state is queued behind the launches in flight); `tag`
+  // names it in the results.  Creates a row solver when none is idle.
+  void start(void* tag, int M, int G, const double* sigma_t, const double* sigma_s,
+             const double* nu_sigma_f, const double* chi, int max_iters, double tol,
+             bool want_flux);
+  // Collects the solves that stopped in completed launches and keeps up to
+  // two graph launches in flight.  Never waits for the device.
+  void progress(std::vector<BatchResult>& done);
+  // Waits for the launches in flight and collects them (a batch without
+  // live solves only; their rows have all stopped).
+  void drain();
+  // Frees the idle row solvers that no launch in flight refers to; returns
+  // the bytes freed.
+  size_t releaseIdle();
+
+ private:
+  struct Row {
+    void* tag;
+    uint64_t serial;  // this batch's number of the solve (tags may be reused)
+    int si;           // row solver: solvers_[si], its table entry slots_[si]
+    int max_iters;
+    bool want_flux;
+  };
+  struct Launch {
+    hipEvent_t ev;
+    int buf;
+    std::vector<std::pair<uint64_t, int>> rows;  // (serial, solver) of row j of this launch
+  };
+  void fillRow(BatchSlot& b, HipSolver& s, double tol) const;
+  void setRow(int row, const BatchSlot& b);
+  void launchStep(hipStream_t s, int n, int parity);
+  hipGraphExec_t graphFor(int n);
+  void launchGraph();
+  void finish(size_t idx, const IterState& st, std::vector<BatchResult>& done);
+  int newSolver();
+
+  HipSolver* geo_;
+  int device_ = 0;
+  int capacity_ = 0;
+  int res_type_ = MOC_FISSION_SOURCE;
+  size_t bytes_per_solve_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipStream_t capture_stream_ = nullptr;
+  BatchSlot* d_table_ = nullptr;       // [capacity]
+  IterState* d_snap_ = nullptr;        // [2][capacity]
+  IterState* h_snap_ = nullptr;        // [2][capacity] pinned
+  int next_buf_ = 0;
+  uint64_t next_serial_ = 1;
+  std::vector<std::unique_ptr<HipSolver>> solvers_;  // null where one was released
+  std::vector<std::unique_ptr<BatchSlot>> slots_;    // host copy of each solver's row
+  std::vector<int> idle_;              // solvers without a solve
+  std::vector<Row> live_;              // row j of the table = live_[j]
+  std::vector<Launch> inflight_;       // oldest first
+  std::vector<hipEvent_t> free_events_;
+  std::map<int, hipGraphExec_t> graphs_;  // by live-row count
 };
 
 }  // namespace moc

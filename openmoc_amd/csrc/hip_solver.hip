@@ -147,6 +147,8 @@ __device__ __forceinline__ float attenuation_hw(float y) {
 // ---------------------------------------------------------------------------
 // Transport sweep
 // ---------------------------------------------------------------------------
+}  // namespace
+
 struct SweepArgs {
   const int32_t* work;      // [ntiles*tpw] directed track ids (t*2+d) or -1, longest first
   int ntiles;               // wave tiles of tpw directed tracks
@@ -165,6 +167,9 @@ struct SweepArgs {
   float* leak;              // [2T*G] vacuum leakage w(a,0) * sum_p psi_out, or null
   float* tally;             // global path: [tally_copies][N*G]
   int tally_copies;         // global path: private copies (block b -> copy b % copies)
+  float* segout;            // global path, per-segment store: [slots][seg_stride], or null
+  const int32_t* seg_slot;  // tile-interleaved like seg_len: each segment's store row
+  int seg_stride;           // floats per store row (G rounded up to 32)
   size_t tally_stride;      // N*G
   const float* weights;     // [num_azim/2][P]
   float pf0, pf1, pf2;      // polar factors: 1/sin(theta_p) (times log2 e for HW exp)
@@ -173,6 +178,8 @@ struct SweepArgs {
   int G, P, fpt, lanes_per_track, tracks_per_wave;
   int N;
 };
+
+namespace {
 
 // Segment streams carry a packed key: FSR id in the low 24 bits, material in
 // the high 8 bits (N < 2^24, M <= 256 are checked on the host).
@@ -220,6 +227,7 @@ struct SubsetSource {  // global subset-major Q/sigma_t [N][GB]; LDS iv [N], sig
 
 // Tally policies: where a lane's weighted delta-psi of one segment goes.
 struct GlobalTally {   // device-scope float atomics into this block's copy
+  static constexpr bool kStore = false;
   float* t;
   int G;
   __device__ void add(int fsr, int g, int /*loc*/, float v, float /*iv*/) const {
@@ -232,6 +240,7 @@ struct GlobalTally {   // device-scope float atomics into this block's copy
 // integer sums are exact and order-independent, so the tally is bitwise
 // reproducible.
 struct LdsFixedTally {
+  static constexpr bool kStore = false;
   unsigned long long* t;
   int GB;
   __device__ void add(int fsr, int /*g*/, int loc, float v, float iv) const {
@@ -245,6 +254,17 @@ struct LdsFixedTally {
         ((unsigned long long)(unsigned)(int)hi << 32) | (unsigned long long)lo;
     atomicAdd(&t[__umul24(fsr, GB) + loc], x);
   }
+};
+
+// Per-segment store: the weighted delta-psi of a segment and group g goes to
+// row `slot` of out, out[slot * stride + g], no atomics.  Slots are laid out
+// FSR by FSR (each FSR's segments in a fixed order), so the per-FSR step sums
+// a contiguous run of rows and the tally is deterministic.
+struct SegmentStore {
+  static constexpr bool kStore = true;
+  float* out;
+  int stride;
+  __device__ void store(int slot, int g, float v) const { out[(size_t)slot * stride + g] = v; }
 };
 
 // Integrates directed track dt (tile slot `slot`) for the lane's groups
@@ -321,19 +341,23 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
       __builtin_amdgcn_make_buffer_rsrc((void*)a.seg_key, (short)0, (int)a.stream_bytes, 0x00020000);
   const unsigned row_bytes = (unsigned)a.tpw * 4u;
   const unsigned lane_off = (unsigned)ti.x * row_bytes + (unsigned)slot * 4u;
-  auto load_chunk = [&](int c, float* L, int* K) {
+  const __amdgpu_buffer_rsrc_t rs_slot = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)a.seg_slot, (short)0, Tally::kStore ? (int)a.stream_bytes : 0, 0x00020000);
+  auto load_chunk = [&](int c, float* L, int* K, int* S) {
     const unsigned vo = lane_off + (unsigned)(c * D) * row_bytes;
 #pragma unroll
     for (int j = 0; j < D; j++) {
       L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, j * row_bytes, 0));
       K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, j * row_bytes, 0);
+      if constexpr (Tally::kStore)
+        S[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_slot, vo, j * row_bytes, 0);
     }
   };
   auto fetch = [&](int key, SrcVal* q) {
 #pragma unroll
     for (int k = 0; k < GPL; k++) q[k] = src.load(key, g[k], loc[k]);
   };
-  auto integrate = [&](float len, int key, const SrcVal* q) {
+  auto integrate = [&](float len, int key, const SrcVal* q, int sidx) {
     const int fsr = key & kFsrMask;
 #pragma unroll
     for (int k = 0; k < GPL; k++) {
@@ -343,26 +367,35 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
       for (int p = 0; p < P; p++) {
         const float x = tau * pf[p];
         const float A = EXPM == MOC_EXP_RATIONAL ? attenuation_rational(x) : attenuation_hw(x);
-        const float d = (psi[k][p] - q[k].q) * A;
-        psi[k][p] -= d;
+        // delta = (psi - Q/sigma_t) A; the flux update is one fused
+        // multiply-add on the same difference and the tally takes the
+        // rounded product.  Written out so that every kernel built on this
+        // loop rounds alike, whatever the compiler would fuse on its own.
+        const float dq = psi[k][p] - q[k].q;
+        const float d = dq * A;
+        psi[k][p] = fmaf(-dq, A, psi[k][p]);
         acc = fmaf(w[p], d, acc);
       }
-      if (GPL > 1 && !ok[k]) acc = 0.f;
-      tally.add(fsr, g[k], loc[k], acc, q[k].iv);
+      if constexpr (Tally::kStore) {
+        if (ok[k]) tally.store(sidx, g[k], acc);
+      } else {
+        if (GPL > 1 && !ok[k]) acc = 0.f;
+        tally.add(fsr, g[k], loc[k], acc, q[k].iv);
+      }
     }
   };
   // Integrates one chunk.  Local sources: read one segment ahead.  Global
   // sources: `cur` holds this chunk's sources (fetched one chunk earlier) and
   // the next chunk's sources are fetched into `nxt` first.
-  auto process = [&](const float* L, const int* K, const int* Knext, SrcVal (*cur)[GPL],
-                     SrcVal (*nxt)[GPL]) {
+  auto process = [&](const float* L, const int* K, const int* S, const int* Knext,
+                     SrcVal (*cur)[GPL], SrcVal (*nxt)[GPL]) {
     if constexpr (Src::kLocal) {
       SrcVal qa[GPL], qb[GPL];
       fetch(K[0], qa);
 #pragma unroll
       for (int j = 0; j < D; j++) {
         if (j + 1 < D) fetch(K[j + 1], (j & 1) ? qa : qb);
-        integrate(L[j], K[j], (j & 1) ? qb : qa);
+        integrate(L[j], K[j], (j & 1) ? qb : qa, Tally::kStore ? S[j] : 0);
       }
       (void)Knext;
       (void)cur;
@@ -371,7 +404,7 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
 #pragma unroll
       for (int j = 0; j < D; j++) fetch(Knext[j], nxt[j]);
 #pragma unroll
-      for (int j = 0; j < D; j++) integrate(L[j], K[j], cur[j]);
+      for (int j = 0; j < D; j++) integrate(L[j], K[j], cur[j], Tally::kStore ? S[j] : 0);
     }
   };
 
@@ -380,10 +413,11 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
     // three rotating sets: chunk c lives in set c % 3 (streams and sources)
     float L0[D], L1[D], L2[D];
     int K0[D], K1[D], K2[D];
+    int S0[D], S1[D], S2[D];  // store rows (per-segment store only)
     SrcVal Q0[Src::kLocal ? 1 : D][GPL], Q1[Src::kLocal ? 1 : D][GPL], Q2[Src::kLocal ? 1 : D][GPL];
-    load_chunk(0, L0, K0);
-    load_chunk(1, L1, K1);
-    load_chunk(2, L2, K2);
+    load_chunk(0, L0, K0, S0);
+    load_chunk(1, L1, K1, S1);
+    load_chunk(2, L2, K2, S2);
     if constexpr (!Src::kLocal) {
 #pragma unroll
       for (int j = 0; j < D; j++) fetch(K0[j], Q0[j]);
@@ -396,18 +430,18 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
     // head, where it has to wait for the newest loads.
     int c = 0;
     for (; c + 3 <= nch; c += 3) {
-      process(L0, K0, K1, Q0, Q1);
-      load_chunk(c + 3, L0, K0);
+      process(L0, K0, S0, K1, Q0, Q1);
+      load_chunk(c + 3, L0, K0, S0);
       __builtin_amdgcn_sched_barrier(0);
-      process(L1, K1, K2, Q1, Q2);
-      load_chunk(c + 4, L1, K1);
+      process(L1, K1, S1, K2, Q1, Q2);
+      load_chunk(c + 4, L1, K1, S1);
       __builtin_amdgcn_sched_barrier(0);
-      process(L2, K2, K0, Q2, Q0);
-      load_chunk(c + 5, L2, K2);
+      process(L2, K2, S2, K0, Q2, Q0);
+      load_chunk(c + 5, L2, K2, S2);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (c < nch) process(L0, K0, K1, Q0, Q1);
-    if (c + 1 < nch) process(L1, K1, K2, Q1, Q2);
+    if (c < nch) process(L0, K0, S0, K1, Q0, Q1);
+    if (c + 1 < nch) process(L1, K1, S1, K2, Q1, Q2);
   }
 
   const int out = a.out_wp[dt];
@@ -446,8 +480,8 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
 
 // Global-atomic sweep: one wave per tile of tpw directed tracks,
 // lanes_per_track lanes per track, GPL groups per lane.
-template <int GPL, int EXPM, int NP>
-__global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
+template <int GPL, int EXPM, int NP, int STORE>
+__device__ __forceinline__ void sweep2d_kernel_body(const SweepArgs& a) {
   const int lane = threadIdx.x & 63;
   const int tile = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int slot = lane / a.lanes_per_track;
@@ -457,8 +491,18 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
   const int dt = a.work[tile * a.tpw + slot];
   if (dt < 0) return;
   GatherSource src{a.qs, a.G};
-  GlobalTally tally{a.tally + (size_t)(blockIdx.x % a.tally_copies) * a.tally_stride, a.G};
-  sweep_track<GPL, EXPM, NP>(a, tile, slot, dt, gl, a.lanes_per_track, 0, a.G, src, tally);
+  if constexpr (STORE) {
+    SegmentStore tally{a.segout, a.seg_stride};
+    sweep_track<GPL, EXPM, NP>(a, tile, slot, dt, gl, a.lanes_per_track, 0, a.G, src, tally);
+  } else {
+    GlobalTally tally{a.tally + (size_t)(blockIdx.x % a.tally_copies) * a.tally_stride, a.G};
+    sweep_track<GPL, EXPM, NP>(a, tile, slot, dt, gl, a.lanes_per_track, 0, a.G, src, tally);
+  }
+}
+
+template <int GPL, int EXPM, int NP, int STORE>
+__global__ __launch_bounds__(kSweepBlockMax) void sweep2d_kernel(SweepArgs a) {
+  sweep2d_kernel_body<GPL, EXPM, NP, STORE>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,6 +533,11 @@ struct LsArgs {
   float* tally_xy;          // [tally_copies][2][N*G] (x plane, y plane)
   const float* azim_cos;    // [num_azim/2]
   const float* azim_sin;
+  // per-segment store instead of the atomics, or null: planes (flat, x, y)
+  // of [slots][seg_stride] floats, seg_plane floats apart
+  float* segout;
+  int seg_stride;
+  size_t seg_plane;
 };
 
 // One lane per (directed track, group subset of GPL groups); float atomics
@@ -543,9 +592,12 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
   const float* lp = a.seg_len + (size_t)ti.x * a.tpw + slot;
   const int* kp = a.seg_key + (size_t)ti.x * a.tpw + slot;
   const float2* pp = L.seg_pos + (size_t)ti.x * a.tpw + slot;
+  // store rows of the per-segment store (the key stream stands in without it)
+  const int* sp = (L.segout ? a.seg_slot : a.seg_key) + (size_t)ti.x * a.tpw + slot;
   const size_t tpw = (size_t)a.tpw;
   float len_a = lp[0], len_b = lp[tpw];
   int key_a = kp[0], key_b = kp[tpw];
+  int sl_a = sp[0], sl_b = sp[tpw];
   float2 pos_a = pp[0], pos_b = pp[tpw];
   float4 q_a[GPL], q_b[GPL];
 #pragma unroll
@@ -554,6 +606,7 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
     const size_t o2 = (size_t)(r + 2) * tpw;
     const float len_c = lp[o2];
     const int key_c = kp[o2];
+    const int sl_c = sp[o2];
     const float2 pos_c = pp[o2];
 #pragma unroll
     for (int k = 0; k < GPL; k++) q_b[k] = L.qls[(size_t)(key_b & kFsrMask) * G + g[k]];
@@ -585,7 +638,14 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
         ax += h * dx + d * pos.x;
         ay += h * dy + d * pos.y;
       }
-      if (ok[k] && len > 0.f) {
+      if (L.segout) {
+        if (ok[k]) {
+          float* o = L.segout + (size_t)sl_a * L.seg_stride + g[k];
+          o[0] = acc;
+          o[L.seg_plane] = ax;
+          o[2 * L.seg_plane] = ay;
+        }
+      } else if (ok[k] && len > 0.f) {
         const size_t i = (size_t)fsr * G + g[k];
         unsafeAtomicAdd(&tf[i], acc);
         unsafeAtomicAdd(&tx[i], ax);
@@ -594,9 +654,11 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
     }
     len_a = len_b;
     key_a = key_b;
+    sl_a = sl_b;
     pos_a = pos_b;
     len_b = len_c;
     key_b = key_c;
+    sl_b = sl_c;
     pos_b = pos_c;
 #pragma unroll
     for (int k = 0; k < GPL; k++) q_a[k] = q_b[k];
@@ -628,6 +690,8 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
 // buffer, which the per-FSR step sums, or adds the slice's non-zero entries to
 // one global int64 tally with atomics.  Both sums are exact integer sums, so
 // the two flushes give the same bits.
+}  // namespace
+
 struct LdsParams {
   unsigned long long* partials;  // [nsub][blocks_per_sub][N*GB] fixed-point sums, or null
   unsigned long long* tally64;   // [N*G] fixed-point sums (atomic flush), or null
@@ -640,8 +704,10 @@ struct LdsParams {
   int blocks_per_sub;
 };
 
+namespace {
+
 template <int EXPM, int QLDS, int NP>
-__global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParams L) {
+__device__ __forceinline__ void sweep2d_lds_kernel_body(const SweepArgs& a, const LdsParams& L) {
   extern __shared__ unsigned long long lds64[];
   if (a.state->converged) return;  // uniform: read by every thread of the block
   // Workgroups are dispatched to the 8 XCDs round-robin by block id.  With
@@ -711,6 +777,11 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
   }
   unsigned long long* out = L.partials + ((size_t)sub * L.blocks_per_sub + bslot) * nl;
   for (int i = threadIdx.x; i < nl; i += blockDim.x) out[i] = lds_tally[i];
+}
+
+template <int EXPM, int QLDS, int NP>
+__global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParams L) {
+  sweep2d_lds_kernel_body<EXPM, QLDS, NP>(a, L);
 }
 
 // 3D z-stack sweep.  A group is the set of 3D tracks over a pair of
@@ -839,8 +910,19 @@ struct FsrArgs {
   float* tally;               // [tally_copies][N*G] (global-atomic sweep)
   int tally_copies;
   const unsigned long long* lds_partials;  // [nsub][bps][N*GB] fixed point (LDS sweep), or null
-  unsigned long long* tally_i64;           // [N*G] fixed point (3D z-stack sweep), or null
+  unsigned long long* tally_i64;           // [N*G] fixed point (3D z-stack sweep, LDS flush), or null
   int lds_bps;
+  // per-segment store of the global-tally sweep: [planes][slots][seg_stride]
+  // floats (planes flat, x, y); FSR r's segments are rows
+  // csr_off[r] .. csr_off[r+1] - 1, summed here in that order; or null
+  const float* segout;
+  const int64_t* csr_off;
+  int seg_stride;
+  size_t seg_plane;
+  // post_sweep_kernel's threads run in the LDS sweep's subset-major order
+  // (sub, r, l) (both of its flushes), so its FP64 partial sums are added in
+  // the same order whichever flush ran
+  int lds_order;
   const float* leak;          // [2T*G] vacuum leakage per directed track and group, or null
   int64_t leak_n;
   double* partials;           // [kNumPartials][pstride] per-block sums
@@ -868,6 +950,16 @@ struct FsrArgs {
   // residual against a fixed reference flux [N*G] instead of the old flux
   // (setResidualByReference), or null
   const double* ref_flux;
+};
+
+// Device buffers of one sweep that replace the solver's own (the
+// allocation-free sweep): Q/sigma_t tables, tallies and the iteration state.
+struct SweepBuffers {
+  const float2* qs;               // [N*G] (Q/sigma_t, sigma_t): global-tally sweep
+  float* qsub;                    // [nsub][N][GB] Q/sigma_t: LDS and z-stack sweeps
+  float* tally;                   // [copies][N*G]: global-tally sweep
+  unsigned long long* tally_i64;  // [N*G] fixed point: LDS (atomic flush) and z-stack sweeps
+  IterState* state;
 };
 
 namespace {
@@ -911,6 +1003,30 @@ __device__ double block_max(double v) {
   return r;  // valid in thread 0
 }
 
+// Tally of FSR r, group g from the per-segment store: the sum over the FSR's
+// rows, in FP64, in a fixed order (eight interleaved partial sums keep eight
+// loads in flight), so it does not depend on scheduling.  The threads of one
+// FSR read its contiguous rows of G floats one after the other.
+__device__ __forceinline__ double segment_sum(const FsrArgs& a, int64_t r, int g, int plane) {
+  const float* base = a.segout + (size_t)plane * a.seg_plane + g;
+  const int64_t e0 = a.csr_off[r], e1 = a.csr_off[r + 1];
+  constexpr int kAcc = 8;
+  double acc[kAcc];
+#pragma unroll
+  for (int u = 0; u < kAcc; u++) acc[u] = 0.0;
+  int64_t e = e0;
+  for (; e + kAcc <= e1; e += kAcc) {
+#pragma unroll
+    for (int u = 0; u < kAcc; u++) acc[u] += (double)base[(size_t)(e + u) * a.seg_stride];
+  }
+  for (; e < e1; e++) acc[0] += (double)base[(size_t)e * a.seg_stride];
+#pragma unroll
+  for (int w = kAcc / 2; w > 0; w /= 2)
+#pragma unroll
+    for (int u = 0; u < w; u++) acc[u] += acc[u + w];
+  return acc[0];
+}
+
 // One thread per (FSR, group), in the LDS sweep's subset-major order
 // (sub, r, l) so the fixed-point partials are read coalesced.
 //   from_tally=1: the tally of the sweep becomes the scalar flux
@@ -923,7 +1039,7 @@ __device__ double block_max(double v) {
 //   from_tally=0: phi is the current array (initial guess).
 // Per-block partials: fission rate before and after stabilisation,
 // absorption rate and vacuum leakage (computeKeff, src/CPUSolver.cpp:2258-2328).
-__global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int from_tally,
+__device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int from_tally,
                                                                  int balance) {
   if (a.state->converged) return;
   const int G = a.G;
@@ -938,7 +1054,7 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
   int64_t sub = 0, j = 0, r;
   int g;
   bool valid;
-  if (a.lds_partials) {
+  if (a.lds_order) {
     sub = i / nl;
     j = i - sub * nl;
     r = j / GB;
@@ -985,13 +1101,19 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
       } else if (a.linear) {
         // CPULSSolver::addSourceToScalarFlux (src/CPULSSolver.cpp:787-882)
         double t = 0.0, tx = 0.0, ty = 0.0;
-        for (int c = 0; c < a.tally_copies; c++) {
-          t += (double)a.tally[c * NG + k];
-          tx += (double)a.tally_xy[2 * c * NG + k];
-          ty += (double)a.tally_xy[(2 * c + 1) * NG + k];
-          a.tally[c * NG + k] = 0.f;
-          a.tally_xy[2 * c * NG + k] = 0.f;
-          a.tally_xy[(2 * c + 1) * NG + k] = 0.f;
+        if (a.segout) {
+          t = segment_sum(a, r, g, 0);
+          tx = segment_sum(a, r, g, 1);
+          ty = segment_sum(a, r, g, 2);
+        } else {
+          for (int c = 0; c < a.tally_copies; c++) {
+            t += (double)a.tally[c * NG + k];
+            tx += (double)a.tally_xy[2 * c * NG + k];
+            ty += (double)a.tally_xy[(2 * c + 1) * NG + k];
+            a.tally[c * NG + k] = 0.f;
+            a.tally_xy[2 * c * NG + k] = 0.f;
+            a.tally_xy[(2 * c + 1) * NG + k] = 0.f;
+          }
         }
         f = t / vol;
         f += kFourPi * a.Q[k];
@@ -1015,6 +1137,8 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
         }
         a.phi_xy[2 * k] = mx;
         a.phi_xy[2 * k + 1] = my;
+      } else if (a.segout) {
+        f = segment_sum(a, r, g, 0) / (st * vol) + kFourPi * a.Q[k] / st;
       } else {
         double t = 0.0;
         for (int c = 0; c < a.tally_copies; c++) {
@@ -1075,11 +1199,16 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
   if (neg) atomicAdd((unsigned long long*)&a.state->negative_fluxes, (unsigned long long)neg);
 }
 
+__global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int from_tally,
+                                                                 int balance) {
+  post_sweep_kernel_body(a, from_tally, balance);
+}
+
 // phase 0: initial guess normalisation (k unchanged); phase 1: computeKeff
 // (k *= F/N, or k = F / (A + L) from the neutron balance; src/CPUSolver.cpp:
 // 2258-2328) then normalizeFluxes' factor N/F (src/CPUSolver.cpp:1860-1931)
 // from the (stabilised) fluxes.
-__global__ __launch_bounds__(kReduceBlock) void keff_kernel(const double* partials, int nparts,
+__device__ __forceinline__ void keff_kernel_body(const double* partials, int nparts,
                                                            int pstride, IterState* state,
                                                            double num_fsrs, int phase,
                                                            int balance) {
@@ -1111,6 +1240,13 @@ __global__ __launch_bounds__(kReduceBlock) void keff_kernel(const double* partia
     state->norm_factor = num_fsrs / fiss_norm;
     state->psi_scale = num_fsrs / fiss_norm;
   }
+}
+
+__global__ __launch_bounds__(kReduceBlock) void keff_kernel(const double* partials, int nparts,
+                                                           int pstride, IterState* state,
+                                                           double num_fsrs, int phase,
+                                                           int balance) {
+  keff_kernel_body(partials, nparts, pstride, state, num_fsrs, phase, balance);
 }
 
 // From this many groups on, the per-FSR source residual terms are summed by
@@ -1157,7 +1293,7 @@ __device__ double source_residual_term(const FsrArgs& a, int64_t r, int res_type
 // Source residual terms for G >= kFsrResidualGroups, one thread per FSR,
 // after update_kernel (which leaves phi and the current old flux as they
 // were and writes zero partials); per-block sums replace its partials.
-__global__ __launch_bounds__(kReduceBlock) void fsr_residual_kernel(FsrArgs a, int normalize,
+__device__ __forceinline__ void fsr_residual_kernel_body(const FsrArgs& a, int normalize,
                                                                    int res_type) {
   if (a.state->converged) return;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1169,6 +1305,11 @@ __global__ __launch_bounds__(kReduceBlock) void fsr_residual_kernel(FsrArgs a, i
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
+__global__ __launch_bounds__(kReduceBlock) void fsr_residual_kernel(FsrArgs a, int normalize,
+                                                                   int res_type) {
+  fsr_residual_kernel_body(a, normalize, res_type);
+}
+
 // Source kinds of update_kernel
 constexpr int kSrcTotal = 0;    // computeFSRSources (src/CPUSolver.cpp:1939-2023)
 constexpr int kSrcFission = 1;  // computeFSRFissionSources (src/CPUSolver.cpp:2030-2066)
@@ -1178,7 +1319,7 @@ constexpr int kSrcScatter = 2;  // computeFSRScatterSources (src/CPUSolver.cpp:2
 // partials, store the normalised flux as the next "old" flux, sources of the
 // next sweep.  Reads phi and the current old flux only, writes the other
 // old buffer, so threads of one FSR never see each other's writes.
-__global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int normalize,
+__device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normalize,
                                                              int res_type, int do_residual,
                                                              int compute_sources,
                                                              int source_iteration_offset,
@@ -1316,6 +1457,15 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
   if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
 }
 
+__global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int normalize,
+                                                             int res_type, int do_residual,
+                                                             int compute_sources,
+                                                             int source_iteration_offset,
+                                                             int source_kind) {
+  update_kernel_body(a, normalize, res_type, do_residual, compute_sources, source_iteration_offset,
+                     source_kind);
+}
+
 // ---------------------------------------------------------------------------
 // Scattering-source contraction on the matrix cores
 // ---------------------------------------------------------------------------
@@ -1337,6 +1487,8 @@ constexpr int kMfmaMinGroups = 32;   // automatic choice: matrix cores from 32 g
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
+}  // namespace
+
 struct ScatArgs {
   int64_t N;
   int G;
@@ -1356,6 +1508,8 @@ struct ScatArgs {
   float* out;                  // [C][N][G + 1]
   const IterState* state;      // nothing to do once converged (null: always run)
 };
+
+namespace {
 
 __device__ __forceinline__ void split_f16(double v, _Float16& hi, _Float16& lo) {
   const float f = (float)v;
@@ -1409,7 +1563,7 @@ __global__ void scatter_frag_kernel(const double* sigma_s, const double* nsf, in
 // One wave per tile of 16 FSRs (4 waves per block).  Lane l loads row
 // i = l % 16 of the tile at k = 16 ks + 4 (l / 16) + j, the A operand layout;
 // the accumulator holds rows 4 (l / 16) + j of column 16 nt + l % 16.
-__global__ __launch_bounds__(256) void scatter_mfma_kernel(ScatArgs a) {
+__device__ __forceinline__ void scatter_mfma_kernel_body(const ScatArgs& a) {
   if (a.state && a.state->converged) return;
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1479,9 +1633,13 @@ __global__ __launch_bounds__(256) void scatter_mfma_kernel(ScatArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void scatter_mfma_kernel(ScatArgs a) {
+  scatter_mfma_kernel_body(a);
+}
+
 // FP32 fallback of the same contraction on the vector ALUs: one thread per
 // (component, FSR, column), FP32 products and sums.
-__global__ __launch_bounds__(256) void scatter_fp32_kernel(ScatArgs a) {
+__device__ __forceinline__ void scatter_fp32_kernel_body(const ScatArgs& a) {
   if (a.state && a.state->converged) return;
   const int G = a.G;
   const int64_t per = a.N * (G + 1);
@@ -1497,6 +1655,10 @@ __global__ __launch_bounds__(256) void scatter_fp32_kernel(ScatArgs a) {
   float acc = 0.f;
   for (int g = 0; g < G; g++) acc = fmaf((float)coef[g], (float)x[(size_t)g * a.xs[c]], acc);
   a.out[t] = acc;
+}
+
+__global__ __launch_bounds__(256) void scatter_fp32_kernel(ScatArgs a) {
+  scatter_fp32_kernel_body(a);
 }
 
 // Fixed-point scale of the next sweep's LDS tally.  Over one sweep,
@@ -1527,7 +1689,7 @@ __global__ void tally_scale_kernel(IterState* state, double sigma_t_max, int tak
 
 // mode 0 eigen: stop when res < tol && |int(1e5 dk)| < 1 (src/Solver.cpp:1643-1681)
 // mode 1/2 flux/source: stop when i > 1 && res < tol, _num_iterations = i.
-__global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* partials, int nparts,
+__device__ __forceinline__ void residual_kernel_body(const double* partials, int nparts,
                                                                IterState* state, double norm,
                                                                double tol, int mode,
                                                                double sigma_t_max, int take_qmax) {
@@ -1556,8 +1718,102 @@ __global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* pa
         state->reported_iters = i;
       }
     }
+    // a batched solve stops at its iteration limit (converged = 2: stopped
+    // without meeting the stopping rule)
+    if (!state->converged && state->max_iters > 0 && state->iteration >= state->max_iters)
+      state->converged = 2;
     set_tally_scale(state, sigma_t_max, take_qmax);
   }
+}
+
+__global__ __launch_bounds__(kReduceBlock) void residual_kernel(const double* partials, int nparts,
+                                                               IterState* state, double norm,
+                                                               double tol, int mode,
+                                                               double sigma_t_max, int take_qmax) {
+  residual_kernel_body(partials, nparts, state, norm, tol, mode, sigma_t_max, take_qmax);
+}
+
+// ---------------------------------------------------------------------------
+// Batched iteration: the solves of one geometry advance together, one launch
+// per step for all of them.  Solve j of a launch is row j of a device table
+// of per-solve arguments (blockIdx.y = j); each kernel returns at entry for a
+// solve that has stopped, so a row's work depends only on its own arrays.
+// ---------------------------------------------------------------------------
+}  // namespace
+
+struct BatchSlot {
+  SweepArgs sw[2];      // start fluxes in psi[0] / psi[1] (parity of the iteration)
+  LdsParams lds;        // LDS sweep (lds_groups > 0)
+  ScatArgs scat;        // scattering contraction on the matrix cores / FP32 (from 32 groups)
+  FsrArgs fsr;
+  int nparts_post;      // post_sweep_kernel blocks: partials summed by keff_kernel
+  int nparts_upd;       // update_kernel blocks: partials summed by residual_kernel
+  double num_fsrs;      // N (keff and normalisation)
+  double norm;          // residual normaliser (FSRs, or fissionable FSRs)
+  double tol;           // convergence threshold of this solve
+  double sigma_t_max;   // largest sigma_t of this solve's materials
+};
+
+namespace {
+
+// a launch argument of set_batch_slot_kernel
+static_assert(sizeof(BatchSlot) <= 2048, "batch row arguments exceed the kernel-argument budget");
+
+template <int GPL, int EXPM, int NP, int STORE>
+__global__ __launch_bounds__(kSweepBlockMax) void sweep2d_batch_kernel(const BatchSlot* __restrict__ t,
+                                                                       int parity) {
+  sweep2d_kernel_body<GPL, EXPM, NP, STORE>(t[blockIdx.y].sw[parity]);
+}
+
+template <int EXPM, int QLDS, int NP>
+__global__ __launch_bounds__(1024) void sweep2d_lds_batch_kernel(const BatchSlot* __restrict__ t,
+                                                                 int parity) {
+  const BatchSlot& b = t[blockIdx.y];
+  sweep2d_lds_kernel_body<EXPM, QLDS, NP>(b.sw[parity], b.lds);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void post_sweep_batch_kernel(const BatchSlot* __restrict__ t) {
+  post_sweep_kernel_body(t[blockIdx.y].fsr, 1, 0);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void keff_batch_kernel(const BatchSlot* __restrict__ t) {
+  const BatchSlot& b = t[blockIdx.y];
+  keff_kernel_body(b.fsr.partials, b.nparts_post, b.fsr.pstride, b.fsr.state, b.num_fsrs, 1, 0);
+}
+
+__global__ __launch_bounds__(256) void scatter_mfma_batch_kernel(const BatchSlot* __restrict__ t) {
+  scatter_mfma_kernel_body(t[blockIdx.y].scat);
+}
+
+__global__ __launch_bounds__(256) void scatter_fp32_batch_kernel(const BatchSlot* __restrict__ t) {
+  scatter_fp32_kernel_body(t[blockIdx.y].scat);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void update_batch_kernel(const BatchSlot* __restrict__ t,
+                                                                   int res_type) {
+  update_kernel_body(t[blockIdx.y].fsr, 1, res_type, 1, 1, 1, kSrcTotal);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void fsr_residual_batch_kernel(const BatchSlot* __restrict__ t,
+                                                                         int res_type) {
+  fsr_residual_kernel_body(t[blockIdx.y].fsr, 1, res_type);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void residual_batch_kernel(const BatchSlot* __restrict__ t) {
+  const BatchSlot& b = t[blockIdx.y];
+  residual_kernel_body(b.fsr.partials, b.nparts_upd, b.fsr.state, b.norm, b.tol, 0, b.sigma_t_max, 1);
+}
+
+// Row `row` of the table := v (a launch argument, so the host's copy of v is
+// free as soon as the launch is queued).
+__global__ void set_batch_slot_kernel(BatchSlot* t, int row, BatchSlot v) {
+  if (threadIdx.x == 0) t[row] = v;
+}
+
+// Iteration states of rows 0..n-1, gathered for one copy to the host.
+__global__ void gather_states_kernel(const BatchSlot* __restrict__ t, int n, IterState* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = *t[j].fsr.state;
 }
 
 // Plain device copy (16-byte vector loads and stores), the reference kernel
@@ -1593,6 +1849,14 @@ __global__ void make_qs_kernel(const double* Q, const int32_t* fsr_mat, const do
   }
 }
 
+// Raw FSR tally [N*G] from the per-segment store (single-sweep entry point).
+__global__ void segment_tally_kernel(FsrArgs a, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N * a.G) return;
+  const int64_t r = i / a.G;
+  out[i] = segment_sum(a, r, (int)(i - r * a.G), 0);
+}
+
 __global__ void fission_rates_kernel(const double* phi, const int32_t* fsr_mat,
                                      const double* nsf, const double* vol, double* out,
                                      int64_t N, int G) {
@@ -1621,6 +1885,7 @@ void ensure_lds_attribute(const void* kernel, int device, size_t bytes) {
 }
 
 }  // namespace
+
 
 // ===========================================================================
 // Host driver
@@ -1669,8 +1934,13 @@ HipSolver::~HipSolver() {
     d_fsr_vol_ = nullptr;
     d_inv_vol_ = nullptr;
     d_out_wp_ = nullptr;
+    d_csr_off_ = nullptr;
+    d_seg_slot_ = nullptr;
   }
   dev_free(d_seg_len_);
+  dev_free(d_csr_off_);
+  dev_free(d_seg_slot_);
+  dev_free(d_segout_);
   dev_free(d_seg_key_);
   dev_free(d_work_);
   dev_free(d_tile_);
@@ -1725,7 +1995,16 @@ HipSolver::~HipSolver() {
   dev_free(d_tally64_);
   dev_free(d_state_);
   if (h_state_) (void)hipHostFree(h_state_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (stream_ && !stream_borrowed_) (void)hipStreamDestroy(stream_);
+}
+
+// Runs this solver's launches on `s` (owned by the caller) from now on.
+void HipSolver::adoptStream(hipStream_t s) {
+  HIP_CHECK(hipSetDevice(device_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  if (!stream_borrowed_) HIP_CHECK(hipStreamDestroy(stream_));
+  stream_ = s;
+  stream_borrowed_ = true;
 }
 
 void HipSolver::pickShape() {
@@ -1743,6 +2022,8 @@ void HipSolver::pickShape() {
   s.lanes_per_track = (G_ + s.groups_per_lane - 1) / s.groups_per_lane;
   s.tracks_per_wave = 64 / s.lanes_per_track;
   s.block = 256;
+  // the global-tally path stores per segment by default (deterministic)
+  s.seg_store = 1;
   // LDS-privatised tally when at least one group's slice fits in LDS: the
   // largest subset that fits, evened out over the subsets.
   // LDS-privatised tally when a group's slice fits in LDS: with the source
@@ -1800,7 +2081,7 @@ void HipSolver::setShape(const LaunchShape& s) {
       s.tally_copies > 64)
     throw DeviceError(MOC_ERR_INVALID, "invalid sweep launch shape");
   if (s.lds_groups < 0 || s.lds_groups > 64 || s.lds_groups > G_ || s.lds_q < 0 || s.lds_q > 1 ||
-      s.lds_flush < 0 || s.lds_flush > 1 ||
+      s.lds_flush < 0 || s.lds_flush > 1 || s.seg_store < 0 || s.seg_store > 1 ||
       (s.lds_groups > 0 &&
        (ldsBytes(s.lds_groups, s.lds_q) > lds_max_ || s.lds_block <= 0 ||
         s.lds_block % 64 != 0 || s.lds_block > 1024 || s.lds_blocks_per_sub <= 0)))
@@ -2127,7 +2408,50 @@ void HipSolver::buildTileLayout(int tpw) {
   h_work_.swap(work);
   h_tile_.swap(tile);
   ls_ready_ = false;
+  csr_ready_ = false;  // the per-segment index follows the stream layout
   state_valid_ = false;  // start fluxes are laid out by work position
+}
+
+// Rows of the per-segment store: FSR by FSR (a counting sort of the
+// segments by FSR, each FSR's segments in stream order), one more row at the
+// end for the padding segments; csr_off[r] is FSR r's first row and the
+// seg_slot stream gives each stream position its row.  Geometry data, shared
+// with solvers that share the streams.
+void HipSolver::buildSegmentIndex() {
+  seg_stride_ = (G_ + 31) / 32 * 32;  // rows of whole 128-byte lines
+  if (csr_ready_) return;
+  if (geo_hold_)
+    throw DeviceError(MOC_ERR_STATE, "the shared geometry has no per-segment index for this launch shape");
+  const int tpw = layout_tpw_;
+  auto each = [&](auto fn) {
+    for (int64_t tl = 0; tl < layout_ntiles_; tl++) {
+      const int64_t r0 = h_tile_[2 * tl];
+      const int32_t nr = h_tile_[2 * tl + 1];
+      for (int32_t i = 0; i < nr; i++)
+        for (int sl = 0; sl < tpw; sl++) {
+          const int32_t dt = h_work_[tl * tpw + sl];
+          if (dt < 0) continue;
+          const int32_t t = dt >> 1, dir = dt & 1;
+          const int32_t b = h_trk_[4 * t], n = h_trk_[4 * t + 1];
+          if (i >= n) continue;
+          fn(h_fsr_[dir ? b + n - 1 - i : b + i], (size_t)(r0 + i) * tpw + sl);
+        }
+    }
+  };
+  std::vector<int64_t> off((size_t)N_ + 1, 0);
+  each([&](int32_t f, size_t) { off[(size_t)f + 1]++; });
+  for (int64_t r = 0; r < N_; r++) off[r + 1] += off[r];
+  if (off[N_] >= ((int64_t)1 << 31)) throw DeviceError(MOC_ERR_INVALID, "too many segments");
+  seg_rows_ = off[N_] + 1;
+  const size_t total = (size_t)(layout_rows_ + kTailRows) * tpw;
+  std::vector<int32_t> slot(total, (int32_t)off[N_]);  // padding: the last row
+  std::vector<int64_t> cur(off.begin(), off.end() - 1);
+  each([&](int32_t f, size_t q) { slot[q] = (int32_t)cur[f]++; });
+  dev_alloc(&d_csr_off_, off.size());
+  dev_alloc(&d_seg_slot_, total);
+  HIP_CHECK(hipMemcpy(d_csr_off_, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(d_seg_slot_, slot.data(), total * sizeof(int32_t), hipMemcpyHostToDevice));
+  csr_ready_ = true;
 }
 
 int HipSolver::sweepTracksPerWave() const {
@@ -2358,6 +2682,13 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_tally64_, NG);
     HIP_CHECK(hipMemset(d_tally64_, 0, NG * sizeof(double)));
   }
+  if (segStore()) {
+    buildTileLayout(sweepTracksPerWave());  // positions follow the final shape
+    buildSegmentIndex();
+    const int planes = linear_ ? 3 : 1;
+    dev_alloc(&d_segout_, (size_t)planes * seg_rows_ * seg_stride_);
+    dev_alloc(&d_tally64_, NG);
+  }
   {
     // per-block partials: post_sweep_kernel runs over nsub*N*GB threads,
     // update_kernel over N*G
@@ -2448,12 +2779,13 @@ void HipSolver::prepareScatter() {
 
 // The contraction of the current flux (and moments) the next update_kernel
 // reads; nothing on the FP64 path, whose update_kernel loops over groups.
-void HipSolver::launchScatter(hipStream_t s) {
+// Arguments of the scattering contraction; false on the FP64 path (no
+// contraction launch).
+bool HipSolver::scatArgs(ScatArgs& a) const {
   const int path = scatterPrecisionInUse();
-  if (path == MOC_SCATTER_FP64) return;
+  if (path == MOC_SCATTER_FP64) return false;
   if (!d_scat_ || (path == MOC_SCATTER_MFMA_F16 && !(scat_tiles_ok_ && scat_frag_ok_)))
     throw DeviceError(MOC_ERR_STATE, "scattering contraction buffers are not prepared");
-  ScatArgs a;
   a.N = N_;
   a.G = G_;
   a.ks_n = (G_ + 15) / 16;
@@ -2475,6 +2807,13 @@ void HipSolver::launchScatter(hipStream_t s) {
   a.xs[2] = 2;
   a.out = d_scat_;
   a.state = d_state_;
+  return true;
+}
+
+void HipSolver::launchScatter(hipStream_t s) {
+  ScatArgs a;
+  if (!scatArgs(a)) return;
+  const int path = scatterPrecisionInUse();
   if (path == MOC_SCATTER_MFMA_F16)
     hipLaunchKernelGGL(scatter_mfma_kernel, dim3(blocks_for(scat_ntiles_, 4)), dim3(256), 0, s, a);
   else
@@ -2591,9 +2930,10 @@ void HipSolver::resetFixedSources() {
                         hipMemcpyHostToDevice));
 }
 
-void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
-                            hipEvent_t ev_after_sweep, float* leak) {
-  SweepArgs a;
+// Arguments of the 2D / 3D sweep kernels over this solver's arrays, or over
+// caller-provided buffers (the allocation-free sweep).
+void HipSolver::sweepArgs(SweepArgs& a, const float* psi_in, float* psi_out, float* leak,
+                          const SweepBuffers* bufs) const {
   if (layout_tpw_ != sweepTracksPerWave())
     throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
   a.work = d_work_;
@@ -2614,23 +2954,53 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   a.tally = d_tally_;
   a.tally_copies = shape_.tally_copies;
   a.tally_stride = (size_t)N_ * G_;
+  a.segout = (!bufs && segStore()) ? d_segout_ : nullptr;
+  a.seg_slot = d_seg_slot_;
+  a.seg_stride = seg_stride_;
   a.weights = d_weights_;
   const float l2e = exp_mode_ == MOC_EXP_HW ? 1.4426950408889634f : 1.0f;
   a.pf0 = inv_sin_[0] * l2e;
   a.pf1 = inv_sin_[1] * l2e;
   a.pf2 = inv_sin_[2] * l2e;
   a.state = d_state_;
-  a.psi_max_bits = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(d_state_) +
-                                               offsetof(IterState, psi_max_bits));
   a.G = G_;
   a.P = P_;
   a.fpt = P_ * G_;
   a.N = (int)N_;
   a.lanes_per_track = shape_.lanes_per_track;
   a.tracks_per_wave = shape_.tracks_per_wave;
+  if (bufs) {
+    // caller-provided buffers (the allocation-free sweep): the LDS sweep then
+    // always flushes into the int64 tally
+    a.qs = bufs->qs;
+    a.tally = bufs->tally;
+    a.state = bufs->state;
+  }
+  a.psi_max_bits = &const_cast<IterState*>(a.state)->psi_max_bits;
+}
+
+// Parameters of the 2D LDS sweep (shape_.lds_groups > 0).
+void HipSolver::ldsParams(LdsParams& L, const SweepBuffers* bufs) const {
+  L.GB = shape_.lds_groups;
+  L.nsub = (G_ + L.GB - 1) / L.GB;
+  L.blocks_per_sub = shape_.lds_blocks_per_sub;
+  L.partials = (shape_.lds_flush || bufs) ? nullptr : d_lds_partials_;
+  L.tally64 = bufs ? bufs->tally_i64 : shape_.lds_flush ? d_tally_i64_ : nullptr;
+  L.qsub = bufs ? bufs->qsub : d_qsub_;
+  L.inv_vol = d_inv_vol_;
+  L.sigma_t = d_sigma_t_;
+  L.M = M_;
+}
+
+void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
+                            hipEvent_t ev_after_sweep, float* leak, const SweepBuffers* bufs) {
+  SweepArgs a;
+  sweepArgs(a, psi_in, psi_out, leak, bufs);
   if (linear_) {
+    if (bufs) throw DeviceError(MOC_ERR_INVALID, "the linear source has no caller-buffer sweep");
     if (!ls_ready_) throw DeviceError(MOC_ERR_STATE, "linear-source data not prepared");
-    LsArgs L{d_seg_pos_, d_qls_, d_tally_xy_, d_azim_cos_, d_azim_sin_};
+    LsArgs L{d_seg_pos_, d_qls_, d_tally_xy_, d_azim_cos_, d_azim_sin_,
+             a.segout, seg_stride_, (size_t)seg_rows_ * seg_stride_};
     // the reference's LS exponentials take plain 1/sin(theta)
     a.pf0 = inv_sin_[0];
     a.pf1 = inv_sin_[1];
@@ -2658,8 +3028,8 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     StackParams L;
     L.grp = reinterpret_cast<const int4*>(d_grp_);
     L.slot_fsr = d_slot_fsr_;
-    L.tally = d_tally_i64_;
-    L.qsub = d_qsub_;
+    L.tally = bufs ? bufs->tally_i64 : d_tally_i64_;
+    L.qsub = bufs ? bufs->qsub : d_qsub_;
     L.inv_vol = d_inv_vol_;
     L.sigma_t = d_sigma_t_;
     L.M = M_;
@@ -2687,15 +3057,7 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
   }
   if (shape_.lds_groups > 0) {
     LdsParams L;
-    L.GB = shape_.lds_groups;
-    L.nsub = (G_ + L.GB - 1) / L.GB;
-    L.blocks_per_sub = shape_.lds_blocks_per_sub;
-    L.partials = shape_.lds_flush ? nullptr : d_lds_partials_;
-    L.tally64 = shape_.lds_flush ? d_tally_i64_ : nullptr;
-    L.qsub = d_qsub_;
-    L.inv_vol = d_inv_vol_;
-    L.sigma_t = d_sigma_t_;
-    L.M = M_;
+    ldsParams(L, bufs);
     const size_t lds_bytes = ldsBytes(L.GB, shape_.lds_q);
     if (lds_bytes > lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
     const unsigned grid = (unsigned)(L.nsub * L.blocks_per_sub);
@@ -2736,12 +3098,19 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
         default: hipLaunchKernelGGL(kernel3, dim3(grid), blk, 0, s, a); break;
       }
     };
-    if (exp_mode_ == MOC_EXP_HW)
-      by_np(sweep2d_kernel<GPL, MOC_EXP_HW, 1>, sweep2d_kernel<GPL, MOC_EXP_HW, 2>,
-            sweep2d_kernel<GPL, MOC_EXP_HW, 3>);
+    auto by_exp = [&](auto st) {
+      constexpr int ST = decltype(st)::value;
+      if (exp_mode_ == MOC_EXP_HW)
+        by_np(sweep2d_kernel<GPL, MOC_EXP_HW, 1, ST>, sweep2d_kernel<GPL, MOC_EXP_HW, 2, ST>,
+              sweep2d_kernel<GPL, MOC_EXP_HW, 3, ST>);
+      else
+        by_np(sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 1, ST>, sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 2, ST>,
+              sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 3, ST>);
+    };
+    if (a.segout)
+      by_exp(std::integral_constant<int, 1>{});
     else
-      by_np(sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 1>, sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 2>,
-            sweep2d_kernel<GPL, MOC_EXP_RATIONAL, 3>);
+      by_exp(std::integral_constant<int, 0>{});
   };
   switch (shape_.groups_per_lane) {
     case 1: go(std::integral_constant<int, 1>{}); break;
@@ -2799,6 +3168,11 @@ FsrArgs HipSolver::fsrArgs() {
   a.ss_colsum = d_ss_colsum_;
   a.ref_flux = (!ref_flux_file_.empty() && ref_flux_n_ == (size_t)N_ * G_) ? d_ref_flux_ : nullptr;
   a.tally_i64 = nullptr;
+  a.lds_order = 0;
+  a.segout = segStore() ? d_segout_ : nullptr;
+  a.csr_off = d_csr_off_;
+  a.seg_stride = seg_stride_;
+  a.seg_plane = (size_t)seg_rows_ * seg_stride_;
   if (stackMode()) {
     a.qs = nullptr;
     a.qsub = d_qsub_;
@@ -2813,6 +3187,7 @@ FsrArgs HipSolver::fsrArgs() {
     a.lds_partials = shape_.lds_flush ? nullptr : d_lds_partials_;
     a.lds_bps = shape_.lds_flush ? 0 : shape_.lds_blocks_per_sub;
     a.tally_i64 = shape_.lds_flush ? d_tally_i64_ : nullptr;
+    a.lds_order = 1;
   } else {
     a.qs = d_qs_;
     a.qsub = nullptr;
@@ -2879,6 +3254,7 @@ void HipSolver::launchInit(Mode mode, double k0) {
   st.norm_factor = 1.0;
   st.psi_scale = 1.0;
   st.scale_cap = scale_cap_;
+  st.max_iters = device_max_iters_;
   HIP_CHECK(hipMemcpyAsync(d_state_, &st, sizeof(st), hipMemcpyHostToDevice, stream_));
   // fresh angular fluxes and tally (initializeFluxArrays, src/CPUSolver.cpp:281-380)
   hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(psi_n, 256)), dim3(256), 0, stream_, d_psi_[0], (int64_t)psi_n);
@@ -3390,6 +3766,11 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
     HIP_CHECK(hipGetLastError());
     // the fixed-point tally holds sum / V; the raw tally is sum
     HIP_CHECK(hipMemcpyAsync(tally, d_tally64_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  } else if (segStore()) {
+    hipLaunchKernelGGL(segment_tally_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, stream_, fsrArgs(),
+                       d_tally64_);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(tally, d_tally64_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
   } else {
     t32.resize(NG * shape_.tally_copies);
     HIP_CHECK(hipMemcpyAsync(t32.data(), d_tally_, NG * shape_.tally_copies * sizeof(float),
@@ -3405,6 +3786,8 @@ void HipSolver::transportSweepOnce(const double* Q, float* psi, double* tally) {
     }
     hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(2 * NG, 256)), dim3(256), 0, stream_,
                        reinterpret_cast<float*>(d_tally_i64_), (int64_t)(2 * NG));
+  } else if (segStore()) {
+    // the raw tally came back as is
   } else if (shape_.lds_groups == 0) {
     for (size_t i = 0; i < NG; i++) {
       double t = 0.0;
@@ -3563,7 +3946,8 @@ void HipSolver::shareGeometry(HipSolver& src) {
     src.geo_hold_->device = src.device_;
     src.geo_hold_->bufs = {src.d_seg_len_, src.d_seg_key_, src.d_work_,  src.d_tile_,
                            src.d_trk_,     src.d_trk_azim_, src.d_weights_, src.d_fsr_mat_,
-                           src.d_fsr_vol_, src.d_inv_vol_, src.d_out_wp_};
+                           src.d_fsr_vol_, src.d_inv_vol_, src.d_out_wp_, src.d_csr_off_,
+                           src.d_seg_slot_};
   }
   geo_hold_ = src.geo_hold_;
   d_seg_len_ = src.d_seg_len_;
@@ -3577,6 +3961,11 @@ void HipSolver::shareGeometry(HipSolver& src) {
   d_fsr_vol_ = src.d_fsr_vol_;
   d_inv_vol_ = src.d_inv_vol_;
   d_out_wp_ = src.d_out_wp_;
+  d_csr_off_ = src.d_csr_off_;
+  d_seg_slot_ = src.d_seg_slot_;
+  csr_ready_ = src.csr_ready_;
+  seg_stride_ = src.seg_stride_;
+  seg_rows_ = src.seg_rows_;
   psi_w_ = src.psi_w_;
   h_wpos_ = src.h_wpos_;
   T_ = src.T_;
@@ -3612,6 +4001,8 @@ size_t HipSolver::solveBytes() const {
              (size_t)M_ * G_ * (G_ + 4) * sizeof(double);
   if (shape_.lds_groups > 0)
     b += nsub * shape_.lds_blocks_per_sub * N_ * gb * sizeof(uint64_t) + nsub * N_ * gb * sizeof(float);
+  if (segStore())
+    b += (size_t)(linear_ ? 3 : 1) * seg_rows_ * seg_stride_ * sizeof(float) + NG * sizeof(double);
   if (scatterPrecisionInUse() != MOC_SCATTER_FP64)
     b += (size_t)N_ * (G_ + 1) * sizeof(float) +
          (size_t)M_ * ((G_ + 15) / 16) * ((G_ + 16) / 16) * 64 * 16;
@@ -3763,7 +4154,7 @@ void HipSolver::finishSolve(int max_iters) {
   old_sel_ = h_state_->old_sel;
   phi_is_old_ = true;
   // a restart counts from 1 (src/Solver.cpp:1554)
-  num_iterations_ = (h_state_->converged ? h_state_->reported_iters : max_iters) +
+  num_iterations_ = (h_state_->converged == 1 ? h_state_->reported_iters : max_iters) +
                     (restart_active_ ? 1 : 0);
   // iterations launched after convergence return at entry, so the start
   // fluxes of a following sweep are where the last live iteration put them
@@ -3991,6 +4382,358 @@ void HipSolver::getCentroids(double* out, int64_t n) {
   if (linear_ && !ls_ready_) prepareLinearSource();
   if (h_centroid_.size() != (size_t)(2 * N_)) throw DeviceError(MOC_ERR_STATE, "no centroids yet");
   std::memcpy(out, h_centroid_.data(), 2 * N_ * sizeof(double));
+}
+
+// ===========================================================================
+// SolveBatch
+// ===========================================================================
+SolveBatch::SolveBatch(HipSolver& geometry, int capacity, int res_type)
+    : geo_(&geometry), device_(geometry.device()), capacity_(capacity), res_type_(res_type) {
+  if (capacity <= 0 || capacity > 65535) throw DeviceError(MOC_ERR_INVALID, "batch capacity out of range");
+  if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
+    throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
+  geometry.requireReady();
+  if (geometry.is3d_ || geometry.linear_)
+    throw DeviceError(MOC_ERR_INVALID, "batched solves are 2D flat-source solves");
+  if (geometry.layout_tpw_ != geometry.sweepTracksPerWave())
+    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
+  HIP_CHECK(hipSetDevice(device_));
+  bytes_per_solve_ = geometry.solveBytes();
+  HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking));
+  dev_alloc(&d_table_, (size_t)capacity);
+  dev_alloc(&d_snap_, (size_t)2 * capacity);
+  HIP_CHECK(hipHostMalloc((void**)&h_snap_, (size_t)2 * capacity * sizeof(IterState), hipHostMallocDefault));
+  std::memset(h_snap_, 0, (size_t)2 * capacity * sizeof(IterState));
+  // the graphs of every row count are captured now, so a change of the
+  // number of live solves never waits for a capture
+  for (int n = 1; n <= capacity; n++) (void)graphFor(n);
+}
+
+SolveBatch::~SolveBatch() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+  for (auto& l : inflight_) (void)hipEventDestroy(l.ev);
+  for (hipEvent_t e : free_events_) (void)hipEventDestroy(e);
+  solvers_.clear();  // their stream is stream_: destroyed below
+  dev_free(d_table_);
+  dev_free(d_snap_);
+  if (h_snap_) (void)hipHostFree(h_snap_);
+  if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+size_t SolveBatch::bytesHeld() const {
+  size_t n = 0;
+  for (const auto& s : solvers_)
+    if (s) n++;
+  return n * bytes_per_solve_;
+}
+
+int SolveBatch::newSolver() {
+  auto s = std::make_unique<HipSolver>(device_);
+  s->shareGeometry(*geo_);
+  s->adoptStream(stream_);
+  for (size_t i = 0; i < solvers_.size(); i++)
+    if (!solvers_[i]) {
+      solvers_[i] = std::move(s);
+      return (int)i;
+    }
+  solvers_.push_back(std::move(s));
+  slots_.push_back(std::make_unique<BatchSlot>());
+  return (int)solvers_.size() - 1;
+}
+
+// Row arguments of solver s: both start-flux parities of the sweep, the LDS
+// tally, the scattering contraction and the per-FSR steps, as launchIteration
+// passes them for an eigenvalue iteration.
+void SolveBatch::fillRow(BatchSlot& b, HipSolver& s, double tol) const {
+  std::memset(&b, 0, sizeof(b));
+  s.sweepArgs(b.sw[0], s.d_psi_[0], s.d_psi_[1], nullptr, nullptr);
+  s.sweepArgs(b.sw[1], s.d_psi_[1], s.d_psi_[0], nullptr, nullptr);
+  if (s.shape_.lds_groups > 0) s.ldsParams(b.lds, nullptr);
+  (void)s.scatArgs(b.scat);
+  b.fsr = s.fsrArgs();
+  b.nparts_post = s.num_partials_;
+  b.nparts_upd = (int)blocks_for(s.N_ * s.G_, kReduceBlock);
+  b.num_fsrs = (double)s.N_;
+  b.norm = res_type_ == MOC_FISSION_SOURCE ? (double)s.num_fissionable_ : (double)s.N_;
+  b.tol = tol;
+  b.sigma_t_max = s.sigma_t_max_;
+}
+
+void SolveBatch::setRow(int row, const BatchSlot& b) {
+  hipLaunchKernelGGL(set_batch_slot_kernel, dim3(1), dim3(64), 0, stream_, d_table_, row, b);
+  HIP_CHECK(hipGetLastError());
+}
+
+void SolveBatch::start(void* tag, int M, int G, const double* sigma_t, const double* sigma_s,
+                       const double* nu_sigma_f, const double* chi, int max_iters, double tol,
+                       bool want_flux) {
+  if ((int)live_.size() >= capacity_) throw DeviceError(MOC_ERR_STATE, "no free row in the batch");
+  if (max_iters <= 0 || !(tol > 0.0)) throw DeviceError(MOC_ERR_INVALID, "invalid solve options");
+  HIP_CHECK(hipSetDevice(device_));
+  int si;
+  if (!idle_.empty()) {
+    si = idle_.back();
+    idle_.pop_back();
+  } else {
+    si = newSolver();
+  }
+  HipSolver& s = *solvers_[si];
+  try {
+    // the solver's previous solve (if any) has stopped: launches still in
+    // flight return at entry for its row, so its arrays can be re-armed
+    s.setMaterials(M, G, sigma_t, sigma_s, nu_sigma_f, chi);
+    s.setConvergenceThreshold(tol);
+    if (s.num_fissionable_ == 0)
+      throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
+    s.device_max_iters_ = max_iters;
+    s.launchInit(HipSolver::MODE_EIGEN, 1.0);
+    s.device_max_iters_ = 0;
+    fillRow(*slots_[si], s, tol);
+  } catch (...) {
+    s.device_max_iters_ = 0;
+    idle_.push_back(si);
+    throw;
+  }
+  setRow((int)live_.size(), *slots_[si]);
+  live_.push_back(Row{tag, next_serial_++, si, max_iters, want_flux});
+}
+
+// One eigenvalue iteration of rows 0..n-1 (launchIteration's sequence).
+void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
+  const HipSolver& g = *geo_;
+  const LaunchShape& sh = g.shape_;
+  const int G = g.G_;
+  const int64_t N = g.N_;
+  if (sh.lds_groups > 0) {
+    const int nsub = (G + sh.lds_groups - 1) / sh.lds_groups;
+    const size_t lds_bytes = g.ldsBytes(sh.lds_groups, sh.lds_q);
+    if (lds_bytes > g.lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
+    const dim3 grid((unsigned)(nsub * sh.lds_blocks_per_sub), (unsigned)n);
+    auto launch = [&](auto kernel) {
+      ensure_lds_attribute((const void*)kernel, device_, g.lds_max_);
+      hipLaunchKernelGGL(kernel, grid, dim3(sh.lds_block), lds_bytes, st, d_table_, parity);
+    };
+    auto by_np = [&](auto e, auto q) {
+      constexpr int E = decltype(e)::value, Q = decltype(q)::value;
+      switch (g.P_) {
+        case 1: launch(sweep2d_lds_batch_kernel<E, Q, 1>); break;
+        case 2: launch(sweep2d_lds_batch_kernel<E, Q, 2>); break;
+        default: launch(sweep2d_lds_batch_kernel<E, Q, 3>); break;
+      }
+    };
+    using EH = std::integral_constant<int, MOC_EXP_HW>;
+    using ER = std::integral_constant<int, MOC_EXP_RATIONAL>;
+    using Q1 = std::integral_constant<int, 1>;
+    using Q0 = std::integral_constant<int, 0>;
+    if (g.exp_mode_ == MOC_EXP_HW)
+      sh.lds_q ? by_np(EH{}, Q1{}) : by_np(EH{}, Q0{});
+    else
+      sh.lds_q ? by_np(ER{}, Q1{}) : by_np(ER{}, Q0{});
+  } else {
+    const int wpb = sh.block / 64;
+    const dim3 grid((unsigned)((g.layout_ntiles_ + wpb - 1) / wpb), (unsigned)n);
+    auto go = [&](auto gpl) {
+      constexpr int GPL = decltype(gpl)::value;
+      auto by_np = [&](auto k1, auto k2, auto k3) {
+        switch (g.P_) {
+          case 1: hipLaunchKernelGGL(k1, grid, dim3(sh.block), 0, st, d_table_, parity); break;
+          case 2: hipLaunchKernelGGL(k2, grid, dim3(sh.block), 0, st, d_table_, parity); break;
+          default: hipLaunchKernelGGL(k3, grid, dim3(sh.block), 0, st, d_table_, parity); break;
+        }
+      };
+      auto by_exp = [&](auto stv) {
+        constexpr int ST = decltype(stv)::value;
+        if (g.exp_mode_ == MOC_EXP_HW)
+          by_np(sweep2d_batch_kernel<GPL, MOC_EXP_HW, 1, ST>, sweep2d_batch_kernel<GPL, MOC_EXP_HW, 2, ST>,
+                sweep2d_batch_kernel<GPL, MOC_EXP_HW, 3, ST>);
+        else
+          by_np(sweep2d_batch_kernel<GPL, MOC_EXP_RATIONAL, 1, ST>,
+                sweep2d_batch_kernel<GPL, MOC_EXP_RATIONAL, 2, ST>,
+                sweep2d_batch_kernel<GPL, MOC_EXP_RATIONAL, 3, ST>);
+      };
+      if (g.segStore())
+        by_exp(std::integral_constant<int, 1>{});
+      else
+        by_exp(std::integral_constant<int, 0>{});
+    };
+    switch (sh.groups_per_lane) {
+      case 1: go(std::integral_constant<int, 1>{}); break;
+      case 2: go(std::integral_constant<int, 2>{}); break;
+      case 4: go(std::integral_constant<int, 4>{}); break;
+      case 9: go(std::integral_constant<int, 9>{}); break;
+      case 16: go(std::integral_constant<int, 16>{}); break;
+      default: throw DeviceError(MOC_ERR_INVALID, "unsupported groups per lane");
+    }
+  }
+  const unsigned ny = (unsigned)n;
+  hipLaunchKernelGGL(post_sweep_batch_kernel, dim3((unsigned)g.num_partials_, ny), dim3(kReduceBlock), 0,
+                     st, d_table_);
+  hipLaunchKernelGGL(keff_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
+  const int path = g.scatterPrecisionInUse();
+  if (path == MOC_SCATTER_MFMA_F16)
+    hipLaunchKernelGGL(scatter_mfma_batch_kernel, dim3(blocks_for(g.scat_ntiles_, 4), ny), dim3(256), 0,
+                       st, d_table_);
+  else if (path == MOC_SCATTER_FP32)
+    hipLaunchKernelGGL(scatter_fp32_batch_kernel, dim3(blocks_for(N * (G + 1), 256), ny), dim3(256), 0,
+                       st, d_table_);
+  hipLaunchKernelGGL(update_batch_kernel, dim3(blocks_for(N * G, kReduceBlock), ny), dim3(kReduceBlock), 0,
+                     st, d_table_, res_type_);
+  if (res_type_ != MOC_SCALAR_FLUX && G >= kFsrResidualGroups)
+    hipLaunchKernelGGL(fsr_residual_batch_kernel, dim3(blocks_for(N, kReduceBlock), ny), dim3(kReduceBlock),
+                       0, st, d_table_, res_type_);
+  hipLaunchKernelGGL(residual_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
+  HIP_CHECK(hipGetLastError());
+}
+
+// The graph of kGraphIters iterations of n rows (an even count, so every
+// row's start fluxes are back in psi[0] after it).
+hipGraphExec_t SolveBatch::graphFor(int n) {
+  auto it = graphs_.find(n);
+  if (it != graphs_.end()) return it->second;
+  {
+    // the LDS opt-in is set before the capture
+    const void* ks[] = {
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 0, 1>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 0, 2>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 0, 3>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 1, 1>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 1, 2>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_RATIONAL, 1, 3>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 0, 1>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 0, 2>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 0, 3>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 1>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 2>,
+        (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 3>};
+    for (const void* k : ks) ensure_lds_attribute(k, device_, geo_->lds_max_);
+  }
+  hipGraph_t g = nullptr;
+  HIP_CHECK(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int i = 0; i < kGraphIters; i++) launchStep(capture_stream_, n, i & 1);
+  } catch (...) {
+    (void)hipStreamEndCapture(capture_stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(capture_stream_, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) throw DeviceError(MOC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  graphs_[n] = ex;
+  return ex;
+}
+
+void SolveBatch::launchGraph() {
+  const int n = (int)live_.size();
+  hipGraphExec_t ex = graphFor(n);
+  HIP_CHECK(hipGraphLaunch(ex, stream_));
+  const int buf = next_buf_;
+  next_buf_ ^= 1;
+  IterState* d = d_snap_ + (size_t)buf * capacity_;
+  hipLaunchKernelGGL(gather_states_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, stream_,
+                     (const BatchSlot*)d_table_, n, d);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(h_snap_ + (size_t)buf * capacity_, d, (size_t)n * sizeof(IterState),
+                           hipMemcpyDeviceToHost, stream_));
+  hipEvent_t ev;
+  if (!free_events_.empty()) {
+    ev = free_events_.back();
+    free_events_.pop_back();
+  } else {
+    HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  HIP_CHECK(hipEventRecord(ev, stream_));
+  Launch l{ev, buf, {}};
+  l.rows.reserve(n);
+  for (const Row& r : live_) l.rows.emplace_back(r.serial, r.si);
+  inflight_.push_back(std::move(l));
+}
+
+// Result of row idx, whose solve stopped with state st; the last row moves
+// into its place.
+void SolveBatch::finish(size_t idx, const IterState& st, std::vector<BatchResult>& done) {
+  const Row r = live_[idx];
+  HipSolver& s = *solvers_[r.si];
+  BatchResult res;
+  res.tag = r.tag;
+  res.keff = st.k;
+  res.num_iterations = st.converged == 1 ? st.reported_iters : r.max_iters;
+  if (r.want_flux) {
+    // the normalised flux of the last live iteration; no launch writes this
+    // solver's arrays any more (its rows return at entry)
+    res.flux.resize((size_t)s.N_ * s.G_);
+    HIP_CHECK(hipMemcpy(res.flux.data(), s.d_phi_old_[st.old_sel], res.flux.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  }
+  s.keff_ = st.k;
+  s.old_sel_ = st.old_sel;
+  s.phi_is_old_ = true;
+  s.num_iterations_ = res.num_iterations;
+  const size_t last = live_.size() - 1;
+  if (idx != last) {
+    live_[idx] = live_[last];
+    setRow((int)idx, *slots_[live_[idx].si]);
+  }
+  live_.pop_back();
+  idle_.push_back(r.si);
+  done.push_back(std::move(res));
+}
+
+void SolveBatch::progress(std::vector<BatchResult>& done) {
+  HIP_CHECK(hipSetDevice(device_));
+  while (!inflight_.empty()) {
+    Launch& l = inflight_.front();
+    const hipError_t q = hipEventQuery(l.ev);
+    if (q == hipErrorNotReady) break;
+    HIP_CHECK(q);
+    const IterState* snap = h_snap_ + (size_t)l.buf * capacity_;
+    for (size_t j = 0; j < l.rows.size(); j++) {
+      if (!snap[j].converged) continue;
+      // still live (a finished row's solver may hold a newer solve by now)
+      for (size_t i = 0; i < live_.size(); i++)
+        if (live_[i].serial == l.rows[j].first) {
+          finish(i, snap[j], done);
+          break;
+        }
+    }
+    free_events_.push_back(l.ev);
+    inflight_.erase(inflight_.begin());
+  }
+  while (inflight_.size() < 2 && !live_.empty()) launchGraph();
+}
+
+void SolveBatch::drain() {
+  if (!live_.empty() || inflight_.empty()) return;
+  HIP_CHECK(hipSetDevice(device_));
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<BatchResult> none;
+  progress(none);  // pops the finished launches; no row is live, none is launched
+}
+
+size_t SolveBatch::releaseIdle() {
+  HIP_CHECK(hipSetDevice(device_));
+  size_t freed = 0;
+  std::vector<int> keep;
+  for (int si : idle_) {
+    // a launch in flight may still read the row's state
+    bool referenced = false;
+    for (const Launch& l : inflight_)
+      for (const auto& row : l.rows) referenced |= row.second == si;
+    if (referenced) {
+      keep.push_back(si);
+      continue;
+    }
+    solvers_[si].reset();
+    freed += bytes_per_solve_;
+  }
+  idle_.swap(keep);
+  return freed;
 }
 
 double deviceCopyCalibration(int device, int64_t bytes, int repeats) {

@@ -1,16 +1,16 @@
 // Continuous-batching solve engine: many eigenvalue solves in flight on one
-// GPU, each on its own stream with device-resident state, admitted as slots
-// and device memory free up.
+// GPU with device-resident state, admitted as rows and device memory free up.
 //
 // A geometry (tracks, FSRs and the launch shape) is uploaded once and shared
-// by every solve on it; a request brings only its cross sections.  Each
-// admitted request is bound to a solver from the geometry's pool (re-armed
-// in place, so serving makes no hipMalloc/hipFree), started with
-// HipSolver::startSolve and advanced by pollSolve, whose iteration batches
-// are captured HIP graphs.  Solves on different streams overlap on the
-// device; a solve's arithmetic does not depend on what else runs (integer
-// fixed-point tallies and fixed-order reductions), so its result is the same
-// alone and in the mix.
+// by every solve on it; a request brings only its cross sections.  The solves
+// of one geometry (and residual type) form a SolveBatch: every launch of an
+// iteration step carries all of them, one row each, in captured graphs of
+// iteration steps, and a row whose solve stopped is refilled with the next
+// request between graph launches.  Batches of different geometries run on
+// their own streams and overlap on the device.  A solve's arithmetic does not
+// depend on what else runs (integer fixed-point tallies and fixed-order
+// reductions on the 7-group path), so its result is the same alone and in the
+// mix.
 #include "serve.h"
 
 #include <algorithm>
@@ -57,7 +57,8 @@ int ServeEngine::addGeometry(const moc_track_data_t& tracks, int64_t num_fsrs,
                              const double* sigma_t, const double* sigma_s, const double* nu_sigma_f,
                              const double* chi) {
   auto geo = std::make_unique<Geometry>();
-  // the template solver owns the shared buffers; it is also a pooled solver
+  // the template solver owns the shared buffers; the batches' row solvers
+  // share them
   geo->tmpl = std::make_unique<HipSolver>(device_);
   geo->tmpl->setTracks(tracks);
   geo->tmpl->setMaterials(M, G, sigma_t, sigma_s, nu_sigma_f, chi);
@@ -142,73 +143,113 @@ ServeStats ServeEngine::stats() {
   std::lock_guard<std::mutex> lk(mu_);
   ServeStats s = stats_;
   s.pending = (int)pending_.size();
+  s.budget = (int64_t)budget_;
+  s.held = held_.load();
+  s.held_max = held_max_.load();
   return s;
 }
 
-// Binds a request to a pooled solver of its geometry and starts it.
-void ServeEngine::admit(const std::shared_ptr<Request>& r) {
+void ServeEngine::noteHeld() {
+  const int64_t h = (int64_t)bytesHeld();
+  held_.store(h);
+  if (h > held_max_.load()) held_max_.store(h);
+}
+
+size_t ServeEngine::bytesHeld() {
+  std::lock_guard<std::mutex> lk(geo_mu_);
+  size_t b = 0;
+  for (auto& g : geos_)
+    for (auto& kv : g->batches) b += kv.second->bytesHeld();
+  return b;
+}
+
+// Starts r in the batch of its geometry and residual type.  A row solver that
+// has to be created counts against the memory budget with every row solver
+// already held (idle ones included); idle solvers not referenced by a launch
+// in flight are freed first when that helps.  With no solve live anywhere a
+// request is started whatever the budget, so the queue always moves.
+bool ServeEngine::admit(const std::shared_ptr<Request>& r, int live) {
   Geometry* g;
   {
     std::lock_guard<std::mutex> lk(geo_mu_);
     g = geos_[r->geo].get();
   }
-  HipSolver* s = nullptr;
-  if (!g->tmpl_busy) {
-    s = g->tmpl.get();
-    g->tmpl_busy = true;
-  } else if (!g->idle.empty()) {
-    s = g->idle.back();
-    g->idle.pop_back();
-  } else {
-    g->pool.push_back(std::make_unique<HipSolver>(device_));
-    s = g->pool.back().get();
-    s->shareGeometry(*g->tmpl);
+  std::unique_ptr<SolveBatch>& b = g->batches[r->res_type];
+  if (!b) {
+    try {
+      b = std::make_unique<SolveBatch>(*g->tmpl, max_active_, r->res_type);
+    } catch (const std::exception& e) {
+      g->batches.erase(r->res_type);
+      finishRequest(r, nullptr, e.what());
+      return true;
+    }
   }
-  Active a{r, s, g};
+  if (!b->hasIdleSolver()) {
+    size_t held = bytesHeld();
+    if (held + b->bytesPerSolve() > budget_) {
+      std::lock_guard<std::mutex> lk(geo_mu_);
+      for (auto& gg : geos_)
+        for (auto& kv : gg->batches) {
+          // with nothing live, the launches still in flight only finish
+          // stopped rows: let them end so every idle solver can go
+          if (live == 0) kv.second->drain();
+          if (held + b->bytesPerSolve() > budget_) held -= kv.second->releaseIdle();
+        }
+      held_.store((int64_t)held);
+    }
+    if (held + b->bytesPerSolve() > budget_ && live > 0) return false;
+  }
+  r->t_start = now_s();
   try {
-    s->setMaterials(g->M, g->G, r->st.data(), r->ss.data(), r->nsf.data(), r->chi.data());
-    s->setConvergenceThreshold(r->tol);
-    r->t_start = now_s();
-    s->startSolve(0, r->max_iters, r->res_type, 1.0);
+    b->start(r.get(), g->M, g->G, r->st.data(), r->ss.data(), r->nsf.data(), r->chi.data(),
+             r->max_iters, r->tol, r->want_flux);
   } catch (const std::exception& e) {
-    release(a);
-    std::lock_guard<std::mutex> lk(mu_);
-    r->state = kFailed;
-    r->error = e.what();
-    stats_.failed++;
-    cv_done_.notify_all();
-    return;
+    finishRequest(r, nullptr, e.what());
+    return true;
   }
-  mem_used_ += g->bytes;
-  active_.push_back(a);
+  running_[r.get()] = r;
+  noteHeld();
+  return true;
 }
 
-void ServeEngine::release(const Active& a) {
-  if (a.solver == a.geo->tmpl.get())
-    a.geo->tmpl_busy = false;
-  else
-    a.geo->idle.push_back(a.solver);
+void ServeEngine::finishRequest(const std::shared_ptr<Request>& r, const BatchResult* res,
+                                const std::string& err) {
+  const double t = now_s();
+  std::lock_guard<std::mutex> lk(mu_);
+  r->t_done = t;
+  if (res && err.empty()) {
+    r->keff = res->keff;
+    r->iters = res->num_iterations;
+    r->flux = res->flux;
+    r->state = kDone;
+    stats_.completed++;
+    stats_.iterations += r->iters;
+  } else {
+    r->state = kFailed;
+    r->error = err;
+    stats_.failed++;
+  }
 }
 
 void ServeEngine::loop() {
   (void)hipSetDevice(device_);
+  std::vector<BatchResult> done;
   while (true) {
-    // admission: free slots and memory.  Longest first: the pending request
+    // admission: free rows and memory.  Longest first: the pending request
     // with the most work per sweep starts next, so long solves overlap the
     // short ones instead of trailing them; a request that has been passed
     // over max_active_ times is admitted before anything younger.
     std::vector<std::shared_ptr<Request>> admit_now;
+    int live = (int)running_.size();
     {
       std::unique_lock<std::mutex> lk(mu_);
-      if (active_.empty() && pending_.empty()) {
+      if (running_.empty() && pending_.empty()) {
         cv_work_.wait(lk, [&] { return stop_ || !pending_.empty(); });
       }
-      if (stop_ && pending_.empty() && active_.empty()) break;
-      size_t mem = mem_used_;
-      size_t slots = active_.size();
-      while (!pending_.empty() && slots < (size_t)max_active_) {
+      if (stop_ && pending_.empty() && running_.empty()) break;
+      int slots = live;
+      while (!pending_.empty() && slots < max_active_) {
         size_t pick = 0;
-        size_t need;
         {
           std::lock_guard<std::mutex> gl(geo_mu_);
           if (pending_.front()->passed_over < max_active_) {
@@ -221,10 +262,7 @@ void ServeEngine::loop() {
               }
             }
           }
-          need = geos_[pending_[pick]->geo]->bytes;
         }
-        if (slots > 0 && mem + need > budget_) break;  // wait for memory
-        mem += need;
         slots++;
         for (size_t i = 0; i < pick; i++) pending_[i]->passed_over++;
         admit_now.push_back(pending_[pick]);
@@ -240,57 +278,45 @@ void ServeEngine::loop() {
         cv_done_.notify_all();
       }
     }
-    for (auto& r : admit_now) admit(r);
-    // progress every active solve
-    bool finished_any = false;
-    for (size_t i = 0; i < active_.size();) {
-      Active& a = active_[i];
-      bool done = false;
-      std::string err;
-      try {
-        done = a.solver->pollSolve();
-      } catch (const std::exception& e) {
-        done = true;
-        err = e.what();
-      }
-      if (!done) {
-        i++;
-        continue;
-      }
-      std::shared_ptr<Request> r = a.req;
-      if (err.empty()) {
-        r->keff = a.solver->getKeff();
-        r->iters = a.solver->getNumIterations();
-        if (r->want_flux) {
-          r->flux.resize((size_t)a.geo->N * a.geo->G);
-          try {
-            a.solver->getFluxes(r->flux.data(), (int64_t)r->flux.size());
-          } catch (const std::exception& e) {
-            err = e.what();
-          }
-        }
-      }
-      const double t = now_s();
-      mem_used_ -= a.geo->bytes;
-      release(a);
-      active_.erase(active_.begin() + (long)i);
-      {
+    bool changed = false;
+    for (size_t i = 0; i < admit_now.size(); i++) {
+      if (!admit(admit_now[i], live)) {
+        // short of memory: the rest go back to the front of the queue, in order
         std::lock_guard<std::mutex> lk(mu_);
-        r->t_done = t;
-        if (err.empty()) {
-          r->state = kDone;
-          stats_.completed++;
-          stats_.iterations += r->iters;
-        } else {
-          r->state = kFailed;
-          r->error = err;
-          stats_.failed++;
-        }
+        pending_.insert(pending_.begin(), admit_now.begin() + (long)i, admit_now.end());
+        break;
       }
-      finished_any = true;
+      live = (int)running_.size();
+      changed = true;
     }
-    if (finished_any) cv_done_.notify_all();
-    if (!active_.empty() && !finished_any) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    // progress every batch: collect stopped solves, keep launches in flight
+    done.clear();
+    std::vector<SolveBatch*> batches;
+    {
+      std::lock_guard<std::mutex> lk(geo_mu_);
+      for (auto& g : geos_)
+        for (auto& kv : g->batches) batches.push_back(kv.second.get());
+    }
+    for (SolveBatch* b : batches) {
+      try {
+        b->progress(done);
+      } catch (const std::exception& e) {
+        // a device error: every solve of the engine is lost
+        for (auto& kv : running_) finishRequest(kv.second, nullptr, e.what());
+        running_.clear();
+        changed = true;
+        break;
+      }
+    }
+    for (const BatchResult& res : done) {
+      auto it = running_.find(res.tag);
+      if (it == running_.end()) continue;
+      finishRequest(it->second, &res, "");
+      running_.erase(it);
+      changed = true;
+    }
+    if (changed) cv_done_.notify_all();
+    if (!running_.empty() && !changed) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
 }
 

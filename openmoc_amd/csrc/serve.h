@@ -2,6 +2,7 @@
 #ifndef OPENMOC_AMD_SERVE_H_
 #define OPENMOC_AMD_SERVE_H_
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -28,6 +29,9 @@ struct ServeStats {
   int64_t failed = 0;
   int64_t iterations = 0;
   int pending = 0;
+  int64_t budget = 0;     // device bytes for the solves' own arrays
+  int64_t held = 0;       // bytes of the row solvers held now (live or idle)
+  int64_t held_max = 0;   // the largest `held` so far
 };
 
 class ServeEngine {
@@ -71,38 +75,36 @@ class ServeEngine {
     int passed_over = 0;  // admissions that went to a later request
   };
   struct Geometry {
-    std::unique_ptr<HipSolver> tmpl;                 // owns the shared buffers
-    bool tmpl_busy = false;
-    std::vector<std::unique_ptr<HipSolver>> pool;    // solvers sharing tmpl's geometry
-    std::vector<HipSolver*> idle;
+    std::unique_ptr<HipSolver> tmpl;  // owns the shared geometry buffers
+    // batches of this geometry's solves, by residual type
+    std::map<int, std::unique_ptr<SolveBatch>> batches;
     int M = 0, G = 0;
     int64_t N = 0;
-    size_t bytes = 0;                                // device bytes of one solve
-    double work = 0.0;                               // integrations per sweep
-  };
-  struct Active {
-    std::shared_ptr<Request> req;
-    HipSolver* solver;
-    Geometry* geo;
+    size_t bytes = 0;                 // device bytes of one solve
+    double work = 0.0;                // integrations per sweep
   };
   void loop();
-  void admit(const std::shared_ptr<Request>& r);
-  void release(const Active& a);
+  // Starts r in its geometry's batch; false when device memory is short.
+  bool admit(const std::shared_ptr<Request>& r, int live);
+  size_t bytesHeld();                 // all row solvers of all batches
+  void finishRequest(const std::shared_ptr<Request>& r, const BatchResult* res,
+                     const std::string& err);
 
   int device_;
   int max_active_;
   size_t budget_ = 0;
-  size_t mem_used_ = 0;  // worker thread only
   std::mutex geo_mu_;
   std::vector<std::unique_ptr<Geometry>> geos_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::deque<std::shared_ptr<Request>> pending_;
   std::map<int64_t, std::shared_ptr<Request>> all_;
-  std::vector<Active> active_;  // worker thread only
+  std::map<void*, std::shared_ptr<Request>> running_;  // worker thread only: by batch tag
   int64_t next_id_ = 1;
   bool stop_ = false;
   ServeStats stats_;
+  std::atomic<int64_t> held_{0}, held_max_{0};  // written by the worker thread
+  void noteHeld();
   std::thread worker_;
 };
 

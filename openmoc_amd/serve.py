@@ -1,9 +1,12 @@
 """Continuous-batching serving of eigenvalue solves on one MI355X.
 
 A geometry is uploaded once (``add_geometry``); each request brings its own
-cross sections and is admitted as solve slots and device memory free up.
-Solves in flight overlap on the device; a result is the same alone and in
-the mix (bitwise: integer fixed-point tallies, fixed-order reductions).
+cross sections and is admitted as batch rows and device memory free up.  The
+solves in flight of one geometry advance together, one launch per iteration
+step for all of them (captured graphs of iteration steps); batches of
+different geometries overlap on the device.  A result is the same alone and
+in the mix (bitwise on the 7-group path: integer fixed-point tallies,
+fixed-order reductions).
 There is no counterpart in the reference (its GPUSolver runs one blocking
 solve); each request follows Solver::computeEigenvalue
 (src/Solver.cpp:1542-1689).
@@ -136,3 +139,10 @@ class ServeEngine:
         check(lib().moc_engine_stats(self._h, C.byref(c), C.byref(f), C.byref(it), C.byref(p)))
         return {"completed": c.value, "failed": f.value, "iterations": it.value,
                 "pending": p.value}
+
+    def memory(self) -> dict:
+        """Device bytes of the solves: budget, held now (live or idle
+        solvers of all batches) and the most held at once."""
+        b, h, m = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().moc_engine_memory(self._h, C.byref(b), C.byref(h), C.byref(m)))
+        return {"budget": b.value, "held": h.value, "held_max": m.value}

@@ -675,7 +675,7 @@ class HipSolver:
 
     SHAPE_KEYS = ("groups_per_lane", "lanes_per_track", "tracks_per_wave", "block",
                   "tally_copies", "lds_groups", "lds_block", "lds_blocks_per_sub", "lds_q",
-                  "lds_flush")
+                  "lds_flush", "seg_store")
 
     def setLaunchShape(self, **shape) -> None:
         """Override the sweep launch shape (keys of getLaunchShape())."""

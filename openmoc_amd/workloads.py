@@ -346,14 +346,15 @@ def water_box_source() -> Problem:
                    azim_spacing=0.1, tolerance=1e-5, max_iters=500, res_type="TOTAL_SOURCE")
 
 
-def pin_cell_7g(seed: int = 1) -> Problem:
+def pin_cell_7g(seed: int = 1, num_azim: int = 32) -> Problem:
     """Single 1.26 cm pin cell, r=0.54 (3 rings x 8 sectors), moderator 8
-    sectors, reflective; 7 synthetic groups; 32 azimuthal, 0.05 cm, TY 6."""
+    sectors, reflective; 7 synthetic groups; 32 azimuthal (default), 0.05 cm,
+    TY 6."""
     rng = np.random.default_rng(seed)
     fuel = synthetic_material(rng, 7, "fuel", True)
     mod = synthetic_material(rng, 7, "moderator", False)
     return Problem("pin_cell_7g", [fuel, mod], [_pin(0, 1)], 1, 1, PITCH, PITCH,
-                   np.zeros((1, 1), np.int32), num_azim=32, azim_spacing=0.05,
+                   np.zeros((1, 1), np.int32), num_azim=num_azim, azim_spacing=0.05,
                    tolerance=1e-5, max_iters=1000)
 
 
@@ -394,18 +395,18 @@ def minicore_70g(seed: int = 3, num_azim: int = 32, spacing: float = 0.05) -> Pr
                    tolerance=1e-5, max_iters=1000)
 
 
-def pin_cell_70g(seed: int = 6) -> Problem:
+def pin_cell_70g(seed: int = 6, num_azim: int = 16) -> Problem:
     """pin_cell_7g geometry with 70 synthetic groups (dense lower-triangular
     plus band scattering): the 70-group parity case."""
     rng = np.random.default_rng(seed)
     fuel = synthetic_material(rng, 70, "fuel", True)
     mod = synthetic_material(rng, 70, "moderator", False)
     return Problem("pin_cell_70g", [fuel, mod], [_pin(0, 1)], 1, 1, PITCH, PITCH,
-                   np.zeros((1, 1), np.int32), num_azim=16, azim_spacing=0.05,
+                   np.zeros((1, 1), np.int32), num_azim=num_azim, azim_spacing=0.05,
                    tolerance=1e-5, max_iters=1000)
 
 
-def lattice_3x3_vac_7g(seed: int = 4) -> Problem:
+def lattice_3x3_vac_7g(seed: int = 4, num_azim: int = 16) -> Problem:
     """3x3 pin lattice (centre position moderator-filled), vacuum on all four
     sides, 7 synthetic groups; 16 azimuthal angles, 0.05 cm.  Leaks
     neutrons, so it exercises the neutron-balance keff and the Krylov
@@ -417,7 +418,7 @@ def lattice_3x3_vac_7g(seed: int = 4) -> Problem:
     cells = np.zeros((3, 3), np.int32)
     cells[1, 1] = 1
     return Problem("lattice_3x3_vac_7g", [fuel, mod], [_pin(0, 1), _pin(1, 1)], 3, 3,
-                   PITCH, PITCH, cells, bc=("V", "V", "V", "V"), num_azim=16,
+                   PITCH, PITCH, cells, bc=("V", "V", "V", "V"), num_azim=num_azim,
                    azim_spacing=0.05, tolerance=1e-5, max_iters=1000)
 
 

@@ -1,6 +1,7 @@
 """Serving engine (continuous batching) through the C ABI: each result equals
 the plain single solve of the same inputs, alone and in a mix, and stays
-within the parity bar of the oracle."""
+within the parity bar of the oracle; the memory budget holds across
+geometries."""
 import numpy as np
 import pytest
 
@@ -97,6 +98,57 @@ def test_memory_admission_serialises_when_budget_is_one_solve(tracks):
         res = [eng.wait(eng.submit(gid, p), timeout=120) for p in probs]
     for p, r in zip(probs, res):
         assert r.keff == plain_solve(tg, p)[0]
+
+
+def test_memory_budget_holds_across_two_geometries(tracks):
+    """A budget of one solve of each of two geometries: the solvers of both
+    batches, idle ones included, never hold more than the budget, and every
+    result still equals its plain solve."""
+    tg_pin, tg_lat = tracks("pin_cell_7g"), tracks("lattice_3x3_vac_7g")
+    with serve.ServeEngine(max_active=8) as eng:
+        g_pin, g_lat = eng.add_geometry(tg_pin), eng.add_geometry(tg_lat)
+        eng.wait(eng.submit(g_pin, perturbed("pin_cell_7g", 40)), timeout=120)
+        eng.wait(eng.submit(g_lat, perturbed("lattice_3x3_vac_7g", 41)), timeout=120)
+        one_each = eng.memory()["held"]
+    reqs = [("pin_cell_7g", 50 + i) if i % 2 else ("lattice_3x3_vac_7g", 50 + i)
+            for i in range(8)]
+    with serve.ServeEngine(max_active=8, mem_budget_bytes=one_each) as eng:
+        gids = {"pin_cell_7g": eng.add_geometry(tg_pin),
+                "lattice_3x3_vac_7g": eng.add_geometry(tg_lat)}
+        probs = [perturbed(n, seed) for n, seed in reqs]
+        ids = [eng.submit(gids[n], p) for (n, _), p in zip(reqs, probs)]
+        res = [eng.wait(i, timeout=300) for i in ids]
+        mem = eng.memory()
+        st = eng.stats()
+    assert st["failed"] == 0
+    assert mem["budget"] == one_each
+    assert 0 < mem["held_max"] <= one_each
+    for (n, _), p, r in zip(reqs, probs, res):
+        assert r.keff == plain_solve(tracks(n), p)[0]
+
+
+def test_idle_solvers_of_one_geometry_give_way_to_another(tracks):
+    """Budget of one lattice solve: pin-cell requests fill their batch with
+    solvers first; the lattice request that follows gets the memory once those
+    solvers are idle (they are freed), and the budget is never exceeded after
+    the first solver."""
+    tg_pin, tg_lat = tracks("pin_cell_7g"), tracks("lattice_3x3_vac_7g")
+    with serve.ServeEngine(max_active=4) as eng:
+        g = eng.add_geometry(tg_lat)
+        eng.wait(eng.submit(g, perturbed("lattice_3x3_vac_7g", 60)), timeout=120)
+        lat_bytes = eng.memory()["held"]
+    with serve.ServeEngine(max_active=4, mem_budget_bytes=lat_bytes) as eng:
+        g_pin, g_lat = eng.add_geometry(tg_pin), eng.add_geometry(tg_lat)
+        pins = [eng.submit(g_pin, perturbed("pin_cell_7g", 70 + i)) for i in range(4)]
+        for i in pins:
+            eng.wait(i, timeout=120)
+        held_pins = eng.memory()["held"]
+        p = perturbed("lattice_3x3_vac_7g", 61)
+        r = eng.wait(eng.submit(g_lat, p), timeout=120)
+        mem = eng.memory()
+    assert 0 < held_pins <= lat_bytes
+    assert mem["held"] <= lat_bytes
+    assert r.keff == plain_solve(tg_lat, p)[0]
 
 
 def test_submit_validation(tracks):
