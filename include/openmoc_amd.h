@@ -398,6 +398,30 @@ int moc_solver_get_timing(const moc_solver_t* solver, double* sweep_seconds,
 int moc_solver_transport_sweep(moc_solver_t* solver, const double* Q, float* psi,
                                double* tally);
 
+/* Bytes of device scratch the allocation-free sweep holds per GPU. */
+#define MOC_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+
+/* Allocation-free form of moc_solver_transport_sweep.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Q [N*G], psi_in and psi_out [T*2*(P*G)] (reference layout, as above) and
+ * tally [N*G] are DEVICE pointers; psi_out may equal psi_in.  The work is
+ * queued on `stream` (a hipStream_t; NULL: the solver's own stream) with no
+ * host synchronisation, and the results equal moc_solver_transport_sweep's
+ * bit for bit.  The call makes no hipMalloc/hipFree and no memset: the
+ * sweep's own buffers (start-flux slots, Q/sigma_t table, tally, iteration
+ * state) live in one static device array of MOC_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  MOC_ERR_INVALID when the problem
+ * needs more scratch than that (2*psi slots + tables + tally; e.g. the
+ * 17x17 lattice at 7 groups needs about 11 MB), or for the linear source. */
+int moc_solver_transport_sweep_noalloc(moc_solver_t* solver, const double* Q,
+                                       const float* psi_in, float* psi_out, double* tally,
+                                       void* stream);
+/* Scratch bytes (<= MOC_NOALLOC_SCRATCH_BYTES when it fits) that
+ * moc_solver_transport_sweep_noalloc needs for this solver's problem. */
+int moc_solver_noalloc_scratch_bytes(const moc_solver_t* solver, int64_t* bytes);
 
 /* Sweep launch shape, shape11 = {groups_per_lane, lanes_per_track,
  * tracks_per_wave, block, tally_copies, lds_groups, lds_block,

@@ -134,6 +134,8 @@ _SIGS = [
     ("moc_solver_get_timing", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),
     ("moc_solver_transport_sweep", C.c_int, [_P, _P, _P, _P]),
+    ("moc_solver_transport_sweep_noalloc", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("moc_solver_noalloc_scratch_bytes", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("moc_solver_set_launch_shape", C.c_int, [_P, _P]),
     ("moc_solver_get_launch_shape", C.c_int, [_P, _P]),
     ("moc_solver_run_iterations", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),

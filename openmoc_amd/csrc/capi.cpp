@@ -581,6 +581,18 @@ int moc_solver_transport_sweep(moc_solver_t* s, const double* Q, float* psi, dou
   MOC_SOLVER_CALL(s, s->impl->transportSweepOnce(Q, psi, tally));
 }
 
+int moc_solver_transport_sweep_noalloc(moc_solver_t* s, const double* Q, const float* psi_in,
+                                       float* psi_out, double* tally, void* stream) {
+  MOC_SOLVER_CALL(s, s->impl->transportSweepNoAlloc(Q, psi_in, psi_out, tally,
+                                                    static_cast<hipStream_t>(stream)));
+}
+
+int moc_solver_noalloc_scratch_bytes(const moc_solver_t* s, int64_t* bytes) {
+  if (!s || !s->impl || !bytes) return fail(MOC_ERR_INVALID, "null argument");
+  *bytes = (int64_t)s->impl->noAllocScratchBytes();
+  return MOC_OK;
+}
+
 int moc_solver_set_launch_shape(moc_solver_t* s, const int32_t* shape11) {
   if (!shape11) return fail(MOC_ERR_INVALID, "null argument");
   MOC_SOLVER_CALL(s, {

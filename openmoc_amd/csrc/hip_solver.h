@@ -180,6 +180,16 @@ class HipSolver {
   void timing(double* sweep_s, double* total_s, int64_t* integrations) const;
 
   void transportSweepOnce(const double* Q, float* psi, double* tally);
+  // transportSweepOnce on device buffers (Q [N*G], psi_in/psi_out in the
+  // reference layout, tally [N*G]), asynchronous on `s` (null: the solver's
+  // stream), allocating nothing: the sweep's own buffers (start-flux slots,
+  // Q/sigma_t table, fixed-point tally, iteration state) are carved from a
+  // static device array of kNoAllocScratchBytes per GPU, shared by every call
+  // on that GPU (overlapping calls on one GPU race).
+  void transportSweepNoAlloc(const double* Q, const float* psi_in, float* psi_out,
+                             double* tally, hipStream_t s);
+  // Bytes of that device array a call on this solver needs.
+  size_t noAllocScratchBytes() const;
   // Runs n eigenvalue source iterations with the stopping rule disabled
   // (benchmark form), as a solve runs them (graph batches).  Returns the mean
   // device time per iteration; *sweep_seconds (nullable) receives the mean

@@ -1884,8 +1884,145 @@ void ensure_lds_attribute(const void* kernel, int device, size_t bytes) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free single sweep: kernels over caller buffers and the static
+// scratch array
+// ---------------------------------------------------------------------------
+// Fresh iteration state of a single sweep (psi_scale 1, nothing converged)
+// and cleared bound words: bounds[0] max |Q/sigma_t| (double bits), bounds[1]
+// max |psi| (float bits).
+__global__ void noalloc_begin_kernel(IterState* st, unsigned long long* bounds, double scale_cap) {
+  IterState z = {};
+  z.k = 1.0;
+  z.psi_scale = 1.0;
+  z.scale_cap = scale_cap;
+  *st = z;
+  bounds[0] = 0ull;
+  bounds[1] = 0ull;
+}
+
+// Maxima of |Q / sigma_t| (FP64, as the host computes them) and |psi| over
+// the inputs: per-thread maxima, a wave reduction, one atomic per wave.
+__global__ __launch_bounds__(256) void noalloc_bounds_kernel(const double* Q, const int32_t* fsr_mat,
+                                                             const double* sigma_t, int64_t N, int G,
+                                                             const float* psi, int64_t psi_n,
+                                                             unsigned long long* bounds) {
+  double qm = 0.0;
+  float pm = 0.f;
+  const int64_t NG = N * G;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < NG; i += stride) {
+    const int64_t r = i / G;
+    const int g = (int)(i - r * G);
+    qm = fmax(qm, fabs(Q[i] / sigma_t[(size_t)fsr_mat[r] * G + g]));
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < psi_n; i += stride)
+    pm = fmaxf(pm, fabsf(psi[i]));
+  for (int off = 32; off > 0; off >>= 1) {
+    qm = fmax(qm, __shfl_xor(qm, off, 64));
+    pm = fmaxf(pm, __shfl_xor(pm, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    // non-negative floating-point values order like their bit patterns
+    atomicMax(&bounds[0], (unsigned long long)__double_as_longlong(qm));
+    atomicMax(&bounds[1], (unsigned long long)__float_as_uint(pm));
+  }
+}
+
+// Fixed-point tally scale from the two maxima, with the host's arithmetic
+// (transportSweepOnce).
+__global__ void noalloc_scale_kernel(IterState* st, const unsigned long long* bounds,
+                                     double sigma_t_max) {
+  const double qmax = __longlong_as_double((long long)bounds[0]);
+  const double pmax = (double)__uint_as_float((unsigned)bounds[1]);
+  st->qmax = (float)(qmax * (1 + 1e-6));
+  st->psi_max = (float)(pmax * (1 + 1e-6));
+  const double bound = 2.0 * kFourPi * sigma_t_max * ((double)st->psi_max + (double)st->qmax);
+  st->tally_scale = tally_scale_for(bound, st->scale_cap);
+}
+
+// Reference layout (dt*P*G + p*G + g) <-> start-flux slots [G][W][P] by work
+// position (work[i] = directed track at position i, -1 padding).
+__global__ void psi_to_slots_kernel(const float* psi, const int32_t* work, int64_t W, int P, int G,
+                                    float* s0, float* s1) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)G * W * P) return;
+  const int p = (int)(e % P);
+  const int64_t i = (e / P) % W;
+  const int g = (int)(e / ((int64_t)P * W));
+  const int dt = work[i];
+  if (dt < 0) return;
+  const float v = psi[(size_t)dt * P * G + (size_t)p * G + g];
+  s0[e] = v;
+  s1[e] = v;
+}
+
+__global__ void slots_to_psi_kernel(const float* s, const int32_t* work, int64_t W, int P, int G,
+                                    float* psi) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)G * W * P) return;
+  const int p = (int)(e % P);
+  const int64_t i = (e / P) % W;
+  const int g = (int)(e / ((int64_t)P * W));
+  const int dt = work[i];
+  if (dt >= 0) psi[(size_t)dt * P * G + (size_t)p * G + g] = s[e];
+}
+
+// The raw FSR tally (sum of contributions, CPUSolver::_scalar_flux after the
+// sweep) from the fixed-point tally (sum / V in units of 1/tally_scale) or
+// from the FP32 tally copies summed in copy order.
+__global__ void noalloc_tally_kernel(const unsigned long long* t64, const float* t32, int copies,
+                                     const double* vol, const IterState* st, int64_t N, int G,
+                                     double* out) {
+  const int64_t NG = N * G;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NG) return;
+  if (t64) {
+    const double v = vol[i / G];
+    out[i] = (double)(long long)t64[i] / st->tally_scale * (v < kFltEpsilon ? 0.0 : v);
+  } else {
+    double t = 0.0;
+    for (int c = 0; c < copies; c++) t += t32[(size_t)c * NG + i];
+    out[i] = t;
+  }
+}
+
 }  // namespace
 
+// Scratch memory of the allocation-free sweep (transportSweepNoAlloc): one
+// copy per GPU, held until the process ends.  Host code never uses this
+// name as an address; it uses the per-device pointer from
+// hipGetSymbolAddress (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[MOC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device.
+unsigned char* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw DeviceError(MOC_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw DeviceError(MOC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+// Byte offsets of the allocation-free sweep's buffers within the scratch
+// array (each 256-byte aligned).
+struct NoAllocLayout {
+  size_t state, bounds, tally, table, psi0, psi1, total;
+};
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
 
 // ===========================================================================
 // Host driver
@@ -4734,6 +4871,94 @@ size_t SolveBatch::releaseIdle() {
   }
   idle_.swap(keep);
   return freed;
+}
+
+size_t HipSolver::noAllocScratchBytes() const {
+  const size_t NG = (size_t)N_ * G_;
+  const bool i64 = shape_.lds_groups > 0;  // LDS and z-stack sweeps: int64 tally, qsub table
+  const int gb = std::max(1, shape_.lds_groups);
+  const size_t nsub = (size_t)(G_ + gb - 1) / gb;
+  size_t b = align256(sizeof(IterState)) + 256;
+  b += align256(i64 ? NG * sizeof(unsigned long long) : NG * shape_.tally_copies * sizeof(float));
+  b += align256(i64 ? nsub * N_ * gb * sizeof(float) : NG * sizeof(float2));
+  b += 2 * align256(psiCount() * sizeof(float));
+  return b;
+}
+
+// The sweep of transportSweepOnce on device buffers, with every buffer of
+// its own in the static scratch array: no allocation, no host
+// synchronisation, no memset (fill kernels only), so it can be captured into
+// a HIP graph.  The fixed-point scale comes from the same bounds, computed
+// with the same arithmetic on the device, so the results are the bits of
+// transportSweepOnce (the LDS sweep always flushes its slice into the int64
+// tally here; both flushes are exact integer sums).
+void HipSolver::transportSweepNoAlloc(const double* Q, const float* psi_in, float* psi_out,
+                                      double* tally, hipStream_t s) {
+  requireReady();
+  if (linear_) throw DeviceError(MOC_ERR_INVALID, "the single-sweep entry point is flat-source only");
+  if (!Q || !psi_in || !psi_out || !tally) throw DeviceError(MOC_ERR_INVALID, "null argument");
+  const size_t need = noAllocScratchBytes();
+  if (need > (size_t)MOC_NOALLOC_SCRATCH_BYTES)
+    throw DeviceError(MOC_ERR_INVALID,
+                      "problem too large for the allocation-free sweep: needs " + std::to_string(need) +
+                          " bytes of scratch, the maximum is " +
+                          std::to_string((size_t)MOC_NOALLOC_SCRATCH_BYTES));
+  if (layout_tpw_ != sweepTracksPerWave())
+    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
+  HIP_CHECK(hipSetDevice(device_));
+  unsigned char* base = noalloc_scratch(device_);
+  if (!s) s = stream_;
+  const size_t NG = (size_t)N_ * G_;
+  const bool i64 = shape_.lds_groups > 0;
+  const int gb = std::max(1, shape_.lds_groups);
+  const size_t nsub = (size_t)(G_ + gb - 1) / gb;
+  const size_t psi_n = psiCount();
+  size_t off = 0;
+  IterState* st = reinterpret_cast<IterState*>(base + off);
+  off += align256(sizeof(IterState));
+  unsigned long long* bounds = reinterpret_cast<unsigned long long*>(base + off);
+  off += 256;
+  unsigned char* tally_mem = base + off;
+  const size_t tally_bytes = i64 ? NG * sizeof(unsigned long long) : NG * shape_.tally_copies * sizeof(float);
+  off += align256(tally_bytes);
+  unsigned char* table_mem = base + off;
+  off += align256(i64 ? nsub * N_ * gb * sizeof(float) : NG * sizeof(float2));
+  float* slot0 = reinterpret_cast<float*>(base + off);
+  off += align256(psi_n * sizeof(float));
+  float* slot1 = reinterpret_cast<float*>(base + off);
+
+  SweepBuffers B;
+  B.qs = i64 ? nullptr : reinterpret_cast<const float2*>(table_mem);
+  B.qsub = i64 ? reinterpret_cast<float*>(table_mem) : nullptr;
+  B.tally = i64 ? nullptr : reinterpret_cast<float*>(tally_mem);
+  B.tally_i64 = i64 ? reinterpret_cast<unsigned long long*>(tally_mem) : nullptr;
+  B.state = st;
+
+  const int64_t ref_n = (int64_t)2 * T_ * P_ * G_;
+  hipLaunchKernelGGL(noalloc_begin_kernel, dim3(1), dim3(1), 0, s, st, bounds, scale_cap_);
+  const unsigned nb = (unsigned)std::min<int64_t>(
+      std::max<int64_t>(1, (std::max<int64_t>((int64_t)NG, ref_n) + 255) / 256), 2048);
+  hipLaunchKernelGGL(noalloc_bounds_kernel, dim3(nb), dim3(256), 0, s, Q, d_fsr_mat_, d_sigma_t_,
+                     N_, G_, psi_in, ref_n, bounds);
+  hipLaunchKernelGGL(noalloc_scale_kernel, dim3(1), dim3(1), 0, s, st, (const unsigned long long*)bounds,
+                     sigma_t_max_);
+  hipLaunchKernelGGL(make_qs_kernel, dim3(blocks_for(NG, 256)), dim3(256), 0, s, Q, d_fsr_mat_,
+                     d_sigma_t_, i64 ? nullptr : reinterpret_cast<float2*>(table_mem), B.qsub, gb, N_, G_);
+  // the tally starts at zero (what the last call left there is cleared)
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for((int64_t)(tally_bytes / 4), 256)), dim3(256), 0, s,
+                     reinterpret_cast<float*>(tally_mem), (int64_t)(tally_bytes / 4));
+  // both slot buffers get the inputs: slots no track writes keep their value
+  // (vacuum inflow), as CPUSolver::_start_flux does
+  hipLaunchKernelGGL(psi_to_slots_kernel, dim3(blocks_for((int64_t)psi_n, 256)), dim3(256), 0, s, psi_in,
+                     d_work_, psi_w_, P_, G_, slot0, slot1);
+  HIP_CHECK(hipGetLastError());
+  launchSweep(slot0, slot1, s, nullptr, nullptr, &B);
+  hipLaunchKernelGGL(noalloc_tally_kernel, dim3(blocks_for((int64_t)NG, 256)), dim3(256), 0, s,
+                     B.tally_i64, B.tally, shape_.tally_copies, d_fsr_vol_, (const IterState*)st, N_, G_,
+                     tally);
+  hipLaunchKernelGGL(slots_to_psi_kernel, dim3(blocks_for((int64_t)psi_n, 256)), dim3(256), 0, s, slot1,
+                     d_work_, psi_w_, P_, G_, psi_out);
+  HIP_CHECK(hipGetLastError());
 }
 
 double deviceCopyCalibration(int device, int64_t bytes, int repeats) {
