@@ -443,8 +443,8 @@ int moc_solver_noalloc_scratch_bytes(const moc_solver_t* solver, int64_t* bytes)
  *     (<= 256).  seg_store = 1 (default): each segment's weighted
  *     delta-psi per group is stored (no atomics) and the per-FSR step sums
  *     an FSR's segments in a fixed order in FP64, so repeated solves give the
- *     same bits (device memory: 4 * segments * 2 * ceil(G/32)*32 bytes, three
- *     times that with the linear source); seg_store = 0: device-scope float
+ *     same bits (device memory: 4 * segments * 2 * G bytes, three times that
+ *     with the linear source); seg_store = 0: device-scope float
  *     atomics into tally_copies private tallies (block b uses copy
  *     b % copies), whose order varies from run to run. */
 int moc_solver_set_launch_shape(moc_solver_t* solver, const int32_t* shape11);

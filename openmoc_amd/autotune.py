@@ -33,8 +33,17 @@ def _signature(solver) -> Tuple:
             i.num_azim, i.num_polar)
 
 
-def candidates(num_groups: int, num_cus: int = 256) -> List[dict]:
+def candidates(num_groups: int, num_cus: int = 256, is_3d: bool = False) -> List[dict]:
     out = []
+    if is_3d:
+        # z-stack sweep: one block per (track group, group subset), its
+        # sources in LDS, so only the subset width and the block size vary
+        for gb in sorted({1, 2, 3, 4, 7, min(num_groups, 64)}):
+            if gb > num_groups:
+                continue
+            for block in (512, 1024):
+                out.append(dict(lds_groups=gb, lds_q=1, lds_block=block))
+        return out
     for gb in sorted({1, 2, 3, 4, 7, min(num_groups, 64)}):
         if gb > num_groups:
             continue
@@ -68,7 +77,8 @@ def autotune(solver, repeats: int = 5, num_cus: int = 256, verbose: bool = False
         return shape, t
     best: Optional[Tuple[dict, float]] = None
     start = solver.getLaunchShape()
-    for c in (cands or candidates(solver._num_groups, num_cus)):
+    is_3d = bool(solver._tg is not None and solver._tg.info.is_3d)
+    for c in (cands or candidates(solver._num_groups, num_cus, is_3d)):
         shape = dict(start)
         shape.update(c)
         try:

@@ -413,8 +413,8 @@ class HipSolver {
   int64_t* d_csr_off_ = nullptr;       // [N+1] each FSR's rows of d_segout_ (geometry)
   int32_t* d_seg_slot_ = nullptr;      // tile-interleaved: each segment's row (geometry)
   bool csr_ready_ = false;             // d_csr_off_, d_seg_slot_ match the stream layout
-  int seg_stride_ = 0;                 // floats per row: G rounded up to 32
-  int64_t seg_rows_ = 0;               // segments + 1 (a row for the padding)
+  int seg_stride_ = 0;                 // floats per row: G
+  int64_t seg_rows_ = 0;               // stored segment passes + the padding rows
   void buildSegmentIndex();
   int num_cus_ = 0;
   size_t lds_max_ = 0;                 // max dynamic LDS per block (bytes)
@@ -430,6 +430,7 @@ class HipSolver {
   std::vector<int32_t> h_fsr_, h_trk_, h_order_;
   int layout_tpw_ = 0, layout_ntiles_ = 0;
   bool layout_stack_ = false;            // streams laid out by z-stack group (3D)
+  bool layout_shared_ = false;           // one stream for both directions of a track
   std::vector<int32_t> layout_grp_ntiles_;
   int64_t layout_rows_ = 0;
   std::vector<int32_t> mat_fissionable_;  // host copy, per material
@@ -457,6 +458,13 @@ class HipSolver {
   int64_t integrations_ = 0;
 
   // batched solves (SolveBatch)
+  // New cross sections of the same material and group counts, with nothing
+  // allocated and nothing synchronous: the tables are staged in pinned host
+  // memory and copied on the solver's stream (a re-armed row solver).
+  void rearmMaterials(const double* sigma_t, const double* sigma_s, const double* nu_sigma_f,
+                      const double* chi);
+  double* h_xs_stage_ = nullptr;       // pinned staging of rearmMaterials
+  size_t h_xs_stage_n_ = 0;
   bool stream_borrowed_ = false;       // stream_ belongs to a SolveBatch
   int device_max_iters_ = 0;           // IterState::max_iters of the next launchInit
   void adoptStream(hipStream_t s);
@@ -470,36 +478,49 @@ struct BatchResult {
   std::vector<double> flux;  // normalised FSR scalar flux [N*G] when requested
 };
 
-// Eigenvalue solves of one geometry advanced together (serving).  Every
-// launch of an iteration step carries all live solves, one row each of a
-// device table of per-solve arguments (blockIdx.y = row); the steps of
-// kGraphIters iterations are one captured graph per live-row count.  Each
-// solve stops on the device (stopping rule or iteration limit), after which
-// its rows return at entry, and a finished row is refilled with the next
-// request between graph launches (continuous batching).  Every row keeps
+// Eigenvalue solves advanced together (serving).  Every launch of an
+// iteration step carries all live solves, one row each of a device table of
+// per-solve kernel arguments (blockIdx.y = row); the steps of kGraphIters
+// iterations are one captured graph per live-row count.  The rows may belong
+// to different geometries whose kernels are the same (batchKey): each row
+// brings its own tracks, FSRs and cross sections, launches are sized for the
+// largest member, and a row's blocks beyond its own work return at entry.
+// Each solve stops on the device (stopping rule or iteration limit), after
+// which its rows return at entry, and a finished row is refilled with the
+// next request between graph launches (continuous batching).  Every row keeps
 // its angular and scalar fluxes resident; its arithmetic is that of a plain
 // HipSolver solve of the same inputs.
 class SolveBatch {
  public:
-  // geometry: a solver with tracks, FSRs and materials set; the rows share
-  // its geometry buffers and launch shape.  res_type: residual of every solve.
-  SolveBatch(HipSolver& geometry, int capacity, int res_type);
+  SolveBatch(int device, int capacity, int res_type);
   ~SolveBatch();
   SolveBatch(const SolveBatch&) = delete;
   SolveBatch& operator=(const SolveBatch&) = delete;
 
+  // The kernels (and their template arguments) a geometry's solves run; only
+  // geometries with equal keys can share a batch.  Throws for geometries a
+  // batch cannot run (3D, linear source).
+  static std::string batchKey(const HipSolver& geometry);
+  // Whether a geometry's solves are small enough to share launches with
+  // other geometries' (an LDS sweep of at most 64 blocks and 64 KB of LDS per
+  // solve); larger ones fill launches on their own.
+  static bool shareable(const HipSolver& geometry);
+  // Lets this batch run solves on `geometry` (tracks, FSRs and materials set;
+  // its rows share its geometry buffers and launch shape).
+  void addGeometry(HipSolver& geometry);
+  bool hasGeometry(const HipSolver* geometry) const;
+
   int capacity() const { return capacity_; }
   int live() const { return (int)live_.size(); }
   bool busy() const { return !live_.empty() || !inflight_.empty(); }
-  bool hasIdleSolver() const { return !idle_.empty(); }
-  size_t bytesPerSolve() const { return bytes_per_solve_; }
+  bool hasIdleSolver(const HipSolver* geometry) const;
   size_t bytesHeld() const;
-  // Starts an eigenvalue solve with these cross sections in a free row
-  // (its initial state is queued behind the launches in flight); `tag`
-  // names it in the results.  Creates a row solver when none is idle.
-  void start(void* tag, int M, int G, const double* sigma_t, const double* sigma_s,
-             const double* nu_sigma_f, const double* chi, int max_iters, double tol,
-             bool want_flux);
+  // Starts an eigenvalue solve on `geometry` with these cross sections in a
+  // free row (its initial state is queued behind the launches in flight);
+  // `tag` names it in the results.  Creates a row solver when none is idle.
+  void start(HipSolver* geometry, void* tag, int M, int G, const double* sigma_t,
+             const double* sigma_s, const double* nu_sigma_f, const double* chi, int max_iters,
+             double tol, bool want_flux);
   // Collects the solves that stopped in completed launches and keeps up to
   // two graph launches in flight.  Never waits for the device.
   void progress(std::vector<BatchResult>& done);
@@ -523,19 +544,27 @@ class SolveBatch {
     int buf;
     std::vector<std::pair<uint64_t, int>> rows;  // (serial, solver) of row j of this launch
   };
+  // Launch sizes: the largest over the member geometries.
+  struct Dims {
+    int sweep_blocks = 0, sweep_threads = 0;
+    size_t lds_bytes = 0;
+    int post_blocks = 0, upd_blocks = 0;
+    int mfma_blocks = 0, fp32_blocks = 0, fsr_blocks = 0;  // 0: no member needs the kernel
+  };
   void fillRow(BatchSlot& b, HipSolver& s, double tol) const;
   void setRow(int row, const BatchSlot& b);
   void launchStep(hipStream_t s, int n, int parity);
   hipGraphExec_t graphFor(int n);
   void launchGraph();
   void finish(size_t idx, const IterState& st, std::vector<BatchResult>& done);
-  int newSolver();
+  int newSolver(HipSolver* geometry);
 
-  HipSolver* geo_;
   int device_ = 0;
   int capacity_ = 0;
   int res_type_ = MOC_FISSION_SOURCE;
-  size_t bytes_per_solve_ = 0;
+  std::string key_;
+  std::vector<HipSolver*> geos_;       // member geometries (not owned)
+  Dims dims_;
   hipStream_t stream_ = nullptr;
   hipStream_t capture_stream_ = nullptr;
   BatchSlot* d_table_ = nullptr;       // [capacity]
@@ -544,12 +573,14 @@ class SolveBatch {
   int next_buf_ = 0;
   uint64_t next_serial_ = 1;
   std::vector<std::unique_ptr<HipSolver>> solvers_;  // null where one was released
+  std::vector<HipSolver*> solver_geo_;               // geometry of each solver
   std::vector<std::unique_ptr<BatchSlot>> slots_;    // host copy of each solver's row
   std::vector<int> idle_;              // solvers without a solve
   std::vector<Row> live_;              // row j of the table = live_[j]
   std::vector<Launch> inflight_;       // oldest first
   std::vector<hipEvent_t> free_events_;
-  std::map<int, hipGraphExec_t> graphs_;  // by live-row count
+  std::map<int, hipGraphExec_t> graphs_;    // by live-row count, for dims_
+  std::vector<hipGraphExec_t> retired_;     // graphs of smaller dims, destroyed when idle
 };
 
 }  // namespace moc

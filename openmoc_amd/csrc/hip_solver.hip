@@ -56,11 +56,17 @@ constexpr int kReduceBlock = 256;
 constexpr int kSweepBlockMax = 256;  // __launch_bounds__ of the sweep kernel
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 // Segment-stream rows: every tile's row count is a multiple of kRowPad (the
-// pipeline chunk sizes divide it), and kTailRows rows follow the last tile,
-// so the pipeline's look-ahead loads (at most 3 chunks of 4 rows past a
-// tile's end) stay inside the stream without clamping.
+// pipeline chunk sizes divide it), and kTailRows rows precede the first tile
+// and follow the last, so the pipeline's look-ahead loads (at most 3 chunks
+// of 4 rows past a tile's end, or before its start in a backward pass) stay
+// inside the stream without clamping.
 constexpr int kRowPad = 4;
 constexpr int kTailRows = 12;
+// Tile table entries are (first row, rows | kTileBackward).  Where both
+// directions of a track read one stream, a tile has two work tiles: the
+// forward pass reads its rows first to last, the backward pass last to first.
+constexpr int kTileBackward = 1 << 30;
+constexpr int kTileRowsMask = kTileBackward - 1;
 // Source iterations captured once into a graph and relaunched as a batch:
 // one launch call per batch instead of six per iteration.  The batch length
 // is even, so the ping-pong start-flux buffers return to their places.
@@ -169,7 +175,7 @@ struct SweepArgs {
   int tally_copies;         // global path: private copies (block b -> copy b % copies)
   float* segout;            // global path, per-segment store: [slots][seg_stride], or null
   const int32_t* seg_slot;  // tile-interleaved like seg_len: each segment's store row
-  int seg_stride;           // floats per store row (G rounded up to 32)
+  int seg_stride;           // floats per store row (G)
   size_t tally_stride;      // N*G
   const float* weights;     // [num_azim/2][P]
   float pf0, pf1, pf2;      // polar factors: 1/sin(theta_p) (times log2 e for HW exp)
@@ -332,7 +338,11 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   // same trip count with unconditional loads.
   constexpr int D = GPL >= 4 ? 2 : 4;
   static_assert(kRowPad % D == 0 && kTailRows >= 3 * D, "stream padding");
-  const int n = ti.y;  // multiple of kRowPad; tile is wave-uniform
+  const int n = ti.y & kTileRowsMask;  // multiple of kRowPad; tile is wave-uniform
+  // backward pass over a stream shared by both directions: chunk c is rows
+  // [n - (c+1) D, n - c D) of the tile, last row first; the tile's padding
+  // rows (length 0, exact no-ops) then come before its segments
+  const bool rev = (ti.y & kTileBackward) != 0;
   // Buffer loads: 32-bit lane offset plus a scalar row offset, bounds-checked
   // by the hardware against the stream size (out-of-range reads give 0).
   const __amdgpu_buffer_rsrc_t rs_len =
@@ -343,14 +353,19 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   const unsigned lane_off = (unsigned)ti.x * row_bytes + (unsigned)slot * 4u;
   const __amdgpu_buffer_rsrc_t rs_slot = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.seg_slot, (short)0, Tally::kStore ? (int)a.stream_bytes : 0, 0x00020000);
+  // (a backward pass's first row may lie before the tile: the wrap of the
+  // unsigned row offset cancels in the sum, which stays inside the stream)
   auto load_chunk = [&](int c, float* L, int* K, int* S) {
-    const unsigned vo = lane_off + (unsigned)(c * D) * row_bytes;
+    const int r = rev ? n - (c + 1) * D : c * D;
+    const unsigned vo = lane_off + (unsigned)r * row_bytes;
 #pragma unroll
     for (int j = 0; j < D; j++) {
-      L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, j * row_bytes, 0));
-      K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, j * row_bytes, 0);
+      const int so = (rev ? D - 1 - j : j) * (int)row_bytes;
+      L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, so, 0));
+      K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, so, 0);
+      // a shared stream's backward pass stores to the row after the forward one
       if constexpr (Tally::kStore)
-        S[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_slot, vo, j * row_bytes, 0);
+        S[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_slot, vo, so, 0) + (rev ? 1 : 0);
     }
   };
   auto fetch = [&](int key, SrcVal* q) {
@@ -583,7 +598,7 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_ls_kernel(SweepArgs a,
 #pragma unroll
     for (int p = 0; p < P; p++) psi[k][p] = ok[k] ? pin[p] * scale : 0.f;
   }
-  const int n = ti.y;
+  const int n = ti.y & kTileRowsMask;  // one stream per direction: never a backward pass
   // Two-deep pipeline: the streams of segment r + 2 and the sources of
   // segment r + 1 are loaded before segment r is integrated.  Rows past the
   // tile (the next tile's or the zero-length tail rows keyed to valid FSRs)
@@ -791,7 +806,10 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
 // Block (group, group subset) keeps the fixed-point tally and the
 // {Q/sigma_t, 1/V} table of its slots in LDS, sweeps the group's tiles, and
 // adds its tally to the global int64 tally (exact, order-independent).  The
-// segment keys of the stream are local slots | material << 24.
+// segment keys of the stream are local slots | material << 24.  Sources are
+// read from LDS only: the pipeline's look-ahead reads keys of rows past the
+// tile, which may be slots of another group, so a key is only ever used as
+// an index into the block's own LDS slice (never to address global memory).
 struct StackParams {
   const int4* grp;             // [ngrp] first tile, tiles, first slot, slots
   const int32_t* slot_fsr;     // global FSR of each slot
@@ -1039,8 +1057,10 @@ __device__ __forceinline__ double segment_sum(const FsrArgs& a, int64_t r, int g
 //   from_tally=0: phi is the current array (initial guess).
 // Per-block partials: fission rate before and after stabilisation,
 // absorption rate and vacuum leakage (computeKeff, src/CPUSolver.cpp:2258-2328).
-__device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int from_tally,
-                                                                 int balance) {
+// lds_bps: blocks per subset whose partial slices are summed (the LDS
+// sweep's launch; a.lds_bps unless a batched launch used fewer)
+__device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int from_tally, int balance,
+                                                       int lds_bps) {
   if (a.state->converged) return;
   const int G = a.G;
   const int GB = a.qsub_groups;
@@ -1074,7 +1094,7 @@ __device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int fro
     double f;
     if (from_tally) {
       if (a.lds_partials) {
-        const unsigned long long* p = a.lds_partials + (size_t)sub * a.lds_bps * nl + j;
+        const unsigned long long* p = a.lds_partials + (size_t)sub * lds_bps * nl + j;
         // exact (two's complement) sums, so any order gives the same bits;
         // sixteen independent accumulators keep sixteen loads in flight
         constexpr int kAcc = 16;
@@ -1082,11 +1102,11 @@ __device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int fro
 #pragma unroll
         for (int u = 0; u < kAcc; u++) acc[u] = 0ull;
         int b = 0;
-        for (; b + kAcc <= a.lds_bps; b += kAcc) {
+        for (; b + kAcc <= lds_bps; b += kAcc) {
 #pragma unroll
           for (int u = 0; u < kAcc; u++) acc[u] += p[(size_t)(b + u) * nl];
         }
-        for (; b < a.lds_bps; b++) acc[0] += p[(size_t)b * nl];
+        for (; b < lds_bps; b++) acc[0] += p[(size_t)b * nl];
 #pragma unroll
         for (int w = kAcc / 2; w > 0; w /= 2)
 #pragma unroll
@@ -1201,7 +1221,7 @@ __device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int fro
 
 __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int from_tally,
                                                                  int balance) {
-  post_sweep_kernel_body(a, from_tally, balance);
+  post_sweep_kernel_body(a, from_tally, balance, a.lds_bps);
 }
 
 // phase 0: initial guess normalisation (k unchanged); phase 1: computeKeff
@@ -1748,6 +1768,8 @@ struct BatchSlot {
   FsrArgs fsr;
   int nparts_post;      // post_sweep_kernel blocks: partials summed by keff_kernel
   int nparts_upd;       // update_kernel blocks: partials summed by residual_kernel
+  int nparts_fsr;       // fsr_residual_kernel blocks (0 below kFsrResidualGroups groups)
+  int scat_path;        // MOC_SCATTER_* of the contraction (FP64: no contraction launch)
   double num_fsrs;      // N (keff and normalisation)
   double norm;          // residual normaliser (FSRs, or fissionable FSRs)
   double tol;           // convergence threshold of this solve
@@ -1765,15 +1787,27 @@ __global__ __launch_bounds__(kSweepBlockMax) void sweep2d_batch_kernel(const Bat
   sweep2d_kernel_body<GPL, EXPM, NP, STORE>(t[blockIdx.y].sw[parity]);
 }
 
+// bps > 0: every row's subsets get bps blocks instead of the row's own count
+// (a launch of many rows of one geometry needs fewer blocks per row to fill
+// the device; the tally is an exact integer sum, so the bits are the same)
 template <int EXPM, int QLDS, int NP>
 __global__ __launch_bounds__(1024) void sweep2d_lds_batch_kernel(const BatchSlot* __restrict__ t,
-                                                                 int parity) {
+                                                                 int parity, int bps) {
   const BatchSlot& b = t[blockIdx.y];
-  sweep2d_lds_kernel_body<EXPM, QLDS, NP>(b.sw[parity], b.lds);
+  LdsParams L = b.lds;
+  if (bps > 0) L.blocks_per_sub = bps;
+  // rows of smaller geometries than the largest in the batch use fewer blocks
+  if ((int)blockIdx.x >= L.nsub * L.blocks_per_sub) return;
+  sweep2d_lds_kernel_body<EXPM, QLDS, NP>(b.sw[parity], L);
 }
 
-__global__ __launch_bounds__(kReduceBlock) void post_sweep_batch_kernel(const BatchSlot* __restrict__ t) {
-  post_sweep_kernel_body(t[blockIdx.y].fsr, 1, 0);
+// bps > 0: the LDS sweep of this launch used bps blocks per subset for every
+// row (see sweep2d_lds_batch_kernel)
+__global__ __launch_bounds__(kReduceBlock) void post_sweep_batch_kernel(const BatchSlot* __restrict__ t,
+                                                                       int bps) {
+  const BatchSlot& b = t[blockIdx.y];
+  if ((int)blockIdx.x >= b.nparts_post) return;  // its partials end there
+  post_sweep_kernel_body(b.fsr, 1, 0, bps > 0 ? bps : b.fsr.lds_bps);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void keff_batch_kernel(const BatchSlot* __restrict__ t) {
@@ -1782,21 +1816,29 @@ __global__ __launch_bounds__(kReduceBlock) void keff_batch_kernel(const BatchSlo
 }
 
 __global__ __launch_bounds__(256) void scatter_mfma_batch_kernel(const BatchSlot* __restrict__ t) {
-  scatter_mfma_kernel_body(t[blockIdx.y].scat);
+  const BatchSlot& b = t[blockIdx.y];
+  if (b.scat_path != MOC_SCATTER_MFMA_F16) return;
+  scatter_mfma_kernel_body(b.scat);
 }
 
 __global__ __launch_bounds__(256) void scatter_fp32_batch_kernel(const BatchSlot* __restrict__ t) {
-  scatter_fp32_kernel_body(t[blockIdx.y].scat);
+  const BatchSlot& b = t[blockIdx.y];
+  if (b.scat_path != MOC_SCATTER_FP32) return;
+  scatter_fp32_kernel_body(b.scat);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void update_batch_kernel(const BatchSlot* __restrict__ t,
                                                                    int res_type) {
-  update_kernel_body(t[blockIdx.y].fsr, 1, res_type, 1, 1, 1, kSrcTotal);
+  const BatchSlot& b = t[blockIdx.y];
+  if ((int)blockIdx.x >= b.nparts_upd) return;
+  update_kernel_body(b.fsr, 1, res_type, 1, 1, 1, kSrcTotal);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void fsr_residual_batch_kernel(const BatchSlot* __restrict__ t,
                                                                          int res_type) {
-  fsr_residual_kernel_body(t[blockIdx.y].fsr, 1, res_type);
+  const BatchSlot& b = t[blockIdx.y];
+  if ((int)blockIdx.x >= b.nparts_fsr) return;  // none below kFsrResidualGroups groups
+  fsr_residual_kernel_body(b.fsr, 1, res_type);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void residual_batch_kernel(const BatchSlot* __restrict__ t) {
@@ -2132,7 +2174,92 @@ HipSolver::~HipSolver() {
   dev_free(d_tally64_);
   dev_free(d_state_);
   if (h_state_) (void)hipHostFree(h_state_);
+  if (h_xs_stage_) (void)hipHostFree(h_xs_stage_);
   if (stream_ && !stream_borrowed_) (void)hipStreamDestroy(stream_);
+}
+
+// setMaterials for a solver that already has materials of these counts:
+// the same tables (production matrices, absorption, column sums, fissionable
+// flags), built in a pinned staging buffer and copied asynchronously on the
+// solver's stream; the matrix-core operands are rebuilt there too.  The
+// iteration arrays are kept (launchInit starts the next solve afresh).
+void HipSolver::rearmMaterials(const double* sigma_t, const double* sigma_s, const double* nu_sigma_f,
+                               const double* chi) {
+  if (!materials_set_) throw DeviceError(MOC_ERR_STATE, "materials have not been set");
+  if (!sigma_t || !sigma_s || !nu_sigma_f || !chi) throw DeviceError(MOC_ERR_INVALID, "invalid material arrays");
+  const int M = M_, G = G_;
+  const size_t MG = (size_t)M * G;
+  std::vector<int32_t> fis(M, 0);
+  double st_max = 0.0;
+  for (int m = 0; m < M; m++)
+    for (int g = 0; g < G; g++) {
+      if (!(sigma_t[(size_t)m * G + g] > 0.0)) throw DeviceError(MOC_ERR_INVALID, "sigma_t must be positive");
+      if (nu_sigma_f[(size_t)m * G + g] != 0.0) fis[m] = 1;
+      st_max = std::max(st_max, sigma_t[(size_t)m * G + g]);
+    }
+  h_sigma_t_.assign(sigma_t, sigma_t + MG);
+  h_sigma_s_fwd_.assign(sigma_s, sigma_s + MG * G);
+  h_nsf_fwd_.assign(nu_sigma_f, nu_sigma_f + MG);
+  h_chi_fwd_.assign(chi, chi + MG);
+  const bool adj = solver_mode_ == MOC_ADJOINT;
+  // staging: sigma_t | nu-sigma-f | sigma_s (dest-major) | chi | fission nsf |
+  // sigma_a | column sums | fissionable flags (as doubles' worth of int32)
+  const size_t need = MG * 7 + MG * G + ((size_t)M + 1) / 2;
+  HIP_CHECK(hipSetDevice(device_));
+  if (h_xs_stage_n_ < need) {
+    if (h_xs_stage_) HIP_CHECK(hipHostFree(h_xs_stage_));
+    h_xs_stage_ = nullptr;
+    HIP_CHECK(hipHostMalloc((void**)&h_xs_stage_, need * sizeof(double), hipHostMallocDefault));
+    h_xs_stage_n_ = need;
+  }
+  double* st = h_xs_stage_;
+  double* nsf = st + MG;
+  double* ss = nsf + MG;
+  double* ch = ss + MG * G;
+  double* fn = ch + MG;
+  double* sa = fn + MG;
+  double* cs = sa + MG;
+  int32_t* fl = reinterpret_cast<int32_t*>(cs + MG);
+  std::memcpy(st, sigma_t, MG * sizeof(double));
+  std::memcpy(nsf, nu_sigma_f, MG * sizeof(double));
+  for (int m = 0; m < M; m++)
+    for (int d = 0; d < G; d++)
+      for (int g = 0; g < G; g++)
+        ss[((size_t)m * G + d) * G + g] =
+            adj ? sigma_s[((size_t)m * G + g) * G + d] : sigma_s[((size_t)m * G + d) * G + g];
+  std::memcpy(ch, adj ? nu_sigma_f : chi, MG * sizeof(double));
+  std::memcpy(fn, adj ? chi : nu_sigma_f, MG * sizeof(double));
+  for (int m = 0; m < M; m++)
+    for (int g = 0; g < G; g++) {
+      double v = sigma_t[(size_t)m * G + g], c = 0.0;
+      for (int d = 0; d < G; d++) {
+        v -= ss[((size_t)m * G + d) * G + g];
+        c += ss[((size_t)m * G + d) * G + g];
+      }
+      sa[(size_t)m * G + g] = v;
+      cs[(size_t)m * G + g] = c;
+    }
+  std::memcpy(fl, fis.data(), M * sizeof(int32_t));
+  h_sigma_s_.assign(ss, ss + MG * G);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+  };
+  up(d_sigma_t_, st, MG * sizeof(double));
+  up(d_nsf_, nsf, MG * sizeof(double));
+  up(d_sigma_s_, ss, MG * G * sizeof(double));
+  up(d_chi_, ch, MG * sizeof(double));
+  up(d_fis_nsf_, fn, MG * sizeof(double));
+  up(d_sigma_a_, sa, MG * sizeof(double));
+  up(d_ss_colsum_, cs, MG * sizeof(double));
+  up(d_mat_fissionable_, fl, M * sizeof(int32_t));
+  mat_fissionable_ = fis;
+  sigma_t_max_ = st_max;
+  countFissionable();
+  scat_frag_ok_ = false;
+  prepareScatter();  // matrix-core operands of the new matrices, on stream_
+  graph_dirty_ = true;
+  state_valid_ = false;
+  solved_ = false;
 }
 
 // Runs this solver's launches on `s` (owned by the caller) from now on.
@@ -2192,13 +2319,22 @@ void HipSolver::pickShape() {
       // a multiple of the XCD count keeps a slot's subset blocks on one XCD
       s.lds_blocks_per_sub = (s.lds_blocks_per_sub + kNumXcd - 1) / kNumXcd * kNumXcd;
     }
+    // z-stack sweep: one group per block.  Narrow slices keep several blocks
+    // on a CU: on the 3D 17x17 lattice 1 group per block sweeps in 1.10 ms,
+    // 2 or 4 (one block per CU) in 1.43-1.45 ms.
+    if (is3d_ && ngrp_ > 0 && s.lds_groups > 0) {
+      s.lds_groups = 1;
+      s.lds_q = 1;
+    }
   }
   shape_ = s;
 }
 
 // LDS of the LDS sweep: int64 tally [N*gb] | 1/V [N] | sigma_t [M*gb] | Q/sigma_t [N*gb]
 size_t HipSolver::ldsBytes(int gb, int with_q) const {
-  if (is3d_ && ngrp_ > 0)  // z-stack sweep: tally [slots*gb] i64 | {Q/sigma_t, iv} [slots*gb] | sigma_t [M*gb]
+  // z-stack sweep (sources always in LDS): tally [slots*gb] i64 |
+  // {Q/sigma_t, iv} [slots*gb] | sigma_t [M*gb]
+  if (is3d_ && ngrp_ > 0)
     return (size_t)max_grp_slots_ * gb * (sizeof(uint64_t) + 2 * sizeof(float)) +
            (size_t)std::max(M_, 1) * gb * sizeof(float);
   return (size_t)N_ * gb * sizeof(uint64_t) +
@@ -2414,58 +2550,77 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
   allocateIterationArrays();
 }
 
-// Tile-interleaved segment streams for tiles of `tpw` directed tracks taken
-// in work order (longest first): row i of a tile holds the i-th segment, in
-// traversal order, of each of its tracks; shorter tracks are padded with
-// length-0 segments that repeat their last FSR.
+// Tile-interleaved segment streams for tiles of `tpw` tracks taken in work
+// order (longest first): row i of a tile holds the i-th segment of each of
+// its tracks; shorter tracks are padded with length-0 segments that repeat
+// their last FSR.  Flat-source sweeps read one stream for both directions of
+// a track: each stream tile has a forward and a backward work tile (the
+// backward pass reads the rows last to first), which halves the stream
+// traffic.  The linear-source sweep keeps one stream per directed track (its
+// segment entry points differ by direction).
 void HipSolver::buildTileLayout(int tpw) {
   const bool stack = stackMode();
-  if (tpw == layout_tpw_ && stack == layout_stack_) return;
+  const bool shared = !linear_;
+  if (tpw == layout_tpw_ && stack == layout_stack_ && shared == layout_shared_) return;
   if (geo_hold_)
     throw DeviceError(MOC_ERR_INVALID,
                       "shared geometry is laid out for another launch shape (same group count needed)");
   if (!(tracks_set_ && fsrs_set_)) return;
   if (N_ >= ((int64_t)1 << kFsrBits) || M_ > 256)
     throw DeviceError(MOC_ERR_INVALID, "at most 2^24 FSRs and 256 materials");
-  // work order: directed tracks longest first; for the z-stack sweep, group
-  // by group (each group's tracks longest first, its last tile padded)
+  // stream units in work order: directed tracks longest first, or with a
+  // shared stream each track once (as its forward direction); for the
+  // z-stack sweep, group by group (each group's units longest first, its
+  // last tile padded)
+  const int reps = shared ? 2 : 1;  // work tiles per stream tile
+  std::vector<int32_t> units;
+  units.reserve(h_order_.size() / reps);
+  for (int32_t dt : h_order_)
+    if (!shared || (dt & 1) == 0) units.push_back(dt);
   std::vector<int32_t> order_grp;
-  std::vector<int32_t> grp_tile0, grp_ntiles;
-  const std::vector<int32_t>* order = &h_order_;
+  std::vector<int32_t> grp_tile0, grp_ntiles;  // in work tiles
+  const std::vector<int32_t>* order = &units;
   if (stack) {
     std::vector<std::vector<int32_t>> mem(ngrp_);
-    for (int32_t dt : h_order_) mem[h_trk_grp_[dt >> 1]].push_back(dt);
+    for (int32_t u : units) mem[h_trk_grp_[u >> 1]].push_back(u);
     for (int32_t g = 0; g < ngrp_; g++) {
-      grp_tile0.push_back((int32_t)(order_grp.size() / tpw));
+      const int32_t t0 = (int32_t)(order_grp.size() / tpw);
       order_grp.insert(order_grp.end(), mem[g].begin(), mem[g].end());
       while (order_grp.size() % tpw) order_grp.push_back(-1);
-      grp_ntiles.push_back((int32_t)(order_grp.size() / tpw) - grp_tile0.back());
+      grp_tile0.push_back(t0 * reps);
+      grp_ntiles.push_back(((int32_t)(order_grp.size() / tpw) - t0) * reps);
     }
     order = &order_grp;
   }
-  const int64_t W = (int64_t)order->size();
-  const int64_t ntiles = (W + tpw - 1) / tpw;
-  if (ntiles >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "too many tiles");
+  const int64_t U = (int64_t)order->size();
+  const int64_t nst = (U + tpw - 1) / tpw;  // stream tiles
+  const int64_t ntiles = nst * reps;        // work tiles
+  if (ntiles >= (int64_t)1 << 30) throw DeviceError(MOC_ERR_INVALID, "too many tiles");
   std::vector<int32_t> work(ntiles * tpw, -1);
   std::vector<int32_t> tile(2 * ntiles);
-  int64_t rows = 0;
-  for (int64_t tl = 0; tl < ntiles; tl++) {
+  std::vector<int64_t> st_row0(nst);
+  int64_t rows = kTailRows;  // head rows: a backward pass's look-ahead before the first tile
+  for (int64_t ts = 0; ts < nst; ts++) {
     int32_t nmax = 0;
     for (int sl = 0; sl < tpw; sl++) {
-      const int64_t w = tl * tpw + sl;
-      if (w >= W) break;
-      const int32_t dt = (*order)[w];
-      if (dt < 0) continue;
-      work[tl * tpw + sl] = dt;
-      nmax = std::max(nmax, h_trk_[4 * (dt >> 1) + 1]);
+      const int64_t w = ts * tpw + sl;
+      if (w >= U) break;
+      const int32_t u = (*order)[w];
+      if (u < 0) continue;
+      for (int d = 0; d < reps; d++) work[(ts * reps + d) * tpw + sl] = u + d;
+      nmax = std::max(nmax, h_trk_[4 * (u >> 1) + 1]);
     }
     nmax = (nmax + kRowPad - 1) / kRowPad * kRowPad;
-    tile[2 * tl] = (int32_t)rows;
-    tile[2 * tl + 1] = nmax;
+    st_row0[ts] = rows;
+    for (int d = 0; d < reps; d++) {
+      tile[2 * (ts * reps + d)] = (int32_t)rows;
+      tile[2 * (ts * reps + d) + 1] = nmax | (d ? kTileBackward : 0);
+    }
     rows += nmax;
-    if (rows >= (int64_t)1 << 31) throw DeviceError(MOC_ERR_INVALID, "segment streams too large");
+    if (rows >= (int64_t)1 << 30) throw DeviceError(MOC_ERR_INVALID, "segment streams too large");
   }
-  // zero-length tail rows keyed to FSR 0 (read by the look-ahead, never integrated)
+  // zero-length head and tail rows keyed to FSR 0 (read by the look-ahead,
+  // never integrated)
   const size_t total = (size_t)(rows + kTailRows) * tpw;
   if (total * sizeof(float) >= ((size_t)1 << 31))
     throw DeviceError(MOC_ERR_INVALID, "segment streams exceed 2 GiB");
@@ -2479,16 +2634,16 @@ void HipSolver::buildTileLayout(int tpw) {
     if (!stack) return key(h_fsr_[k]);
     return (int32_t)((uint32_t)h_slot_[k] | ((uint32_t)h_fsr_mat_[h_fsr_[k]] << kFsrBits));
   };
-  for (int64_t tl = 0; tl < ntiles; tl++) {
-    const int64_t r0 = tile[2 * tl];
-    const int32_t nr = tile[2 * tl + 1];
+  for (int64_t ts = 0; ts < nst; ts++) {
+    const int64_t r0 = st_row0[ts];
+    const int32_t nr = tile[2 * ts * reps + 1] & kTileRowsMask;
     for (int sl = 0; sl < tpw; sl++) {
-      const int32_t dt = work[tl * tpw + sl];
-      if (dt < 0) continue;
-      const int32_t t = dt >> 1, dir = dt & 1;
+      const int32_t u = work[ts * reps * tpw + sl];
+      if (u < 0) continue;
+      const int32_t t = u >> 1, dir = u & 1;  // dir 0 for a shared stream
       const int32_t b = h_trk_[4 * t], n = h_trk_[4 * t + 1];
-      // padding repeats the last segment's key (the last segment in
-      // traversal order, which is segment b for a backward track)
+      // padding repeats the key of the stream's last segment (the last
+      // segment in traversal order, which is segment b for a backward track)
       const int32_t lastk = n > 0 ? (dir ? b : b + n - 1) : -1;
       const int32_t lastkey = lastk >= 0 ? seg_key(lastk) : 0;
       for (int32_t i = 0; i < nr; i++) {
@@ -2514,6 +2669,7 @@ void HipSolver::buildTileLayout(int tpw) {
   HIP_CHECK(hipMemcpy(d_tile_, tile.data(), tile.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   layout_tpw_ = tpw;
   layout_stack_ = stack;
+  layout_shared_ = shared;
   layout_ntiles_ = (int)ntiles;
   layout_rows_ = rows;
   layout_grp_ntiles_.clear();
@@ -2550,20 +2706,27 @@ void HipSolver::buildTileLayout(int tpw) {
 }
 
 // Rows of the per-segment store: FSR by FSR (a counting sort of the
-// segments by FSR, each FSR's segments in stream order), one more row at the
+// segments by FSR, each FSR's segments in stream order), then rows at the
 // end for the padding segments; csr_off[r] is FSR r's first row and the
-// seg_slot stream gives each stream position its row.  Geometry data, shared
-// with solvers that share the streams.
+// seg_slot stream gives each stream position its row.  With a stream shared
+// by both directions each position has two adjacent rows, the forward pass's
+// and, after it, the backward pass's.  Geometry data, shared with solvers
+// that share the streams.
 void HipSolver::buildSegmentIndex() {
-  seg_stride_ = (G_ + 31) / 32 * 32;  // rows of whole 128-byte lines
+  // rows packed back to back: the per-FSR sums read an FSR's rows, and the
+  // FSRs, in row order, so rows that share a cache line are read together
+  seg_stride_ = G_;
   if (csr_ready_) return;
   if (geo_hold_)
     throw DeviceError(MOC_ERR_STATE, "the shared geometry has no per-segment index for this launch shape");
   const int tpw = layout_tpw_;
+  const int per = layout_shared_ ? 2 : 1;  // rows per stream position
   auto each = [&](auto fn) {
     for (int64_t tl = 0; tl < layout_ntiles_; tl++) {
+      // a shared stream's positions are visited once, by the forward pass
+      if (h_tile_[2 * tl + 1] & kTileBackward) continue;
       const int64_t r0 = h_tile_[2 * tl];
-      const int32_t nr = h_tile_[2 * tl + 1];
+      const int32_t nr = h_tile_[2 * tl + 1] & kTileRowsMask;
       for (int32_t i = 0; i < nr; i++)
         for (int sl = 0; sl < tpw; sl++) {
           const int32_t dt = h_work_[tl * tpw + sl];
@@ -2576,14 +2739,17 @@ void HipSolver::buildSegmentIndex() {
     }
   };
   std::vector<int64_t> off((size_t)N_ + 1, 0);
-  each([&](int32_t f, size_t) { off[(size_t)f + 1]++; });
+  each([&](int32_t f, size_t) { off[(size_t)f + 1] += per; });
   for (int64_t r = 0; r < N_; r++) off[r + 1] += off[r];
-  if (off[N_] >= ((int64_t)1 << 31)) throw DeviceError(MOC_ERR_INVALID, "too many segments");
-  seg_rows_ = off[N_] + 1;
+  if (off[N_] + per >= ((int64_t)1 << 31)) throw DeviceError(MOC_ERR_INVALID, "too many segments");
+  seg_rows_ = off[N_] + per;
   const size_t total = (size_t)(layout_rows_ + kTailRows) * tpw;
-  std::vector<int32_t> slot(total, (int32_t)off[N_]);  // padding: the last row
+  std::vector<int32_t> slot(total, (int32_t)off[N_]);  // padding: the last rows
   std::vector<int64_t> cur(off.begin(), off.end() - 1);
-  each([&](int32_t f, size_t q) { slot[q] = (int32_t)cur[f]++; });
+  each([&](int32_t f, size_t q) {
+    slot[q] = (int32_t)cur[f];
+    cur[f] += per;
+  });
   dev_alloc(&d_csr_off_, off.size());
   dev_alloc(&d_seg_slot_, total);
   HIP_CHECK(hipMemcpy(d_csr_off_, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -2711,8 +2877,9 @@ void HipSolver::setSolverMode(int mode) {
     // the matrix-core operands follow the matrices in use
     if (tracks_set_ && fsrs_set_) prepareScatter();
   }
-  state_valid_ = false;
-  solved_ = false;
+  // the fluxes of the last solve stay (Solver::setSolverMode only records the
+  // mode, src/Solver.cpp:597-602): a restart in the other mode starts from
+  // them, with sources from the matrices of the new mode
   krylov_ready_ = false;
 }
 
@@ -2793,9 +2960,9 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_Qxy_, NG * 2);
     dev_alloc(&d_qls_, NG);
     dev_alloc(&d_tally_xy_, NG * 2 * shape_.tally_copies);
-    HIP_CHECK(hipMemset(d_tally_xy_, 0, NG * 2 * shape_.tally_copies * sizeof(float)));
-    HIP_CHECK(hipMemset(d_phi_xy_n_, 0, NG * 2 * sizeof(double)));
-    HIP_CHECK(hipMemset(d_phi_xy_, 0, NG * 2 * sizeof(double)));
+    HIP_CHECK(hipMemsetAsync(d_tally_xy_, 0, NG * 2 * shape_.tally_copies * sizeof(float), stream_));
+    HIP_CHECK(hipMemsetAsync(d_phi_xy_n_, 0, NG * 2 * sizeof(double), stream_));
+    HIP_CHECK(hipMemsetAsync(d_phi_xy_, 0, NG * 2 * sizeof(double), stream_));
     ls_ready_ = false;
   }
   dev_alloc(&d_Q_, NG);
@@ -2803,21 +2970,21 @@ void HipSolver::allocateIterationArrays() {
   if (stackMode()) {
     const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
     dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
-    HIP_CHECK(hipMemset(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float)));
+    HIP_CHECK(hipMemsetAsync(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float), stream_));
     dev_alloc(&d_tally_i64_, NG);
-    HIP_CHECK(hipMemset(d_tally_i64_, 0, NG * sizeof(unsigned long long)));
+    HIP_CHECK(hipMemsetAsync(d_tally_i64_, 0, NG * sizeof(unsigned long long), stream_));
   } else if (shape_.lds_groups > 0) {
     const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
     if (shape_.lds_flush) {
       dev_alloc(&d_tally_i64_, NG);
-      HIP_CHECK(hipMemset(d_tally_i64_, 0, NG * sizeof(unsigned long long)));
+      HIP_CHECK(hipMemsetAsync(d_tally_i64_, 0, NG * sizeof(unsigned long long), stream_));
     } else {
       dev_alloc(&d_lds_partials_, (size_t)nsub * shape_.lds_blocks_per_sub * N_ * shape_.lds_groups);
     }
     dev_alloc(&d_qsub_, (size_t)nsub * N_ * shape_.lds_groups);
-    HIP_CHECK(hipMemset(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float)));
+    HIP_CHECK(hipMemsetAsync(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float), stream_));
     dev_alloc(&d_tally64_, NG);
-    HIP_CHECK(hipMemset(d_tally64_, 0, NG * sizeof(double)));
+    HIP_CHECK(hipMemsetAsync(d_tally64_, 0, NG * sizeof(double), stream_));
   }
   if (segStore()) {
     buildTileLayout(sweepTracksPerWave());  // positions follow the final shape
@@ -2836,23 +3003,26 @@ void HipSolver::allocateIterationArrays() {
     dev_alloc(&d_partials_, (size_t)kNumPartials * pstride_);
   }
   buildTileLayout(sweepTracksPerWave());
-  HIP_CHECK(hipMemset(d_psi_[0], 0, psi_n * sizeof(float)));
-  HIP_CHECK(hipMemset(d_psi_[1], 0, psi_n * sizeof(float)));
-  HIP_CHECK(hipMemset(d_tally_, 0, NG * shape_.tally_copies * sizeof(float)));
-  HIP_CHECK(hipMemset(d_phi_, 0, NG * sizeof(double)));
-  HIP_CHECK(hipMemset(d_phi_old_[0], 0, NG * sizeof(double)));
-  HIP_CHECK(hipMemset(d_phi_old_[1], 0, NG * sizeof(double)));
-  HIP_CHECK(hipMemset(d_leak_, 0, (size_t)2 * T_ * G_ * sizeof(float)));
+  HIP_CHECK(hipMemsetAsync(d_psi_[0], 0, psi_n * sizeof(float), stream_));
+  HIP_CHECK(hipMemsetAsync(d_psi_[1], 0, psi_n * sizeof(float), stream_));
+  HIP_CHECK(hipMemsetAsync(d_tally_, 0, NG * shape_.tally_copies * sizeof(float), stream_));
+  HIP_CHECK(hipMemsetAsync(d_phi_, 0, NG * sizeof(double), stream_));
+  HIP_CHECK(hipMemsetAsync(d_phi_old_[0], 0, NG * sizeof(double), stream_));
+  HIP_CHECK(hipMemsetAsync(d_phi_old_[1], 0, NG * sizeof(double), stream_));
+  HIP_CHECK(hipMemsetAsync(d_leak_, 0, (size_t)2 * T_ * G_ * sizeof(float), stream_));
   phi_is_old_ = false;
   krylov_ready_ = false;
-  HIP_CHECK(hipMemset(d_Q_, 0, NG * sizeof(double)));
-  HIP_CHECK(hipMemset(d_qs_, 0, NG * sizeof(float2)));
+  HIP_CHECK(hipMemsetAsync(d_Q_, 0, NG * sizeof(double), stream_));
+  HIP_CHECK(hipMemsetAsync(d_qs_, 0, NG * sizeof(float2), stream_));
   if (fixed_on_ && h_fixed_.size() == NG) {
     dev_alloc(&d_fixed_, NG);
     HIP_CHECK(hipMemcpy(d_fixed_, h_fixed_.data(), NG * sizeof(double), hipMemcpyHostToDevice));
   }
   fixed_dirty_ = fixed_on_;
   psi_cur_ = 0;
+  // the clears run on the solver's stream, which does not wait for the
+  // default stream: they are complete before any later upload or launch
+  HIP_CHECK(hipStreamSynchronize(stream_));
   prepareScatter();
 }
 
@@ -4118,6 +4288,8 @@ void HipSolver::shareGeometry(HipSolver& src) {
   layout_tpw_ = src.layout_tpw_;
   layout_ntiles_ = src.layout_ntiles_;
   layout_rows_ = src.layout_rows_;
+  layout_shared_ = src.layout_shared_;
+  layout_stack_ = src.layout_stack_;
   h_fsr_mat_ = src.h_fsr_mat_;
   h_fsr_vol_ = src.h_fsr_vol_;
   scale_cap_ = src.scale_cap_;
@@ -4524,33 +4696,25 @@ void HipSolver::getCentroids(double* out, int64_t n) {
 // ===========================================================================
 // SolveBatch
 // ===========================================================================
-SolveBatch::SolveBatch(HipSolver& geometry, int capacity, int res_type)
-    : geo_(&geometry), device_(geometry.device()), capacity_(capacity), res_type_(res_type) {
+SolveBatch::SolveBatch(int device, int capacity, int res_type)
+    : device_(device), capacity_(capacity), res_type_(res_type) {
   if (capacity <= 0 || capacity > 65535) throw DeviceError(MOC_ERR_INVALID, "batch capacity out of range");
   if (res_type < MOC_SCALAR_FLUX || res_type > MOC_TOTAL_SOURCE)
     throw DeviceError(MOC_ERR_INVALID, "unknown residual type");
-  geometry.requireReady();
-  if (geometry.is3d_ || geometry.linear_)
-    throw DeviceError(MOC_ERR_INVALID, "batched solves are 2D flat-source solves");
-  if (geometry.layout_tpw_ != geometry.sweepTracksPerWave())
-    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
   HIP_CHECK(hipSetDevice(device_));
-  bytes_per_solve_ = geometry.solveBytes();
   HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_CHECK(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking));
   dev_alloc(&d_table_, (size_t)capacity);
   dev_alloc(&d_snap_, (size_t)2 * capacity);
   HIP_CHECK(hipHostMalloc((void**)&h_snap_, (size_t)2 * capacity * sizeof(IterState), hipHostMallocDefault));
   std::memset(h_snap_, 0, (size_t)2 * capacity * sizeof(IterState));
-  // the graphs of every row count are captured now, so a change of the
-  // number of live solves never waits for a capture
-  for (int n = 1; n <= capacity; n++) (void)graphFor(n);
 }
 
 SolveBatch::~SolveBatch() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+  for (hipGraphExec_t g : retired_) (void)hipGraphExecDestroy(g);
   for (auto& l : inflight_) (void)hipEventDestroy(l.ev);
   for (hipEvent_t e : free_events_) (void)hipEventDestroy(e);
   solvers_.clear();  // their stream is stream_: destroyed below
@@ -4561,39 +4725,125 @@ SolveBatch::~SolveBatch() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-size_t SolveBatch::bytesHeld() const {
-  size_t n = 0;
-  for (const auto& s : solvers_)
-    if (s) n++;
-  return n * bytes_per_solve_;
+std::string SolveBatch::batchKey(const HipSolver& g) {
+  g.requireReady();
+  if (g.is3d_ || g.linear_) throw DeviceError(MOC_ERR_INVALID, "batched solves are 2D flat-source solves");
+  if (g.layout_tpw_ != g.sweepTracksPerWave())
+    throw DeviceError(MOC_ERR_STATE, "segment streams are not laid out for the launch shape");
+  const LaunchShape& sh = g.shape_;
+  std::string k = "P" + std::to_string(g.P_) + ".e" + std::to_string(g.exp_mode_);
+  if (sh.lds_groups > 0)
+    k += ".lds.q" + std::to_string(sh.lds_q) + ".b" + std::to_string(sh.lds_block);
+  else
+    k += ".glob.g" + std::to_string(sh.groups_per_lane) + ".b" + std::to_string(sh.block) + ".s" +
+         std::to_string(g.segStore() ? 1 : 0);
+  return k;
 }
 
-int SolveBatch::newSolver() {
+bool SolveBatch::shareable(const HipSolver& g) {
+  const LaunchShape& sh = g.shape_;
+  if (sh.lds_groups <= 0) return false;
+  const int nsub = (g.G_ + sh.lds_groups - 1) / sh.lds_groups;
+  return nsub * sh.lds_blocks_per_sub <= 64 && g.ldsBytes(sh.lds_groups, sh.lds_q) <= ((size_t)64 << 10);
+}
+
+bool SolveBatch::hasGeometry(const HipSolver* g) const {
+  return std::find(geos_.begin(), geos_.end(), g) != geos_.end();
+}
+
+void SolveBatch::addGeometry(HipSolver& g) {
+  if (hasGeometry(&g)) return;
+  const std::string k = batchKey(g);
+  if (!geos_.empty() && k != key_)
+    throw DeviceError(MOC_ERR_INVALID, "the geometry runs other kernels than the batch's");
+  if (g.device() != device_) throw DeviceError(MOC_ERR_INVALID, "geometry on another device");
+  key_ = k;
+  geos_.push_back(&g);
+  // launch sizes: the largest over the members
+  Dims d;
+  for (const HipSolver* m : geos_) {
+    const LaunchShape& sh = m->shape_;
+    if (sh.lds_groups > 0) {
+      const int nsub = (m->G_ + sh.lds_groups - 1) / sh.lds_groups;
+      d.sweep_blocks = std::max(d.sweep_blocks, nsub * sh.lds_blocks_per_sub);
+      d.sweep_threads = sh.lds_block;
+      d.lds_bytes = std::max(d.lds_bytes, m->ldsBytes(sh.lds_groups, sh.lds_q));
+      if (d.lds_bytes > m->lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
+    } else {
+      const int wpb = sh.block / 64;
+      d.sweep_blocks = std::max(d.sweep_blocks, (m->layout_ntiles_ + wpb - 1) / wpb);
+      d.sweep_threads = sh.block;
+    }
+    d.post_blocks = std::max(d.post_blocks, m->num_partials_);
+    d.upd_blocks = std::max(d.upd_blocks, (int)blocks_for(m->N_ * m->G_, kReduceBlock));
+    const int path = m->scatterPrecisionInUse();
+    if (path == MOC_SCATTER_MFMA_F16)
+      d.mfma_blocks = std::max(d.mfma_blocks, (int)blocks_for(m->scat_ntiles_, 4));
+    else if (path == MOC_SCATTER_FP32)
+      d.fp32_blocks = std::max(d.fp32_blocks, (int)blocks_for(m->N_ * (m->G_ + 1), 256));
+    if (res_type_ != MOC_SCALAR_FLUX && m->G_ >= kFsrResidualGroups)
+      d.fsr_blocks = std::max(d.fsr_blocks, (int)blocks_for(m->N_, kReduceBlock));
+  }
+  const bool grew = d.sweep_blocks != dims_.sweep_blocks || d.lds_bytes != dims_.lds_bytes ||
+                    d.post_blocks != dims_.post_blocks || d.upd_blocks != dims_.upd_blocks ||
+                    d.mfma_blocks != dims_.mfma_blocks || d.fp32_blocks != dims_.fp32_blocks ||
+                    d.fsr_blocks != dims_.fsr_blocks || d.sweep_threads != dims_.sweep_threads;
+  dims_ = d;
+  if (grew) {
+    // graphs of the old sizes may still be in flight: kept until the batch
+    // is idle or destroyed
+    for (auto& gr : graphs_) retired_.push_back(gr.second);
+    graphs_.clear();
+    // the graphs of every row count are captured now, so a change of the
+    // number of live solves never waits for a capture
+    for (int n = 1; n <= capacity_; n++) (void)graphFor(n);
+  }
+}
+
+size_t SolveBatch::bytesHeld() const {
+  size_t b = 0;
+  for (size_t i = 0; i < solvers_.size(); i++)
+    if (solvers_[i]) b += solver_geo_[i]->solveBytes();
+  return b;
+}
+
+bool SolveBatch::hasIdleSolver(const HipSolver* g) const {
+  for (int si : idle_)
+    if (solver_geo_[si] == g) return true;
+  return false;
+}
+
+int SolveBatch::newSolver(HipSolver* g) {
   auto s = std::make_unique<HipSolver>(device_);
-  s->shareGeometry(*geo_);
+  s->shareGeometry(*g);
   s->adoptStream(stream_);
   for (size_t i = 0; i < solvers_.size(); i++)
     if (!solvers_[i]) {
       solvers_[i] = std::move(s);
+      solver_geo_[i] = g;
       return (int)i;
     }
   solvers_.push_back(std::move(s));
+  solver_geo_.push_back(g);
   slots_.push_back(std::make_unique<BatchSlot>());
   return (int)solvers_.size() - 1;
 }
 
 // Row arguments of solver s: both start-flux parities of the sweep, the LDS
 // tally, the scattering contraction and the per-FSR steps, as launchIteration
-// passes them for an eigenvalue iteration.
+// passes them for an eigenvalue iteration, with the row's own block counts.
 void SolveBatch::fillRow(BatchSlot& b, HipSolver& s, double tol) const {
   std::memset(&b, 0, sizeof(b));
   s.sweepArgs(b.sw[0], s.d_psi_[0], s.d_psi_[1], nullptr, nullptr);
   s.sweepArgs(b.sw[1], s.d_psi_[1], s.d_psi_[0], nullptr, nullptr);
   if (s.shape_.lds_groups > 0) s.ldsParams(b.lds, nullptr);
-  (void)s.scatArgs(b.scat);
+  b.scat_path = s.scatArgs(b.scat) ? s.scatterPrecisionInUse() : MOC_SCATTER_FP64;
   b.fsr = s.fsrArgs();
   b.nparts_post = s.num_partials_;
   b.nparts_upd = (int)blocks_for(s.N_ * s.G_, kReduceBlock);
+  b.nparts_fsr = (res_type_ != MOC_SCALAR_FLUX && s.G_ >= kFsrResidualGroups)
+                     ? (int)blocks_for(s.N_, kReduceBlock)
+                     : 0;
   b.num_fsrs = (double)s.N_;
   b.norm = res_type_ == MOC_FISSION_SOURCE ? (double)s.num_fissionable_ : (double)s.N_;
   b.tol = tol;
@@ -4605,24 +4855,29 @@ void SolveBatch::setRow(int row, const BatchSlot& b) {
   HIP_CHECK(hipGetLastError());
 }
 
-void SolveBatch::start(void* tag, int M, int G, const double* sigma_t, const double* sigma_s,
-                       const double* nu_sigma_f, const double* chi, int max_iters, double tol,
-                       bool want_flux) {
+void SolveBatch::start(HipSolver* g, void* tag, int M, int G, const double* sigma_t,
+                       const double* sigma_s, const double* nu_sigma_f, const double* chi,
+                       int max_iters, double tol, bool want_flux) {
+  if (!hasGeometry(g)) throw DeviceError(MOC_ERR_INVALID, "geometry not in the batch");
   if ((int)live_.size() >= capacity_) throw DeviceError(MOC_ERR_STATE, "no free row in the batch");
   if (max_iters <= 0 || !(tol > 0.0)) throw DeviceError(MOC_ERR_INVALID, "invalid solve options");
   HIP_CHECK(hipSetDevice(device_));
-  int si;
-  if (!idle_.empty()) {
-    si = idle_.back();
-    idle_.pop_back();
-  } else {
-    si = newSolver();
-  }
+  int si = -1;
+  for (size_t i = 0; i < idle_.size(); i++)
+    if (solver_geo_[idle_[i]] == g) {
+      si = idle_[i];
+      idle_.erase(idle_.begin() + (long)i);
+      break;
+    }
+  if (si < 0) si = newSolver(g);
   HipSolver& s = *solvers_[si];
   try {
     // the solver's previous solve (if any) has stopped: launches still in
     // flight return at entry for its row, so its arrays can be re-armed
-    s.setMaterials(M, G, sigma_t, sigma_s, nu_sigma_f, chi);
+    if (s.materials_set_ && s.M_ == M && s.G_ == G)
+      s.rearmMaterials(sigma_t, sigma_s, nu_sigma_f, chi);
+    else
+      s.setMaterials(M, G, sigma_t, sigma_s, nu_sigma_f, chi);
     s.setConvergenceThreshold(tol);
     if (s.num_fissionable_ == 0)
       throw DeviceError(MOC_ERR_INVALID, "eigenvalue problem without fissionable material");
@@ -4639,20 +4894,26 @@ void SolveBatch::start(void* tag, int M, int G, const double* sigma_t, const dou
   live_.push_back(Row{tag, next_serial_++, si, max_iters, want_flux});
 }
 
-// One eigenvalue iteration of rows 0..n-1 (launchIteration's sequence).
+// One eigenvalue iteration of rows 0..n-1 (launchIteration's sequence),
+// sized for the largest member geometry.
 void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
-  const HipSolver& g = *geo_;
+  const HipSolver& g = *geos_.front();  // kernels and template arguments are the members' own
   const LaunchShape& sh = g.shape_;
-  const int G = g.G_;
-  const int64_t N = g.N_;
+  // one geometry's LDS sweep: n rows share the blocks one solve would use
+  // (about two per CU), a multiple of the XCD count per subset, at least 8
+  int bps = 0;
+  dim3 sgrid((unsigned)dims_.sweep_blocks, (unsigned)n);
+  if (sh.lds_groups > 0 && geos_.size() == 1 && sh.lds_blocks_per_sub % kNumXcd == 0) {
+    const int nsub = (g.G_ + sh.lds_groups - 1) / sh.lds_groups;
+    bps = std::max(kNumXcd, (sh.lds_blocks_per_sub / n + kNumXcd - 1) / kNumXcd * kNumXcd);
+    bps = std::min(bps, sh.lds_blocks_per_sub);
+    sgrid.x = (unsigned)(nsub * bps);
+  }
+  const dim3 sblock((unsigned)dims_.sweep_threads);
   if (sh.lds_groups > 0) {
-    const int nsub = (G + sh.lds_groups - 1) / sh.lds_groups;
-    const size_t lds_bytes = g.ldsBytes(sh.lds_groups, sh.lds_q);
-    if (lds_bytes > g.lds_max_) throw DeviceError(MOC_ERR_INVALID, "LDS slice exceeds the device limit");
-    const dim3 grid((unsigned)(nsub * sh.lds_blocks_per_sub), (unsigned)n);
     auto launch = [&](auto kernel) {
       ensure_lds_attribute((const void*)kernel, device_, g.lds_max_);
-      hipLaunchKernelGGL(kernel, grid, dim3(sh.lds_block), lds_bytes, st, d_table_, parity);
+      hipLaunchKernelGGL(kernel, sgrid, sblock, dims_.lds_bytes, st, d_table_, parity, bps);
     };
     auto by_np = [&](auto e, auto q) {
       constexpr int E = decltype(e)::value, Q = decltype(q)::value;
@@ -4671,15 +4932,13 @@ void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
     else
       sh.lds_q ? by_np(ER{}, Q1{}) : by_np(ER{}, Q0{});
   } else {
-    const int wpb = sh.block / 64;
-    const dim3 grid((unsigned)((g.layout_ntiles_ + wpb - 1) / wpb), (unsigned)n);
     auto go = [&](auto gpl) {
       constexpr int GPL = decltype(gpl)::value;
       auto by_np = [&](auto k1, auto k2, auto k3) {
         switch (g.P_) {
-          case 1: hipLaunchKernelGGL(k1, grid, dim3(sh.block), 0, st, d_table_, parity); break;
-          case 2: hipLaunchKernelGGL(k2, grid, dim3(sh.block), 0, st, d_table_, parity); break;
-          default: hipLaunchKernelGGL(k3, grid, dim3(sh.block), 0, st, d_table_, parity); break;
+          case 1: hipLaunchKernelGGL(k1, sgrid, sblock, 0, st, d_table_, parity); break;
+          case 2: hipLaunchKernelGGL(k2, sgrid, sblock, 0, st, d_table_, parity); break;
+          default: hipLaunchKernelGGL(k3, sgrid, sblock, 0, st, d_table_, parity); break;
         }
       };
       auto by_exp = [&](auto stv) {
@@ -4707,20 +4966,19 @@ void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
     }
   }
   const unsigned ny = (unsigned)n;
-  hipLaunchKernelGGL(post_sweep_batch_kernel, dim3((unsigned)g.num_partials_, ny), dim3(kReduceBlock), 0,
-                     st, d_table_);
+  hipLaunchKernelGGL(post_sweep_batch_kernel, dim3((unsigned)dims_.post_blocks, ny), dim3(kReduceBlock), 0,
+                     st, d_table_, bps);
   hipLaunchKernelGGL(keff_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
-  const int path = g.scatterPrecisionInUse();
-  if (path == MOC_SCATTER_MFMA_F16)
-    hipLaunchKernelGGL(scatter_mfma_batch_kernel, dim3(blocks_for(g.scat_ntiles_, 4), ny), dim3(256), 0,
-                       st, d_table_);
-  else if (path == MOC_SCATTER_FP32)
-    hipLaunchKernelGGL(scatter_fp32_batch_kernel, dim3(blocks_for(N * (G + 1), 256), ny), dim3(256), 0,
-                       st, d_table_);
-  hipLaunchKernelGGL(update_batch_kernel, dim3(blocks_for(N * G, kReduceBlock), ny), dim3(kReduceBlock), 0,
-                     st, d_table_, res_type_);
-  if (res_type_ != MOC_SCALAR_FLUX && G >= kFsrResidualGroups)
-    hipLaunchKernelGGL(fsr_residual_batch_kernel, dim3(blocks_for(N, kReduceBlock), ny), dim3(kReduceBlock),
+  if (dims_.mfma_blocks > 0)
+    hipLaunchKernelGGL(scatter_mfma_batch_kernel, dim3((unsigned)dims_.mfma_blocks, ny), dim3(256), 0, st,
+                       d_table_);
+  if (dims_.fp32_blocks > 0)
+    hipLaunchKernelGGL(scatter_fp32_batch_kernel, dim3((unsigned)dims_.fp32_blocks, ny), dim3(256), 0, st,
+                       d_table_);
+  hipLaunchKernelGGL(update_batch_kernel, dim3((unsigned)dims_.upd_blocks, ny), dim3(kReduceBlock), 0, st,
+                     d_table_, res_type_);
+  if (dims_.fsr_blocks > 0)
+    hipLaunchKernelGGL(fsr_residual_batch_kernel, dim3((unsigned)dims_.fsr_blocks, ny), dim3(kReduceBlock),
                        0, st, d_table_, res_type_);
   hipLaunchKernelGGL(residual_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
   HIP_CHECK(hipGetLastError());
@@ -4746,7 +5004,7 @@ hipGraphExec_t SolveBatch::graphFor(int n) {
         (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 1>,
         (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 2>,
         (const void*)sweep2d_lds_batch_kernel<MOC_EXP_HW, 1, 3>};
-    for (const void* k : ks) ensure_lds_attribute(k, device_, geo_->lds_max_);
+    for (const void* k : ks) ensure_lds_attribute(k, device_, geos_.front()->lds_max_);
   }
   hipGraph_t g = nullptr;
   HIP_CHECK(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal));
@@ -4824,6 +5082,10 @@ void SolveBatch::finish(size_t idx, const IterState& st, std::vector<BatchResult
 
 void SolveBatch::progress(std::vector<BatchResult>& done) {
   HIP_CHECK(hipSetDevice(device_));
+  if (!retired_.empty() && inflight_.empty()) {
+    for (hipGraphExec_t g : retired_) (void)hipGraphExecDestroy(g);
+    retired_.clear();
+  }
   while (!inflight_.empty()) {
     Launch& l = inflight_.front();
     const hipError_t q = hipEventQuery(l.ev);
@@ -4866,8 +5128,8 @@ size_t SolveBatch::releaseIdle() {
       keep.push_back(si);
       continue;
     }
+    freed += solver_geo_[si]->solveBytes();
     solvers_[si].reset();
-    freed += bytes_per_solve_;
   }
   idle_.swap(keep);
   return freed;

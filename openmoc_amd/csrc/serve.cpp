@@ -2,11 +2,12 @@
 // GPU with device-resident state, admitted as rows and device memory free up.
 //
 // A geometry (tracks, FSRs and the launch shape) is uploaded once and shared
-// by every solve on it; a request brings only its cross sections.  The solves
-// of one geometry (and residual type) form a SolveBatch: every launch of an
-// iteration step carries all of them, one row each, in captured graphs of
-// iteration steps, and a row whose solve stopped is refilled with the next
-// request between graph launches.  Batches of different geometries run on
+// by every solve on it; a request brings only its cross sections.  Solves form
+// SolveBatches (by residual type): the solves of all small geometries that run
+// the same kernels share one batch, a large geometry has a batch of its own.
+// Every launch of an iteration step carries all solves of a batch, one row
+// each, in captured graphs of iteration steps, and a row whose solve stopped
+// is refilled with the next request between graph launches.  Batches run on
 // their own streams and overlap on the device.  A solve's arithmetic does not
 // depend on what else runs (integer fixed-point tallies and fixed-order
 // reductions on the 7-group path), so its result is the same alone and in the
@@ -68,7 +69,9 @@ int ServeEngine::addGeometry(const moc_track_data_t& tracks, int64_t num_fsrs,
   geo->N = num_fsrs;
   geo->bytes = geo->tmpl->solveBytes();
   geo->work = 2.0 * (double)tracks.num_segments * G * (tracks.polar ? 1 : tracks.num_polar / 2);
+  const std::string key = SolveBatch::batchKey(*geo->tmpl);  // throws for what batches cannot run
   std::lock_guard<std::mutex> lk(geo_mu_);
+  geo->batch_key = SolveBatch::shareable(*geo->tmpl) ? key : "geometry " + std::to_string(geos_.size());
   geos_.push_back(std::move(geo));
   return (int)geos_.size() - 1;
 }
@@ -156,14 +159,12 @@ void ServeEngine::noteHeld() {
 }
 
 size_t ServeEngine::bytesHeld() {
-  std::lock_guard<std::mutex> lk(geo_mu_);
   size_t b = 0;
-  for (auto& g : geos_)
-    for (auto& kv : g->batches) b += kv.second->bytesHeld();
+  for (auto& kv : batches_) b += kv.second->bytesHeld();
   return b;
 }
 
-// Starts r in the batch of its geometry and residual type.  A row solver that
+// Starts r in the batch of its geometry's key and residual type.  A row solver that
 // has to be created counts against the memory budget with every row solver
 // already held (idle ones included); idle solvers not referenced by a launch
 // in flight are freed first when that helps.  With no solve live anywhere a
@@ -174,35 +175,33 @@ bool ServeEngine::admit(const std::shared_ptr<Request>& r, int live) {
     std::lock_guard<std::mutex> lk(geo_mu_);
     g = geos_[r->geo].get();
   }
-  std::unique_ptr<SolveBatch>& b = g->batches[r->res_type];
-  if (!b) {
-    try {
-      b = std::make_unique<SolveBatch>(*g->tmpl, max_active_, r->res_type);
-    } catch (const std::exception& e) {
-      g->batches.erase(r->res_type);
-      finishRequest(r, nullptr, e.what());
-      return true;
-    }
+  const auto key = std::make_pair(g->batch_key, r->res_type);
+  std::unique_ptr<SolveBatch>& b = batches_[key];
+  try {
+    if (!b) b = std::make_unique<SolveBatch>(device_, max_active_, r->res_type);
+    if (!b->hasGeometry(g->tmpl.get())) b->addGeometry(*g->tmpl);
+  } catch (const std::exception& e) {
+    if (b && !b->busy() && b->bytesHeld() == 0) batches_.erase(key);
+    finishRequest(r, nullptr, e.what());
+    return true;
   }
-  if (!b->hasIdleSolver()) {
+  if (!b->hasIdleSolver(g->tmpl.get())) {
     size_t held = bytesHeld();
-    if (held + b->bytesPerSolve() > budget_) {
-      std::lock_guard<std::mutex> lk(geo_mu_);
-      for (auto& gg : geos_)
-        for (auto& kv : gg->batches) {
-          // with nothing live, the launches still in flight only finish
-          // stopped rows: let them end so every idle solver can go
-          if (live == 0) kv.second->drain();
-          if (held + b->bytesPerSolve() > budget_) held -= kv.second->releaseIdle();
-        }
+    if (held + g->bytes > budget_) {
+      for (auto& kv : batches_) {
+        // with nothing live, the launches still in flight only finish
+        // stopped rows: let them end so every idle solver can go
+        if (live == 0) kv.second->drain();
+        if (held + g->bytes > budget_) held -= kv.second->releaseIdle();
+      }
       held_.store((int64_t)held);
     }
-    if (held + b->bytesPerSolve() > budget_ && live > 0) return false;
+    if (held + g->bytes > budget_ && live > 0) return false;
   }
   r->t_start = now_s();
   try {
-    b->start(r.get(), g->M, g->G, r->st.data(), r->ss.data(), r->nsf.data(), r->chi.data(),
-             r->max_iters, r->tol, r->want_flux);
+    b->start(g->tmpl.get(), r.get(), g->M, g->G, r->st.data(), r->ss.data(), r->nsf.data(),
+             r->chi.data(), r->max_iters, r->tol, r->want_flux);
   } catch (const std::exception& e) {
     finishRequest(r, nullptr, e.what());
     return true;
@@ -291,15 +290,9 @@ void ServeEngine::loop() {
     }
     // progress every batch: collect stopped solves, keep launches in flight
     done.clear();
-    std::vector<SolveBatch*> batches;
-    {
-      std::lock_guard<std::mutex> lk(geo_mu_);
-      for (auto& g : geos_)
-        for (auto& kv : g->batches) batches.push_back(kv.second.get());
-    }
-    for (SolveBatch* b : batches) {
+    for (auto& kv : batches_) {
       try {
-        b->progress(done);
+        kv.second->progress(done);
       } catch (const std::exception& e) {
         // a device error: every solve of the engine is lost
         for (auto& kv : running_) finishRequest(kv.second, nullptr, e.what());

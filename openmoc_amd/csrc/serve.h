@@ -76,8 +76,9 @@ class ServeEngine {
   };
   struct Geometry {
     std::unique_ptr<HipSolver> tmpl;  // owns the shared geometry buffers
-    // batches of this geometry's solves, by residual type
-    std::map<int, std::unique_ptr<SolveBatch>> batches;
+    // batch of its solves (with the residual type): small geometries that
+    // run the same kernels share one, a large geometry has its own
+    std::string batch_key;
     int M = 0, G = 0;
     int64_t N = 0;
     size_t bytes = 0;                 // device bytes of one solve
@@ -100,6 +101,8 @@ class ServeEngine {
   std::deque<std::shared_ptr<Request>> pending_;
   std::map<int64_t, std::shared_ptr<Request>> all_;
   std::map<void*, std::shared_ptr<Request>> running_;  // worker thread only: by batch tag
+  // worker thread only: batches by (batch key, residual type)
+  std::map<std::pair<std::string, int>, std::unique_ptr<SolveBatch>> batches_;
   int64_t next_id_ = 1;
   bool stop_ = false;
   ServeStats stats_;

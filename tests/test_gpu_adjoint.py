@@ -34,6 +34,26 @@ def test_adjoint_matches_reference(tracks, golden, name):
     assert rel_l2(s.getFluxes(), fwd["flux"]) < 1e-4
 
 
+def test_restart_across_a_solver_mode_switch(tracks, golden):
+    """setSolverMode only records the mode (src/Solver.cpp:597-602): after a
+    forward solve, setSolverMode(ADJOINT) with setRestartStatus(True) starts
+    the adjoint solve from the forward fluxes and reaches the adjoint
+    solution; switching back and restarting reaches the forward one."""
+    name = "lattice_3x3_vac_7g"
+    adj, fwd = golden(f"{name}_adjoint"), golden(name)
+    s = om.HipSolver(tracks(name))
+    s.computeEigenvalue(1000)
+    s.setSolverMode(om.ADJOINT)
+    s.setRestartStatus(True)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - adj["keff"]) < 1e-5
+    assert rel_l2(s.getFluxes(), adj["flux"]) < 1e-4
+    s.setSolverMode(om.FORWARD)
+    s.computeEigenvalue(1000)
+    assert abs(s.getKeff() - fwd["keff"]) < 1e-5
+    assert rel_l2(s.getFluxes(), fwd["flux"]) < 1e-4
+
+
 def test_adjoint_scatter_precisions_agree(tracks):
     tg = tracks("pin_cell_70g")
     out = []
@@ -86,7 +106,12 @@ def test_chi_spectrum_initial_guess_matches_reference(tracks, golden):
     assert abs(s.getNumIterations() - int(ref["num_iterations"])) <= 1
     assert abs(s.getKeff() - ref["keff"]) < 1e-6
     assert rel_l2(s.getFluxes(), ref["flux"]) < 1e-4
-    s.setChiSpectrumMaterial("fuel")       # by name
+    # the same material by name gives the same solve, bit for bit
+    k_idx, f_idx = s.getKeff(), s.getFluxes()
+    s.setChiSpectrumMaterial("fuel")
+    s.computeEigenvalue(1000)
+    assert s.getKeff() == k_idx
+    np.testing.assert_array_equal(s.getFluxes(), f_idx)
     s.setChiSpectrumMaterial(None)         # flat guess again
     s.computeEigenvalue(1000)
     assert abs(s.getKeff() - golden("lattice_3x3_vac_7g")["keff"]) < 1e-6

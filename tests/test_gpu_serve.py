@@ -160,3 +160,30 @@ def test_submit_validation(tracks):
             eng.submit(gid + 5, (st, ss, nsf, chi))
         with pytest.raises(RuntimeError):
             eng.submit(gid, (st[:, :3], ss, nsf, chi))
+
+
+def test_small_geometries_share_one_batch_with_plain_results(tracks):
+    """Pin cells at 32 and 64 azimuthal angles and a 3x3 lattice: small
+    geometries whose solves share every launch of one batch, rows of
+    different sizes side by side; each result equals its plain solve bit for
+    bit."""
+    kinds = [("pin_cell_7g", {"num_azim": 32}), ("pin_cell_7g", {"num_azim": 64}),
+             ("lattice_3x3_vac_7g", {})]
+    tgs = [tracks(n, **kw) for n, kw in kinds]
+    reqs = [(i % 3, 200 + i) for i in range(12)]
+    probs = []
+    for k, seed in reqs:
+        n, kw = kinds[k]
+        p = workloads.get(n, **kw)
+        rng = np.random.default_rng(seed)
+        p.materials = [workloads.perturb_material(m, rng) for m in p.materials]
+        probs.append(p)
+    with serve.ServeEngine(max_active=12) as eng:
+        gids = [eng.add_geometry(tg) for tg in tgs]
+        ids = [eng.submit(gids[k], p, want_flux=True) for (k, _), p in zip(reqs, probs)]
+        res = [eng.wait(i, timeout=300) for i in ids]
+        assert eng.stats()["failed"] == 0
+    for (k, _), p, r in zip(reqs, probs, res):
+        kk, it, flux = plain_solve(tgs[k], p)
+        assert (r.keff, r.num_iterations) == (kk, it)
+        np.testing.assert_array_equal(r.flux.ravel(), flux)
