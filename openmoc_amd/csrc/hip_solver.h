@@ -409,6 +409,9 @@ class HipSolver {
   double* d_tally64_ = nullptr;        // [N*G] reduced LDS-path tally
   // per-segment store of the global-tally sweep (shape_.seg_store)
   bool segStore() const { return shape_.seg_store && shape_.lds_groups == 0 && !is3d_; }
+  unsigned long long* d_stack_partial_ = nullptr;  // z-stack sweep: [slots][G] per-slot sums
+  int64_t* d_stack_off_ = nullptr;                 // [N+1] first entry of each FSR's slot list
+  int32_t* d_stack_slots_ = nullptr;               // slots ordered by FSR
   float* d_segout_ = nullptr;          // [planes][seg_rows_][seg_stride_]
   int64_t* d_csr_off_ = nullptr;       // [N+1] each FSR's rows of d_segout_ (geometry)
   int32_t* d_seg_slot_ = nullptr;      // tile-interleaved: each segment's row (geometry)

@@ -353,21 +353,6 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
   const unsigned lane_off = (unsigned)ti.x * row_bytes + (unsigned)slot * 4u;
   const __amdgpu_buffer_rsrc_t rs_slot = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.seg_slot, (short)0, Tally::kStore ? (int)a.stream_bytes : 0, 0x00020000);
-  // (a backward pass's first row may lie before the tile: the wrap of the
-  // unsigned row offset cancels in the sum, which stays inside the stream)
-  auto load_chunk = [&](int c, float* L, int* K, int* S) {
-    const int r = rev ? n - (c + 1) * D : c * D;
-    const unsigned vo = lane_off + (unsigned)r * row_bytes;
-#pragma unroll
-    for (int j = 0; j < D; j++) {
-      const int so = (rev ? D - 1 - j : j) * (int)row_bytes;
-      L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, so, 0));
-      K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, so, 0);
-      // a shared stream's backward pass stores to the row after the forward one
-      if constexpr (Tally::kStore)
-        S[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_slot, vo, so, 0) + (rev ? 1 : 0);
-    }
-  };
   auto fetch = [&](int key, SrcVal* q) {
 #pragma unroll
     for (int k = 0; k < GPL; k++) q[k] = src.load(key, g[k], loc[k]);
@@ -423,6 +408,25 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
     }
   };
 
+  // The pipeline, with the row order a compile-time choice (a run-time one
+  // makes the compiler split the steady-state loop into branches).  A
+  // backward pass's first row may lie before the tile: the wrap of the
+  // unsigned row offset cancels in the sum, which stays inside the stream.
+  auto pipeline = [&](auto rev_tag) {
+  constexpr bool REV = decltype(rev_tag)::value;
+  auto load_chunk = [&](int c, float* L, int* K, int* S) {
+    const int r = REV ? n - (c + 1) * D : c * D;
+    const unsigned vo = lane_off + (unsigned)r * row_bytes;
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      const int so = (REV ? D - 1 - j : j) * (int)row_bytes;
+      L[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_len, vo, so, 0));
+      K[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_key, vo, so, 0);
+      // a shared stream's backward pass stores to the row after the forward one
+      if constexpr (Tally::kStore)
+        S[j] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_slot, vo, so, 0) + (REV ? 1 : 0);
+    }
+  };
   if (n > 0) {
     const int nch = n / D;
     // three rotating sets: chunk c lives in set c % 3 (streams and sources)
@@ -458,6 +462,11 @@ __device__ __forceinline__ void sweep_track(const SweepArgs& a, int tile, int sl
     if (c < nch) process(L0, K0, S0, K1, Q0, Q1);
     if (c + 1 < nch) process(L1, K1, S1, K2, Q1, Q2);
   }
+  };
+  if (rev)
+    pipeline(std::true_type{});
+  else
+    pipeline(std::false_type{});
 
   const int out = a.out_wp[dt];
   float amax = 0.f;
@@ -813,7 +822,8 @@ __global__ __launch_bounds__(1024) void sweep2d_lds_kernel(SweepArgs a, LdsParam
 struct StackParams {
   const int4* grp;             // [ngrp] first tile, tiles, first slot, slots
   const int32_t* slot_fsr;     // global FSR of each slot
-  unsigned long long* tally;   // [N*G] fixed-point sums of contribution / V
+  unsigned long long* tally;   // [N*G] fixed-point sums of contribution / V (atomic flush)
+  unsigned long long* partial; // [slots][G] each slot's sums, written once per sweep, or null
   const float* qsub;           // [nsub][N][GB] Q/sigma_t, subset-major
   const float* inv_vol;        // [N]
   const double* sigma_t;       // [M][G]
@@ -870,12 +880,48 @@ __global__ __launch_bounds__(1024) void sweep3d_stack_kernel(SweepArgs a, StackP
     }
   }
   __syncthreads();
+  if (L.partial) {
+    // every (slot, group) of the sweep has exactly one block: its sums are
+    // written, zero or not, and stack_reduce_kernel adds an FSR's slots
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+      const int s = i / L.GB, l = i - s * L.GB;
+      if (l < gb) L.partial[(size_t)(gi.z + s) * a.G + g0 + l] = lds_tally[i];
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int s = i / L.GB, l = i - s * L.GB;
     const unsigned long long v = lds_tally[i];
     if (l < gb && v != 0ull)
       atomicAdd(&L.tally[(size_t)L.slot_fsr[gi.z + s] * a.G + g0 + l], v);
   }
+}
+
+// tally[r][g] = sum over FSR r's z-stack slots of their sums (an exact
+// integer sum, so the bits equal those of the atomic flush).  off [N+1] and
+// slots list each FSR's slots; reads are one slot row of G sums per step.
+__global__ void stack_reduce_kernel(const unsigned long long* __restrict__ partial,
+                                    const int64_t* __restrict__ off, const int32_t* __restrict__ slots,
+                                    unsigned long long* __restrict__ tally, int64_t N, int G,
+                                    const IterState* state) {
+  if (state->converged) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N * G) return;
+  const int64_t r = i / G;
+  const int g = (int)(i - r * G);
+  // four independent loads in flight (integer sums: the order is free)
+  unsigned long long t0 = 0ull, t1 = 0ull, t2 = 0ull, t3 = 0ull;
+  int64_t q = off[r];
+  const int64_t e = off[r + 1];
+  for (; q + 4 <= e; q += 4) {
+    const int s0 = slots[q], s1 = slots[q + 1], s2 = slots[q + 2], s3 = slots[q + 3];
+    t0 += partial[(size_t)s0 * G + g];
+    t1 += partial[(size_t)s1 * G + g];
+    t2 += partial[(size_t)s2 * G + g];
+    t3 += partial[(size_t)s3 * G + g];
+  }
+  for (; q < e; q++) t0 += partial[(size_t)slots[q] * G + g];
+  tally[i] = (t0 + t1) + (t2 + t3);
 }
 
 // tally64[r*G + g] = sum over the subset's blocks of partials (fixed order).
@@ -2149,6 +2195,9 @@ HipSolver::~HipSolver() {
   dev_free(d_grp_);
   dev_free(d_slot_fsr_);
   dev_free(d_tally_i64_);
+  dev_free(d_stack_partial_);
+  dev_free(d_stack_off_);
+  dev_free(d_stack_slots_);
   dev_free(d_scat_rows_);
   dev_free(d_scat_tile_mat_);
   dev_free(d_scat_b_);
@@ -2283,6 +2332,20 @@ void HipSolver::pickShape() {
     }
   }
   if (s.groups_per_lane == 0) throw DeviceError(MOC_ERR_INVALID, "too many energy groups (max 1024)");
+  // Up to 4 groups per lane, the count that keeps the most lanes busy: at 70
+  // groups 2 per lane fill 35 lanes of a wave (one track), 4 per lane 54 of
+  // them (three tracks of 18 lanes); measured on the 70-group mini-core the
+  // flat sweep takes 1.52 ms instead of 1.72 ms, the linear-source one 4.38
+  // instead of 4.47.  More groups per lane hold more registers and measured
+  // slower (9: 1.82 ms).
+  {
+    auto busy = [&](int gpl) {
+      const int lpt = (G_ + gpl - 1) / gpl;
+      return lpt > 64 ? 0.0 : (double)G_ / (lpt * gpl) * (64 / lpt) * lpt / 64.0;
+    };
+    for (int gpl : {2, 4})
+      if (gpl > s.groups_per_lane && busy(gpl) > busy(s.groups_per_lane) + 0.05) s.groups_per_lane = gpl;
+  }
   s.lanes_per_track = (G_ + s.groups_per_lane - 1) / s.groups_per_lane;
   s.tracks_per_wave = 64 / s.lanes_per_track;
   s.block = 256;
@@ -2973,6 +3036,25 @@ void HipSolver::allocateIterationArrays() {
     HIP_CHECK(hipMemsetAsync(d_qsub_, 0, (size_t)nsub * N_ * shape_.lds_groups * sizeof(float), stream_));
     dev_alloc(&d_tally_i64_, NG);
     HIP_CHECK(hipMemsetAsync(d_tally_i64_, 0, NG * sizeof(unsigned long long), stream_));
+    // per-slot sums of the z-stack sweep and each FSR's slots (a counting
+    // sort of the slots by FSR)
+    const size_t nslots = h_slot_fsr_.size();
+    std::vector<int64_t> off((size_t)N_ + 1, 0);
+    for (int32_t f : h_slot_fsr_) {
+      if (f < 0 || f >= N_) throw DeviceError(MOC_ERR_INVALID, "segment FSR id exceeds the number of FSRs");
+      off[(size_t)f + 1]++;
+    }
+    for (int64_t r = 0; r < N_; r++) off[r + 1] += off[r];
+    std::vector<int32_t> slots(std::max<size_t>(nslots, 1), 0);
+    {
+      std::vector<int64_t> cur(off.begin(), off.end() - 1);
+      for (size_t q = 0; q < nslots; q++) slots[cur[h_slot_fsr_[q]]++] = (int32_t)q;
+    }
+    dev_alloc(&d_stack_partial_, std::max<size_t>(nslots, 1) * G_);
+    dev_alloc(&d_stack_off_, off.size());
+    dev_alloc(&d_stack_slots_, slots.size());
+    HIP_CHECK(hipMemcpy(d_stack_off_, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(d_stack_slots_, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   } else if (shape_.lds_groups > 0) {
     const int nsub = (G_ + shape_.lds_groups - 1) / shape_.lds_groups;
     if (shape_.lds_flush) {
@@ -3336,6 +3418,10 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     L.grp = reinterpret_cast<const int4*>(d_grp_);
     L.slot_fsr = d_slot_fsr_;
     L.tally = bufs ? bufs->tally_i64 : d_tally_i64_;
+    // the solver's own sweep writes per-slot sums and reduces them per FSR;
+    // on caller buffers (no room for the slot sums) the blocks add theirs
+    // to the tally with atomics
+    L.partial = bufs ? nullptr : d_stack_partial_;
     L.qsub = bufs ? bufs->qsub : d_qsub_;
     L.inv_vol = d_inv_vol_;
     L.sigma_t = d_sigma_t_;
@@ -3358,6 +3444,9 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
       launch(sweep3d_stack_kernel<MOC_EXP_HW>);
     else
       launch(sweep3d_stack_kernel<MOC_EXP_RATIONAL>);
+    if (L.partial)
+      hipLaunchKernelGGL(stack_reduce_kernel, dim3(blocks_for((int64_t)N_ * G_, 256)), dim3(256), 0, s,
+                         L.partial, d_stack_off_, d_stack_slots_, d_tally_i64_, N_, G_, a.state);
     HIP_CHECK(hipGetLastError());
     if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
     return;
@@ -4308,7 +4397,10 @@ size_t HipSolver::solveBytes() const {
   size_t b = 2 * psi + NG * (6 * sizeof(double) + sizeof(float2)) +
              NG * shape_.tally_copies * sizeof(float) + (size_t)2 * T_ * G_ * sizeof(float) +
              (size_t)M_ * G_ * (G_ + 4) * sizeof(double);
-  if (shape_.lds_groups > 0)
+  if (stackMode())
+    b += h_slot_fsr_.size() * (G_ * sizeof(uint64_t) + sizeof(int32_t)) + NG * sizeof(uint64_t) +
+         nsub * N_ * gb * sizeof(float);
+  else if (shape_.lds_groups > 0)
     b += nsub * shape_.lds_blocks_per_sub * N_ * gb * sizeof(uint64_t) + nsub * N_ * gb * sizeof(float);
   if (segStore())
     b += (size_t)(linear_ ? 3 : 1) * seg_rows_ * seg_stride_ * sizeof(float) + NG * sizeof(double);
