@@ -257,6 +257,9 @@ def main():
                     help="fixed sweep launch shape (keys of getLaunchShape), no autotuning")
     ap.add_argument("--serving", type=int, default=48, metavar="N",
                     help="requests in the serving-stream measurement (0: skip)")
+    ap.add_argument("--max-active", type=int, default=48, metavar="K",
+                    help="solves in flight at once in the serving measurement (48: the whole "
+                         "stream may be in flight; 16 and 32 measured 2.84-2.89x and 2.92x)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 3 times (a plain copy kernel "
                          "of known size, the reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -371,7 +374,7 @@ def main():
     serving = None
     if args.serving > 0:
         log(f"serving: {args.serving} requests ...")
-        serving = serving_bench(args.serving)
+        serving = serving_bench(args.serving, max_active=args.max_active)
         log(f"serving: {serving}")
 
     cpu = None

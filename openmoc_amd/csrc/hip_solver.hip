@@ -882,7 +882,11 @@ __global__ __launch_bounds__(1024) void sweep3d_stack_kernel(SweepArgs a, StackP
   __syncthreads();
   if (L.partial) {
     // every (slot, group) of the sweep has exactly one block: its sums are
-    // written, zero or not, and stack_reduce_kernel adds an FSR's slots
+    // written, zero or not, and stack_reduce_kernel adds an FSR's slots.
+    // (Rows of G sums per slot: the reduction then reads one row per slot.
+    // Writing each block's slice contiguously instead, [subset][slot][GB],
+    // cut these writes by a third but scattered the reduction's reads: 1.2 GB
+    // fetched against 0.24 GB.)
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
       const int s = i / L.GB, l = i - s * L.GB;
       if (l < gb) L.partial[(size_t)(gi.z + s) * a.G + g0 + l] = lds_tally[i];
@@ -909,18 +913,19 @@ __global__ void stack_reduce_kernel(const unsigned long long* __restrict__ parti
   if (i >= N * G) return;
   const int64_t r = i / G;
   const int g = (int)(i - r * G);
+  const unsigned long long* pl = partial + g;
   // four independent loads in flight (integer sums: the order is free)
   unsigned long long t0 = 0ull, t1 = 0ull, t2 = 0ull, t3 = 0ull;
   int64_t q = off[r];
   const int64_t e = off[r + 1];
   for (; q + 4 <= e; q += 4) {
     const int s0 = slots[q], s1 = slots[q + 1], s2 = slots[q + 2], s3 = slots[q + 3];
-    t0 += partial[(size_t)s0 * G + g];
-    t1 += partial[(size_t)s1 * G + g];
-    t2 += partial[(size_t)s2 * G + g];
-    t3 += partial[(size_t)s3 * G + g];
+    t0 += pl[(size_t)s0 * G];
+    t1 += pl[(size_t)s1 * G];
+    t2 += pl[(size_t)s2 * G];
+    t3 += pl[(size_t)s3 * G];
   }
-  for (; q < e; q++) t0 += partial[(size_t)slots[q] * G + g];
+  for (; q < e; q++) t0 += pl[(size_t)slots[q] * G];
   tally[i] = (t0 + t1) + (t2 + t3);
 }
 
