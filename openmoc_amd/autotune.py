@@ -48,11 +48,18 @@ def candidates(num_groups: int, num_cus: int = 256, is_3d: bool = False) -> List
         if gb > num_groups:
             continue
         nsub = -(-num_groups // gb)
-        for q, block, per_cu, flush in itertools.product((1, 0), (512, 1024), (1, 2, 4), (0, 1)):
-            # blocks per subset in multiples of 8: the subsets of one track
-            # slot then run on one XCD and share its L2 for the segment stream
-            # (about a fifth of the HBM fetch of other counts, at equal speed)
-            bps = max(8, ((num_cus * per_cu) // nsub + 7) // 8 * 8)
+        # Blocks per subset, multiples of 8: the subsets of one track slot
+        # then run on one XCD and share its L2 for the segment stream (other
+        # counts fetch several times the stream: on the 17x17 lattice 7 x 73
+        # blocks sweep 3% faster than 7 x 72 but fetch 357 MB against 47).
+        # A slice that fills most of a CU's LDS leaves one block per CU, so
+        # the sweep runs in rounds of num_cus blocks: counts rounded down
+        # fill whole rounds, counts rounded up give smaller blocks.
+        counts = set()
+        for per_cu in (1, 2, 4):
+            counts.add(max(8, ((num_cus * per_cu) // nsub + 7) // 8 * 8))
+            counts.add(max(8, (num_cus * per_cu) // nsub // 8 * 8))
+        for q, block, bps, flush in itertools.product((1, 0), (512, 1024), sorted(counts), (0, 1)):
             out.append(dict(lds_groups=gb, lds_q=q, lds_block=block, lds_blocks_per_sub=bps,
                             lds_flush=flush))
     for gpl in (1, 2, 4, 9):

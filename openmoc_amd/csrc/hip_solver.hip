@@ -2373,9 +2373,17 @@ void HipSolver::pickShape() {
       s.lds_groups = gb;
       s.lds_q = q;
       s.lds_block = 1024;
-      // about two blocks per CU: the strided tile assignment then balances
-      // the longest-first tiles across blocks (measured on the 17x17 lattice)
-      s.lds_blocks_per_sub = std::max(1, (2 * num_cus_ + nsub - 1) / nsub);
+      // Blocks run in rounds of (CUs x blocks resident per CU): about two
+      // rounds, rounded down to a multiple of the XCD count, which places
+      // the subsets of one track slot on one XCD so they share its L2 for
+      // the segment stream.  On the 17x17 lattice (one block per CU): 7 x 72
+      // blocks 150 us per source iteration, 47 MB fetched per sweep; one
+      // round (7 x 32 or 7 x 36) 163-168 us; 7 x 73 without the XCD placement
+      // 145 us but 357 MB fetched.
+      const int64_t resident =
+          std::max<int64_t>(1, std::min<int64_t>((int64_t)(lds_max_ / ldsBytes(gb, q)), 2048 / s.lds_block));
+      s.lds_blocks_per_sub =
+          (int)std::max<int64_t>(kNumXcd, 2 * num_cus_ * resident / nsub / kNumXcd * kNumXcd);
       // no more blocks than the subset has waves of tiles: a small problem
       // (a pin cell) then launches a few blocks instead of hundreds of idle
       // ones, each of which would clear and write a partial tally slice
@@ -2383,9 +2391,8 @@ void HipSolver::pickShape() {
         const int64_t tiles = (2 * T_ + 64 / gb - 1) / (64 / gb);
         const int64_t waves = std::max<int64_t>(1, (tiles + s.lds_block / 64 - 1) / (s.lds_block / 64));
         s.lds_blocks_per_sub = (int)std::min<int64_t>(s.lds_blocks_per_sub, waves);
+        s.lds_blocks_per_sub = (s.lds_blocks_per_sub + kNumXcd - 1) / kNumXcd * kNumXcd;
       }
-      // a multiple of the XCD count keeps a slot's subset blocks on one XCD
-      s.lds_blocks_per_sub = (s.lds_blocks_per_sub + kNumXcd - 1) / kNumXcd * kNumXcd;
     }
     // z-stack sweep: one group per block.  Narrow slices keep several blocks
     // on a CU: on the 3D 17x17 lattice 1 group per block sweeps in 1.10 ms,
@@ -2575,6 +2582,20 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
       h_trk_grp_[t] = it->second;
     }
     ngrp_ = (int32_t)gid.size();
+    // groups numbered by work, most segments first: blocks are dispatched in
+    // group order, so the largest groups start first and the last blocks of
+    // the launch are short ones
+    {
+      std::vector<int64_t> work(ngrp_, 0);
+      for (int64_t t = 0; t < T; t++) work[h_trk_grp_[t]] += td.seg_offset[t + 1] - td.seg_offset[t];
+      std::vector<int32_t> by_work(ngrp_);
+      std::iota(by_work.begin(), by_work.end(), 0);
+      std::stable_sort(by_work.begin(), by_work.end(),
+                       [&](int32_t x, int32_t y) { return work[x] > work[y]; });
+      std::vector<int32_t> rank(ngrp_);
+      for (int32_t i = 0; i < ngrp_; i++) rank[by_work[i]] = i;
+      for (int64_t t = 0; t < T; t++) h_trk_grp_[t] = rank[h_trk_grp_[t]];
+    }
     std::vector<std::vector<int64_t>> members(ngrp_);
     for (int64_t t = 0; t < T; t++) members[h_trk_grp_[t]].push_back(t);
     h_slot_.assign(S, 0);
@@ -5000,7 +5021,13 @@ void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
   // (about two per CU), a multiple of the XCD count per subset, at least 8
   int bps = 0;
   dim3 sgrid((unsigned)dims_.sweep_blocks, (unsigned)n);
-  if (sh.lds_groups > 0 && geos_.size() == 1 && sh.lds_blocks_per_sub % kNumXcd == 0) {
+  if (sh.lds_groups > 0 && geos_.size() == 1 && n > 1) {
+    // n rows of one geometry read the same segment stream: each row gets
+    // about a 1/n share of one solve's blocks per subset, rounded up to a
+    // multiple of the XCD count, so the blocks of one track slot (every
+    // subset of every row) run on one XCD and share its L2 for the stream
+    // (measured on five 17x17 solves: 16 blocks per row and subset 55.7 ms,
+    // 14 without that placement 58.8 ms)
     const int nsub = (g.G_ + sh.lds_groups - 1) / sh.lds_groups;
     bps = std::max(kNumXcd, (sh.lds_blocks_per_sub / n + kNumXcd - 1) / kNumXcd * kNumXcd);
     bps = std::min(bps, sh.lds_blocks_per_sub);
