@@ -138,12 +138,13 @@ def copy_calibration(mib: int):
 # iterations of 3 ms) fills the GPU alone, so it would time one solve, not
 # the serving of a stream; its served result is checked in the test suite.
 SERVING_MIX = [
-    ("pin_cell_7g@32", "pin_cell_7g", {"num_azim": 32}, 0.25),
-    ("pin_cell_7g@64", "pin_cell_7g", {"num_azim": 64}, 0.15),
+    ("pin_cell_7g@32", "pin_cell_7g", {"num_azim": 32}, 0.22),
+    ("pin_cell_7g@64", "pin_cell_7g", {"num_azim": 64}, 0.14),
     ("pin_cell_7g@128", "pin_cell_7g", {"num_azim": 128}, 0.10),
     ("pin_cell_70g@32", "pin_cell_70g", {"num_azim": 32}, 0.10),
-    ("lattice_3x3_vac_7g@16", "lattice_3x3_vac_7g", {"num_azim": 16}, 0.15),
-    ("lattice_3x3_vac_7g@32", "lattice_3x3_vac_7g", {"num_azim": 32}, 0.15),
+    ("lattice_3x3_vac_7g@32", "lattice_3x3_vac_7g", {"num_azim": 32}, 0.14),
+    ("lattice_3x3_vac_7g@64", "lattice_3x3_vac_7g", {"num_azim": 64}, 0.14),
+    ("minicore_7g@32", "minicore_7g", {"num_azim": 32}, 0.06),
     ("lattice_17x17_7g@64", "lattice_17x17_7g", {"num_azim": 64}, 0.10),
 ]
 

@@ -66,9 +66,13 @@ def candidates(num_groups: int, num_cus: int = 256, is_3d: bool = False) -> List
         lpt = -(-num_groups // gpl)
         if lpt > 64:
             continue
-        for block, (store, copies) in itertools.product((64, 256), ((1, 1), (0, 1), (0, 8))):
+        # a full wave of tracks, or half of one: a problem of a few thousand
+        # tracks has only a few waves per CU, and twice as many half-empty
+        # waves hide more of the gathers' latency
+        tpws = sorted({64 // lpt, max(1, (64 // lpt) // 2)})
+        for block, (store, copies), tpw in itertools.product((64, 256), ((1, 1), (0, 1), (0, 8)), tpws):
             out.append(dict(lds_groups=0, groups_per_lane=gpl, lanes_per_track=lpt,
-                            tracks_per_wave=64 // lpt, block=block, tally_copies=copies,
+                            tracks_per_wave=tpw, block=block, tally_copies=copies,
                             seg_store=store))
     return out
 

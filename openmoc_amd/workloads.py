@@ -395,6 +395,18 @@ def minicore_70g(seed: int = 3, num_azim: int = 32, spacing: float = 0.05) -> Pr
                    tolerance=1e-5, max_iters=1000)
 
 
+def minicore_7g(seed: int = 7, num_azim: int = 32, spacing: float = 0.4) -> Problem:
+    """The minicore_70g geometry (2x2 fuel assemblies, reflector column and
+    row, vacuum on xmax/ymin) with 7 synthetic groups; 32 azimuthal angles at
+    0.4 cm by default: the 3x3 mini-core kind of the serving stream."""
+    p = minicore_70g(num_azim=num_azim, spacing=spacing)
+    rng = np.random.default_rng(seed)
+    p.name = "minicore_7g"
+    p.materials = [synthetic_material(rng, 7, "fuel", True), synthetic_material(rng, 7, "moderator", False),
+                   synthetic_material(rng, 7, "reflector", False)]
+    return p
+
+
 def pin_cell_70g(seed: int = 6, num_azim: int = 16) -> Problem:
     """pin_cell_7g geometry with 70 synthetic groups (dense lower-triangular
     plus band scattering): the 70-group parity case."""
@@ -492,6 +504,7 @@ WORKLOADS = {
     "pin_cell_7g": pin_cell_7g,
     "lattice_17x17_7g": lattice_17x17_7g,
     "minicore_70g": minicore_70g,
+    "minicore_7g": minicore_7g,
 }
 
 
