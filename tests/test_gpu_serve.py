@@ -187,3 +187,23 @@ def test_small_geometries_share_one_batch_with_plain_results(tracks):
         kk, it, flux = plain_solve(tgs[k], p)
         assert (r.keff, r.num_iterations) == (kk, it)
         np.testing.assert_array_equal(r.flux.ravel(), flux)
+
+
+def test_mini_core_served_with_pin_cells_equals_plain_solves(tracks):
+    # the serving stream's 7-group 3x3 mini-core (global-tally sweep, its own
+    # batch) next to pin cells (LDS sweep): two rows of the mini-core batch
+    # and the pin-cell rows give the bits of their plain solves
+    tg_mc = tracks("minicore_7g")
+    tg_pin = tracks("pin_cell_7g")
+    probs = [("mc", perturbed("minicore_7g", 40)), ("pin", perturbed("pin_cell_7g", 41)),
+             ("mc", perturbed("minicore_7g", 42)), ("pin", perturbed("pin_cell_7g", 43))]
+    with serve.ServeEngine(max_active=4) as eng:
+        gids = {"mc": eng.add_geometry(tg_mc), "pin": eng.add_geometry(tg_pin)}
+        ids = [eng.submit(gids[kind], p, want_flux=True) for kind, p in probs]
+        res = [eng.wait(i, timeout=120) for i in ids]
+    for (kind, p), r in zip(probs, res):
+        print(f"mini-core serving: {kind} keff {r.keff:.8f} iterations {r.num_iterations}", flush=True)
+        k, it, flux = plain_solve(tg_mc if kind == "mc" else tg_pin, p)
+        assert r.keff == k
+        assert r.num_iterations == it
+        np.testing.assert_array_equal(r.flux.ravel(), flux)
