@@ -258,7 +258,8 @@ def test_linear_source_matches_reference(tracks, golden, name):
     assert abs(s.getNumIterations() - int(g["num_iterations"])) <= 2
     assert rel_l2(s.getFluxes(), g["flux"]) < FLUX_RL2
     # moments: FP32 sweep arithmetic on the difference terms of F2 and H
-    assert rel_l2(s.getFluxMoments(), g["flux_moments"]) < 5e-3
+    # (measured 1.5e-6 on the pin cell and 1.8e-6 on the 3x3 lattice)
+    assert rel_l2(s.getFluxMoments(), g["flux_moments"]) < 2e-5
     np.testing.assert_allclose(s.getFSRCentroids(), g["fsr_centroid"], atol=1e-7)
 
 
@@ -270,7 +271,7 @@ def test_linear_source_matches_oracle_on_same_tracks(tracks):
     s.computeEigenvalue(1000)
     assert abs(s.getKeff() - sol.keff) < K_TOL
     assert rel_l2(s.getFluxes(), sol.flux) < FLUX_RL2
-    assert rel_l2(s.getFluxMoments(), mom) < 5e-3
+    assert rel_l2(s.getFluxMoments(), mom) < 2e-5  # measured 7e-7
     np.testing.assert_allclose(s.getFSRCentroids(), cen, atol=1e-12)
 
 
