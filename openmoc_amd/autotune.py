@@ -59,6 +59,7 @@ def candidates(num_groups: int, num_cus: int = 256, is_3d: bool = False) -> List
         for per_cu in (1, 2, 4):
             counts.add(max(8, ((num_cus * per_cu) // nsub + 7) // 8 * 8))
             counts.add(max(8, (num_cus * per_cu) // nsub // 8 * 8))
+        counts.add(max(8, (4 * num_cus) // (3 * nsub) // 8 * 8))
         for q, block, bps, flush in itertools.product((1, 0), (512, 1024), sorted(counts), (0, 1)):
             out.append(dict(lds_groups=gb, lds_q=q, lds_block=block, lds_blocks_per_sub=bps,
                             lds_flush=flush))

@@ -2373,17 +2373,19 @@ void HipSolver::pickShape() {
       s.lds_groups = gb;
       s.lds_q = q;
       s.lds_block = 1024;
-      // Blocks run in rounds of (CUs x blocks resident per CU): about two
-      // rounds, rounded down to a multiple of the XCD count, which places
-      // the subsets of one track slot on one XCD so they share its L2 for
-      // the segment stream.  On the 17x17 lattice (one block per CU): 7 x 72
-      // blocks 150 us per source iteration, 47 MB fetched per sweep; one
-      // round (7 x 32 or 7 x 36) 163-168 us; 7 x 73 without the XCD placement
-      // 145 us but 357 MB fetched.
+      // Blocks run in rounds of (CUs x blocks resident per CU).  About 4/3
+      // of a round, rounded down to a multiple of the XCD count (which
+      // places the subsets of one track slot on one XCD, where they share its
+      // L2 for the segment stream): on the 17x17 lattice (one block per CU)
+      // 7 x 48 to 7 x 80 blocks all take 146-150 us per source iteration,
+      // 7 x 40 153-163 us and one round (7 x 32 or 36) 163-168 us; the
+      // fewest blocks that keep the speed write the fewest partial slices.
+      // 7 x 73 without the XCD placement took 145 us but fetched 357 MB per
+      // sweep against 47 MB.
       const int64_t resident =
           std::max<int64_t>(1, std::min<int64_t>((int64_t)(lds_max_ / ldsBytes(gb, q)), 2048 / s.lds_block));
       s.lds_blocks_per_sub =
-          (int)std::max<int64_t>(kNumXcd, 2 * num_cus_ * resident / nsub / kNumXcd * kNumXcd);
+          (int)std::max<int64_t>(kNumXcd, 4 * num_cus_ * resident / (3 * nsub) / kNumXcd * kNumXcd);
       // no more blocks than the subset has waves of tiles: a small problem
       // (a pin cell) then launches a few blocks instead of hundreds of idle
       // ones, each of which would clear and write a partial tally slice
