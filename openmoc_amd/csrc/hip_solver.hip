@@ -157,12 +157,12 @@ __device__ __forceinline__ float attenuation_hw(float y) {
 
 struct SweepArgs {
   const int32_t* work;      // [ntiles*tpw] directed track ids (t*2+d) or -1, longest first
-  int ntiles;               // wave tiles of tpw directed tracks
+  int ntiles;               // work tiles of tpw directed tracks (one wave each)
   int tpw;                  // directed tracks per tile (= per wave)
-  const int2* tile;         // [ntiles] (first row, rows) of the interleaved streams
+  const int2* tile;         // [ntiles] (first row, rows | kTileBackward) of the interleaved streams
   const int4* trk;          // seg_begin, seg_count, out slot fwd, out slot bwd
   const int32_t* trk_azim;
-  const float* seg_len;     // tile-interleaved: [row][tpw], traversal order, 0-padded
+  const float* seg_len;     // tile-interleaved: [row][tpw], forward order (or traversal order), 0-padded
   const int32_t* seg_key;   // tile-interleaved: [row][tpw], FSR | material << 24
   unsigned stream_bytes;    // bytes of each stream (rows + tail) * tpw * 4
   const float2* qs;         // [N*G] (Q/sigma_t, sigma_t)
