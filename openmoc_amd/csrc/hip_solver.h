@@ -289,6 +289,8 @@ class HipSolver {
   int32_t* d_slot_fsr_ = nullptr;      // [total slots] global FSR of each slot
   unsigned long long* d_tally_i64_ = nullptr;  // [N*G] fixed-point tally (stack sweep)
   bool stackMode() const { return is3d_ && ngrp_ > 0 && shape_.lds_groups > 0; }
+  // z-stack per-slot sums: entries per slot row (G rounded up to 8)
+  int stackRow() const { return (G_ + 7) / 8 * 8; }
   // the sweep tallies into the global int64 fixed-point tally d_tally_i64_
   // (3D z-stack sweep, or the 2D LDS sweep with the atomic flush)
   bool int64Tally() const { return stackMode() || (shape_.lds_groups > 0 && shape_.lds_flush); }

@@ -823,7 +823,8 @@ struct StackParams {
   const int4* grp;             // [ngrp] first tile, tiles, first slot, slots
   const int32_t* slot_fsr;     // global FSR of each slot
   unsigned long long* tally;   // [N*G] fixed-point sums of contribution / V (atomic flush)
-  unsigned long long* partial; // [slots][G] each slot's sums, written once per sweep, or null
+  unsigned long long* partial; // [slots][prow] each slot's sums, written once per sweep, or null
+  int prow;                    // row stride: G rounded up to 8 (64-byte rows)
   const float* qsub;           // [nsub][N][GB] Q/sigma_t, subset-major
   const float* inv_vol;        // [N]
   const double* sigma_t;       // [M][G]
@@ -889,7 +890,7 @@ __global__ __launch_bounds__(1024) void sweep3d_stack_kernel(SweepArgs a, StackP
     // fetched against 0.24 GB.)
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
       const int s = i / L.GB, l = i - s * L.GB;
-      if (l < gb) L.partial[(size_t)(gi.z + s) * a.G + g0 + l] = lds_tally[i];
+      if (l < gb) L.partial[(size_t)(gi.z + s) * L.prow + g0 + l] = lds_tally[i];
     }
     return;
   }
@@ -904,10 +905,12 @@ __global__ __launch_bounds__(1024) void sweep3d_stack_kernel(SweepArgs a, StackP
 // tally[r][g] = sum over FSR r's z-stack slots of their sums (an exact
 // integer sum, so the bits equal those of the atomic flush).  off [N+1] and
 // slots list each FSR's slots; reads are one slot row of G sums per step.
+// Rows of prow sums (G rounded up to 8, 64 bytes at 7 groups) keep the 7
+// blocks' pieces of a row in one 64-byte line.
 __global__ void stack_reduce_kernel(const unsigned long long* __restrict__ partial,
                                     const int64_t* __restrict__ off, const int32_t* __restrict__ slots,
                                     unsigned long long* __restrict__ tally, int64_t N, int G,
-                                    const IterState* state) {
+                                    int prow, const IterState* state) {
   if (state->converged) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N * G) return;
@@ -920,12 +923,12 @@ __global__ void stack_reduce_kernel(const unsigned long long* __restrict__ parti
   const int64_t e = off[r + 1];
   for (; q + 4 <= e; q += 4) {
     const int s0 = slots[q], s1 = slots[q + 1], s2 = slots[q + 2], s3 = slots[q + 3];
-    t0 += pl[(size_t)s0 * G];
-    t1 += pl[(size_t)s1 * G];
-    t2 += pl[(size_t)s2 * G];
-    t3 += pl[(size_t)s3 * G];
+    t0 += pl[(size_t)s0 * prow];
+    t1 += pl[(size_t)s1 * prow];
+    t2 += pl[(size_t)s2 * prow];
+    t3 += pl[(size_t)s3 * prow];
   }
-  for (; q < e; q++) t0 += pl[(size_t)slots[q] * G];
+  for (; q < e; q++) t0 += pl[(size_t)slots[q] * prow];
   tally[i] = (t0 + t1) + (t2 + t3);
 }
 
@@ -3078,7 +3081,7 @@ void HipSolver::allocateIterationArrays() {
       std::vector<int64_t> cur(off.begin(), off.end() - 1);
       for (size_t q = 0; q < nslots; q++) slots[cur[h_slot_fsr_[q]]++] = (int32_t)q;
     }
-    dev_alloc(&d_stack_partial_, std::max<size_t>(nslots, 1) * G_);
+    dev_alloc(&d_stack_partial_, std::max<size_t>(nslots, 1) * stackRow());
     dev_alloc(&d_stack_off_, off.size());
     dev_alloc(&d_stack_slots_, slots.size());
     HIP_CHECK(hipMemcpy(d_stack_off_, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -3450,6 +3453,7 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
     // on caller buffers (no room for the slot sums) the blocks add theirs
     // to the tally with atomics
     L.partial = bufs ? nullptr : d_stack_partial_;
+    L.prow = stackRow();
     L.qsub = bufs ? bufs->qsub : d_qsub_;
     L.inv_vol = d_inv_vol_;
     L.sigma_t = d_sigma_t_;
@@ -3474,7 +3478,7 @@ void HipSolver::launchSweep(const float* psi_in, float* psi_out, hipStream_t s,
       launch(sweep3d_stack_kernel<MOC_EXP_RATIONAL>);
     if (L.partial)
       hipLaunchKernelGGL(stack_reduce_kernel, dim3(blocks_for((int64_t)N_ * G_, 256)), dim3(256), 0, s,
-                         L.partial, d_stack_off_, d_stack_slots_, d_tally_i64_, N_, G_, a.state);
+                         L.partial, d_stack_off_, d_stack_slots_, d_tally_i64_, N_, G_, L.prow, a.state);
     HIP_CHECK(hipGetLastError());
     if (ev_after_sweep) HIP_CHECK(hipEventRecord(ev_after_sweep, s));
     return;
@@ -4426,7 +4430,7 @@ size_t HipSolver::solveBytes() const {
              NG * shape_.tally_copies * sizeof(float) + (size_t)2 * T_ * G_ * sizeof(float) +
              (size_t)M_ * G_ * (G_ + 4) * sizeof(double);
   if (stackMode())
-    b += h_slot_fsr_.size() * (G_ * sizeof(uint64_t) + sizeof(int32_t)) + NG * sizeof(uint64_t) +
+    b += h_slot_fsr_.size() * (stackRow() * sizeof(uint64_t) + sizeof(int32_t)) + NG * sizeof(uint64_t) +
          nsub * N_ * gb * sizeof(float);
   else if (shape_.lds_groups > 0)
     b += nsub * shape_.lds_blocks_per_sub * N_ * gb * sizeof(uint64_t) + nsub * N_ * gb * sizeof(float);
