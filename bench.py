@@ -178,16 +178,19 @@ def serving_bench(n_requests: int, seed: int = 0, max_active: int = 16):
     solvers = {n: om.HipSolver(tg) for n, tg in tgs.items()}
     for n, s in solvers.items():       # warm (code objects, graphs)
         s.computeEigenvalue(1000)
-    plain = []
+    plain, t_req = [], []
     t0 = time.perf_counter()
     for name, xs in reqs:
+        t1 = time.perf_counter()
         s = solvers[name]
         s.setMaterials(*xs)
         s.computeEigenvalue(1000)
         plain.append((s.getKeff(), s.getFluxes()))
+        t_req.append(time.perf_counter() - t1)
     t_plain = time.perf_counter() - t0
 
-    def run_engine(arrivals):
+    def run_engine(arrivals, subset=None):
+        stream = [reqs[i] for i in subset] if subset is not None else reqs
         with serve.ServeEngine(max_active=max_active) as eng:
             gids = {n: eng.add_geometry(tg) for n, tg in tgs.items()}
             # warm-up: as many solves of each kind at once as the stream will
@@ -197,13 +200,13 @@ def serving_bench(n_requests: int, seed: int = 0, max_active: int = 16):
             for n in gids:
                 name, kw = problems[n]
                 xs = serve.material_arrays(workloads.get(name, **kw))
-                for _ in range(min(int(np.sum(names == n)), max_active)):
+                for _ in range(min(sum(1 for k, _ in stream if k == n), max_active)):
                     warm.append(eng.submit(gids[n], xs))
             for i in warm:
                 eng.wait(i, 300)
             ids = []
             t0 = time.perf_counter()
-            for (name, xs), ta in zip(reqs, arrivals):
+            for (name, xs), ta in zip(stream, arrivals):
                 d = t0 + ta - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
@@ -219,6 +222,15 @@ def serving_bench(n_requests: int, seed: int = 0, max_active: int = 16):
     t_poi, res_p = run_engine(np.cumsum(gaps) - gaps[0])
     lat = np.array([r.latency_s for r in res_p])
 
+    # (4) the same stream without its mini-cores (the longest solves,
+    # latency-bound on the device: they set the ratio of the whole stream)
+    sub = [i for i, (k, _) in enumerate(reqs) if not k.startswith("minicore")]
+    t_sub, _ = run_engine(np.zeros(len(sub)), sub)
+    t_sub_plain = float(sum(t_req[i] for i in sub))
+    by_kind = {}
+    for (k, _), t in zip(reqs, t_req):
+        by_kind[k] = by_kind.get(k, 0.0) + t
+
     dk = max(abs(r.keff - k) for r, (k, _) in zip(res + res_p, plain + plain))
     dphi = max(float(np.linalg.norm(r.flux.ravel() - f) / np.linalg.norm(f))
                for r, (_, f) in zip(res + res_p, plain + plain))
@@ -233,6 +245,9 @@ def serving_bench(n_requests: int, seed: int = 0, max_active: int = 16):
         "latency_p50_s": float(np.percentile(lat, 50)),
         "latency_p99_s": float(np.percentile(lat, 99)),
         "max_dk_pcm_vs_plain": dk * 1e5, "max_flux_rel_l2_vs_plain": dphi,
+        "plain_seconds_by_kind": by_kind,
+        "without_minicores": {"requests": len(sub),
+                              "speedup_vs_plain": t_sub_plain / t_sub if t_sub > 0 else None},
         "note": "each served result compared with the plain single solve of the same "
                 "inputs (bitwise equal expected); parity of plain solves vs the "
                 "compiled reference is in the solve object and the test suite",
