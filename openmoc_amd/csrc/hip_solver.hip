@@ -10,7 +10,9 @@
 //                      (transferBoundaryFlux fused, src/CPUSolver.cpp:2560-2601)
 //   post_sweep_kernel  addSourceToScalarFlux (FP64) + per-FSR fission rate
 //                      partials + zeroing of the tally for the next sweep
-//   keff_kernel        computeKeff + normalisation factor (FP64, fixed order)
+//   keff_kernel        computeKeff + normalisation factor (FP64, fixed order);
+//                      in an eigen iteration every update_kernel block forms
+//                      the same sums itself and block 0 stores them
 //   update_kernel      normalizeFluxes (phi), computeResidual partials,
 //                      storeFSRFluxes and computeFSRSources for the next
 //                      iteration (FP64), and the FP32 (Q/sigma_t, sigma_t)
@@ -1037,11 +1039,12 @@ struct SweepBuffers {
 namespace {
 
 // per-block partial sums written by post_sweep_kernel / update_kernel
-constexpr int kPartFission = 0;     // sum V nu-sigma-f phi (unstabilised) / residual terms
+constexpr int kPartFission = 0;     // sum V nu-sigma-f phi (unstabilised)
 constexpr int kPartAbsorption = 1;  // sum V sigma_a phi
 constexpr int kPartNormFission = 2; // sum V nu-sigma-f phi after stabilisation
 constexpr int kPartLeakage = 3;     // sum of vacuum leakage
-constexpr int kNumPartials = 4;
+constexpr int kPartResidual = 4;    // residual terms (update_kernel, fsr_residual_kernel)
+constexpr int kNumPartials = 5;
 
 // stabilizationType (src/Solver.h:103-113)
 constexpr int kStabDiagonal = 0;
@@ -1060,6 +1063,20 @@ __device__ double block_sum(double v) {
     for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
   __syncthreads();
   return r;  // valid in thread 0
+}
+
+// The sum of x[0..n) formed by one wave alone, in every lane with the same
+// bits: lane l adds x[l], x[l + 64], ... in order, then six butterfly steps
+// add the lanes pairwise (each step adds the same two values in every lane of
+// a pair, and IEEE addition commutes).  No LDS and no barrier, so the other
+// loads of the calling block stay in flight.
+__device__ __forceinline__ double wave_sum_all(const double* x, int n) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+#pragma unroll 4
+  for (int i = lane; i < n; i += 64) v += x[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
 __device__ double block_max(double v) {
@@ -1282,27 +1299,34 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
 // (k *= F/N, or k = F / (A + L) from the neutron balance; src/CPUSolver.cpp:
 // 2258-2328) then normalizeFluxes' factor N/F (src/CPUSolver.cpp:1860-1931)
 // from the (stabilised) fluxes.
+// The sums of post_sweep_kernel's per-block partials (every thread of the
+// block gets them, with the same bits in every block and launch): fission before and after stabilisation, and with the
+// neutron balance absorption and leakage.  One order for every caller, so a
+// block of update_kernel that takes keff_kernel's place gets its bits.
+struct KeffSums {
+  double fiss, fiss_norm, absr, leak;
+};
+__device__ __forceinline__ KeffSums keff_sums(const double* partials, int nparts, int pstride,
+                                              int balance) {
+  KeffSums r;
+  r.fiss = wave_sum_all(partials + (size_t)kPartFission * pstride, nparts);
+  r.fiss_norm = wave_sum_all(partials + (size_t)kPartNormFission * pstride, nparts);
+  r.absr = 0.0;
+  r.leak = 0.0;
+  if (balance) {
+    r.absr = wave_sum_all(partials + (size_t)kPartAbsorption * pstride, nparts);
+    r.leak = wave_sum_all(partials + (size_t)kPartLeakage * pstride, nparts);
+  }
+  return r;
+}
+
 __device__ __forceinline__ void keff_kernel_body(const double* partials, int nparts,
                                                            int pstride, IterState* state,
                                                            double num_fsrs, int phase,
                                                            int balance) {
   if (state->converged) return;
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    v0 += partials[kPartFission * pstride + i];
-    v2 += partials[kPartNormFission * pstride + i];
-    if (balance) {
-      v1 += partials[kPartAbsorption * pstride + i];
-      v3 += partials[kPartLeakage * pstride + i];
-    }
-  }
-  const double fiss = block_sum(v0);
-  const double fiss_norm = block_sum(v2);
-  double absr = 0.0, leak = 0.0;
-  if (balance) {
-    absr = block_sum(v1);
-    leak = block_sum(v3);
-  }
+  const KeffSums ks = keff_sums(partials, nparts, pstride, balance);
+  const double fiss = ks.fiss, fiss_norm = ks.fiss_norm, absr = ks.absr, leak = ks.leak;
   if (threadIdx.x == 0) {
     if (phase == 1) {
       if (balance)
@@ -1376,7 +1400,7 @@ __device__ __forceinline__ void fsr_residual_kernel_body(const FsrArgs& a, int n
     res = source_residual_term(a, r, res_type, normalize, normalize ? a.state->norm_factor : 1.0,
                                a.state->k);
   const double s = block_sum(res);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) a.partials[kPartResidual * a.pstride + blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(kReduceBlock) void fsr_residual_kernel(FsrArgs a, int normalize,
@@ -1393,23 +1417,44 @@ constexpr int kSrcScatter = 2;  // computeFSRScatterSources (src/CPUSolver.cpp:2
 // partials, store the normalised flux as the next "old" flux, sources of the
 // next sweep.  Reads phi and the current old flux only, writes the other
 // old buffer, so threads of one FSR never see each other's writes.
+// keff_parts > 0: this launch also does keff_kernel's phase-1 step (eigen
+// iteration) from the first keff_parts post-sweep partials.  Every block
+// forms the sums with keff_sums (the same bits in every block) and block 0
+// stores k and the normalisation factor, so the state after the launch holds
+// what keff_kernel would store.  k_prev is read for k: the previous residual
+// step (or the solve's set-up) made them equal, and unlike k it is not
+// written during the launch.
 __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normalize,
                                                              int res_type, int do_residual,
                                                              int compute_sources,
                                                              int source_iteration_offset,
-                                                             int source_kind) {
+                                                             int source_kind, int keff_parts = 0,
+                                                             int balance = 0,
+                                                             double num_fsrs = 0.0) {
   if (a.state->converged) return;
   const int G = a.G;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0;
   double qmax = 0.0;
   int64_t negsrc = 0;
+  double k_new = 0.0, f_new = 0.0;
+  if (keff_parts > 0) {
+    const KeffSums ks = keff_sums(a.partials, keff_parts, a.pstride, balance);
+    k_new = balance ? ks.fiss / (ks.absr + ks.leak) : a.state->k_prev * ks.fiss / num_fsrs;
+    f_new = num_fsrs / ks.fiss_norm;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.state->k = k_new;
+      a.state->fission_total = ks.fiss_norm;
+      a.state->norm_factor = f_new;
+      a.state->psi_scale = f_new;
+    }
+  }
   if (i < a.N * G) {
     const int64_t r = i / G;
     const int gd = (int)(i - r * G);
     const int m = a.fsr_mat[r];
-    const double f = normalize ? a.state->norm_factor : 1.0;
-    const double k = a.state->k;
+    const double f = normalize ? (keff_parts > 0 ? f_new : a.state->norm_factor) : 1.0;
+    const double k = keff_parts > 0 ? k_new : a.state->k;
     const double st = a.sigma_t[(size_t)m * G + gd];
     const double* ss = a.sigma_s + (size_t)m * G * G;
     // fission source: chi(gd) sum_g fnsf(g) x(g) (transposed roles when adjoint)
@@ -1521,7 +1566,7 @@ __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normali
   const double s = block_sum(res);
   const double qm = block_max(qmax);
   if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = s;
+    a.partials[kPartResidual * a.pstride + blockIdx.x] = s;
     if (qm > 0.0) {
       // round up so the float bound is not below the double value
       const float qf = __uint_as_float(__float_as_uint((float)qm) + 1u);
@@ -1535,9 +1580,10 @@ __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int nor
                                                              int res_type, int do_residual,
                                                              int compute_sources,
                                                              int source_iteration_offset,
-                                                             int source_kind) {
+                                                             int source_kind, int keff_parts,
+                                                             int balance, double num_fsrs) {
   update_kernel_body(a, normalize, res_type, do_residual, compute_sources, source_iteration_offset,
-                     source_kind);
+                     source_kind, keff_parts, balance, num_fsrs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1864,11 +1910,6 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_batch_kernel(const Ba
   post_sweep_kernel_body(b.fsr, 1, 0, bps > 0 ? bps : b.fsr.lds_bps);
 }
 
-__global__ __launch_bounds__(kReduceBlock) void keff_batch_kernel(const BatchSlot* __restrict__ t) {
-  const BatchSlot& b = t[blockIdx.y];
-  keff_kernel_body(b.fsr.partials, b.nparts_post, b.fsr.pstride, b.fsr.state, b.num_fsrs, 1, 0);
-}
-
 __global__ __launch_bounds__(256) void scatter_mfma_batch_kernel(const BatchSlot* __restrict__ t) {
   const BatchSlot& b = t[blockIdx.y];
   if (b.scat_path != MOC_SCATTER_MFMA_F16) return;
@@ -1885,7 +1926,8 @@ __global__ __launch_bounds__(kReduceBlock) void update_batch_kernel(const BatchS
                                                                    int res_type) {
   const BatchSlot& b = t[blockIdx.y];
   if ((int)blockIdx.x >= b.nparts_upd) return;
-  update_kernel_body(b.fsr, 1, res_type, 1, 1, 1, kSrcTotal);
+  // with keff_kernel's step (k from the row's post-sweep partials)
+  update_kernel_body(b.fsr, 1, res_type, 1, 1, 1, kSrcTotal, b.nparts_post, 0, b.num_fsrs);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void fsr_residual_batch_kernel(const BatchSlot* __restrict__ t,
@@ -1897,7 +1939,8 @@ __global__ __launch_bounds__(kReduceBlock) void fsr_residual_batch_kernel(const 
 
 __global__ __launch_bounds__(kReduceBlock) void residual_batch_kernel(const BatchSlot* __restrict__ t) {
   const BatchSlot& b = t[blockIdx.y];
-  residual_kernel_body(b.fsr.partials, b.nparts_upd, b.fsr.state, b.norm, b.tol, 0, b.sigma_t_max, 1);
+  residual_kernel_body(b.fsr.partials + (size_t)kPartResidual * b.fsr.pstride, b.nparts_upd, b.fsr.state,
+                       b.norm, b.tol, 0, b.sigma_t_max, 1);
 }
 
 // Row `row` of the table := v (a launch argument, so the host's copy of v is
@@ -3636,9 +3679,7 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
   psi_cur_ ^= 1;
   hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1,
                      balance ? 1 : 0);
-  if (mode == MODE_EIGEN)
-    hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
-                       num_partials_, pstride_, d_state_, (double)N_, 1, balance ? 1 : 0);
+  // eigen: keff_kernel's step runs inside update_kernel (one launch fewer)
   const double norm = (mode == MODE_EIGEN && res_type == MOC_FISSION_SOURCE)
                           ? (double)num_fissionable_
                           : (double)N_;
@@ -3646,11 +3687,13 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
   const unsigned nb_upd = blocks_for((int64_t)N_ * G_, kReduceBlock);
   if (recompute_sources) launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(nb_upd), dim3(kReduceBlock), 0, stream_, fa,
-                     mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1, kSrcTotal);
+                     mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1, kSrcTotal,
+                     mode == MODE_EIGEN ? num_partials_ : 0, balance ? 1 : 0, (double)N_);
   if (rt != MOC_SCALAR_FLUX && G_ >= kFsrResidualGroups)
     hipLaunchKernelGGL(fsr_residual_kernel, dim3(blocks_for(N_, kReduceBlock)), dim3(kReduceBlock), 0,
                        stream_, fa, mode == MODE_EIGEN ? 1 : 0, rt);
-  hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
+  hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(kReduceBlock), 0, stream_,
+                     d_partials_ + (size_t)kPartResidual * pstride_,
                      (int)nb_upd, d_state_, rt == MOC_FISSION_SOURCE ? (double)num_fissionable_ : norm,
                      tol, mode == MODE_EIGEN ? 0 : 1, sigma_t_max_, recompute_sources ? 1 : 0);
   HIP_CHECK(hipGetLastError());
@@ -3726,7 +3769,7 @@ void HipSolver::launchInit(Mode mode, double k0) {
   // normalise, store old flux, sources of iteration 0
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, normalize ? 1 : 0, 0, 0, 1, 0, kSrcTotal);
+                     stream_, fa, normalize ? 1 : 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   // fixed-point scale of the first sweep (start fluxes are zero); the old
   // flux just written becomes the current one
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
@@ -3788,7 +3831,7 @@ void HipSolver::launchRestart(Mode mode, double k0) {
   // storeFSRFluxes and the sources of iteration 0, no normalisation
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -3841,7 +3884,7 @@ void HipSolver::launchStartupSweeps() {
   // computeFSRSources(0) of this flux, unnormalised; the old flux is kept
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
   HIP_CHECK(hipGetLastError());
 }
@@ -3926,7 +3969,7 @@ void HipSolver::krylovSweep(int source_kind) {
   fa.stab_type = -1;
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, 0, 0, 0, 1, 0, source_kind);
+                     stream_, fa, 0, 0, 0, 1, 0, source_kind, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
   launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
   psi_cur_ ^= 1;
@@ -3969,7 +4012,7 @@ void HipSolver::computeFSRSources(int iteration, double* out, int64_t n) {
   fa.stab_type = -1;
   launchScatter(stream_);
   hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
-                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal);
+                     stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(out, d_Q_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HIP_CHECK(hipStreamSynchronize(stream_));
@@ -5098,7 +5141,6 @@ void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
   const unsigned ny = (unsigned)n;
   hipLaunchKernelGGL(post_sweep_batch_kernel, dim3((unsigned)dims_.post_blocks, ny), dim3(kReduceBlock), 0,
                      st, d_table_, bps);
-  hipLaunchKernelGGL(keff_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
   if (dims_.mfma_blocks > 0)
     hipLaunchKernelGGL(scatter_mfma_batch_kernel, dim3((unsigned)dims_.mfma_blocks, ny), dim3(256), 0, st,
                        d_table_);
