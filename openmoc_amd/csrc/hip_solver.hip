@@ -1299,13 +1299,22 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_kernel(FsrArgs a, int
 // (k *= F/N, or k = F / (A + L) from the neutron balance; src/CPUSolver.cpp:
 // 2258-2328) then normalizeFluxes' factor N/F (src/CPUSolver.cpp:1860-1931)
 // from the (stabilised) fluxes.
-// The sums of post_sweep_kernel's per-block partials (every thread of the
-// block gets them, with the same bits in every block and launch): fission before and after stabilisation, and with the
+// The sums of post_sweep_kernel's per-block partials for the keff step inside
+// update_kernel: formed by every wave on its own, so every thread of every
+// block gets the same bits (keff_kernel, one block, sums the same partials
+// with block_sum; the two orders differ in the last bits, and a solve always
+// takes the same one): fission before and after stabilisation, and with the
 // neutron balance absorption and leakage.  One order for every caller, so a
 // block of update_kernel that takes keff_kernel's place gets its bits.
 struct KeffSums {
   double fiss, fiss_norm, absr, leak;
 };
+// The keff step runs inside update_kernel only up to this many post-sweep
+// partials: every update block reads all of them, so the reads grow with the
+// square of the block count (at 254 partials, the 17x17 lattice, the fused
+// step saves 2.1 us an iteration; at 1 052, the 7-group mini-core, it breaks
+// even; far above, 70 groups, it would read gigabytes).
+constexpr int kFuseKeffMaxParts = 512;
 __device__ __forceinline__ KeffSums keff_sums(const double* partials, int nparts, int pstride,
                                               int balance) {
   KeffSums r;
@@ -1325,8 +1334,22 @@ __device__ __forceinline__ void keff_kernel_body(const double* partials, int npa
                                                            double num_fsrs, int phase,
                                                            int balance) {
   if (state->converged) return;
-  const KeffSums ks = keff_sums(partials, nparts, pstride, balance);
-  const double fiss = ks.fiss, fiss_norm = ks.fiss_norm, absr = ks.absr, leak = ks.leak;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    v0 += partials[kPartFission * pstride + i];
+    v2 += partials[kPartNormFission * pstride + i];
+    if (balance) {
+      v1 += partials[kPartAbsorption * pstride + i];
+      v3 += partials[kPartLeakage * pstride + i];
+    }
+  }
+  const double fiss = block_sum(v0);
+  const double fiss_norm = block_sum(v2);
+  double absr = 0.0, leak = 0.0;
+  if (balance) {
+    absr = block_sum(v1);
+    leak = block_sum(v3);
+  }
   if (threadIdx.x == 0) {
     if (phase == 1) {
       if (balance)
@@ -1420,10 +1443,11 @@ constexpr int kSrcScatter = 2;  // computeFSRScatterSources (src/CPUSolver.cpp:2
 // keff_parts > 0: this launch also does keff_kernel's phase-1 step (eigen
 // iteration) from the first keff_parts post-sweep partials.  Every block
 // forms the sums with keff_sums (the same bits in every block) and block 0
-// stores k and the normalisation factor, so the state after the launch holds
-// what keff_kernel would store.  k_prev is read for k: the previous residual
-// step (or the solve's set-up) made them equal, and unlike k it is not
-// written during the launch.
+// stores k and the normalisation factor.  k_prev is read for k: the previous
+// residual step (or the solve's set-up) made them equal, and unlike k it is
+// not written during the launch.  FUSE = 0 compiles the step out (keff_parts
+// is then ignored), so launches without it keep the plain kernel.
+template <int FUSE>
 __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normalize,
                                                              int res_type, int do_residual,
                                                              int compute_sources,
@@ -1438,7 +1462,8 @@ __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normali
   double qmax = 0.0;
   int64_t negsrc = 0;
   double k_new = 0.0, f_new = 0.0;
-  if (keff_parts > 0) {
+  const bool fused = FUSE && keff_parts > 0;
+  if (fused) {
     const KeffSums ks = keff_sums(a.partials, keff_parts, a.pstride, balance);
     k_new = balance ? ks.fiss / (ks.absr + ks.leak) : a.state->k_prev * ks.fiss / num_fsrs;
     f_new = num_fsrs / ks.fiss_norm;
@@ -1453,8 +1478,8 @@ __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normali
     const int64_t r = i / G;
     const int gd = (int)(i - r * G);
     const int m = a.fsr_mat[r];
-    const double f = normalize ? (keff_parts > 0 ? f_new : a.state->norm_factor) : 1.0;
-    const double k = keff_parts > 0 ? k_new : a.state->k;
+    const double f = normalize ? (fused ? f_new : a.state->norm_factor) : 1.0;
+    const double k = fused ? k_new : a.state->k;
     const double st = a.sigma_t[(size_t)m * G + gd];
     const double* ss = a.sigma_s + (size_t)m * G * G;
     // fission source: chi(gd) sum_g fnsf(g) x(g) (transposed roles when adjoint)
@@ -1576,14 +1601,15 @@ __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normali
   if (negsrc) atomicAdd((unsigned long long*)&a.state->negative_sources, (unsigned long long)negsrc);
 }
 
+template <int FUSE>
 __global__ __launch_bounds__(kReduceBlock) void update_kernel(FsrArgs a, int normalize,
                                                              int res_type, int do_residual,
                                                              int compute_sources,
                                                              int source_iteration_offset,
                                                              int source_kind, int keff_parts,
                                                              int balance, double num_fsrs) {
-  update_kernel_body(a, normalize, res_type, do_residual, compute_sources, source_iteration_offset,
-                     source_kind, keff_parts, balance, num_fsrs);
+  update_kernel_body<FUSE>(a, normalize, res_type, do_residual, compute_sources,
+                           source_iteration_offset, source_kind, keff_parts, balance, num_fsrs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1910,6 +1936,13 @@ __global__ __launch_bounds__(kReduceBlock) void post_sweep_batch_kernel(const Ba
   post_sweep_kernel_body(b.fsr, 1, 0, bps > 0 ? bps : b.fsr.lds_bps);
 }
 
+// rows with few partials take the keff step in update_batch_kernel instead
+__global__ __launch_bounds__(kReduceBlock) void keff_batch_kernel(const BatchSlot* __restrict__ t) {
+  const BatchSlot& b = t[blockIdx.y];
+  if (b.nparts_post <= kFuseKeffMaxParts) return;
+  keff_kernel_body(b.fsr.partials, b.nparts_post, b.fsr.pstride, b.fsr.state, b.num_fsrs, 1, 0);
+}
+
 __global__ __launch_bounds__(256) void scatter_mfma_batch_kernel(const BatchSlot* __restrict__ t) {
   const BatchSlot& b = t[blockIdx.y];
   if (b.scat_path != MOC_SCATTER_MFMA_F16) return;
@@ -1922,12 +1955,15 @@ __global__ __launch_bounds__(256) void scatter_fp32_batch_kernel(const BatchSlot
   scatter_fp32_kernel_body(b.scat);
 }
 
+// FUSE: with keff_kernel's step for rows with few post-sweep partials, as in
+// the row's single solve (launchIteration)
+template <int FUSE>
 __global__ __launch_bounds__(kReduceBlock) void update_batch_kernel(const BatchSlot* __restrict__ t,
                                                                    int res_type) {
   const BatchSlot& b = t[blockIdx.y];
   if ((int)blockIdx.x >= b.nparts_upd) return;
-  // with keff_kernel's step (k from the row's post-sweep partials)
-  update_kernel_body(b.fsr, 1, res_type, 1, 1, 1, kSrcTotal, b.nparts_post, 0, b.num_fsrs);
+  update_kernel_body<FUSE>(b.fsr, 1, res_type, 1, 1, 1, kSrcTotal,
+                           b.nparts_post <= kFuseKeffMaxParts ? b.nparts_post : 0, 0, b.num_fsrs);
 }
 
 __global__ __launch_bounds__(kReduceBlock) void fsr_residual_batch_kernel(const BatchSlot* __restrict__ t,
@@ -3680,15 +3716,21 @@ void HipSolver::launchIteration(Mode mode, int res_type, bool recompute_sources,
   hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1,
                      balance ? 1 : 0);
   // eigen: keff_kernel's step runs inside update_kernel (one launch fewer)
+  // when the post-sweep partials are few enough
+  const int keff_parts = mode == MODE_EIGEN && num_partials_ <= kFuseKeffMaxParts ? num_partials_ : 0;
+  if (mode == MODE_EIGEN && keff_parts == 0)
+    hipLaunchKernelGGL(keff_kernel, dim3(1), dim3(kReduceBlock), 0, stream_, d_partials_,
+                       num_partials_, pstride_, d_state_, (double)N_, 1, balance ? 1 : 0);
   const double norm = (mode == MODE_EIGEN && res_type == MOC_FISSION_SOURCE)
                           ? (double)num_fissionable_
                           : (double)N_;
   const int rt = mode == MODE_FLUX ? MOC_SCALAR_FLUX : res_type;
   const unsigned nb_upd = blocks_for((int64_t)N_ * G_, kReduceBlock);
   if (recompute_sources) launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(nb_upd), dim3(kReduceBlock), 0, stream_, fa,
+  auto* const upd = keff_parts > 0 ? update_kernel<1> : update_kernel<0>;
+  hipLaunchKernelGGL(upd, dim3(nb_upd), dim3(kReduceBlock), 0, stream_, fa,
                      mode == MODE_EIGEN ? 1 : 0, rt, 1, recompute_sources ? 1 : 0, 1, kSrcTotal,
-                     mode == MODE_EIGEN ? num_partials_ : 0, balance ? 1 : 0, (double)N_);
+                     keff_parts, balance ? 1 : 0, (double)N_);
   if (rt != MOC_SCALAR_FLUX && G_ >= kFsrResidualGroups)
     hipLaunchKernelGGL(fsr_residual_kernel, dim3(blocks_for(N_, kReduceBlock)), dim3(kReduceBlock), 0,
                        stream_, fa, mode == MODE_EIGEN ? 1 : 0, rt);
@@ -3768,7 +3810,7 @@ void HipSolver::launchInit(Mode mode, double k0) {
   }
   // normalise, store old flux, sources of iteration 0
   launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+  hipLaunchKernelGGL(update_kernel<0>, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, normalize ? 1 : 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   // fixed-point scale of the first sweep (start fluxes are zero); the old
   // flux just written becomes the current one
@@ -3830,7 +3872,7 @@ void HipSolver::launchRestart(Mode mode, double k0) {
   const FsrArgs fa = fsrArgs();
   // storeFSRFluxes and the sources of iteration 0, no normalisation
   launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+  hipLaunchKernelGGL(update_kernel<0>, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 1);
   HIP_CHECK(hipGetLastError());
@@ -3883,7 +3925,7 @@ void HipSolver::launchStartupSweeps() {
   hipLaunchKernelGGL(post_sweep_kernel, dim3(num_partials_), dim3(kReduceBlock), 0, stream_, fa, 1, 0);
   // computeFSRSources(0) of this flux, unnormalised; the old flux is kept
   launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+  hipLaunchKernelGGL(update_kernel<0>, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
   HIP_CHECK(hipGetLastError());
@@ -3968,7 +4010,7 @@ void HipSolver::krylovSweep(int source_kind) {
   fa.leak = nullptr;
   fa.stab_type = -1;
   launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+  hipLaunchKernelGGL(update_kernel<0>, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, 0, 0, 0, 1, 0, source_kind, 0, 0, 0.0);
   hipLaunchKernelGGL(tally_scale_kernel, dim3(1), dim3(1), 0, stream_, d_state_, sigma_t_max_, 1, 0);
   launchSweep(d_psi_[psi_cur_], d_psi_[psi_cur_ ^ 1], stream_);
@@ -4011,7 +4053,7 @@ void HipSolver::computeFSRSources(int iteration, double* out, int64_t n) {
   fa.leak = nullptr;
   fa.stab_type = -1;
   launchScatter(stream_);
-  hipLaunchKernelGGL(update_kernel, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
+  hipLaunchKernelGGL(update_kernel<0>, dim3(blocks_for(NG, kReduceBlock)), dim3(kReduceBlock), 0,
                      stream_, fa, 0, 0, 0, 1, 0, kSrcTotal, 0, 0, 0.0);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(out, d_Q_, NG * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -5141,14 +5183,19 @@ void SolveBatch::launchStep(hipStream_t st, int n, int parity) {
   const unsigned ny = (unsigned)n;
   hipLaunchKernelGGL(post_sweep_batch_kernel, dim3((unsigned)dims_.post_blocks, ny), dim3(kReduceBlock), 0,
                      st, d_table_, bps);
+  // rows with few partials take the keff step inside update_kernel
+  if (dims_.post_blocks > kFuseKeffMaxParts)
+    hipLaunchKernelGGL(keff_batch_kernel, dim3(1, ny), dim3(kReduceBlock), 0, st, d_table_);
   if (dims_.mfma_blocks > 0)
     hipLaunchKernelGGL(scatter_mfma_batch_kernel, dim3((unsigned)dims_.mfma_blocks, ny), dim3(256), 0, st,
                        d_table_);
   if (dims_.fp32_blocks > 0)
     hipLaunchKernelGGL(scatter_fp32_batch_kernel, dim3((unsigned)dims_.fp32_blocks, ny), dim3(256), 0, st,
                        d_table_);
-  hipLaunchKernelGGL(update_batch_kernel, dim3((unsigned)dims_.upd_blocks, ny), dim3(kReduceBlock), 0, st,
-                     d_table_, res_type_);
+  bool fuse_any = false;  // some row takes the keff step in update_kernel
+  for (const HipSolver* m : geos_) fuse_any = fuse_any || m->num_partials_ <= kFuseKeffMaxParts;
+  auto* const upd = fuse_any ? update_batch_kernel<1> : update_batch_kernel<0>;
+  hipLaunchKernelGGL(upd, dim3((unsigned)dims_.upd_blocks, ny), dim3(kReduceBlock), 0, st, d_table_, res_type_);
   if (dims_.fsr_blocks > 0)
     hipLaunchKernelGGL(fsr_residual_batch_kernel, dim3((unsigned)dims_.fsr_blocks, ny), dim3(kReduceBlock),
                        0, st, d_table_, res_type_);
