@@ -1079,17 +1079,32 @@ __device__ __forceinline__ double wave_sum_all(const double* x, int n) {
   return v;
 }
 
-__device__ double block_max(double v) {
-  __shared__ double sh[kReduceBlock / 64];
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+// block_sum of two values in one pass (one pair of barriers): each result
+// has block_sum's bits and is valid in thread 0.  MX: the second is a
+// maximum, taken over the same tree.
+template <bool MX>
+__device__ void block_reduce2(double& a, double& b) {
+  __shared__ double sh[2][kReduceBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    const double o = __shfl_down(b, off, 64);
+    b = MX ? fmax(b, o) : b + o;
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) sh[wid] = v;
+  if (lane == 0) {
+    sh[0][wid] = a;
+    sh[1][wid] = b;
+  }
   __syncthreads();
-  double r = 0;
+  double ra = 0, rb = 0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmax(r, sh[i]);
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      ra += sh[0][i];
+      rb = MX ? fmax(rb, sh[1][i]) : rb + sh[1][i];
+    }
   __syncthreads();
-  return r;  // valid in thread 0
+  a = ra;
+  b = rb;
 }
 
 // Tally of FSR r, group g from the per-segment store: the sum over the FSR's
@@ -1274,12 +1289,13 @@ __device__ __forceinline__ void post_sweep_kernel_body(const FsrArgs& a, int fro
   }
   if (balance && a.leak)
     for (int64_t q = i; q < a.leak_n; q += (int64_t)gridDim.x * blockDim.x) leak += (double)a.leak[q];
-  const double s0 = block_sum(rate);
-  const double s2 = block_sum(rate_s);
+  double s0 = rate, s2 = rate_s;
+  block_reduce2<false>(s0, s2);
   double s1 = 0.0, s3 = 0.0;
   if (balance) {
-    s1 = block_sum(absr);
-    s3 = block_sum(leak);
+    s1 = absr;
+    s3 = leak;
+    block_reduce2<false>(s1, s3);
   }
   if (threadIdx.x == 0) {
     a.partials[kPartFission * a.pstride + blockIdx.x] = s0;
@@ -1588,8 +1604,8 @@ __device__ __forceinline__ void update_kernel_body(const FsrArgs& a, int normali
       }
     }
   }
-  const double s = block_sum(res);
-  const double qm = block_max(qmax);
+  double s = res, qm = qmax;
+  block_reduce2<true>(s, qm);
   if (threadIdx.x == 0) {
     a.partials[kPartResidual * a.pstride + blockIdx.x] = s;
     if (qm > 0.0) {
