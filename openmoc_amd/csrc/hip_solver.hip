@@ -2715,6 +2715,21 @@ void HipSolver::setTracks(const moc_track_data_t& td) {
           h_slot_[k] = sl;
         }
       const int32_t n = (int32_t)h_slot_fsr_.size() - s0;
+      {
+        // slots in FSR order within the group: lanes on neighbouring tracks
+        // then meet on fewer LDS banks than with first-encounter numbering
+        std::vector<int32_t> perm(n), newpos(n), sorted(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::sort(perm.begin(), perm.end(),
+                  [&](int32_t x, int32_t y) { return h_slot_fsr_[s0 + x] < h_slot_fsr_[s0 + y]; });
+        for (int32_t i = 0; i < n; i++) {
+          newpos[perm[i]] = i;
+          sorted[i] = h_slot_fsr_[s0 + perm[i]];
+        }
+        std::copy(sorted.begin(), sorted.end(), h_slot_fsr_.begin() + s0);
+        for (int64_t t : members[g])
+          for (int64_t k = td.seg_offset[t]; k < td.seg_offset[t + 1]; k++) h_slot_[k] = newpos[h_slot_[k]];
+      }
       for (int32_t i = 0; i < n; i++) slot_of[h_slot_fsr_[s0 + i]] = -1;
       h_grp_nslots_.push_back(n);
       max_grp_slots_ = std::max(max_grp_slots_, n);
